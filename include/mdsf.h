@@ -1,0 +1,155 @@
+/*
+ * mdsf.h - C ABI of the MI355X (gfx950) structure-factor hot path.
+ *
+ * Drop-in replacement for the compute core of joeyelk/MD-Structure-Factor
+ * `dens.compute_sf` (dens.py:166-346): per-frame Gaussian deposition onto a
+ * periodic grid with the reference's border fold, a 3-D real-to-complex FFT,
+ * and the frame sum of |F(q)|^2 into an FP64 structure factor.
+ *
+ * Conventions
+ *   - Plain C types only.  Pointers named `d_*` are device pointers owned by
+ *     the caller; the plan owns its workspaces and twiddle tables.
+ *   - Every entry point returns MDSF_OK or an error code; the message of the
+ *     last error on the calling thread is returned by mdsf_last_error().
+ *   - Work is enqueued on the given HIP stream (NULL = legacy default stream);
+ *     nothing synchronises the device except where stated.
+ *   - A plan is not thread-safe; use one plan per host thread.
+ *   - Grid arrays are C-order [x][y][z]; the spectrum keeps the reference's
+ *     half-spectrum shape (Nx, Ny, Nz/2+1).
+ */
+#ifndef MDSF_H
+#define MDSF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDSF_OK 0
+#define MDSF_ERR_ARG 1        /* invalid argument (the reference raises ValueError)   */
+#define MDSF_ERR_HIP 2        /* HIP runtime error                                    */
+#define MDSF_ERR_RANGE 3      /* an atom window leaves the padded grid (ValueError)   */
+#define MDSF_ERR_TYPE 4       /* an atom type index is out of range (KeyError)        */
+#define MDSF_ERR_UNSUPPORTED 5
+
+#define MDSF_F32 0
+#define MDSF_F64 1
+
+typedef struct mdsf_plan mdsf_plan;
+typedef void* mdsf_stream; /* a hipStream_t */
+
+/* One atom type: replaces the per-type values `compute_sf` reads from the
+ * radii dictionary and `get_borders` (dens.py:38-43, 287, 305-308). */
+typedef struct mdsf_type_desc {
+  double amp;        /* Nel / sigma**3                               dens.py:308 */
+  double two_sig2;   /* 2 * sigma**2                                 dens.py:308 */
+  int32_t half[3];   /* A = int(sigma*sqrt(ln(Nel/1e-24))/dr) per axis  dens.py:42,287 */
+  int32_t count;     /* atoms of this type in the system             */
+} mdsf_type_desc;
+
+/* One system: grid, border, cell and types (dens.py:181-263). */
+typedef struct mdsf_grid_desc {
+  int32_t n[3];          /* Nspatialgrid, even                        dens.py:181-189 */
+  int32_t nborder;       /* Nborder                                   dens.py:231     */
+  double dr[3];          /* L / N                                     dens.py:202     */
+  double ucell[9];       /* row-major 3x3; buf2 = buf @ ucell          dens.py:301     */
+  int32_t ntypes;
+  const mdsf_type_desc* types;
+  int32_t natoms;
+} mdsf_grid_desc;
+
+/* Plan lifecycle.  Validates the descriptor (Nborder <= N, even N, cell
+ * metric without xz / yz coupling), allocates workspaces on the current
+ * device and generates the FFT twiddles in FP64 on the GPU. */
+int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out);
+int mdsf_plan_destroy(mdsf_plan* plan);
+const char* mdsf_last_error(void);
+int mdsf_version(void);
+
+/* Reset the plan's FP64 S(q) accumulator to zero (the `sf = np.zeros` of
+ * dens.py:241). */
+int mdsf_sf_reset(mdsf_plan* plan, mdsf_stream stream);
+
+/* Synchronises `stream` and reports (then clears) the error flags the
+ * kernels raised since the last check: MDSF_ERR_TYPE for a type index out of
+ * range (the reference's KeyError, dens.py:287) and MDSF_ERR_RANGE for an
+ * atom whose window leaves the padded grid (the reference's broadcast
+ * ValueError at dens.py:308). */
+int mdsf_check_errors(mdsf_plan* plan, mdsf_stream stream);
+
+/* The frame loop of dens.py:277-321 for `nframes` frames.
+ *   d_coords   (nframes, natoms, 3) of dtype MDSF_F32 / MDSF_F64; rescaled and
+ *              wrapped IN PLACE exactly as dens.py:54-58 and 209-221 do.
+ *   d_types    int32[natoms] type index of each atom (the `typ[im]` lookup).
+ *   d_scale    double[nframes*3] per-frame rescale factors avg(dims)/dims[t]
+ *              (dens.py:53).
+ *   box        host double[3], the mean box L used by the wrap (dens.py:211-212).
+ *   arith      MDSF_F32 when coords and dims are both float32 (numpy then does
+ *              the rescale and wrap in float32), else MDSF_F64.
+ *   wrap_lo/hi atom index range the wrap touches (all atoms below 1e6 atoms;
+ *              see dens.py:213-221 for the buffered branch).
+ *   d_occupancy optional uint32[Nx*Ny*Nz]: += 1 at (ir mod N) per atom and frame.
+ *   d_home_cells optional int32[natoms*3]: ir of the LAST frame (dens.py:285).
+ * Adds sum_frames |rfftn(folded density)|^2 into the plan accumulator. */
+int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype,
+                           int64_t nframes, const int32_t* d_types,
+                           const double* d_scale, const double box[3], int arith,
+                           int64_t wrap_lo, int64_t wrap_hi,
+                           uint32_t* d_occupancy, int32_t* d_home_cells,
+                           mdsf_stream stream);
+
+/* Only the in-place rescale and wrap of dens.py:179,206-221 for `nframes`
+ * frames (arguments as for mdsf_accumulate_frames).  The RANDOM_NOISE branch
+ * of the reference replaces the density but still mutates the coordinates. */
+int mdsf_rescale_wrap(mdsf_plan* plan, void* d_coords, int dtype, int64_t nframes,
+                      const double* d_scale, const double box[3], int arith, int64_t wrap_lo,
+                      int64_t wrap_hi, mdsf_stream stream);
+
+/* One frame of deposition + periodic fold (dens.py:283-311), no FFT:
+ * writes the folded FP32 density d_density[Nx*Ny*Nz].  Coordinates are
+ * processed as in mdsf_accumulate_frames (rescale with d_scale[0..2], wrap). */
+int mdsf_deposit_fold(mdsf_plan* plan, void* d_coords, int dtype,
+                      const int32_t* d_types, const double* d_scale,
+                      const double box[3], int arith, int64_t wrap_lo,
+                      int64_t wrap_hi, float* d_density, int32_t* d_home_cells,
+                      mdsf_stream stream);
+
+/* |rfftn(density)|^2 summed into the accumulator for a caller-supplied FP32
+ * periodic density (the RANDOM_NOISE branch, dens.py:279-280, 313-318). */
+int mdsf_accumulate_density(mdsf_plan* plan, const float* d_density,
+                            mdsf_stream stream);
+
+/* Forward R2C transform of one FP32 density (np.fft.rfftn, dens.py:313):
+ * d_spectrum receives complex64 (Nx, Ny, Nz/2+1) interleaved re/im. */
+int mdsf_rfft3(mdsf_plan* plan, const float* d_density, float* d_spectrum,
+               mdsf_stream stream);
+
+/* The accumulated S(q) in the reference layout: d_sf double (Nx, Ny, Nz/2+1)
+ * (the `sf` key of the .npz, dens.py:318,346). */
+int mdsf_sf_read(mdsf_plan* plan, double* d_sf, mdsf_stream stream);
+
+/* The derived outputs of dens.py:323-344 from an S(q) in reference layout:
+ *   d_sfplt    (Nx-2, Ny-2, Nz-1)        get_dplot(sf), dens.py:142-163
+ *   d_kgrid    (Nx, Ny, Nz/2+1, 4)       dens.py:325-334
+ *   d_kgridplt (Nx-2, Ny-2, Nz-1, 4)     dens.py:336-344
+ * box_f32 != 0 when L is float32 (numpy then divides in float32). Any output
+ * pointer may be NULL. */
+int mdsf_outputs(mdsf_plan* plan, const double* d_sf, const double box[3],
+                 int box_f32, double* d_sfplt, double* d_kgrid,
+                 double* d_kgridplt, mdsf_stream stream);
+
+/* Ewald-sphere sections of a kgridplt volume (plot2d.py:218-263): trilinear
+ * interpolation of channel 3 at the points of the xy, xz and yz sections for
+ * wavelength `wavelength` (Angstrom); NaN outside the grid
+ * (RegularGridInterpolator(..., bounds_error=False)).
+ *   d_kgridplt (nx, ny, nz, 4) double; outputs (nx,ny), (nx,nz), (ny,nz). */
+int mdsf_ewald_slices(const double* d_kgridplt, int32_t nx, int32_t ny,
+                      int32_t nz, double wavelength, double* d_xy, double* d_xz,
+                      double* d_yz, mdsf_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDSF_H */

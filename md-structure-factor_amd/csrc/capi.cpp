@@ -1,0 +1,382 @@
+// C ABI of libmdsf: plan lifecycle and the per-frame pipeline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "mdsf_internal.h"
+
+namespace mdsf {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(MDSF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+int bucket_edge(int half) {
+  if (half < 24) return 16;
+  if (half < 48) return 32;
+  if (half < 96) return 64;
+  return 128;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+void free_plan(Plan* p) {
+  if (!p) return;
+  void* ptrs[] = {p->types, p->ir4,     p->key,     p->rpos,     p->bucket_count, p->bucket_off,
+                  p->bucket_cur, p->s_idx, p->s_ir, p->s_r,   p->tables,       p->err,
+                  p->rho,   p->nyq,     p->sf_main, p->sf_nyq, p->ax_x.tw,      p->ax_y.tw,
+                  p->ax_zc.tw, p->tw_r2c};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete p;
+}
+
+int make_axis(FftAxis& ax, int n, hipStream_t s) {
+  ax.n = n;
+  if (!fft_factor(n, ax.radix, &ax.nstages))
+    return fail(MDSF_ERR_UNSUPPORTED,
+                "FFT length " + std::to_string(n) + " has a prime factor above 31");
+  MDSF_HIP(dalloc(&ax.tw, (size_t)n));
+  MDSF_HIP(fft_make_twiddles(ax.tw, n, s));
+  return MDSF_OK;
+}
+
+int run_fft_accumulate(Plan& p, hipStream_t s) {
+  float2* main = reinterpret_cast<float2*>(p.rho);
+  MDSF_HIP(fft_r2c_z(p, p.rho, main, p.nyq, s));
+  MDSF_HIP(fft_y(p, main, p.nyq, s));
+  MDSF_HIP(fft_x_accum(p, main, p.nyq, s));
+  return MDSF_OK;
+}
+
+int check_frame_args(Plan* p, void* coords, int dtype, const int32_t* types, const double* scale,
+                     const double* box, int arith) {
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  if (p->sys.natoms > 0 && (!coords || !types || !scale))
+    return fail(MDSF_ERR_ARG, "null coordinate / type / scale pointer");
+  if (!box) return fail(MDSF_ERR_ARG, "null box");
+  if (dtype != MDSF_F32 && dtype != MDSF_F64) return fail(MDSF_ERR_ARG, "dtype must be F32 or F64");
+  if (arith != MDSF_F32 && arith != MDSF_F64) return fail(MDSF_ERR_ARG, "arith must be F32 or F64");
+  if (dtype == MDSF_F64 && arith == MDSF_F32)
+    return fail(MDSF_ERR_ARG, "float64 coordinates are rescaled in float64");
+  return MDSF_OK;
+}
+
+}  // namespace
+}  // namespace mdsf
+
+using namespace mdsf;
+
+extern "C" {
+
+const char* mdsf_last_error(void) { return g_last_error.c_str(); }
+
+int mdsf_version(void) { return 10000; }
+
+int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
+  if (!desc || !out) return fail(MDSF_ERR_ARG, "null descriptor or output");
+  *out = nullptr;
+  for (int d = 0; d < 3; ++d) {
+    const int n = desc->n[d];
+    if (n < 2 || (n % 2) != 0) return fail(MDSF_ERR_ARG, "grid sizes must be even and >= 2");
+    if (n > 4096) return fail(MDSF_ERR_UNSUPPORTED, "grid size above 4096");
+    if (desc->nborder > n)
+      return fail(MDSF_ERR_ARG, "Nborder exceeds the grid (the reference's fold needs Nborder <= N)");
+    if (!(desc->dr[d] > 0.0)) return fail(MDSF_ERR_ARG, "dr must be positive");
+  }
+  if (desc->nborder < 0) return fail(MDSF_ERR_ARG, "negative Nborder");
+  if (desc->ntypes < 0 || desc->ntypes > kMaxTypes)
+    return fail(MDSF_ERR_ARG, "between 0 and 64 atom types are supported");
+  if (desc->natoms < 0) return fail(MDSF_ERR_ARG, "negative atom count");
+  if (desc->ntypes > 0 && !desc->types) return fail(MDSF_ERR_ARG, "null type table");
+
+  const double* U = desc->ucell;
+  double G[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) G[i][j] = U[3 * i + 0] * U[3 * j + 0] + U[3 * i + 1] * U[3 * j + 1] + U[3 * i + 2] * U[3 * j + 2];
+  if (G[0][2] != 0.0 || G[1][2] != 0.0)
+    return fail(MDSF_ERR_UNSUPPORTED,
+                "cell matrices coupling z with x or y are not supported (callers use monoclinic "
+                "xy cells)");
+
+  Plan* p = new Plan();
+  SysDev& s = p->sys;
+  for (int d = 0; d < 3; ++d) {
+    s.n[d] = desc->n[d];
+    s.dr[d] = desc->dr[d];
+    s.ntiles[d] = (desc->n[d] + kTile - 1) / kTile;
+  }
+  s.nb = desc->nborder;
+  s.g00 = G[0][0];
+  s.g01 = G[0][1];
+  s.g11 = G[1][1];
+  s.g22 = G[2][2];
+  s.mode = (G[0][1] == 0.0) ? MODE_DIAG : MODE_XY;
+  s.ntypes = desc->ntypes;
+  s.natoms = desc->natoms;
+
+  long long table_floats = 0;
+  int start = 0, bucket_base = 0;
+  long long counted = 0;
+  for (int t = 0; t < desc->ntypes; ++t) {
+    const mdsf_type_desc& in = desc->types[t];
+    TypeDev& ty = p->types_host[t];
+    ty.amp = in.amp;
+    ty.two_sig2 = in.two_sig2;
+    ty.amp_f = (float)in.amp;
+    ty.c2 = (float)(1.4426950408889634 / in.two_sig2);
+    if (!(in.two_sig2 > 0.0)) {
+      free_plan(p);
+      return fail(MDSF_ERR_ARG, "sigma must be positive");
+    }
+    int stride = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (in.half[d] < 0 || in.half[d] > desc->nborder) {
+        free_plan(p);
+        return fail(MDSF_ERR_ARG, "type window half-width outside [0, Nborder]");
+      }
+      ty.half[d] = in.half[d];
+      stride += 2 * in.half[d];
+      ty.bsz[d] = bucket_edge(in.half[d]);
+      ty.nbk[d] = (desc->n[d] + ty.bsz[d] - 1) / ty.bsz[d];
+    }
+    ty.count = in.count;
+    ty.type_start = start;
+    ty.stride = stride;
+    ty.table_base = table_floats;
+    ty.bucket_base = bucket_base;
+    table_floats += (long long)in.count * stride;
+    start += in.count;
+    counted += in.count;
+    bucket_base += ty.nbk[0] * ty.nbk[1] * ty.nbk[2];
+  }
+  if (counted != desc->natoms) {
+    free_plan(p);
+    return fail(MDSF_ERR_ARG, "type counts do not add up to natoms");
+  }
+  s.nbuckets = bucket_base;
+  p->table_floats = table_floats;
+  (void)hipGetDevice(&p->device);
+
+  const size_t nat = (size_t)desc->natoms;
+  const size_t nx = desc->n[0], ny = desc->n[1], nz = desc->n[2];
+  const size_t m = nz / 2;
+  hipError_t e = hipSuccess;
+#define TRY(x)                          \
+  do {                                  \
+    e = (x);                            \
+    if (e != hipSuccess) {              \
+      int rc = hip_fail(e, #x);         \
+      free_plan(p);                     \
+      return rc;                        \
+    }                                   \
+  } while (0)
+  TRY(dalloc(&p->types, (size_t)kMaxTypes));
+  TRY(hipMemcpy(p->types, p->types_host, sizeof(TypeDev) * kMaxTypes, hipMemcpyHostToDevice));
+  TRY(dalloc(&p->ir4, nat));
+  TRY(dalloc(&p->key, nat));
+  TRY(dalloc(&p->rpos, 3 * nat));
+  TRY(dalloc(&p->bucket_count, (size_t)s.nbuckets + 1));
+  TRY(dalloc(&p->bucket_off, (size_t)s.nbuckets + 1));
+  TRY(dalloc(&p->bucket_cur, (size_t)s.nbuckets + 1));
+  TRY(hipMemset(p->bucket_count, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
+  TRY(hipMemset(p->bucket_off, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
+  TRY(dalloc(&p->s_idx, nat));
+  TRY(dalloc(&p->s_ir, nat));
+  TRY(dalloc(&p->s_r, 3 * nat));
+  TRY(dalloc(&p->tables, (size_t)table_floats));
+  TRY(dalloc(&p->err, (size_t)1));
+  TRY(hipMemset(p->err, 0, sizeof(int)));
+  TRY(dalloc(&p->rho, nx * ny * nz));
+  TRY(dalloc(&p->nyq, nx * ny));
+  TRY(dalloc(&p->sf_main, nx * ny * m));
+  TRY(dalloc(&p->sf_nyq, nx * ny));
+  TRY(hipMemset(p->sf_main, 0, sizeof(double) * nx * ny * m));
+  TRY(hipMemset(p->sf_nyq, 0, sizeof(double) * nx * ny));
+#undef TRY
+  int rc;
+  if ((rc = make_axis(p->ax_x, desc->n[0], nullptr)) != MDSF_OK ||
+      (rc = make_axis(p->ax_y, desc->n[1], nullptr)) != MDSF_OK ||
+      (rc = make_axis(p->ax_zc, desc->n[2] / 2, nullptr)) != MDSF_OK) {
+    free_plan(p);
+    return rc;
+  }
+  if ((e = dalloc(&p->tw_r2c, nz)) != hipSuccess || (e = fft_make_twiddles(p->tw_r2c, (int)nz, nullptr)) != hipSuccess) {
+    rc = hip_fail(e, "r2c twiddles");
+    free_plan(p);
+    return rc;
+  }
+  if ((e = hipDeviceSynchronize()) != hipSuccess) {
+    rc = hip_fail(e, "plan set-up");
+    free_plan(p);
+    return rc;
+  }
+  *out = reinterpret_cast<mdsf_plan*>(p);
+  return MDSF_OK;
+}
+
+int mdsf_plan_destroy(mdsf_plan* plan) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return MDSF_OK;
+  (void)hipDeviceSynchronize();
+  free_plan(p);
+  return MDSF_OK;
+}
+
+int mdsf_sf_reset(mdsf_plan* plan, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t nxy = (size_t)p->sys.n[0] * p->sys.n[1];
+  MDSF_HIP(hipMemsetAsync(p->sf_main, 0, sizeof(double) * nxy * (p->sys.n[2] / 2), s));
+  MDSF_HIP(hipMemsetAsync(p->sf_nyq, 0, sizeof(double) * nxy, s));
+  return MDSF_OK;
+}
+
+int mdsf_check_errors(mdsf_plan* plan, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  int flag = 0;
+  MDSF_HIP(hipMemcpyAsync(&flag, p->err, sizeof(int), hipMemcpyDeviceToHost, s));
+  MDSF_HIP(hipStreamSynchronize(s));
+  if (flag) MDSF_HIP(hipMemsetAsync(p->err, 0, sizeof(int), s));
+  if (flag & ERRBIT_TYPE) return fail(MDSF_ERR_TYPE, "atom type index out of range");
+  if (flag & ERRBIT_RANGE)
+    return fail(MDSF_ERR_RANGE,
+                "an atom's window leaves the padded grid (atom more than the border outside the "
+                "cell)");
+  return MDSF_OK;
+}
+
+int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nframes,
+                           const int32_t* d_types, const double* d_scale, const double box[3],
+                           int arith, int64_t wrap_lo, int64_t wrap_hi, uint32_t* d_occupancy,
+                           int32_t* d_home_cells, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  int rc = check_frame_args(p, d_coords, dtype, d_types, d_scale, box, arith);
+  if (rc != MDSF_OK) return rc;
+  if (nframes < 0) return fail(MDSF_ERR_ARG, "negative frame count");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t esz = dtype == MDSF_F32 ? 4 : 8;
+  const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
+  for (int64_t t = 0; t < nframes; ++t) {
+    char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
+    int32_t* home = (t == nframes - 1) ? d_home_cells : nullptr;
+    MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
+                         d_occupancy, home, s));
+    MDSF_HIP(launch_binning(*p, s));
+    MDSF_HIP(launch_deposit(*p, p->rho, s));
+    rc = run_fft_accumulate(*p, s);
+    if (rc != MDSF_OK) return rc;
+  }
+  return MDSF_OK;
+}
+
+int mdsf_rescale_wrap(mdsf_plan* plan, void* d_coords, int dtype, int64_t nframes,
+                      const double* d_scale, const double box[3], int arith, int64_t wrap_lo,
+                      int64_t wrap_hi, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  if (p->sys.natoms > 0 && (!d_coords || !d_scale)) return fail(MDSF_ERR_ARG, "null coordinates");
+  if (!box) return fail(MDSF_ERR_ARG, "null box");
+  if (dtype != MDSF_F32 && dtype != MDSF_F64) return fail(MDSF_ERR_ARG, "dtype must be F32 or F64");
+  if (dtype == MDSF_F64 && arith == MDSF_F32)
+    return fail(MDSF_ERR_ARG, "float64 coordinates are rescaled in float64");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t esz = dtype == MDSF_F32 ? 4 : 8;
+  const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
+  for (int64_t t = 0; t < nframes; ++t) {
+    char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
+    MDSF_HIP(launch_prep(*p, c, dtype, arith, nullptr, d_scale + 3 * t, box, wrap_lo, wrap_hi,
+                         nullptr, nullptr, s, true));
+  }
+  return MDSF_OK;
+}
+
+int mdsf_deposit_fold(mdsf_plan* plan, void* d_coords, int dtype, const int32_t* d_types,
+                      const double* d_scale, const double box[3], int arith, int64_t wrap_lo,
+                      int64_t wrap_hi, float* d_density, int32_t* d_home_cells,
+                      mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  int rc = check_frame_args(p, d_coords, dtype, d_types, d_scale, box, arith);
+  if (rc != MDSF_OK) return rc;
+  if (!d_density) return fail(MDSF_ERR_ARG, "null density");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MDSF_HIP(launch_prep(*p, d_coords, dtype, arith, d_types, d_scale, box, wrap_lo, wrap_hi,
+                       nullptr, d_home_cells, s));
+  MDSF_HIP(launch_binning(*p, s));
+  MDSF_HIP(launch_deposit(*p, d_density, s));
+  return MDSF_OK;
+}
+
+int mdsf_accumulate_density(mdsf_plan* plan, const float* d_density, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !d_density) return fail(MDSF_ERR_ARG, "null plan or density");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t bytes = sizeof(float) * p->sys.n[0] * p->sys.n[1] * p->sys.n[2];
+  MDSF_HIP(hipMemcpyAsync(p->rho, d_density, bytes, hipMemcpyDeviceToDevice, s));
+  return run_fft_accumulate(*p, s);
+}
+
+int mdsf_rfft3(mdsf_plan* plan, const float* d_density, float* d_spectrum, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !d_density || !d_spectrum) return fail(MDSF_ERR_ARG, "null argument");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t bytes = sizeof(float) * p->sys.n[0] * p->sys.n[1] * p->sys.n[2];
+  MDSF_HIP(hipMemcpyAsync(p->rho, d_density, bytes, hipMemcpyDeviceToDevice, s));
+  float2* main = reinterpret_cast<float2*>(p->rho);
+  MDSF_HIP(fft_r2c_z(*p, p->rho, main, p->nyq, s));
+  MDSF_HIP(fft_y(*p, main, p->nyq, s));
+  MDSF_HIP(fft_x(*p, main, p->nyq, s));
+  MDSF_HIP(launch_spectrum_read(*p, main, p->nyq, d_spectrum, s));
+  return MDSF_OK;
+}
+
+int mdsf_sf_read(mdsf_plan* plan, double* d_sf, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !d_sf) return fail(MDSF_ERR_ARG, "null argument");
+  MDSF_HIP(launch_sf_read(*p, d_sf, reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_outputs(mdsf_plan* plan, const double* d_sf, const double box[3], int box_f32,
+                 double* d_sfplt, double* d_kgrid, double* d_kgridplt, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !d_sf || !box) return fail(MDSF_ERR_ARG, "null argument");
+  MDSF_HIP(launch_outputs(p->sys.n, d_sf, box, box_f32, d_sfplt, d_kgrid, d_kgridplt,
+                          reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_ewald_slices(const double* d_kgridplt, int32_t nx, int32_t ny, int32_t nz,
+                      double wavelength, double* d_xy, double* d_xz, double* d_yz,
+                      mdsf_stream stream) {
+  if (!d_kgridplt) return fail(MDSF_ERR_ARG, "null kgridplt");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(MDSF_ERR_ARG, "each axis needs at least 2 points");
+  if (!(wavelength > 0.0)) return fail(MDSF_ERR_ARG, "wavelength must be positive");
+  MDSF_HIP(launch_ewald(d_kgridplt, nx, ny, nz, wavelength, d_xy, d_xz, d_yz,
+                        reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+}  // extern "C"

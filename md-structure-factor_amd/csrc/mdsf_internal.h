@@ -1,0 +1,125 @@
+// Internal declarations shared by the HIP translation units of libmdsf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/mdsf.h"
+
+namespace mdsf {
+
+constexpr int kMaxTypes = 64;
+constexpr int kTile = 16;       // deposition tile edge along x, y and z
+constexpr int kBatch = 32;      // atoms staged per deposition batch
+constexpr int kMaxStages = 16;  // FFT stages per axis
+
+enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
+enum Mode { MODE_DIAG = 0, MODE_XY = 1 };
+
+// Error bits raised by kernels into the plan's device flag word.
+enum ErrBit { ERRBIT_RANGE = 1, ERRBIT_TYPE = 2 };
+
+// Per-type constants on the device.
+struct TypeDev {
+  double amp;        // Nel / sigma^3
+  double two_sig2;   // 2 sigma^2
+  float c2;          // log2(e) / (2 sigma^2), for exp2 in the XY path
+  float amp_f;
+  int half[3];       // window half-widths A
+  int count;         // atoms of this type
+  int type_start;    // first sorted index of this type
+  int stride;        // floats per atom: 2A_x + 2A_y + 2A_z
+  long long table_base;  // float offset of this type's per-atom tables
+  int bsz[3];        // bucket edge per axis
+  int nbk[3];        // buckets per axis
+  int bucket_base;   // first bucket id of this type
+};
+
+// Per-system constants passed by value to kernels.
+struct SysDev {
+  int n[3];
+  int nb;
+  double dr[3];
+  double g00, g01, g11, g22;  // cell metric G = U U^T (no xz / yz coupling)
+  int ntypes;
+  int natoms;
+  int nbuckets;
+  int ntiles[3];
+  int mode;
+};
+
+// One FFT axis: length, radix sequence and twiddles W_n^k (k < n), FP32
+// values generated in FP64 on the device.
+struct FftAxis {
+  int n = 0;
+  int nstages = 0;
+  int radix[kMaxStages] = {};
+  float2* tw = nullptr;
+};
+
+struct Plan {
+  SysDev sys{};
+  TypeDev types_host[kMaxTypes]{};
+  TypeDev* types = nullptr;
+  int device = 0;
+  long long table_floats = 0;
+  // per-atom scratch (unsorted)
+  int4* ir4 = nullptr;
+  int* key = nullptr;
+  double* rpos = nullptr;
+  // binning
+  int* bucket_count = nullptr;
+  int* bucket_off = nullptr;
+  int* bucket_cur = nullptr;
+  int* s_idx = nullptr;
+  int4* s_ir = nullptr;
+  double* s_r = nullptr;
+  float* tables = nullptr;
+  int* err = nullptr;
+  // grids
+  float* rho = nullptr;          // Nx*Ny*Nz floats == Nx*Ny*(Nz/2) complex (in place)
+  float2* nyq = nullptr;         // Nx*Ny complex: the kz = Nz/2 plane
+  double* sf_main = nullptr;     // Nx*Ny*(Nz/2) FP64 accumulator
+  double* sf_nyq = nullptr;      // Nx*Ny FP64 accumulator
+  // FFT
+  FftAxis ax_x, ax_y, ax_zc;     // zc: complex length Nz/2
+  float2* tw_r2c = nullptr;      // W_Nz^k, k in [0, Nz/2]
+};
+
+// error handling -------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define MDSF_HIP(call)                                        \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return ::mdsf::hip_fail(e_, #call); \
+  } while (0)
+
+// kernel launch wrappers (defined in the .hip units) --------------------------
+hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_t* types,
+                       const double* scale3, const double box[3], int64_t wrap_lo,
+                       int64_t wrap_hi, uint32_t* occ, int32_t* home, hipStream_t s,
+                       bool coords_only = false);
+hipError_t launch_binning(Plan& p, hipStream_t s);
+hipError_t launch_deposit(Plan& p, float* out, hipStream_t s);
+
+hipError_t fft_make_twiddles(float2* tw, int n, hipStream_t s);
+hipError_t fft_r2c_z(Plan& p, const float* in, float2* main, float2* nyq, hipStream_t s);
+hipError_t fft_y(Plan& p, float2* main, float2* nyq, hipStream_t s);
+hipError_t fft_x_accum(Plan& p, float2* main, float2* nyq, hipStream_t s);
+hipError_t fft_x(Plan& p, float2* main, float2* nyq, hipStream_t s);
+bool fft_factor(int n, int* radix, int* nstages);
+
+hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s);
+hipError_t launch_spectrum_read(const Plan& p, const float2* main, const float2* nyq,
+                                float* out, hipStream_t s);
+hipError_t launch_outputs(const int n[3], const double* sf, const double box[3], int box_f32,
+                          double* sfplt, double* kgrid, double* kgridplt, hipStream_t s);
+hipError_t launch_ewald(const double* kp, int nx, int ny, int nz, double wl, double* xy,
+                        double* xz, double* yz, hipStream_t s);
+
+}  // namespace mdsf

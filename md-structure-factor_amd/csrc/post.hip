@@ -1,0 +1,228 @@
+// Output assembly on the GPU:
+//   sf in the reference layout                 dens.py:241,318
+//   sfplt = get_dplot(sf)                      dens.py:142-163,323
+//   kgrid, kgridplt                            dens.py:325-344
+//   Ewald-sphere sections of kgridplt          plot2d.py:218-263
+#include <cmath>
+
+#include "mdsf_internal.h"
+
+namespace mdsf {
+
+namespace {
+
+__global__ void k_sf_read(const double* __restrict__ sfm, const double* __restrict__ sfn,
+                          double* __restrict__ out, int nx, int ny, int m) {
+  const long long h = m + 1;
+  const long long total = (long long)nx * ny * h;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / h;
+    const int kz = (int)(i - row * h);
+    out[i] = kz < m ? sfm[row * m + kz] : sfn[row];
+  }
+}
+
+__global__ void k_spec_read(const float2* __restrict__ main, const float2* __restrict__ nyq,
+                            float2* __restrict__ out, int nx, int ny, int m) {
+  const long long h = m + 1;
+  const long long total = (long long)nx * ny * h;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / h;
+    const int kz = (int)(i - row * h);
+    out[i] = kz < m ? main[row * m + kz] : nyq[row];
+  }
+}
+
+__device__ __forceinline__ int pm(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// value of k * 2 pi / L with the reference's operation order; `num` is the
+// Python-float numerator, the division is float32 when L is float32 (NEP 50).
+__device__ __forceinline__ double kdiv(double num, double L, int f32) {
+#pragma clang fp contract(off)
+  if (f32) return (double)((float)num / (float)L);
+  return num / L;
+}
+
+__global__ void k_outputs(const double* __restrict__ sf, int nx, int ny, int nz, double L0,
+                          double L1, double L2, int f32, double* __restrict__ sfplt,
+                          double* __restrict__ kgrid, double* __restrict__ kgridplt) {
+#pragma clang fp contract(off)
+  const double twopi = 6.283185307179586;  // 2.0 * math.pi
+  const double pi = 3.141592653589793;
+  const int h = nz / 2 + 1;
+  const long long nkg = (long long)nx * ny * h;
+  const int px = nx - 2, py = ny - 2, pz = nz - 1;
+  const long long nplt = (px > 0 && py > 0 && pz > 0) ? (long long)px * py * pz : 0;
+  const long long total = nkg > nplt ? nkg : nplt;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (kgrid && i < nkg) {
+      const int iz = (int)(i % h);
+      const int iy = (int)((i / h) % ny);
+      const int ix = (int)(i / ((long long)h * ny));
+      double kx, ky;
+      if (ix < nx / 2) kx = kdiv((double)ix * 2.0 * pi, L0, f32);
+      else kx = kdiv(-((double)(nx - 1 - ix) + 0.5) * 2.0 * pi, L0, f32);
+      if (iy < ny / 2) ky = kdiv((double)iy * 2.0 * pi, L1, f32);
+      else ky = kdiv(-((double)(ny - 1 - iy) + 0.5) * 2.0 * pi, L1, f32);
+      const double kz = kdiv((double)iz * 2.0 * pi, L2, f32);
+      double* o = kgrid + 4 * i;
+      o[0] = kx;
+      o[1] = ky;
+      o[2] = kz;
+      o[3] = 0.0;
+    }
+    if (i < nplt) {
+      const int c = (int)(i % pz);
+      const int b = (int)((i / pz) % py);
+      const int a = (int)(i / ((long long)pz * py));
+      const int kx = a + 1 - nx / 2, ky = b + 1 - ny / 2, kz = c + 1 - nz / 2;
+      double v;
+      if (kx == 0 && ky == 0 && kz == 0) {
+        const double s100 = sf[((long long)1 * ny + 0) * h + 0];
+        const double s010 = sf[((long long)0 * ny + 1) * h + 0];
+        const double s001 = sf[((long long)0 * ny + 0) * h + 1];
+        v = 1.0 / 3.0 * (s100 + s010 + s001);
+      } else if (kz >= 0) {
+        v = sf[((long long)pm(kx, nx) * ny + pm(ky, ny)) * h + kz];
+      } else {
+        v = sf[((long long)pm(-kx, nx) * ny + pm(-ky, ny)) * h + (-kz)];
+      }
+      if (sfplt) sfplt[i] = v;
+      if (kgridplt) {
+        double* o = kgridplt + 4 * i;
+        o[0] = kdiv(((double)a - (double)px / 2.0) * 2.0 * pi, L0, f32);
+        o[1] = kdiv(((double)b - (double)py / 2.0) * 2.0 * pi, L1, f32);
+        o[2] = kdiv(((double)c - (double)pz / 2.0) * 2.0 * pi, L2, f32);
+        o[3] = v;
+      }
+    }
+  }
+  (void)twopi;
+}
+
+// --- Ewald sections: RegularGridInterpolator(method="linear", bounds_error=False)
+__device__ int find_cell(const double* __restrict__ kp, long long stride, int n, double x,
+                         bool* oob) {
+  // grid values g[i] = kp[i * stride]; i = searchsorted(g, x, 'left') - 1, clipped.
+  const double g0 = kp[0], gl = kp[(long long)(n - 1) * stride];
+  if (x < g0 || x > gl || x != x) *oob = true;
+  int lo = 0, hi = n;  // first index with g >= x
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (kp[(long long)mid * stride] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  int i = lo - 1;
+  if (i < 0) i = 0;
+  if (i > n - 2) i = n - 2;
+  return i;
+}
+
+__device__ double interp3(const double* __restrict__ kp, int nx, int ny, int nz, double x,
+                          double y, double z) {
+#pragma clang fp contract(off)
+  const long long sx = (long long)ny * nz * 4, sy = (long long)nz * 4, sz = 4;
+  bool oob = false;
+  const int i = find_cell(kp + 0, sx, nx, x, &oob);
+  const int j = find_cell(kp + 1, sy, ny, y, &oob);
+  const int k = find_cell(kp + 2, sz, nz, z, &oob);
+  if (oob) return nan("");
+  const double tx = (x - kp[i * sx]) / (kp[(i + 1) * sx] - kp[i * sx]);
+  const double ty = (y - kp[1 + j * sy]) / (kp[1 + (j + 1) * sy] - kp[1 + j * sy]);
+  const double tz = (z - kp[2 + k * sz]) / (kp[2 + (k + 1) * sz] - kp[2 + k * sz]);
+  const double wx[2] = {1.0 - tx, tx}, wy[2] = {1.0 - ty, ty}, wz[2] = {1.0 - tz, tz};
+  double value = 0.0;
+  // corner order of itertools.product over ((i, 1-t), (i+1, t)) per axis
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c) {
+        double w = 1.0;
+        w = w * wx[a];
+        w = w * wy[b];
+        w = w * wz[c];
+        const double v = kp[(i + a) * sx + (j + b) * sy + (k + c) * sz + 3];
+        value = value + v * w;
+      }
+  return value;
+}
+
+__global__ void k_ewald(const double* __restrict__ kp, int nx, int ny, int nz, double K,
+                        double* __restrict__ xy, double* __restrict__ xz, double* __restrict__ yz) {
+#pragma clang fp contract(off)
+  const long long sx = (long long)ny * nz * 4, sy = (long long)nz * 4;
+  const long long nxy = (long long)nx * ny, nxz = (long long)nx * nz, nyz = (long long)ny * nz;
+  const long long total = nxy + nxz + nyz;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < nxy) {
+      const int ix = (int)(i / ny), iy = (int)(i % ny);
+      const double X = kp[ix * sx], Y = kp[1 + iy * sy];
+      const double th = atan(sqrt(pow(X, 2.0) + pow(Y, 2.0)) / K);
+      const double c = cos(th);
+      if (xy) xy[i] = interp3(kp, nx, ny, nz, X * c, Y * c, K * (1.0 - c));
+    } else if (i < nxy + nxz) {
+      const long long j = i - nxy;
+      const int ix = (int)(j / nz), iz = (int)(j % nz);
+      const double X = kp[ix * sx], Z = kp[2 + iz * 4];
+      const double th = atan(sqrt(pow(X, 2.0) + pow(Z, 2.0)) / K);
+      const double c = cos(th);
+      if (xz) xz[j] = interp3(kp, nx, ny, nz, X * c, K * (1.0 - c), Z * c);
+    } else {
+      const long long j = i - nxy - nxz;
+      const int iy = (int)(j / nz), iz = (int)(j % nz);
+      const double Y = kp[1 + iy * sy], Z = kp[2 + iz * 4];
+      const double th = atan(sqrt(pow(Y, 2.0) + pow(Z, 2.0)) / K);
+      const double c = cos(th);
+      if (yz) yz[j] = interp3(kp, nx, ny, nz, K * (1.0 - c), Y * c, Z * c);
+    }
+  }
+}
+
+unsigned grid_for(long long total) {
+  long long b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  hipLaunchKernelGGL(k_sf_read, dim3(grid_for((long long)nx * ny * (m + 1))), dim3(256), 0, s,
+                     p.sf_main, p.sf_nyq, sf, nx, ny, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrum_read(const Plan& p, const float2* main, const float2* nyq, float* out,
+                                hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  hipLaunchKernelGGL(k_spec_read, dim3(grid_for((long long)nx * ny * (m + 1))), dim3(256), 0, s,
+                     main, nyq, reinterpret_cast<float2*>(out), nx, ny, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_outputs(const int n[3], const double* sf, const double box[3], int box_f32,
+                          double* sfplt, double* kgrid, double* kgridplt, hipStream_t s) {
+  const long long nkg = (long long)n[0] * n[1] * (n[2] / 2 + 1);
+  hipLaunchKernelGGL(k_outputs, dim3(grid_for(nkg)), dim3(256), 0, s, sf, n[0], n[1], n[2],
+                     box[0], box[1], box[2], box_f32, sfplt, kgrid, kgridplt);
+  return hipGetLastError();
+}
+
+hipError_t launch_ewald(const double* kp, int nx, int ny, int nz, double wl, double* xy,
+                        double* xz, double* yz, hipStream_t s) {
+  const double K = 2.0 * 3.141592653589793 / wl;
+  const long long total = (long long)nx * ny + (long long)nx * nz + (long long)ny * nz;
+  hipLaunchKernelGGL(k_ewald, dim3(grid_for(total)), dim3(256), 0, s, kp, nx, ny, nz, K, xy, xz,
+                     yz);
+  return hipGetLastError();
+}
+
+}  // namespace mdsf

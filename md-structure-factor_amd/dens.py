@@ -1,0 +1,106 @@
+"""Drop-in ``dens`` module: the S(q) path of joeyelk/MD-Structure-Factor on an MI355X.
+
+Same names, signatures, module globals and ``.npz`` outputs as the reference
+``dens.py``; callers (``main_gromacs.py:155-158,183-186,209-212``) keep
+``import dens`` with this directory on ``sys.path``.  The frame loop
+(rescale, wrap, Gaussian deposition, border fold, R2C FFT, |F|^2 frame sum)
+and the output grids run as HIP kernels in ``libmdsf.so``; this module does
+the reference's host-side set-up arithmetic and the ``.npz`` write.
+
+Precision: deposition and FFT in FP32 (twiddles generated in FP64), the
+frame sum of |F|^2 in FP64; home cells, grid sizes and k-grids are exact.
+"""
+import math
+
+import numpy as np
+
+from mdsf import engine
+
+USE_BETTER_RESOLUTION = True          # dens.py:10
+PRINT_DETAILS = True                  # dens.py:11
+RANDOM_NOISE = 0                      # dens.py:12
+Nspatialgrid = np.asarray([0.0, 0.0, 0.0])   # dens.py:15
+theta = np.pi / 3                     # dens.py:16 (set by callers, not read)
+PRECISION = 1.0E-24                   # dens.py:18
+dtyp = np.float64                     # dens.py:20 (FP64 S(q) accumulation)
+
+
+def load_radii(filename):
+    """``{name: (Nel, sigma_angstrom)}`` from ``Nel name radius_pm`` lines (dens.py:23-35)."""
+    out = {}
+    with open(filename) as f:
+        for line in f.readlines():
+            cols = line.strip().split()
+            out[cols[1]] = (float(cols[0]), float(cols[2]) / 100.0)
+    return out
+
+
+def get_borders(ad, dr, keys):
+    """Window half-widths in grid steps per type (dens.py:38-43)."""
+    return engine.borders(ad, dr, keys)
+
+
+def get_dplot(dmag):
+    """Half spectrum -> centred plottable grid (dens.py:142-163), on the GPU."""
+    import torch
+    dmag = np.asarray(dmag, dtype=np.float64)
+    nx, ny, h = dmag.shape
+    n = [nx, ny, 2 * (h - 1)]
+    plan = _output_plan(n)
+    sf = torch.from_numpy(np.ascontiguousarray(dmag)).to(plan.device)
+    sfplt, _, _ = plan.outputs(sf, [1.0, 1.0, 1.0], False)
+    return sfplt.cpu().numpy()
+
+
+def _output_plan(n):
+    return engine.Plan(n, 0, [1.0, 1.0, 1.0], np.eye(3), [], [], [], [])
+
+
+def _print_details(L, Sres, N):
+    print("=" * 50)
+    print("=" * 50)
+    print("Unit cell has dimensions of %s" % L)
+    print("requested resolution: %s Angstroms" % Sres)
+    print("Using a spatial grid of: %s" % N)
+    print("actual resolution: %s Angstroms" % (L / N))
+    print("This corresponds to maximum q vector of %s inverse Angstroms" % (2 * math.pi * N / L))
+    print("=" * 50)
+    print("=" * 50)
+
+
+def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
+    """Compute the 3-D structure factor and save it to ``out_filename + '.npz'``.
+
+    Same contract as the reference (dens.py:166-346): ``r`` (frames, atoms, 3)
+    is rescaled and wrapped in place, ``L`` holds per-frame box lengths, ``typ``
+    names each atom's entry of ``rad``; the ``.npz`` holds ``sf`` (the SUM over
+    frames of |rfftn(rho)|^2), ``sfplt``, ``L``, ``N``, ``kgrid``, ``kgridplt``.
+    ``ucell`` defaults to the identity and ``Sres`` to 1 Angstrom.
+    """
+    import torch
+    if ucell is None:
+        ucell = np.eye(3)
+    s = engine.Setup(r, L, typ, rad, Sres, USE_BETTER_RESOLUTION, ucell)
+    if PRINT_DETAILS:
+        _print_details(s.L, Sres, s.N)
+    plan = engine.Plan.from_setup(s)
+    dev = plan.device
+    coords = torch.from_numpy(np.ascontiguousarray(r)).to(dev)
+    scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
+    box = [float(x) for x in s.L]
+    if RANDOM_NOISE > 0:
+        plan.rescale_wrap(coords, scale, box, s.arith, s.wrap)
+        for _ in range(s.nframes):
+            d1 = np.random.rand(int(s.N[0]), int(s.N[1]), int(s.N[2]))   # dens.py:280
+            plan.accumulate_density(torch.from_numpy(d1.astype(np.float32)))
+    else:
+        types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
+        plan.accumulate_frames(coords, types, scale, box, s.arith, s.wrap)
+    r[...] = coords.cpu().numpy()
+    plan.check_errors()
+    sf_dev = plan.sf()
+    sfplt, kgrid, kgridplt = plan.outputs(sf_dev, box, s.box_f32)
+    sf = sf_dev.cpu().numpy()
+    np.savez_compressed(out_filename, sf=sf, sfplt=sfplt.cpu().numpy(), L=s.L, N=s.N,
+                        kgrid=kgrid.cpu().numpy(), kgridplt=kgridplt.cpu().numpy())
+    plan.close()

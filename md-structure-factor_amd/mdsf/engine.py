@@ -1,0 +1,218 @@
+"""Host side of the S(q) path on one GPU.
+
+``setup`` restates the host arithmetic of ``dens.compute_sf`` (mean box,
+rescale factors, grid size, borders, window half-widths, wrap range) in numpy
+with the reference's dtypes, and ``Plan`` drives ``libmdsf`` with torch device
+memory and the current torch stream.  All per-atom and per-voxel work runs in
+the HIP kernels.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+
+PRECISION = 1.0e-24   # dens.py:18
+BUFFSIZE = 1000000    # dens.py:206
+
+
+def grid_shape(L, sres, better):
+    """Grid points per axis (dens.py:181-189)."""
+    n = (L / sres).astype(int)
+    if better:
+        for i in range(3):
+            if L[i] / n[i] > sres:
+                n[i] += 1
+    for i in range(3):
+        n[i] += n[i] % 2
+    return n
+
+
+def borders(rad, dr, names):
+    """``{name: sigma*sqrt(ln(Nel/1e-24))/dr}`` (dens.py:38-43); KeyError for unknown names."""
+    return {a: rad[a][1] * np.sqrt(np.log(rad[a][0] / PRECISION)) / dr for a in names}
+
+
+def wrap_range(nframes, natoms):
+    """Atoms the wrap touches (dens.py:209-221): all below 1e6 atoms, else the
+    chunk ``[imin, nframes)`` of the loop that iterates frame indices."""
+    if natoms < BUFFSIZE:
+        return 0, natoms
+    lo = hi = 0
+    for imin in range(0, nframes, BUFFSIZE):
+        if imin + BUFFSIZE > nframes:
+            lo, hi = imin, nframes
+    return lo, hi
+
+
+class Setup:
+    """Everything ``compute_sf`` derives on the host before the frame loop."""
+
+    def __init__(self, r, dims, typ, rad, sres, better, ucell):
+        if r.ndim != 3 or r.shape[2] != 3:
+            raise ValueError("coordinates must have shape (frames, atoms, 3)")
+        if r.dtype not in (np.float32, np.float64):
+            raise TypeError("coordinates must be float32 or float64 (in-place rescale)")
+        dims = np.asarray(dims)
+        self.L = np.average(dims, axis=0)                     # dens.py:52
+        self.scale = self.L / dims                            # dens.py:53
+        self.N = grid_shape(self.L, sres, better)             # dens.py:181-189
+        self.dr = np.divide(self.L, self.N)                   # dens.py:202
+        typ = np.asarray(typ)
+        self.names, self.type_index = np.unique(typ, return_inverse=True)
+        self.names = [str(x) for x in self.names]
+        self.bdict = borders(rad, self.dr, self.names)        # dens.py:223
+        if len(self.bdict) == 0:
+            raise ValueError("no atoms")
+        self.nborder = int(max(max(v) for v in self.bdict.values()))  # dens.py:226-231
+        self.amp = np.array([rad[a][0] / np.power(rad[a][1], 3.) for a in self.names])
+        self.two_sig2 = np.array([2 * np.power(rad[a][1], 2.) for a in self.names])
+        self.half = np.array([self.bdict[a].astype(int) for a in self.names], dtype=np.int64)
+        self.counts = np.bincount(self.type_index.ravel(), minlength=len(self.names))
+        self.nframes, self.natoms = r.shape[0], r.shape[1]
+        self.wrap = wrap_range(self.nframes, self.natoms)
+        # numpy rescales / wraps in float32 only when both coords and dims are float32
+        self.arith = _lib.F32 if (r.dtype == np.float32 and self.scale.dtype == np.float32) else _lib.F64
+        self.dtype = _lib.F32 if r.dtype == np.float32 else _lib.F64
+        self.box_f32 = self.L.dtype == np.float32
+        self.ucell = np.asarray(ucell, dtype=np.float64).reshape(3, 3)
+        for d in range(3):
+            if self.nborder > self.N[d]:
+                raise ValueError(
+                    f"Nborder={self.nborder} exceeds the grid N={list(self.N)}: the border fold "
+                    "(dens.py:86-108) needs Nborder <= N; use a finer Sres")
+
+
+class Plan:
+    """One system (grid, border, cell, types) bound to a libmdsf plan on a GPU."""
+
+    def __init__(self, n, nborder, dr, ucell, amp, two_sig2, half, counts, device=None):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("libmdsf needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self.n = [int(x) for x in n]
+        self.natoms = int(np.sum(counts))
+        ntypes = len(amp)
+        tarr = (_lib.TypeDesc * max(ntypes, 1))()
+        for t in range(ntypes):
+            tarr[t].amp = float(amp[t])
+            tarr[t].two_sig2 = float(two_sig2[t])
+            for d in range(3):
+                tarr[t].half[d] = int(half[t][d])
+            tarr[t].count = int(counts[t])
+        desc = _lib.GridDesc()
+        for d in range(3):
+            desc.n[d] = self.n[d]
+            desc.dr[d] = float(dr[d])
+        desc.nborder = int(nborder)
+        u = np.asarray(ucell, dtype=np.float64).reshape(9)
+        for i in range(9):
+            desc.ucell[i] = float(u[i])
+        desc.ntypes = ntypes
+        desc.types = tarr
+        desc.natoms = self.natoms
+        self._handle = _lib.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.mdsf_plan_create(desc, self._handle))
+
+    @classmethod
+    def from_setup(cls, s, device=None):
+        return cls(s.N, s.nborder, s.dr, s.ucell, s.amp, s.two_sig2, s.half, s.counts, device)
+
+    # ----------------------------------------------------------------- utils
+    def _stream(self):
+        return _lib.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    @staticmethod
+    def _ptr(t):
+        return None if t is None else _lib.c_void_p(t.data_ptr())
+
+    def close(self):
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            self.lib.mdsf_plan_destroy(self._handle)
+            self._handle = _lib.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def half_shape(self):
+        return (self.n[0], self.n[1], self.n[2] // 2 + 1)
+
+    # ------------------------------------------------------------- pipeline
+    def reset(self):
+        _lib.check(self.lib.mdsf_sf_reset(self._handle, self._stream()))
+
+    def check_errors(self):
+        _lib.check(self.lib.mdsf_check_errors(self._handle, self._stream()))
+
+    def accumulate_frames(self, coords, types, scale, box, arith, wrap, occupancy=None,
+                          home=None):
+        """Frame loop (dens.py:277-321) over device ``coords`` (F, natoms, 3), in place."""
+        dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
+        _lib.check(self.lib.mdsf_accumulate_frames(
+            self._handle, self._ptr(coords), dtype, coords.shape[0], self._ptr(types),
+            self._ptr(scale), _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]),
+            self._ptr(occupancy), self._ptr(home), self._stream()))
+
+    def rescale_wrap(self, coords, scale, box, arith, wrap):
+        dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
+        _lib.check(self.lib.mdsf_rescale_wrap(
+            self._handle, self._ptr(coords), dtype, coords.shape[0], self._ptr(scale),
+            _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]), self._stream()))
+
+    def deposit_fold(self, coords, types, scale, box, arith, wrap, home=None):
+        """One frame (dens.py:283-311): folded FP32 density (Nx, Ny, Nz)."""
+        out = torch.empty(self.n, dtype=torch.float32, device=self.device)
+        dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
+        _lib.check(self.lib.mdsf_deposit_fold(
+            self._handle, self._ptr(coords), dtype, self._ptr(types), self._ptr(scale),
+            _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]), self._ptr(out),
+            self._ptr(home), self._stream()))
+        return out
+
+    def accumulate_density(self, density):
+        d = density.to(device=self.device, dtype=torch.float32).contiguous()
+        _lib.check(self.lib.mdsf_accumulate_density(self._handle, self._ptr(d), self._stream()))
+        return d
+
+    def rfft3(self, density):
+        d = density.to(device=self.device, dtype=torch.float32).contiguous()
+        out = torch.empty(self.half_shape + (2,), dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.mdsf_rfft3(self._handle, self._ptr(d), self._ptr(out), self._stream()))
+        return torch.view_as_complex(out)
+
+    def sf(self):
+        out = torch.empty(self.half_shape, dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.mdsf_sf_read(self._handle, self._ptr(out), self._stream()))
+        return out
+
+    def outputs(self, sf, box, box_f32):
+        """``sfplt``, ``kgrid``, ``kgridplt`` (dens.py:323-344) on the device."""
+        nx, ny, nz = self.n
+        sfplt = torch.empty((nx - 2, ny - 2, nz - 1), dtype=torch.float64, device=self.device)
+        kgrid = torch.empty((nx, ny, nz // 2 + 1, 4), dtype=torch.float64, device=self.device)
+        kgridplt = torch.empty((nx - 2, ny - 2, nz - 1, 4), dtype=torch.float64,
+                               device=self.device)
+        _lib.check(self.lib.mdsf_outputs(
+            self._handle, self._ptr(sf), _lib.doubles(box), 1 if box_f32 else 0,
+            self._ptr(sfplt), self._ptr(kgrid), self._ptr(kgridplt), self._stream()))
+        return sfplt, kgrid, kgridplt
+
+
+def ewald_slices(kgridplt, wavelength):
+    """Ewald-sphere sections (plot2d.py:218-263) of a device kgridplt (nx, ny, nz, 4)."""
+    lib = _lib.load()
+    kp = kgridplt.to(dtype=torch.float64).contiguous()
+    nx, ny, nz = (int(x) for x in kp.shape[:3])
+    xy = torch.empty((nx, ny), dtype=torch.float64, device=kp.device)
+    xz = torch.empty((nx, nz), dtype=torch.float64, device=kp.device)
+    yz = torch.empty((ny, nz), dtype=torch.float64, device=kp.device)
+    s = _lib.c_void_p(torch.cuda.current_stream(kp.device).cuda_stream)
+    _lib.check(lib.mdsf_ewald_slices(_lib.c_void_p(kp.data_ptr()), nx, ny, nz, float(wavelength),
+                                     _lib.c_void_p(xy.data_ptr()), _lib.c_void_p(xz.data_ptr()),
+                                     _lib.c_void_p(yz.data_ptr()), s))
+    return xy, xz, yz

@@ -1,0 +1,99 @@
+"""Seeded synthetic systems for the measured workloads (all lengths in Angstrom).
+
+* ``water``  : SPC-geometry waters, uniform O, random orientations, fresh
+               frames, written/read through nm float32 like a trajectory
+               (load_traj.py:95-96: coords = xyz * 10 in float32).
+* ``hii``    : inverted-hexagonal phase: 4 pore columns (radius 8) in a
+               104 x 90 x 80 box lined with C/O/Na headgroups, C/H tails
+               elsewhere, 50,000 atoms; frames = base + N(0, 0.2) noise.
+* ``uniform``: uniform-random atoms of given types in a cubic box.
+"""
+import numpy as np
+
+SPC_OH = 1.0
+SPC_HOH_DEG = 109.47
+
+
+def _random_rotations(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)
+
+
+def water(seed=1001, nwater=900, box=30.0, nframes=10):
+    """Returns (coords float32 (F, 3*nwater, 3), dims float32 (F, 3), typ list)."""
+    rng = np.random.default_rng(seed)
+    half = np.radians(SPC_HOH_DEG) / 2
+    geom = np.array([[0.0, 0.0, 0.0],
+                     [SPC_OH * np.sin(half), SPC_OH * np.cos(half), 0.0],
+                     [-SPC_OH * np.sin(half), SPC_OH * np.cos(half), 0.0]])
+    frames = []
+    for _ in range(nframes):
+        o = rng.uniform(0.0, box, size=(nwater, 3))
+        rot = _random_rotations(rng, nwater)
+        mol = o[:, None, :] + np.einsum("nij,aj->nai", rot, geom)
+        nm = (mol.reshape(-1, 3) / 10.0).astype(np.float32)   # as written to .gro/.trr
+        frames.append(nm * np.float32(10))                     # as read back (float32)
+    coords = np.stack(frames)
+    dims = np.tile((np.array([box] * 3) / 10.0).astype(np.float32) * np.float32(10), (nframes, 1))
+    typ = ["O", "H", "H"] * nwater
+    return coords, dims, typ
+
+
+HII_BOX = (104.0, 90.0, 80.0)
+HII_AXES = ((13.0, 11.25), (65.0, 11.25), (39.0, 56.25), (91.0, 56.25))
+HII_PORE = 8.0
+
+
+def hii(seed=2002, nframes=200, natoms=50000, layers=20, per_layer=6, dtype=np.float32):
+    """Returns (coords (F, natoms, 3), dims (F, 3) float64, typ list)."""
+    rng = np.random.default_rng(seed)
+    bx, by, bz = HII_BOX
+    pos, typ = [], []
+    for cx, cy in HII_AXES:
+        for k in range(layers):
+            z = (k + 0.5) * bz / layers
+            for j in range(per_layer):
+                phi = 2 * np.pi * j / per_layer + (np.pi / per_layer) * (k % 2)
+                for name, rad, dphi in (("C", 8.0, 0.0), ("O", 7.5, 0.12), ("O", 7.5, -0.12),
+                                        ("Na", 5.5, 0.0)):
+                    rr = rad + rng.normal(0.0, 0.5)
+                    pos.append(((cx + rr * np.cos(phi + dphi)) % bx,
+                                (cy + rr * np.sin(phi + dphi)) % by, z))
+                    typ.append(name)
+    ntail = natoms - len(pos)
+    tails = []
+    while len(tails) < ntail:
+        cand = rng.uniform(0.0, 1.0, size=(2 * ntail, 3)) * np.array(HII_BOX)
+        ok = np.ones(len(cand), bool)
+        for cx, cy in HII_AXES:
+            dx = (cand[:, 0] - cx + bx / 2) % bx - bx / 2
+            dy = (cand[:, 1] - cy + by / 2) % by - by / 2
+            ok &= dx * dx + dy * dy > HII_PORE ** 2
+        tails.extend(cand[ok][: ntail - len(tails)])
+    pos.extend(tails)
+    typ.extend(["C" if i % 3 == 0 else "H" for i in range(ntail)])
+    base = np.asarray(pos, dtype=np.float64)
+    coords = np.empty((nframes, natoms, 3), dtype=dtype)
+    for t in range(nframes):
+        coords[t] = np.mod(base + rng.normal(0.0, 0.2, size=base.shape), np.array(HII_BOX))
+    dims = np.tile(np.array(HII_BOX), (nframes, 1))
+    return coords, dims, typ
+
+
+def hii_sres():
+    """Per-axis resolution giving exactly 256^3 (USE_BETTER_RESOLUTION = False)."""
+    return np.array(HII_BOX) / 256.0
+
+
+def uniform(seed, natoms, box, names, nframes, dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    typ = [names[i] for i in rng.integers(0, len(names), size=natoms)]
+    coords = rng.uniform(0.0, box, size=(nframes, natoms, 3)).astype(dtype)
+    dims = np.full((nframes, 3), float(box))
+    return coords, dims, typ

@@ -1,0 +1,287 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+reference's own golden outputs.
+
+Tolerances (FP32 deposition and FFT, FP64 frame sum):
+  * S(q): max relative error <= 1e-5 on voxels above 1e-6 * max(S);
+  * folded density: |d - d_oracle| <= 1e-5 * |d_oracle| + 2e-6 * max(d_oracle);
+  * home cells, occupancy, grid sizes, k-grid axes, in-place coordinates: bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden_cases, load_golden
+from oracle import dens_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SQ_REL = 1e-5
+SQ_FLOOR = 1e-6
+
+
+def assert_sq_close(got, want, rel=SQ_REL, floor=SQ_FLOOR):
+    assert got.shape == want.shape
+    m = np.max(want)
+    mask = want > floor * m
+    err = np.abs(got[mask] - want[mask]) / want[mask]
+    assert err.max() <= rel, f"max rel err {err.max():.3e} (median {np.median(err):.3e})"
+    return err
+
+
+def assert_density_close(got, want):
+    tol = 1e-5 * np.abs(want) + 2e-6 * np.max(np.abs(want))
+    bad = np.abs(got - want) > tol
+    assert not bad.any(), (f"{bad.sum()} voxels off; max abs err "
+                           f"{np.max(np.abs(got - want)):.3e}, max {np.max(want):.3e}")
+
+
+def _gpu(r, dims, typ, rad, sres, better, ucell):
+    import torch
+    from mdsf import engine
+    s = engine.Setup(r, dims, typ, rad, sres, better, ucell)
+    plan = engine.Plan.from_setup(s)
+    coords = torch.from_numpy(np.ascontiguousarray(r)).to(plan.device)
+    types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(plan.device)
+    scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(plan.device)
+    return s, plan, coords, types, scale
+
+
+def _oracle_frames(r, dims, typ, rad, sres, better, ucell):
+    rr = np.array(r)
+    o = O.setup(rr, np.array(dims), typ, rad, sres, better)
+    padded = tuple(int(o["N"][i]) + 2 * o["Nb"] for i in range(3))
+    des, ori = O.fold_tables(o["N"], o["Nb"])
+    out = []
+    for t in range(rr.shape[0]):
+        d0, ir = O.deposit_frame(rr[t], typ, rad, o["bdict"], o["dr"], o["Nb"], ucell, padded)
+        out.append((O.fold(d0, des, ori), ir))
+    return rr, o, out
+
+
+def _run_dropin(c, rad, tmp_path):
+    import dens
+    saved = (dens.USE_BETTER_RESOLUTION, dens.RANDOM_NOISE, dens.PRINT_DETAILS)
+    dens.USE_BETTER_RESOLUTION = bool(c["in_better"])
+    dens.RANDOM_NOISE = int(c["in_noise"])
+    dens.PRINT_DETAILS = False
+    try:
+        if int(c["in_seed"]) >= 0:
+            np.random.seed(int(c["in_seed"]))
+        r = np.array(c["in_coords"])
+        out = str(tmp_path / "sf")
+        dens.compute_sf(r, np.array(c["in_dims"]), c["typ"], out, rad, c["in_ucell"], c["sres"])
+        return r, dict(np.load(out + ".npz"))
+    finally:
+        dens.USE_BETTER_RESOLUTION, dens.RANDOM_NOISE, dens.PRINT_DETAILS = saved
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_dropin_matches_reference_goldens(name, rad, gpu, tmp_path):
+    c = load_golden(name)
+    r, out = _run_dropin(c, rad, tmp_path)
+    assert sorted(out) == sorted(["sf", "sfplt", "L", "N", "kgrid", "kgridplt"])
+    assert r.dtype == c["out_r"].dtype and np.array_equal(r, c["out_r"])
+    assert np.array_equal(out["N"], c["out_N"]) and out["N"].dtype == c["out_N"].dtype
+    assert out["L"].dtype == c["out_L"].dtype and np.array_equal(out["L"], c["out_L"])
+    assert np.array_equal(out["kgrid"], c["out_kgrid"])
+    assert np.array_equal(out["kgridplt"][..., :3], c["out_kgridplt"][..., :3])
+    assert_sq_close(out["sf"], c["out_sf"])
+    assert_sq_close(out["sfplt"], c["out_sfplt"])
+    # the derived grids are exact functions of our own sf
+    assert np.array_equal(out["sfplt"], O.get_dplot(out["sf"]))
+    assert np.array_equal(out["kgridplt"][..., 3], out["sfplt"])
+
+
+@pytest.mark.parametrize("name", ["orth_corners_f64", "mono120_f32", "na_wide",
+                                  "aniso_sres_array", "mixed_f32_24"])
+def test_deposit_fold_and_home_cells(name, rad, gpu):
+    import torch
+    c = load_golden(name)
+    ucell = c["in_ucell"]
+    args = (c["in_coords"], c["in_dims"], c["typ"], rad, c["sres"], bool(c["in_better"]), ucell)
+    s, plan, coords, types, scale = _gpu(*args)
+    _, o, ref = _oracle_frames(*args)
+    for t in range(coords.shape[0]):
+        home = torch.empty((s.natoms, 3), dtype=torch.int32, device=plan.device)
+        d = plan.deposit_fold(coords[t], types, scale[t], [float(x) for x in s.L], s.arith,
+                              s.wrap, home=home)
+        plan.check_errors()
+        assert np.array_equal(home.cpu().numpy(), ref[t][1]), "home cells differ"
+        assert_density_close(d.cpu().numpy().astype(np.float64), ref[t][0])
+
+
+def _corner_case(rng, box, n_each, names, theta_deg=None, dtype=np.float64):
+    """Atoms within a few Angstrom of all 8 corners plus a few interior ones."""
+    pts = []
+    for cx in (0.05, 0.9):
+        for cy in (0.1, 0.95):
+            for cz in (0.02, 0.97):
+                for _ in range(n_each):
+                    pts.append(np.array([cx, cy, cz]) * box + rng.normal(0, 0.3, 3))
+    pts += list(rng.uniform(0, 1, (5, 3)) * box)
+    r = np.mod(np.array(pts), box)[None].astype(dtype)
+    typ = [names[i % len(names)] for i in range(r.shape[1])]
+    if theta_deg is None:
+        ucell = np.eye(3)
+    else:
+        th = math.radians(theta_deg)
+        ucell = np.array([[1, 0, 0], [np.cos(th), np.sin(th), 0], [0, 0, 1]])
+    return r, np.array([box]), typ, ucell
+
+
+@pytest.mark.parametrize("theta", [None, 120.0, 90.0])
+@pytest.mark.parametrize("sres", [0.5, 0.3])
+def test_corner_fold_quirk(theta, sres, rad, gpu):
+    rng = np.random.default_rng(11)
+    box = np.array([7.0, 8.0, 9.0])
+    r, dims, typ, ucell = _corner_case(rng, box, 2, ["C", "O", "H", "N"], theta)
+    args = (r, dims, typ, rad, sres, True, ucell)
+    s, plan, coords, types, scale = _gpu(*args)
+    _, o, ref = _oracle_frames(*args)
+    d = plan.deposit_fold(coords[0], types, scale[0], [float(x) for x in s.L], s.arith, s.wrap)
+    plan.check_errors()
+    assert_density_close(d.cpu().numpy().astype(np.float64), ref[0][0])
+
+
+def test_occupancy_histogram_bit_exact(rad, gpu):
+    import torch
+    from mdsf import synth
+    coords, dims, typ = synth.water(seed=5, nwater=300, box=18.0, nframes=3)
+    s, plan, c, types, scale = _gpu(coords, dims, typ, rad, 0.5, True, np.eye(3))
+    occ = torch.zeros(tuple(int(x) for x in s.N), dtype=torch.int32, device=plan.device)
+    plan.accumulate_frames(c, types, scale, [float(x) for x in s.L], s.arith, s.wrap,
+                           occupancy=occ)
+    plan.check_errors()
+    _, o, ref = _oracle_frames(coords, dims, typ, rad, 0.5, True, np.eye(3))
+    want = np.zeros(tuple(int(x) for x in s.N), dtype=np.int64)
+    for _, ir in ref:
+        np.add.at(want, tuple((ir % o["N"]).T), 1)
+    assert np.array_equal(occ.cpu().numpy().astype(np.int64), want)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (64, 64, 64), (96, 160, 192),
+                                   (38, 14, 22), (30, 18, 50), (2, 4, 2), (256, 256, 256)])
+def test_rfft3_matches_numpy(shape, gpu):
+    import torch
+    from mdsf import engine
+    rng = np.random.default_rng(7)
+    x = rng.random(shape).astype(np.float32)
+    plan = engine.Plan(shape, 0, [1.0] * 3, np.eye(3), [], [], [], [])
+    got = plan.rfft3(torch.from_numpy(x)).cpu().numpy().astype(np.complex128)
+    want = np.fft.rfftn(x.astype(np.float64))
+    err = np.abs(got - want).max() / np.abs(want).max()
+    assert err < 2e-6, err
+    # relative error on the non-DC spectrum, against its own scale
+    scale = np.sqrt(np.mean(np.abs(want) ** 2))
+    assert np.abs(got - want).max() < 1e-5 * scale * np.sqrt(np.log2(x.size))
+
+
+def test_accumulate_density_power_sum(gpu):
+    import torch
+    from mdsf import engine
+    rng = np.random.default_rng(8)
+    shape = (24, 20, 16)
+    plan = engine.Plan(shape, 0, [1.0] * 3, np.eye(3), [], [], [], [])
+    want = np.zeros((24, 20, 9))
+    for _ in range(3):
+        x = rng.random(shape).astype(np.float32)
+        plan.accumulate_density(torch.from_numpy(x))
+        want += O.frame_power(x.astype(np.float64))
+    assert_sq_close(plan.sf().cpu().numpy(), want)
+
+
+def test_determinism_bitwise(rad, gpu):
+    from mdsf import synth
+    coords, dims, typ = synth.water(seed=9, nwater=400, box=20.0, nframes=2)
+    outs = []
+    for _ in range(2):
+        s, plan, c, types, scale = _gpu(np.array(coords), dims, typ, rad, 0.4, True, np.eye(3))
+        plan.accumulate_frames(c, types, scale, [float(x) for x in s.L], s.arith, s.wrap)
+        plan.check_errors()
+        outs.append(plan.sf().cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_water_64_end_to_end(rad, gpu, tmp_path):
+    """The CPU-runnable water workload (64^3, 2,700 atoms), 2 frames, vs the oracle."""
+    import dens
+    from mdsf import synth
+    coords, dims, typ = synth.water(seed=1001, nwater=900, box=30.0, nframes=2)
+    ro = np.array(coords)
+    want = O.compute_sf(ro, np.array(dims), typ, None, rad, np.eye(3), 0.46875)
+    r = np.array(coords)
+    dens.PRINT_DETAILS = False
+    dens.compute_sf(r, dims, typ, str(tmp_path / "w"), rad, np.eye(3), 0.46875)
+    got = dict(np.load(str(tmp_path / "w.npz")))
+    assert list(got["N"]) == [64, 64, 64]
+    assert np.array_equal(r, ro)
+    assert_sq_close(got["sf"], want["sf"])
+
+
+def test_range_error_raises(rad, gpu, tmp_path):
+    import dens
+    r = np.array([[[1.0, 1.0, 1.0], [35.0, 2.0, 2.0]]])   # second atom 3.5 boxes out
+    dens.PRINT_DETAILS = False
+    with pytest.raises(ValueError):
+        dens.compute_sf(r, np.array([[10.0, 10.0, 10.0]]), ["O", "H"], str(tmp_path / "e"), rad,
+                        np.eye(3), 0.5)
+
+
+def test_type_index_error_raises(rad, gpu):
+    import torch
+    s, plan, coords, types, scale = _gpu(np.array([[[1.0, 1.0, 1.0]]]), np.array([[8.0] * 3]),
+                                         ["O"], rad, 0.5, True, np.eye(3))
+    bad = torch.tensor([5], dtype=torch.int32, device=plan.device)
+    plan.accumulate_frames(coords, bad, scale, [8.0] * 3, s.arith, s.wrap)
+    with pytest.raises(KeyError):
+        plan.check_errors()
+
+
+def test_unknown_name_raises_keyerror(rad, gpu, tmp_path):
+    import dens
+    dens.PRINT_DETAILS = False
+    with pytest.raises(KeyError):
+        dens.compute_sf(np.ones((1, 2, 3)), np.ones((1, 3)) * 9, ["OW", "HW1"],
+                        str(tmp_path / "k"), rad, np.eye(3), 0.5)
+
+
+def test_get_dplot_on_gpu_exact(gpu):
+    import dens
+    rng = np.random.default_rng(4)
+    sf = rng.random((12, 10, 9))
+    assert np.array_equal(dens.get_dplot(sf), O.get_dplot(sf))
+
+
+def test_ewald_slices_match_scipy(rad, gpu):
+    import torch
+    from scipy.interpolate import RegularGridInterpolator
+    from mdsf import engine
+    rng = np.random.default_rng(12)
+    n = (16, 14, 18)
+    sf = rng.random((n[0], n[1], n[2] // 2 + 1)) + 1.0
+    plan = engine.Plan(n, 0, [1.0] * 3, np.eye(3), [], [], [], [])
+    L = [9.0, 8.0, 10.0]
+    _, _, kp = plan.outputs(torch.from_numpy(sf).to(plan.device), L, False)
+    wl = 1.542
+    xy, xz, yz = engine.ewald_slices(kp, wl)
+    D = kp.cpu().numpy()
+    X, Y, Z = D[:, 0, 0, 0], D[0, :, 0, 1], D[0, 0, :, 2]
+    ES = RegularGridInterpolator((X, Y, Z), D[..., 3], bounds_error=False)
+    K = 2.0 * math.pi / wl
+    th = np.arctan(np.sqrt(X[:, None] ** 2.0 + Y[None, :] ** 2.0) / K)
+    pts = np.stack([X[:, None] * np.cos(th), Y[None, :] * np.cos(th), K * (1.0 - np.cos(th))], -1)
+    want_xy = ES(pts.reshape(-1, 3)).reshape(len(X), len(Y))
+    th = np.arctan(np.sqrt(X[:, None] ** 2.0 + Z[None, :] ** 2.0) / K)
+    pts = np.stack([X[:, None] * np.cos(th), np.broadcast_to(K * (1.0 - np.cos(th)), th.shape),
+                    Z[None, :] * np.cos(th)], -1)
+    want_xz = ES(pts.reshape(-1, 3)).reshape(len(X), len(Z))
+    th = np.arctan(np.sqrt(Y[:, None] ** 2.0 + Z[None, :] ** 2.0) / K)
+    pts = np.stack([np.broadcast_to(K * (1.0 - np.cos(th)), th.shape), Y[:, None] * np.cos(th),
+                    Z[None, :] * np.cos(th)], -1)
+    want_yz = ES(pts.reshape(-1, 3)).reshape(len(Y), len(Z))
+    for got, want in ((xy, want_xy), (xz, want_xz), (yz, want_yz)):
+        g = got.cpu().numpy()
+        assert np.array_equal(np.isnan(g), np.isnan(want))
+        ok = ~np.isnan(want)
+        np.testing.assert_allclose(g[ok], want[ok], rtol=1e-12, atol=0)

@@ -1,0 +1,87 @@
+"""CPU tests of the host side: set-up arithmetic, C-ABI exports, drop-in surface."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden_cases, load_golden
+from oracle import dens_oracle as O
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_setup_matches_oracle(name, rad):
+    from mdsf import engine
+    c = load_golden(name)
+    r = np.array(c["in_coords"])
+    s = engine.Setup(r, c["in_dims"], c["typ"], rad, c["sres"], bool(c["in_better"]),
+                     c["in_ucell"])
+    ro = np.array(c["in_coords"])
+    o = O.setup(ro, np.array(c["in_dims"]), c["typ"], rad, c["sres"], bool(c["in_better"]))
+    assert np.array_equal(s.N, o["N"]) and np.array_equal(s.N, c["out_N"])
+    assert np.array_equal(s.dr, o["dr"])
+    assert s.nborder == o["Nb"]
+    assert s.L.dtype == c["out_L"].dtype and np.array_equal(s.L, c["out_L"])
+    for t, nm in enumerate(s.names):
+        assert list(s.half[t]) == list(o["bdict"][nm].astype(int))
+    assert s.counts.sum() == r.shape[1]
+    assert [s.names[i] for i in s.type_index] == c["typ"]
+    # arithmetic dtype of the in-place rescale / wrap follows numpy promotion
+    want = (r.dtype == np.float32 and np.asarray(c["in_dims"]).dtype == np.float32)
+    assert (s.arith == 0) == want
+
+
+def test_unknown_type_raises_keyerror(rad):
+    from mdsf import engine
+    r = np.zeros((1, 2, 3)) + 1.0
+    with pytest.raises(KeyError):
+        engine.Setup(r, np.ones((1, 3)) * 10, ["H", "OW"], rad, 0.5, True, np.eye(3))
+
+
+def test_border_larger_than_grid_rejected(rad):
+    from mdsf import engine
+    r = np.zeros((1, 1, 3)) + 1.0
+    with pytest.raises(ValueError):
+        engine.Setup(r, np.ones((1, 3)) * 6.0, ["Na"], rad, 1.0, True, np.eye(3))
+
+
+def test_wrap_range_matches_oracle():
+    from mdsf import engine
+    for f, a in [(10, 5), (100, 1000000), (3000000, 1000000), (2500000, 1200000), (7, 999999)]:
+        assert engine.wrap_range(f, a) == O.wrap_atom_range(f, a)
+
+
+def test_library_exports_every_declared_symbol():
+    from mdsf import _lib
+    header = open(os.path.join(ROOT, "include", "mdsf.h")).read()
+    declared = set(re.findall(r"\b(mdsf_[a-z0-9_]+)\s*\(", header))
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(_lib.SIGNATURES)
+    assert _lib.load().mdsf_version() >= 10000
+
+
+def test_struct_layout_matches_header():
+    from mdsf import _lib
+    assert ctypes.sizeof(_lib.TypeDesc) == 32
+    assert _lib.GridDesc.types.offset == 120 and ctypes.sizeof(_lib.GridDesc) == 136
+
+
+def test_dens_module_surface():
+    import dens
+    for name in ("load_radii", "get_borders", "get_dplot", "compute_sf"):
+        assert callable(getattr(dens, name))
+    assert dens.USE_BETTER_RESOLUTION is True and dens.PRINT_DETAILS is True
+    assert dens.RANDOM_NOISE == 0 and dens.PRECISION == 1.0e-24 and dens.dtyp is np.float64
+    assert dens.load_radii(os.path.join(ROOT, "md-structure-factor_amd", "radii.txt"))["O"] == (8.0, 0.6)
+
+
+def test_plan_needs_gpu_without_fallback():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from mdsf import engine
+    with pytest.raises(RuntimeError):
+        engine.Plan([8, 8, 8], 2, [0.5] * 3, np.eye(3), [1.0], [0.5], [[2, 2, 2]], [1])
