@@ -1,0 +1,239 @@
+"""Benchmark: frames/s of the S(q) hot path at 256^3 with 50,000 atoms on one MI355X.
+
+Workload (synthetic, seeded): inverted-hexagonal phase in a 104 x 90 x 80 A
+box, 4 pore columns lined with C/O/Na headgroups, C/H tails elsewhere
+(50,000 atoms), frames = base + N(0, 0.2 A); grid 256^3 (Sres = L/256 per
+axis), FP32 deposition and FFT, FP64 frame sum of |F|^2.
+
+One step = one frame through the whole path (rescale/wrap, binning, factor
+tables, deposition + fold, z/y/x FFT passes, |F|^2 accumulation), with the
+trajectory already resident in HBM.  Each stage is bracketed by HIP events on
+the stream the kernels run on, which gives the per-kernel averages used for
+the roofline figures.  The CPU baseline times the FP64 numpy oracle (the
+reference's per-atom op sequence) on a bounded, type-stratified sample of
+one frame and extrapolates to frames/s.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "md-structure-factor_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+METRIC = "frames/s for S(q) at 256^3, 50k atoms"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def fft_bytes(n):
+    """Algorithmic HBM bytes of the three FP32 passes (x pass fused with the FP64 sf RMW)."""
+    nx, ny, nz = n
+    h = nz // 2 + 1
+    return {
+        "z": 4 * nx * ny * nz + 8 * nx * ny * h,   # read FP32 density, write half spectrum
+        "y": 16 * nx * ny * h,                     # read + write complex64
+        "x": 8 * nx * ny * h + 16 * nx * ny * h,   # read complex64, read + write FP64 sf
+    }
+
+
+def voxel_updates(setup):
+    """Voxel updates per frame: sum over atoms of the window volume prod_d 2A_d (dens.py:292-308)."""
+    per_type = np.prod(2 * setup.half, axis=1)
+    return int(np.sum(per_type * setup.counts))
+
+
+def cpu_baseline(coords0, dims, typ, rad, sres, budget_atoms=None):
+    """Time the FP64 oracle on a type-stratified atom sample of frame 0 plus one
+    fold + rfftn + |F|^2, and extrapolate to frames/s for the full frame."""
+    from oracle import dens_oracle as O
+    budget_atoms = budget_atoms or {"H": 2000, "C": 2000, "O": 960, "Na": 80}
+    r = np.array(coords0[None], copy=True)
+    o = O.setup(r, np.array(dims[:1]), typ, rad, sres, better=False)
+    n, nb = o["N"], o["Nb"]
+    padded = tuple(int(n[i]) + 2 * nb for i in range(3))
+    d0 = np.zeros(padded)
+    typ = np.asarray(typ)
+    t_dep = 0.0
+    sample = {}
+    for name in sorted(set(typ.tolist())):
+        idx = np.nonzero(typ == name)[0]
+        k = min(len(idx), budget_atoms.get(name, 200))
+        sel = idx[:k]
+        t0 = time.perf_counter()
+        O.deposit_frame(r[0][sel], typ[sel].tolist(), rad, o["bdict"], o["dr"], nb, np.eye(3),
+                        padded, out=d0)
+        dt = time.perf_counter() - t0
+        t_dep += dt / k * len(idx)
+        sample[name] = [int(k), int(len(idx)), round(dt / k * 1e3, 4)]
+    t0 = time.perf_counter()
+    d1 = O.fold(d0, *O.fold_tables(n, nb))
+    t_fold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.frame_power(d1)
+    t_fft = time.perf_counter() - t0
+    frame_s = t_dep + t_fold + t_fft
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": 1.0 / frame_s,
+        "unit": "frames/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("FP64 numpy oracle (reference per-atom op sequence) on frame 0: per-type atom "
+                   "samples [timed, total, ms/atom] " + json.dumps(sample) +
+                   f"; fold {t_fold:.2f} s, rfftn+|F|^2 {t_fft:.2f} s; deposition extrapolated "
+                   f"to all atoms: {t_dep:.1f} s/frame"),
+        "host": f"{cpu}; nproc={os.cpu_count()}; {platform.machine()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--fft", choices=["fp32", "fp64"], default="fp32")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the S(q) path runs on a single MI355X here "
+              "(frames would shard across GPUs with one FP64 all-reduce of S(q)); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dens
+    from mdsf import engine, synth
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rad = dens.load_radii(os.path.join(PKG, "radii.txt"))
+    nframes = max(args.steps, args.warmup, 1)
+    log(f"generating workload: {nframes} frames, 50,000 atoms")
+    coords, dims, typ = synth.hii(seed=2002, nframes=nframes)
+    sres = synth.hii_sres()
+    s = engine.Setup(coords, dims, typ, rad, sres, False, np.eye(3))
+    assert list(s.N) == [256, 256, 256], s.N
+    plan = engine.Plan.from_setup(s, fft64=(args.fft == "fp64"))
+    d_coords = torch.from_numpy(coords).to(dev)
+    d_types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
+    d_scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
+    box = [float(x) for x in s.L]
+    stream = torch.cuda.current_stream(dev)
+    stages = ["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"]
+
+    def run(frames, record):
+        evs = []
+        for t in frames:
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+            if record:
+                e[0].record(stream)
+            plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap)
+            if record:
+                e[1].record(stream)
+            plan.deposit()
+            if record:
+                e[2].record(stream)
+            plan.fft_pass(0)
+            if record:
+                e[3].record(stream)
+            plan.fft_pass(1)
+            if record:
+                e[4].record(stream)
+            plan.fft_pass(2)
+            if record:
+                e[5].record(stream)
+                evs.append(e)
+        return evs
+
+    log(f"warmup {args.warmup} frames")
+    run(range(args.warmup), False)
+    plan.reset()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} frames")
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    evs = run(range(args.steps), True)
+    end.record(stream)
+    torch.cuda.synchronize()
+    plan.check_errors()
+    total_ms = start.elapsed_time(end)
+    per_stage = {st: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
+                 for i, st in enumerate(stages)}
+    log("per-stage avg ms: " + json.dumps({k: round(v, 4) for k, v in per_stage.items()}))
+
+    n = [int(x) for x in s.N]
+    fb = fft_bytes(n)
+    fft_roof = {}
+    for key, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc")):
+        gbs = fb[key] / (per_stage[st] * 1e-3) / 1e9
+        fft_roof[key] = {"achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "bytes": fb[key], "avg_ms": round(per_stage[st], 4)}
+    updates = voxel_updates(s)
+    dep_tflops = 2.0 * updates / (per_stage["deposit"] * 1e-3) / 1e12
+    dominant = max(per_stage, key=per_stage.get)
+    if dominant == "deposit":
+        roofline = {"bound": "mfma", "kernel": "k_deposit (VALU FP32 FMA gather)",
+                    "achieved": round(dep_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(dep_tflops / FP32_PEAK_TFLOPS, 4),
+                    "traffic": None,
+                    "basis": f"{updates} voxel updates/frame x 2 flop (one FMA each)"}
+    else:
+        key = {"fft_z": "z", "fft_y": "y", "fft_x_acc": "x"}.get(dominant, "x")
+        roofline = {"bound": "hbm", "kernel": f"FFT {key} pass",
+                    "achieved": fft_roof[key]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": fft_roof[key]["frac"], "traffic": None}
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / (total_ms * 1e-3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.fft == "fp32" else "f32 deposit / f64 fft",
+        "data": "synthetic",
+        "config": {"workload": "hii_phase_256^3_50k_atoms", "grid": n, "atoms": int(s.natoms),
+                   "box_A": [float(x) for x in s.L], "fft_precision": args.fft,
+                   "sf_accumulation": "f64"},
+        "roofline": roofline,
+        "fft_passes": fft_roof,
+        "deposit": {"voxel_updates_per_frame": updates,
+                    "updates_per_s": round(updates / (per_stage["deposit"] * 1e-3), 1),
+                    "tflops_fma": round(dep_tflops, 3)},
+        "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle on a stratified sample of frame 0)")
+        c0, d0, t0 = synth.hii(seed=2002, nframes=1)
+        out["cpu_baseline"] = cpu_baseline(c0[0], d0, t0, rad, sres)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

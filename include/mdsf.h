@@ -16,6 +16,9 @@
  *   - A plan is not thread-safe; use one plan per host thread.
  *   - Grid arrays are C-order [x][y][z]; the spectrum keeps the reference's
  *     half-spectrum shape (Nx, Ny, Nz/2+1).
+ *   - A plan computes in FP32 (default) or FP64 (flag MDSF_FFT_FP64): the
+ *     density arrays passed in or out (`d_density`) have that element type.
+ *     The S(q) frame sum is FP64 in both.
  */
 #ifndef MDSF_H
 #define MDSF_H
@@ -35,6 +38,9 @@ extern "C" {
 
 #define MDSF_F32 0
 #define MDSF_F64 1
+
+/* mdsf_grid_desc.flags */
+#define MDSF_FFT_FP64 1       /* deposition, density and FFT in FP64 (default FP32) */
 
 typedef struct mdsf_plan mdsf_plan;
 typedef void* mdsf_stream; /* a hipStream_t */
@@ -57,6 +63,7 @@ typedef struct mdsf_grid_desc {
   int32_t ntypes;
   const mdsf_type_desc* types;
   int32_t natoms;
+  int32_t flags;         /* MDSF_FFT_FP64 or 0                                          */
 } mdsf_grid_desc;
 
 /* Plan lifecycle.  Validates the descriptor (Nborder <= N, even N, cell
@@ -107,22 +114,40 @@ int mdsf_rescale_wrap(mdsf_plan* plan, void* d_coords, int dtype, int64_t nframe
                       int64_t wrap_hi, mdsf_stream stream);
 
 /* One frame of deposition + periodic fold (dens.py:283-311), no FFT:
- * writes the folded FP32 density d_density[Nx*Ny*Nz].  Coordinates are
+ * writes the folded density d_density[Nx*Ny*Nz] (plan precision).  Coordinates are
  * processed as in mdsf_accumulate_frames (rescale with d_scale[0..2], wrap). */
 int mdsf_deposit_fold(mdsf_plan* plan, void* d_coords, int dtype,
                       const int32_t* d_types, const double* d_scale,
                       const double box[3], int arith, int64_t wrap_lo,
-                      int64_t wrap_hi, float* d_density, int32_t* d_home_cells,
+                      int64_t wrap_hi, void* d_density, int32_t* d_home_cells,
                       mdsf_stream stream);
 
-/* |rfftn(density)|^2 summed into the accumulator for a caller-supplied FP32
- * periodic density (the RANDOM_NOISE branch, dens.py:279-280, 313-318). */
-int mdsf_accumulate_density(mdsf_plan* plan, const float* d_density,
+/* The frame loop of mdsf_accumulate_frames split into its launches, for
+ * callers that interleave their own work or time each stage:
+ *   mdsf_prepare_frame  rescale + wrap in place, home cells, binning and the
+ *                       per-atom factor tables of one frame (dens.py:179-231,285-287);
+ *   mdsf_deposit        deposition + fold of the prepared frame into d_density
+ *                       (NULL: the plan's own density buffer) (dens.py:283-311);
+ *   mdsf_fft_pass       one forward pass over the plan's spectrum: 0 = z (R2C of
+ *                       d_density, NULL: the plan's density), 1 = y, 2 = x with
+ *                       |F|^2 summed into the accumulator (dens.py:313-318). */
+int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t* d_types,
+                       const double* d_scale, const double box[3], int arith, int64_t wrap_lo,
+                       int64_t wrap_hi, uint32_t* d_occupancy, int32_t* d_home_cells,
+                       mdsf_stream stream);
+int mdsf_deposit(mdsf_plan* plan, void* d_density, mdsf_stream stream);
+int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_stream stream);
+
+/* |rfftn(density)|^2 summed into the accumulator for a caller-supplied
+ * periodic density in plan precision (the RANDOM_NOISE branch,
+ * dens.py:279-280, 313-318). */
+int mdsf_accumulate_density(mdsf_plan* plan, const void* d_density,
                             mdsf_stream stream);
 
-/* Forward R2C transform of one FP32 density (np.fft.rfftn, dens.py:313):
- * d_spectrum receives complex64 (Nx, Ny, Nz/2+1) interleaved re/im. */
-int mdsf_rfft3(mdsf_plan* plan, const float* d_density, float* d_spectrum,
+/* Forward R2C transform of one density in plan precision (np.fft.rfftn,
+ * dens.py:313): d_spectrum receives complex128 (Nx, Ny, Nz/2+1) as
+ * interleaved re/im doubles. */
+int mdsf_rfft3(mdsf_plan* plan, const void* d_density, double* d_spectrum,
                mdsf_stream stream);
 
 /* The accumulated S(q) in the reference layout: d_sf double (Nx, Ny, Nz/2+1)

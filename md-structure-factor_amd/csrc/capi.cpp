@@ -41,10 +41,11 @@ hipError_t dalloc(T** p, size_t count) {
 
 void free_plan(Plan* p) {
   if (!p) return;
-  void* ptrs[] = {p->types, p->ir4,     p->key,     p->rpos,     p->bucket_count, p->bucket_off,
-                  p->bucket_cur, p->s_idx, p->s_ir, p->s_r,   p->tables,       p->err,
-                  p->rho,   p->nyq,     p->sf_main, p->sf_nyq, p->ax_x.tw,      p->ax_y.tw,
-                  p->ax_zc.tw, p->tw_r2c};
+  void* ptrs[] = {p->types,     p->ir4,       p->key,        p->rpos,       p->bucket_count,
+                  p->bucket_off, p->bucket_cur, p->s_idx,      p->s_ir,       p->s_r,
+                  p->tables,    p->err,       p->rho,        p->spec_nyq,   p->sf_main,
+                  p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
+                  p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -53,18 +54,15 @@ void free_plan(Plan* p) {
 int make_axis(FftAxis& ax, int n, hipStream_t s) {
   ax.n = n;
   if (!fft_factor(n, ax.radix, &ax.nstages))
-    return fail(MDSF_ERR_UNSUPPORTED,
-                "FFT length " + std::to_string(n) + " has a prime factor above 31");
+    return fail(MDSF_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " needs too many stages");
   MDSF_HIP(dalloc(&ax.tw, (size_t)n));
-  MDSF_HIP(fft_make_twiddles(ax.tw, n, s));
+  MDSF_HIP(dalloc(&ax.tw64, (size_t)n));
+  MDSF_HIP(fft_make_twiddles(ax.tw, ax.tw64, n, s));
   return MDSF_OK;
 }
 
-int run_fft_accumulate(Plan& p, hipStream_t s) {
-  float2* main = reinterpret_cast<float2*>(p.rho);
-  MDSF_HIP(fft_r2c_z(p, p.rho, main, p.nyq, s));
-  MDSF_HIP(fft_y(p, main, p.nyq, s));
-  MDSF_HIP(fft_x_accum(p, main, p.nyq, s));
+int run_fft_accumulate(Plan& p, const void* density, hipStream_t s) {
+  MDSF_HIP(fft_forward(p, density, true, s));
   return MDSF_OK;
 }
 
@@ -133,6 +131,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   s.mode = (G[0][1] == 0.0) ? MODE_DIAG : MODE_XY;
   s.ntypes = desc->ntypes;
   s.natoms = desc->natoms;
+  p->fft64 = (desc->flags & MDSF_FFT_FP64) ? 1 : 0;
 
   long long table_floats = 0;
   int start = 0, bucket_base = 0;
@@ -203,11 +202,33 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   TRY(dalloc(&p->s_idx, nat));
   TRY(dalloc(&p->s_ir, nat));
   TRY(dalloc(&p->s_r, 3 * nat));
-  TRY(dalloc(&p->tables, (size_t)table_floats));
+  if (p->fft64) {
+    double* tb = nullptr;
+    TRY(dalloc(&tb, (size_t)table_floats));
+    p->tables = tb;
+  } else {
+    float* tb = nullptr;
+    TRY(dalloc(&tb, (size_t)table_floats));
+    p->tables = tb;
+  }
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));
-  TRY(dalloc(&p->rho, nx * ny * nz));
-  TRY(dalloc(&p->nyq, nx * ny));
+  if (p->fft64) {
+    double* rho = nullptr;
+    double2* nyq = nullptr;
+    TRY(dalloc(&rho, nx * ny * nz));
+    TRY(dalloc(&nyq, nx * ny));
+    p->rho = rho;
+    p->spec_nyq = nyq;
+  } else {
+    float* rho = nullptr;
+    float2* nyq = nullptr;
+    TRY(dalloc(&rho, nx * ny * nz));
+    TRY(dalloc(&nyq, nx * ny));
+    p->rho = rho;
+    p->spec_nyq = nyq;
+  }
+  p->spec_main = p->rho;  // the z pass writes the half spectrum over the density
   TRY(dalloc(&p->sf_main, nx * ny * m));
   TRY(dalloc(&p->sf_nyq, nx * ny));
   TRY(hipMemset(p->sf_main, 0, sizeof(double) * nx * ny * m));
@@ -220,7 +241,8 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     free_plan(p);
     return rc;
   }
-  if ((e = dalloc(&p->tw_r2c, nz)) != hipSuccess || (e = fft_make_twiddles(p->tw_r2c, (int)nz, nullptr)) != hipSuccess) {
+  if ((e = dalloc(&p->tw_r2c, nz)) != hipSuccess || (e = dalloc(&p->tw_r2c64, nz)) != hipSuccess ||
+      (e = fft_make_twiddles(p->tw_r2c, p->tw_r2c64, (int)nz, nullptr)) != hipSuccess) {
     rc = hip_fail(e, "r2c twiddles");
     free_plan(p);
     return rc;
@@ -286,7 +308,7 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
                          d_occupancy, home, s));
     MDSF_HIP(launch_binning(*p, s));
     MDSF_HIP(launch_deposit(*p, p->rho, s));
-    rc = run_fft_accumulate(*p, s);
+    rc = run_fft_accumulate(*p, p->rho, s);
     if (rc != MDSF_OK) return rc;
   }
   return MDSF_OK;
@@ -315,7 +337,7 @@ int mdsf_rescale_wrap(mdsf_plan* plan, void* d_coords, int dtype, int64_t nframe
 
 int mdsf_deposit_fold(mdsf_plan* plan, void* d_coords, int dtype, const int32_t* d_types,
                       const double* d_scale, const double box[3], int arith, int64_t wrap_lo,
-                      int64_t wrap_hi, float* d_density, int32_t* d_home_cells,
+                      int64_t wrap_hi, void* d_density, int32_t* d_home_cells,
                       mdsf_stream stream) {
   Plan* p = reinterpret_cast<Plan*>(plan);
   int rc = check_frame_args(p, d_coords, dtype, d_types, d_scale, box, arith);
@@ -329,26 +351,49 @@ int mdsf_deposit_fold(mdsf_plan* plan, void* d_coords, int dtype, const int32_t*
   return MDSF_OK;
 }
 
-int mdsf_accumulate_density(mdsf_plan* plan, const float* d_density, mdsf_stream stream) {
+int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t* d_types,
+                       const double* d_scale, const double box[3], int arith, int64_t wrap_lo,
+                       int64_t wrap_hi, uint32_t* d_occupancy, int32_t* d_home_cells,
+                       mdsf_stream stream) {
   Plan* p = reinterpret_cast<Plan*>(plan);
-  if (!p || !d_density) return fail(MDSF_ERR_ARG, "null plan or density");
+  int rc = check_frame_args(p, d_coords, dtype, d_types, d_scale, box, arith);
+  if (rc != MDSF_OK) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const size_t bytes = sizeof(float) * p->sys.n[0] * p->sys.n[1] * p->sys.n[2];
-  MDSF_HIP(hipMemcpyAsync(p->rho, d_density, bytes, hipMemcpyDeviceToDevice, s));
-  return run_fft_accumulate(*p, s);
+  MDSF_HIP(launch_prep(*p, d_coords, dtype, arith, d_types, d_scale, box, wrap_lo, wrap_hi,
+                       d_occupancy, d_home_cells, s));
+  MDSF_HIP(launch_binning(*p, s));
+  return MDSF_OK;
 }
 
-int mdsf_rfft3(mdsf_plan* plan, const float* d_density, float* d_spectrum, mdsf_stream stream) {
+int mdsf_deposit(mdsf_plan* plan, void* d_density, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  MDSF_HIP(launch_deposit(*p, d_density ? d_density : p->rho,
+                          reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  if (pass < 0 || pass > 2) return fail(MDSF_ERR_ARG, "pass must be 0 (z), 1 (y) or 2 (x)");
+  MDSF_HIP(fft_pass(*p, pass, d_density ? d_density : p->rho,
+                    reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_accumulate_density(mdsf_plan* plan, const void* d_density, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !d_density) return fail(MDSF_ERR_ARG, "null plan or density");
+  return run_fft_accumulate(*p, d_density, reinterpret_cast<hipStream_t>(stream));
+}
+
+int mdsf_rfft3(mdsf_plan* plan, const void* d_density, double* d_spectrum, mdsf_stream stream) {
   Plan* p = reinterpret_cast<Plan*>(plan);
   if (!p || !d_density || !d_spectrum) return fail(MDSF_ERR_ARG, "null argument");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const size_t bytes = sizeof(float) * p->sys.n[0] * p->sys.n[1] * p->sys.n[2];
-  MDSF_HIP(hipMemcpyAsync(p->rho, d_density, bytes, hipMemcpyDeviceToDevice, s));
-  float2* main = reinterpret_cast<float2*>(p->rho);
-  MDSF_HIP(fft_r2c_z(*p, p->rho, main, p->nyq, s));
-  MDSF_HIP(fft_y(*p, main, p->nyq, s));
-  MDSF_HIP(fft_x(*p, main, p->nyq, s));
-  MDSF_HIP(launch_spectrum_read(*p, main, p->nyq, d_spectrum, s));
+  MDSF_HIP(fft_forward(*p, d_density, false, s));
+  MDSF_HIP(fft_spectrum_read(*p, d_spectrum, s));
   return MDSF_OK;
 }
 

@@ -2,16 +2,19 @@
 //
 // Replaces np.fft.rfftn (dens.py:313) and the |F|^2 frame sum
 // (dens.py:315-318).  Unnormalised forward transform, exponent -2*pi*i*k*n/N.
+// Element type R = float (FP32 storage and arithmetic) or double (FP64); the
+// twiddles of both are generated in FP64 on the device.
 //
 // Layout between passes (internal): the z pass packs each real row of Nz
-// floats as Nz/2 complex values and writes the half spectrum in place,
+// floats as Nz/2 complex values and writes the half spectrum,
 // kz in [0, Nz/2) -> main[x][y][kz], and the kz = Nz/2 plane -> nyq[x][y].
 // The y and x passes are strided pencil transforms over those arrays; the
 // x pass can fold |F|^2 into FP64 accumulators of the same layout.
 //
 // Each workgroup owns a tile of P pencils.  LDS holds re/im planes of
-// P x pitch floats (pitch odd), and threads are mapped pencil-fastest, so
-// every Stockham read and write is conflict-free for any radix and span.
+// P x pitch values (pitch odd) twice (ping-pong between stages), and threads
+// are mapped pencil-fastest, so every Stockham read and write is
+// conflict-free for any radix and span.
 #include "mdsf_internal.h"
 
 namespace mdsf {
@@ -19,83 +22,107 @@ namespace mdsf {
 namespace {
 
 constexpr int kFftThreads = 256;
-constexpr int kRegBudget = 16;  // complex values held per thread per stage
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename R> struct V2;
+template <> struct V2<float> { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+template <typename R>
+struct Cx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> mk(R a, R b) {
+  Cx<R> c;
+  c.x = a;
+  c.y = b;
+  return c;
 }
-__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
-  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
+  return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+template <typename R>
+__device__ __forceinline__ Cx<R> cmulc(Cx<R> a, Cx<R> b) {  // a * conj(b)
+  return mk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
+template <typename R>
+__device__ __forceinline__ Cx<R> csub(Cx<R> a, Cx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
+template <typename R>
+__device__ __forceinline__ Cx<R> cscale(R s, Cx<R> a) { return mk<R>(s * a.x, s * a.y); }
 // multiply by -i (forward) or +i (inverse)
-template <bool INV>
-__device__ __forceinline__ float2 mul_mi(float2 a) {
-  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+template <bool INV, typename R>
+__device__ __forceinline__ Cx<R> mul_mi(Cx<R> a) {
+  return INV ? mk<R>(-a.y, a.x) : mk<R>(a.y, -a.x);
+}
+template <typename R>
+__device__ __forceinline__ Cx<R> ld_tw(const void* tw, int k) {
+  const auto v = static_cast<const typename V2<R>::T*>(tw)[k];
+  return mk<R>(v.x, v.y);
 }
 
 // --- small DFT kernels (in registers) ---------------------------------------
-template <bool INV>
-__device__ __forceinline__ void dft2(float2* v) {
-  const float2 a = v[0], b = v[1];
+template <bool INV, typename R>
+__device__ __forceinline__ void dft2(Cx<R>* v) {
+  const Cx<R> a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft3(float2* v) {
-  const float s = 0.86602540378443864676f;  // sin(2 pi / 3)
-  const float2 t1 = cadd(v[1], v[2]);
-  const float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
-  const float2 d = csub(v[1], v[2]);
-  const float2 t3 = mul_mi<INV>(make_float2(s * d.x, s * d.y));
+template <bool INV, typename R>
+__device__ __forceinline__ void dft3(Cx<R>* v) {
+  const R s = (R)0.86602540378443864676;  // sin(2 pi / 3)
+  const Cx<R> t1 = cadd(v[1], v[2]);
+  const Cx<R> t2 = mk<R>(v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y);
+  const Cx<R> t3 = mul_mi<INV>(cscale(s, csub(v[1], v[2])));
   v[0] = cadd(v[0], t1);
   v[1] = cadd(t2, t3);
   v[2] = csub(t2, t3);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft4(float2* v) {
-  const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-  const float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+template <bool INV, typename R>
+__device__ __forceinline__ void dft4(Cx<R>* v) {
+  const Cx<R> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  const Cx<R> t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
   v[0] = cadd(t0, t2);
   v[2] = csub(t0, t2);
   v[1] = cadd(t1, t3);
   v[3] = csub(t1, t3);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft5(float2* v) {
-  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
-  const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-  const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  const float2 x0 = v[0];
-  const float2 a1 = make_float2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
-  const float2 a2 = make_float2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
-  const float2 b1 = mul_mi<INV>(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
-  const float2 b2 = mul_mi<INV>(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
-  v[0] = make_float2(x0.x + t1.x + t2.x, x0.y + t1.y + t2.y);
+template <bool INV, typename R>
+__device__ __forceinline__ void dft5(Cx<R>* v) {
+  const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+  const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+  const Cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const Cx<R> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const Cx<R> x0 = v[0];
+  const Cx<R> a1 = mk<R>(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+  const Cx<R> a2 = mk<R>(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+  const Cx<R> b1 = mul_mi<INV>(mk<R>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+  const Cx<R> b2 = mul_mi<INV>(mk<R>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+  v[0] = mk<R>(x0.x + t1.x + t2.x, x0.y + t1.y + t2.y);
   v[1] = cadd(a1, b1);
   v[4] = csub(a1, b1);
   v[2] = cadd(a2, b2);
   v[3] = csub(a2, b2);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft8(float2* v) {
-  const float h = 0.70710678118654752440f;
-  float2 e[4] = {v[0], v[2], v[4], v[6]};
-  float2 o[4] = {v[1], v[3], v[5], v[7]};
+template <bool INV, typename R>
+__device__ __forceinline__ void dft8(Cx<R>* v) {
+  const R h = (R)0.70710678118654752440;
+  Cx<R> e[4] = {v[0], v[2], v[4], v[6]};
+  Cx<R> o[4] = {v[1], v[3], v[5], v[7]};
   dft4<INV>(e);
   dft4<INV>(o);
-  // W8^k for k = 0..3 (forward: exp(-i pi k / 4))
-  const float2 w1 = make_float2(h, INV ? h : -h);
-  const float2 w3 = make_float2(-h, INV ? h : -h);
-  const float2 o1 = cmul(o[1], w1);
-  const float2 o2 = mul_mi<INV>(o[2]);
-  const float2 o3 = cmul(o[3], w3);
+  const Cx<R> w1 = mk<R>(h, INV ? h : -h);
+  const Cx<R> w3 = mk<R>(-h, INV ? h : -h);
+  const Cx<R> o1 = cmul(o[1], w1);
+  const Cx<R> o2 = mul_mi<INV>(o[2]);
+  const Cx<R> o3 = cmul(o[3], w3);
   v[0] = cadd(e[0], o[0]);
   v[4] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);
@@ -106,9 +133,9 @@ __device__ __forceinline__ void dft8(float2* v) {
   v[7] = csub(e[3], o3);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft16(float2* v) {
-  float2 e[8], o[8];
+template <bool INV, typename R>
+__device__ __forceinline__ void dft16(Cx<R>* v) {
+  Cx<R> e[8], o[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     e[i] = v[2 * i];
@@ -116,59 +143,58 @@ __device__ __forceinline__ void dft16(float2* v) {
   }
   dft8<INV>(e);
   dft8<INV>(o);
-  // W16^k, k = 0..7, forward exp(-2 pi i k / 16)
-  const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
-  const float h = 0.70710678118654752440f;
-  const float sg = INV ? 1.0f : -1.0f;
-  const float2 w[8] = {make_float2(1.f, 0.f),   make_float2(c1, sg * s1), make_float2(h, sg * h),
-                       make_float2(s1, sg * c1), make_float2(0.f, sg),     make_float2(-s1, sg * c1),
-                       make_float2(-h, sg * h),  make_float2(-c1, sg * s1)};
+  const R c1 = (R)0.92387953251128675613, s1 = (R)0.38268343236508977173;
+  const R h = (R)0.70710678118654752440;
+  const R sg = INV ? (R)1 : (R)-1;
+  const Cx<R> w[8] = {mk<R>(1, 0),        mk<R>(c1, sg * s1),  mk<R>(h, sg * h),
+                      mk<R>(s1, sg * c1), mk<R>(0, sg),        mk<R>(-s1, sg * c1),
+                      mk<R>(-h, sg * h),  mk<R>(-c1, sg * s1)};
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const float2 t = k == 0 ? o[0] : cmul(o[k], w[k]);
+    const Cx<R> t = k == 0 ? o[0] : cmul(o[k], w[k]);
     v[k] = cadd(e[k], t);
     v[k + 8] = csub(e[k], t);
   }
 }
 
-template <int R, bool INV>
-__device__ __forceinline__ void dft(float2* v) {
-  if constexpr (R == 2) dft2<INV>(v);
-  else if constexpr (R == 3) dft3<INV>(v);
-  else if constexpr (R == 4) dft4<INV>(v);
-  else if constexpr (R == 5) dft5<INV>(v);
-  else if constexpr (R == 8) dft8<INV>(v);
+template <int RAD, bool INV, typename R>
+__device__ __forceinline__ void dft(Cx<R>* v) {
+  if constexpr (RAD == 2) dft2<INV>(v);
+  else if constexpr (RAD == 3) dft3<INV>(v);
+  else if constexpr (RAD == 4) dft4<INV>(v);
+  else if constexpr (RAD == 5) dft5<INV>(v);
+  else if constexpr (RAD == 8) dft8<INV>(v);
   else dft16<INV>(v);
 }
 
 // One Stockham stage, out of place (in -> out planes), one butterfly per
-// thread iteration: X[(j/ns)*ns*R + k + r*ns] = DFT_R(x[j + r*n/R] * W^(k r n/(ns R))).
-template <int R, bool INV>
-__device__ void stage_fixed(const float* __restrict__ ire, const float* __restrict__ iim,
-                            float* __restrict__ ore, float* __restrict__ oim, int n, int P,
-                            int pitch, int ns, const float2* __restrict__ tw) {
-  const int nq = n / R;
+// thread iteration: X[(j/ns)*ns*RAD + k + r*ns] = DFT(x[j + r*n/RAD] * W^(k r n/(ns RAD))).
+template <int RAD, bool INV, typename R>
+__device__ void stage_fixed(const R* __restrict__ ire, const R* __restrict__ iim,
+                            R* __restrict__ ore, R* __restrict__ oim, int n, int P, int pitch,
+                            int ns, const void* __restrict__ tw) {
+  const int nq = n / RAD;
   const int nbf = P * nq;
-  const int twstep = n / (ns * R);
+  const int twstep = n / (ns * RAD);
   for (int b = threadIdx.x; b < nbf; b += kFftThreads) {
     const int p = b % P, j = b / P;
     const int k = j % ns;
     const int row = p * pitch;
-    float2 v[R];
+    Cx<R> v[RAD];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < RAD; ++r) {
       const int a = row + j + r * nq;
-      float2 x = make_float2(ire[a], iim[a]);
+      Cx<R> x = mk<R>(ire[a], iim[a]);
       if (r > 0 && k > 0) {
-        const float2 w = tw[k * r * twstep];
+        const Cx<R> w = ld_tw<R>(tw, k * r * twstep);
         x = INV ? cmulc(x, w) : cmul(x, w);
       }
       v[r] = x;
     }
-    dft<R, INV>(v);
-    const int d0 = row + (j / ns) * ns * R + k;
+    dft<RAD, INV>(v);
+    const int d0 = row + (j / ns) * ns * RAD + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < RAD; ++r) {
       ore[d0 + r * ns] = v[r].x;
       oim[d0 + r * ns] = v[r].y;
     }
@@ -177,34 +203,34 @@ __device__ void stage_fixed(const float* __restrict__ ire, const float* __restri
 }
 
 // Any other radix (odd primes): one output element per thread iteration,
-// direct DFT with W_R^m = W_n^(m n / R) from the twiddle table.
-template <bool INV>
-__device__ void stage_generic(const float* __restrict__ ire, const float* __restrict__ iim,
-                              float* __restrict__ ore, float* __restrict__ oim, int n, int P,
-                              int pitch, int ns, int R, const float2* __restrict__ tw) {
-  const int nq = n / R;
-  const int twstep = n / (ns * R);
-  const int step = n / R;
+// direct DFT with W_RAD^m = W_n^(m n / RAD) from the twiddle table.
+template <bool INV, typename R>
+__device__ void stage_generic(const R* __restrict__ ire, const R* __restrict__ iim,
+                              R* __restrict__ ore, R* __restrict__ oim, int n, int P, int pitch,
+                              int ns, int RAD, const void* __restrict__ tw) {
+  const int nq = n / RAD;
+  const int twstep = n / (ns * RAD);
+  const int step = n / RAD;
   for (int o = threadIdx.x; o < P * n; o += kFftThreads) {
     const int p = o % P, rest = o / P;
     const int j = rest % nq, m = rest / nq;
     const int k = j % ns;
     const int row = p * pitch;
-    float2 acc = make_float2(0.f, 0.f);
-    int wm = 0;  // (r * m) mod R
-    for (int r = 0; r < R; ++r) {
+    Cx<R> acc = mk<R>(0, 0);
+    int wm = 0;  // (r * m) mod RAD
+    for (int r = 0; r < RAD; ++r) {
       const int a = row + j + r * nq;
-      float2 x = make_float2(ire[a], iim[a]);
+      Cx<R> x = mk<R>(ire[a], iim[a]);
       if (r > 0 && k > 0) {
-        const float2 w = tw[k * r * twstep];
+        const Cx<R> w = ld_tw<R>(tw, k * r * twstep);
         x = INV ? cmulc(x, w) : cmul(x, w);
       }
-      const float2 w = tw[wm * step];
+      const Cx<R> w = ld_tw<R>(tw, wm * step);
       acc = cadd(acc, INV ? cmulc(x, w) : cmul(x, w));
       wm += m;
-      if (wm >= R) wm -= R;
+      if (wm >= RAD) wm -= RAD;
     }
-    const int d = row + (j / ns) * ns * R + k + m * ns;
+    const int d = row + (j / ns) * ns * RAD + k + m * ns;
     ore[d] = acc.x;
     oim[d] = acc.y;
   }
@@ -212,28 +238,29 @@ __device__ void stage_generic(const float* __restrict__ ire, const float* __rest
 }
 
 // Runs every stage of `ax`, ping-ponging between plane pairs 0 and 1 of
-// `lds` ([re0 | im0 | re1 | im1], P*pitch floats each).  Returns the plane
+// `lds` ([re0 | im0 | re1 | im1], P*pitch values each).  Returns the plane
 // pair holding the result.
-template <bool INV>
-__device__ int run_stages(float* lds, int n, int P, int pitch, const FftAxis& ax) {
+template <bool INV, typename R>
+__device__ int run_stages(R* lds, int n, int P, int pitch, const FftAxis& ax) {
   const int plane = P * pitch;
+  const void* tw = sizeof(R) == 8 ? (const void*)ax.tw64 : (const void*)ax.tw;
   int ns = 1, cur = 0;
   for (int s = 0; s < ax.nstages; ++s) {
-    const int R = ax.radix[s];
-    const float* ire = lds + (2 * cur) * plane;
-    const float* iim = ire + plane;
-    float* ore = lds + (2 * (cur ^ 1)) * plane;
-    float* oim = ore + plane;
-    switch (R) {
-      case 2: stage_fixed<2, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw); break;
-      case 3: stage_fixed<3, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw); break;
-      case 4: stage_fixed<4, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw); break;
-      case 5: stage_fixed<5, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw); break;
-      case 8: stage_fixed<8, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw); break;
-      case 16: stage_fixed<16, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw); break;
-      default: stage_generic<INV>(ire, iim, ore, oim, n, P, pitch, ns, R, ax.tw); break;
+    const int RAD = ax.radix[s];
+    const R* ire = lds + (2 * cur) * plane;
+    const R* iim = ire + plane;
+    R* ore = lds + (2 * (cur ^ 1)) * plane;
+    R* oim = ore + plane;
+    switch (RAD) {
+      case 2: stage_fixed<2, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
+      case 3: stage_fixed<3, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
+      case 4: stage_fixed<4, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
+      case 5: stage_fixed<5, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
+      case 8: stage_fixed<8, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
+      case 16: stage_fixed<16, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
+      default: stage_generic<INV>(ire, iim, ore, oim, n, P, pitch, ns, RAD, tw); break;
     }
-    ns *= R;
+    ns *= RAD;
     cur ^= 1;
   }
   return cur;
@@ -242,149 +269,374 @@ __device__ int run_stages(float* lds, int n, int P, int pitch, const FftAxis& ax
 __host__ __device__ inline int lds_pitch(int n) { return (n % 2 == 0) ? n + 1 : n; }
 
 // --- twiddles: W_n^k = exp(-2 pi i k / n), generated in FP64 ------------------
-__global__ void k_twiddles(float2* __restrict__ tw, int n) {
+__global__ void k_twiddles(float2* __restrict__ tw, double2* __restrict__ tw64, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   double s, c;
   sincospi(-2.0 * (double)k / (double)n, &s, &c);
   tw[k] = make_float2((float)c, (float)s);
+  tw64[k] = make_double2(c, s);
 }
 
-// --- z pass: real rows -> half spectrum (in place) + Nyquist plane ------------
+// --- z pass: real rows -> half spectrum + Nyquist plane ------------------------
+template <typename R>
 struct R2CArgs {
-  const float* in;     // rows of nz reals (may alias main)
-  float2* main;        // rows of m = nz/2 complex
-  float2* nyq;         // one complex per row
+  const R* in;                      // rows of nz reals (plan precision)
+  typename V2<R>::T* main;          // rows of m = nz/2 complex (may alias `in` for R = float)
+  typename V2<R>::T* nyq;           // one complex per row
   long long rows;
-  int m;               // nz / 2
-  int P;               // rows per tile
-  FftAxis ax;          // length m
-  const float2* tw2;   // W_nz^k, k < nz
+  int m;                            // nz / 2
+  int P;                            // rows per tile
+  FftAxis ax;                       // length m
+  const void* tw2;                  // W_nz^k, k < nz
 };
 
-__global__ __launch_bounds__(kFftThreads) void k_r2c_z(R2CArgs a) {
-  extern __shared__ float lds[];
+template <typename R>
+__global__ __launch_bounds__(kFftThreads) void k_r2c_z(R2CArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  R* lds = reinterpret_cast<R*>(lds_raw);
   const int m = a.m, P = a.P, pitch = lds_pitch(m);
-  float* re = lds;
-  float* im = lds + P * pitch;
+  R* re = lds;
+  R* im = lds + P * pitch;
   const long long r0 = (long long)blockIdx.x * P;
   const int pc = (int)min((long long)P, a.rows - r0);
-  const float2* src = reinterpret_cast<const float2*>(a.in) + r0 * m;
+  const auto* src = reinterpret_cast<const typename V2<R>::T*>(a.in) + r0 * m;
   for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
     const int p = idx / m, e = idx - p * m;
-    const float2 v = src[idx];
-    re[p * pitch + e] = v.x;
-    im[p * pitch + e] = v.y;
+    const auto v = src[idx];
+    re[p * pitch + e] = (R)v.x;
+    im[p * pitch + e] = (R)v.y;
   }
   __syncthreads();
   {
-    const int cur = run_stages<false>(lds, m, P, pitch, a.ax);
+    const int cur = run_stages<false, R>(lds, m, P, pitch, a.ax);
     re = lds + 2 * cur * P * pitch;
     im = re + P * pitch;
   }
   // split: X_k = A_k + W_nz^k B_k, A = (Z_k + conj Z_{m-k})/2, B = (Z_k - conj Z_{m-k})/(2i)
-  float2* dst = a.main + r0 * m;
+  auto* dst = a.main + r0 * m;
   for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
     const int p = idx / m, k = idx - p * m;
     const int kk = (k == 0) ? 0 : m - k;
-    const float2 z = make_float2(re[p * pitch + k], im[p * pitch + k]);
-    const float2 zc = make_float2(re[p * pitch + kk], -im[p * pitch + kk]);
-    const float2 A = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
-    const float2 d = make_float2(0.5f * (z.x - zc.x), 0.5f * (z.y - zc.y));
-    const float2 B = make_float2(d.y, -d.x);  // d / i
-    const float2 w = a.tw2[k];
-    dst[idx] = cadd(A, cmul(B, w));
+    const Cx<R> z = mk<R>(re[p * pitch + k], im[p * pitch + k]);
+    const Cx<R> zc = mk<R>(re[p * pitch + kk], -im[p * pitch + kk]);
+    const Cx<R> A = cscale((R)0.5, cadd(z, zc));
+    const Cx<R> d = cscale((R)0.5, csub(z, zc));
+    const Cx<R> B = mk<R>(d.y, -d.x);  // d / i
+    const Cx<R> X = cadd(A, cmul(B, ld_tw<R>(a.tw2, k)));
+    dst[idx].x = X.x;
+    dst[idx].y = X.y;
   }
   for (int p = threadIdx.x; p < pc; p += kFftThreads) {
-    const float2 z = make_float2(re[p * pitch], im[p * pitch]);
-    // X_{m} = A_0 - B_0 = Re z - Im z (both real)
-    a.nyq[r0 + p] = make_float2(z.x - z.y, 0.0f);
+    // X_m = A_0 - B_0 = Re z0 - Im z0 (real)
+    a.nyq[r0 + p].x = re[p * pitch] - im[p * pitch];
+    a.nyq[r0 + p].y = (R)0;
+  }
+}
+
+// --- adjoint z pass: half spectrum -> real rows ----------------------------------
+// Real adjoint of the R2C along z:  x_n = Re sum_{k=0..m} X_k e^{+2 pi i k n / nz}.
+// With X'_0 = Re X_0, X'_m = Re X_m and X'_k = X_k / 2 inside, this is the
+// Hermitian inverse of X': Z_k = (X'_k + conj X'_{m-k}) + i W^{-k} (X'_k - conj X'_{m-k})
+// and x_{2j} + i x_{2j+1} = sum_k Z_k e^{+2 pi i jk/m}.
+template <typename R, typename OUT>
+struct C2RArgs {
+  const typename V2<R>::T* main;
+  const typename V2<R>::T* nyq;
+  OUT* out;                         // rows of nz reals
+  long long rows;
+  int m;
+  int P;
+  FftAxis ax;
+  const void* tw2;
+};
+
+template <typename R, typename OUT>
+__global__ __launch_bounds__(kFftThreads) void k_c2r_z(C2RArgs<R, OUT> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  R* lds = reinterpret_cast<R*>(lds_raw);
+  const int m = a.m, P = a.P, pitch = lds_pitch(m);
+  R* re = lds;
+  R* im = lds + P * pitch;
+  const long long r0 = (long long)blockIdx.x * P;
+  const int pc = (int)min((long long)P, a.rows - r0);
+  const auto* src = a.main + r0 * m;
+  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
+    const int p = idx / m, k = idx - p * m;
+    Cx<R> x, xc;
+    if (k == 0) {
+      x = mk<R>(src[idx].x, (R)0);
+      xc = mk<R>(a.nyq[r0 + p].x, (R)0);
+    } else {
+      const auto v = src[p * m + (m - k)];
+      x = mk<R>((R)0.5 * src[idx].x, (R)0.5 * src[idx].y);
+      xc = mk<R>((R)0.5 * v.x, (R)-0.5 * v.y);
+    }
+    const Cx<R> e = cadd(x, xc);
+    const Cx<R> o = cmulc(csub(x, xc), ld_tw<R>(a.tw2, k));  // (X - conj X') W^{-k}
+    re[p * pitch + k] = e.x - o.y;                            // e + i o
+    im[p * pitch + k] = e.y + o.x;
+  }
+  __syncthreads();
+  {
+    const int cur = run_stages<true, R>(lds, m, P, pitch, a.ax);
+    re = lds + 2 * cur * P * pitch;
+    im = re + P * pitch;
+  }
+  OUT* dst = a.out + r0 * (2 * (long long)m);
+  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
+    const int p = idx / m, j = idx - p * m;
+    dst[2 * (long long)idx] = (OUT)re[p * pitch + j];
+    dst[2 * (long long)idx + 1] = (OUT)im[p * pitch + j];
   }
 }
 
 // --- strided pencil pass ------------------------------------------------------
-struct PencilArgs {
-  float2* data;
-  double* acc;              // if set: acc += |X|^2 (FP64) at the same index
-  int write_back;           // store the transform into data
+// One launch transforms two jobs: the main half spectrum and the Nyquist
+// plane (workgroups [0, blocks0) take job 0, the rest job 1).
+template <typename R>
+struct PencilJob {
+  typename V2<R>::T* data;
+  double* acc;              // EPI_ACC: acc += |X|^2 (FP64) at the same index
+  const double* scale;      // if set: X *= scale[index] before the transform
   long long estride;        // between elements of a pencil
   long long pstride;        // between adjacent pencils of a tile
   long long outer_stride;   // between outer groups
   int pencils;              // pencils per outer group
   int tiles_inner;          // ceil(pencils / P)
   int P;
-  int n;
   int pencil_major;         // elements contiguous (estride == 1)
+};
+
+template <typename R>
+struct PencilArgs {
+  PencilJob<R> job[2];
+  long long blocks0;
+  int n;
   FftAxis ax;
 };
 
-template <bool INV>
-__global__ __launch_bounds__(kFftThreads) void k_pencils(PencilArgs a) {
-  extern __shared__ float lds[];
-  const int n = a.n, P = a.P, pitch = lds_pitch(n);
-  float* re = lds;
-  float* im = lds + P * pitch;
-  const long long tile = blockIdx.x;
-  const long long outer = tile / a.tiles_inner;
-  const int inner = (int)(tile - outer * a.tiles_inner);
+enum Epilogue { EPI_STORE = 0, EPI_ACC = 1 };
+
+template <bool INV, typename R, int EPI>
+__global__ __launch_bounds__(kFftThreads) void k_pencils(PencilArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  R* lds = reinterpret_cast<R*>(lds_raw);
+  const bool second = (long long)blockIdx.x >= a.blocks0;
+  const PencilJob<R>& J = second ? a.job[1] : a.job[0];
+  const long long tile = second ? (long long)blockIdx.x - a.blocks0 : (long long)blockIdx.x;
+  const int n = a.n, P = J.P, pitch = lds_pitch(n);
+  R* re = lds;
+  R* im = lds + P * pitch;
+  const long long outer = tile / J.tiles_inner;
+  const int inner = (int)(tile - outer * J.tiles_inner);
   const int p0 = inner * P;
-  const int pc = min(P, a.pencils - p0);
-  const long long base = outer * a.outer_stride + (long long)p0 * a.pstride;
+  const int pc = min(P, J.pencils - p0);
+  const long long base = outer * J.outer_stride + (long long)p0 * J.pstride;
   const int total = pc * n;
   for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
     int p, e;
-    if (a.pencil_major) {
+    if (J.pencil_major) {
       p = idx / n;
       e = idx - p * n;
     } else {
       e = idx / pc;
       p = idx - e * pc;
     }
-    const float2 v = a.data[base + p * a.pstride + e * a.estride];
-    re[p * pitch + e] = v.x;
-    im[p * pitch + e] = v.y;
+    const long long g = base + p * J.pstride + e * J.estride;
+    const auto v = J.data[g];
+    R sc = (R)1;
+    if (J.scale) sc = (R)J.scale[g];
+    re[p * pitch + e] = v.x * sc;
+    im[p * pitch + e] = v.y * sc;
   }
   __syncthreads();
   {
-    const int cur = run_stages<INV>(lds, n, P, pitch, a.ax);
+    const int cur = run_stages<INV, R>(lds, n, P, pitch, a.ax);
     re = lds + 2 * cur * P * pitch;
     im = re + P * pitch;
   }
   for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
     int p, e;
-    if (a.pencil_major) {
+    if (J.pencil_major) {
       p = idx / n;
       e = idx - p * n;
     } else {
       e = idx / pc;
       p = idx - e * pc;
     }
-    const float xr = re[p * pitch + e], xi = im[p * pitch + e];
-    const long long g = base + p * a.pstride + e * a.estride;
-    if (a.write_back) a.data[g] = make_float2(xr, xi);
-    if (a.acc) a.acc[g] += (double)xr * (double)xr + (double)xi * (double)xi;
+    const R xr = re[p * pitch + e], xi = im[p * pitch + e];
+    const long long g = base + p * J.pstride + e * J.estride;
+    if (EPI == EPI_STORE) {
+      J.data[g].x = xr;
+      J.data[g].y = xi;
+    } else {
+      J.acc[g] += (double)xr * (double)xr + (double)xi * (double)xi;
+    }
   }
 }
 
+template <typename R>
 int tile_pencils(int n) {
-  int P = 2048 / (n > 0 ? n : 1);
+  int P = (int)(8192 / sizeof(R)) / (n > 0 ? n : 1);
   if (P > 16) P = 16;
   if (P < 1) P = 1;
   return P;
 }
 
-hipError_t run_pencils(const PencilArgs& a0, long long outer_count, hipStream_t s, bool inv) {
-  PencilArgs a = a0;
-  a.P = tile_pencils(a.n);
-  a.tiles_inner = (a.pencils + a.P - 1) / a.P;
-  const long long blocks = outer_count * a.tiles_inner;
-  if (blocks == 0) return hipSuccess;
-  const size_t lds = (size_t)a.P * lds_pitch(a.n) * 4 * sizeof(float);
-  if (inv)
-    hipLaunchKernelGGL(k_pencils<true>, dim3((unsigned)blocks), dim3(kFftThreads), lds, s, a);
-  else
-    hipLaunchKernelGGL(k_pencils<false>, dim3((unsigned)blocks), dim3(kFftThreads), lds, s, a);
+template <typename R>
+size_t tile_lds(int n, int P) {
+  return (size_t)P * lds_pitch(n) * 4 * sizeof(R);
+}
+
+// job[0] runs over outer0 outer groups, job[1] over outer1.
+template <typename R>
+hipError_t run_pencils(PencilArgs<R> a, long long outer0, long long outer1, bool inv, int epi,
+                       hipStream_t s) {
+  long long blocks[2];
+  const long long outer[2] = {outer0, outer1};
+  size_t lds = 0;
+  for (int j = 0; j < 2; ++j) {
+    PencilJob<R>& J = a.job[j];
+    J.P = tile_pencils<R>(a.n);
+    J.tiles_inner = (J.pencils + J.P - 1) / J.P;
+    blocks[j] = J.pencils > 0 ? outer[j] * J.tiles_inner : 0;
+    if (blocks[j] > 0) lds = tile_lds<R>(a.n, J.P) > lds ? tile_lds<R>(a.n, J.P) : lds;
+  }
+  a.blocks0 = blocks[0];
+  const long long total = blocks[0] + blocks[1];
+  if (total == 0) return hipSuccess;
+  const dim3 g((unsigned)total), b(kFftThreads);
+  if (inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<true, R, EPI_STORE>), g, b, lds, s, a);
+  else if (!inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<false, R, EPI_STORE>), g, b, lds, s, a);
+  else hipLaunchKernelGGL((k_pencils<false, R, EPI_ACC>), g, b, lds, s, a);
   return hipGetLastError();
+}
+
+template <typename R>
+typename V2<R>::T* spec_main(Plan& p) {
+  return static_cast<typename V2<R>::T*>(p.spec_main);
+}
+template <typename R>
+typename V2<R>::T* spec_nyq(Plan& p) {
+  return static_cast<typename V2<R>::T*>(p.spec_nyq);
+}
+
+template <typename R>
+hipError_t pass_z(Plan& p, const void* density, hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  R2CArgs<R> a;
+  a.in = static_cast<const R*>(density);
+  a.main = spec_main<R>(p);
+  a.nyq = spec_nyq<R>(p);
+  a.rows = (long long)nx * ny;
+  a.m = m;
+  a.P = tile_pencils<R>(m);
+  a.ax = p.ax_zc;
+  a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
+  const long long blocks = (a.rows + a.P - 1) / a.P;
+  hipLaunchKernelGGL((k_r2c_z<R>), dim3((unsigned)blocks), dim3(kFftThreads),
+                     tile_lds<R>(m, a.P), s, a);
+  return hipGetLastError();
+}
+
+// y pass: main (pencils = kz, elements = y) + nyq (pencils = x, elements contiguous)
+template <typename R>
+hipError_t pass_y(Plan& p, bool inv, hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  PencilArgs<R> a{};
+  a.n = ny;
+  a.ax = p.ax_y;
+  PencilJob<R>& J = a.job[0];
+  J.data = spec_main<R>(p);
+  J.estride = m;
+  J.pstride = 1;
+  J.outer_stride = (long long)ny * m;
+  J.pencils = m;
+  PencilJob<R>& K = a.job[1];
+  K.data = spec_nyq<R>(p);
+  K.estride = 1;
+  K.pstride = ny;
+  K.pencils = nx;
+  K.pencil_major = 1;
+  return run_pencils(a, nx, 1, inv, EPI_STORE, s);
+}
+
+// x pass: main (pencils = (y, kz), elements = x) + nyq (pencils = y).  Forward
+// with `accumulate` adds |F|^2 into sf_main / sf_nyq; the inverse applies the
+// per-bin FP64 weights w first.
+template <typename R>
+hipError_t pass_x(Plan& p, bool inv, bool accumulate, const double* w_main, const double* w_nyq,
+                  hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  PencilArgs<R> a{};
+  a.n = nx;
+  a.ax = p.ax_x;
+  PencilJob<R>& J = a.job[0];
+  J.data = spec_main<R>(p);
+  J.acc = p.sf_main;
+  J.scale = w_main;
+  J.estride = (long long)ny * m;
+  J.pstride = 1;
+  J.pencils = ny * m;
+  PencilJob<R>& K = a.job[1];
+  K.data = spec_nyq<R>(p);
+  K.acc = p.sf_nyq;
+  K.scale = w_nyq;
+  K.estride = ny;
+  K.pstride = 1;
+  K.pencils = ny;
+  return run_pencils(a, 1, 1, inv, accumulate ? EPI_ACC : EPI_STORE, s);
+}
+
+template <typename R>
+hipError_t forward_impl(Plan& p, const void* density, bool accumulate, hipStream_t s) {
+  hipError_t e = pass_z<R>(p, density, s);
+  if (e != hipSuccess) return e;
+  e = pass_y<R>(p, false, s);
+  if (e != hipSuccess) return e;
+  return pass_x<R>(p, false, accumulate, nullptr, nullptr, s);
+}
+
+// Real adjoint of the forward transform applied to the stored spectrum, each
+// bin first multiplied by the FP64 weight w (internal layout):
+//   out(r) = Re sum_{kx, ky, kz in [0, Nz/2]} w(q) X(q) e^{+i q.r}.
+template <typename R>
+hipError_t inverse_impl(Plan& p, const double* w_main, const double* w_nyq, float* out,
+                        hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  hipError_t e = pass_x<R>(p, true, false, w_main, w_nyq, s);
+  if (e != hipSuccess) return e;
+  e = pass_y<R>(p, true, s);
+  if (e != hipSuccess) return e;
+  C2RArgs<R, float> a;
+  a.main = spec_main<R>(p);
+  a.nyq = spec_nyq<R>(p);
+  a.out = out;
+  a.rows = (long long)nx * ny;
+  a.m = m;
+  a.P = tile_pencils<R>(m);
+  a.ax = p.ax_zc;
+  a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
+  const long long blocks = (a.rows + a.P - 1) / a.P;
+  hipLaunchKernelGGL((k_c2r_z<R, float>), dim3((unsigned)blocks), dim3(kFftThreads),
+                     tile_lds<R>(m, a.P), s, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+__global__ void k_spec_read(const typename V2<R>::T* __restrict__ main,
+                            const typename V2<R>::T* __restrict__ nyq, double2* __restrict__ out,
+                            int nx, int ny, int m) {
+  const long long h = m + 1;
+  const long long total = (long long)nx * ny * h;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / h;
+    const int kz = (int)(i - row * h);
+    const auto v = kz < m ? main[row * m + kz] : nyq[row];
+    out[i] = make_double2((double)v.x, (double)v.y);
+  }
 }
 
 }  // namespace
@@ -411,91 +663,47 @@ bool fft_factor(int n, int* radix, int* nstages) {
   return true;
 }
 
-hipError_t fft_make_twiddles(float2* tw, int n, hipStream_t s) {
+hipError_t fft_make_twiddles(float2* tw, double2* tw64, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, s, tw, n);
+  hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, s, tw, tw64, n);
   return hipGetLastError();
 }
 
-hipError_t fft_r2c_z(Plan& p, const float* in, float2* main, float2* nyq, hipStream_t s) {
-  R2CArgs a;
-  a.in = in;
-  a.main = main;
-  a.nyq = nyq;
-  a.rows = (long long)p.sys.n[0] * p.sys.n[1];
-  a.m = p.sys.n[2] / 2;
-  a.P = tile_pencils(a.m);
-  a.ax = p.ax_zc;
-  a.tw2 = p.tw_r2c;
-  const long long blocks = (a.rows + a.P - 1) / a.P;
-  const size_t lds = (size_t)a.P * lds_pitch(a.m) * 4 * sizeof(float);
-  hipLaunchKernelGGL(k_r2c_z, dim3((unsigned)blocks), dim3(kFftThreads), lds, s, a);
+hipError_t fft_forward(Plan& p, const void* density, bool accumulate, hipStream_t s) {
+  return p.fft64 ? forward_impl<double>(p, density, accumulate, s)
+                 : forward_impl<float>(p, density, accumulate, s);
+}
+
+hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, float* out,
+                       hipStream_t s) {
+  return p.fft64 ? inverse_impl<double>(p, w_main, w_nyq, out, s)
+                 : inverse_impl<float>(p, w_main, w_nyq, out, s);
+}
+
+hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s) {
+  switch (pass) {
+    case 0: return p.fft64 ? pass_z<double>(p, density, s) : pass_z<float>(p, density, s);
+    case 1: return p.fft64 ? pass_y<double>(p, false, s) : pass_y<float>(p, false, s);
+    default:
+      return p.fft64 ? pass_x<double>(p, false, true, nullptr, nullptr, s)
+                     : pass_x<float>(p, false, true, nullptr, nullptr, s);
+  }
+}
+
+hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  long long total = (long long)nx * ny * (m + 1);
+  long long blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (p.fft64)
+    hipLaunchKernelGGL(k_spec_read<double>, dim3((unsigned)blocks), dim3(256), 0, s,
+                       spec_main<double>(p), spec_nyq<double>(p), reinterpret_cast<double2*>(out),
+                       nx, ny, m);
+  else
+    hipLaunchKernelGGL(k_spec_read<float>, dim3((unsigned)blocks), dim3(256), 0, s,
+                       spec_main<float>(p), spec_nyq<float>(p), reinterpret_cast<double2*>(out),
+                       nx, ny, m);
   return hipGetLastError();
-}
-
-hipError_t fft_y(Plan& p, float2* main, float2* nyq, hipStream_t s) {
-  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
-  PencilArgs a{};
-  a.data = main;
-  a.acc = nullptr;
-  a.write_back = 1;
-  a.estride = m;
-  a.pstride = 1;
-  a.outer_stride = (long long)ny * m;
-  a.pencils = m;
-  a.n = ny;
-  a.pencil_major = 0;
-  a.ax = p.ax_y;
-  hipError_t e = run_pencils(a, nx, s, false);
-  if (e != hipSuccess) return e;
-  PencilArgs b{};
-  b.data = nyq;
-  b.acc = nullptr;
-  b.write_back = 1;
-  b.estride = 1;
-  b.pstride = ny;
-  b.outer_stride = 0;
-  b.pencils = nx;
-  b.n = ny;
-  b.pencil_major = 1;
-  b.ax = p.ax_y;
-  return run_pencils(b, 1, s, false);
-}
-
-static hipError_t fft_x_impl(Plan& p, float2* main, float2* nyq, bool accum, hipStream_t s) {
-  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
-  PencilArgs a{};
-  a.data = main;
-  a.acc = accum ? p.sf_main : nullptr;
-  a.write_back = accum ? 0 : 1;
-  a.estride = (long long)ny * m;
-  a.pstride = 1;
-  a.outer_stride = 0;
-  a.pencils = ny * m;
-  a.n = nx;
-  a.pencil_major = 0;
-  a.ax = p.ax_x;
-  hipError_t e = run_pencils(a, 1, s, false);
-  if (e != hipSuccess) return e;
-  PencilArgs b{};
-  b.data = nyq;
-  b.acc = accum ? p.sf_nyq : nullptr;
-  b.write_back = accum ? 0 : 1;
-  b.estride = ny;
-  b.pstride = 1;
-  b.outer_stride = 0;
-  b.pencils = ny;
-  b.n = nx;
-  b.pencil_major = 0;
-  b.ax = p.ax_x;
-  return run_pencils(b, 1, s, false);
-}
-
-hipError_t fft_x_accum(Plan& p, float2* main, float2* nyq, hipStream_t s) {
-  return fft_x_impl(p, main, nyq, true, s);
-}
-hipError_t fft_x(Plan& p, float2* main, float2* nyq, hipStream_t s) {
-  return fft_x_impl(p, main, nyq, false, s);
 }
 
 }  // namespace mdsf

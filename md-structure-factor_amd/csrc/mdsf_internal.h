@@ -57,6 +57,7 @@ struct FftAxis {
   int nstages = 0;
   int radix[kMaxStages] = {};
   float2* tw = nullptr;
+  double2* tw64 = nullptr;
 };
 
 struct Plan {
@@ -76,16 +77,19 @@ struct Plan {
   int* s_idx = nullptr;
   int4* s_ir = nullptr;
   double* s_r = nullptr;
-  float* tables = nullptr;
+  void* tables = nullptr;         // per-atom factor tables, FP32 or FP64 (fft64)
   int* err = nullptr;
   // grids
-  float* rho = nullptr;          // Nx*Ny*Nz floats == Nx*Ny*(Nz/2) complex (in place)
-  float2* nyq = nullptr;         // Nx*Ny complex: the kz = Nz/2 plane
+  void* rho = nullptr;           // Nx*Ny*Nz folded density (FP32, or FP64 when fft64)
+  void* spec_main = nullptr;     // Nx*Ny*(Nz/2) complex, in place over rho
+  void* spec_nyq = nullptr;      // Nx*Ny complex: the kz = Nz/2 plane
   double* sf_main = nullptr;     // Nx*Ny*(Nz/2) FP64 accumulator
   double* sf_nyq = nullptr;      // Nx*Ny FP64 accumulator
   // FFT
+  int fft64 = 0;                 // deposition, density and FFT in FP64 (else FP32)
   FftAxis ax_x, ax_y, ax_zc;     // zc: complex length Nz/2
-  float2* tw_r2c = nullptr;      // W_Nz^k, k in [0, Nz/2]
+  float2* tw_r2c = nullptr;      // W_Nz^k, k < Nz
+  double2* tw_r2c64 = nullptr;
 };
 
 // error handling -------------------------------------------------------------
@@ -105,18 +109,22 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
                        int64_t wrap_hi, uint32_t* occ, int32_t* home, hipStream_t s,
                        bool coords_only = false);
 hipError_t launch_binning(Plan& p, hipStream_t s);
-hipError_t launch_deposit(Plan& p, float* out, hipStream_t s);
+hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);
 
-hipError_t fft_make_twiddles(float2* tw, int n, hipStream_t s);
-hipError_t fft_r2c_z(Plan& p, const float* in, float2* main, float2* nyq, hipStream_t s);
-hipError_t fft_y(Plan& p, float2* main, float2* nyq, hipStream_t s);
-hipError_t fft_x_accum(Plan& p, float2* main, float2* nyq, hipStream_t s);
-hipError_t fft_x(Plan& p, float2* main, float2* nyq, hipStream_t s);
+hipError_t fft_make_twiddles(float2* tw, double2* tw64, int n, hipStream_t s);
+// density (FP32, Nx*Ny*Nz) -> spectrum; with `accumulate` the x pass adds
+// |F|^2 into sf_main / sf_nyq instead of storing F.
+hipError_t fft_forward(Plan& p, const void* density, bool accumulate, hipStream_t s);
+// out = Re sum_q w(q) F(q) e^{+i q.r} over the stored spectrum (consumes it).
+hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, float* out,
+                       hipStream_t s);
+// one forward pass: 0 = z (R2C from `density`), 1 = y, 2 = x with |F|^2 accumulation
+hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s);
+// stored spectrum -> complex128 (Nx, Ny, Nz/2+1)
+hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s);
 bool fft_factor(int n, int* radix, int* nstages);
 
 hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s);
-hipError_t launch_spectrum_read(const Plan& p, const float2* main, const float2* nyq,
-                                float* out, hipStream_t s);
 hipError_t launch_outputs(const int n[3], const double* sf, const double box[3], int box_f32,
                           double* sfplt, double* kgrid, double* kgridplt, hipStream_t s);
 hipError_t launch_ewald(const double* kp, int nx, int ny, int nz, double wl, double* xy,

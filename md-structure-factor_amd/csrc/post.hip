@@ -23,18 +23,6 @@ __global__ void k_sf_read(const double* __restrict__ sfm, const double* __restri
   }
 }
 
-__global__ void k_spec_read(const float2* __restrict__ main, const float2* __restrict__ nyq,
-                            float2* __restrict__ out, int nx, int ny, int m) {
-  const long long h = m + 1;
-  const long long total = (long long)nx * ny * h;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    const long long row = i / h;
-    const int kz = (int)(i - row * h);
-    out[i] = kz < m ? main[row * m + kz] : nyq[row];
-  }
-}
-
 __device__ __forceinline__ int pm(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
@@ -197,14 +185,6 @@ hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
   hipLaunchKernelGGL(k_sf_read, dim3(grid_for((long long)nx * ny * (m + 1))), dim3(256), 0, s,
                      p.sf_main, p.sf_nyq, sf, nx, ny, m);
-  return hipGetLastError();
-}
-
-hipError_t launch_spectrum_read(const Plan& p, const float2* main, const float2* nyq, float* out,
-                                hipStream_t s) {
-  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
-  hipLaunchKernelGGL(k_spec_read, dim3(grid_for((long long)nx * ny * (m + 1))), dim3(256), 0, s,
-                     main, nyq, reinterpret_cast<float2*>(out), nx, ny, m);
   return hipGetLastError();
 }
 

@@ -157,15 +157,16 @@ __global__ void k_bucket_sort(const int* __restrict__ boff, int* __restrict__ s_
 // Gathers atoms into sorted order and writes their 1-D factor tables:
 //   x: amp * exp(-g00 bx^2 / 2s^2), y: exp(-g11 by^2 / 2s^2), z: exp(-g22 bz^2 / 2s^2)
 // with b = r - (ir - A + k) dr in FP64 exactly as buf = r - grid (dens.py:252-256,299),
-// k in [0, 2A).  Values are rounded to FP32 once.
+// k in [0, 2A).  Values are rounded once to the plan's precision (FP32 or FP64).
 constexpr int kTabThreads = 64;
+template <typename TT>
 __global__ __launch_bounds__(256) void k_gather_tables(SysDev sys, const TypeDev* __restrict__ td,
                                                        const int* __restrict__ s_idx,
                                                        const int4* __restrict__ ir4,
                                                        const double* __restrict__ rpos,
                                                        int4* __restrict__ s_ir,
                                                        double* __restrict__ s_r,
-                                                       float* __restrict__ tables,
+                                                       TT* __restrict__ tables,
                                                        const int* __restrict__ nvalid) {
 #pragma clang fp contract(off)
   const int s = blockIdx.x * (blockDim.x / kTabThreads) + threadIdx.x / kTabThreads;
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(256) void k_gather_tables(SysDev sys, const TypeDev
   // valid; after a range / type error (reported to the host) write nothing
   const int slot = s - ty.type_start;
   if (slot < 0 || slot >= ty.count) return;
-  float* tab = tables + ty.table_base + (long long)slot * ty.stride;
+  TT* tab = tables + ty.table_base + (long long)slot * ty.stride;
   const int ir[3] = {q.x, q.y, q.z};
   const double g[3] = {sys.g00, sys.g11, sys.g22};
   const int len0 = 2 * ty.half[0], len1 = 2 * ty.half[1], len2 = 2 * ty.half[2];
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(256) void k_gather_tables(SysDev sys, const TypeDev
     const double qv = (g[d] * (b * b)) / ty.two_sig2;
     double v = exp(-qv);
     if (d == 0) v = ty.amp * v;
-    tab[e] = (float)v;
+    tab[e] = (TT)v;
   }
 }
 
@@ -250,9 +251,15 @@ hipError_t launch_binning(Plan& p, hipStream_t s) {
     hipLaunchKernelGGL(k_bucket_sort, dim3((nbk + 255) / 256), dim3(256), 0, s, p.bucket_off,
                        p.s_idx, nbk);
     const int per_block = 256 / kTabThreads;
-    hipLaunchKernelGGL(k_gather_tables, dim3((n + per_block - 1) / per_block), dim3(256), 0, s,
-                       p.sys, p.types, p.s_idx, p.ir4, p.rpos, p.s_ir, p.s_r, p.tables,
-                       p.bucket_off + nbk);
+    const dim3 grid((n + per_block - 1) / per_block), block(256);
+    if (p.fft64)
+      hipLaunchKernelGGL(k_gather_tables<double>, grid, block, 0, s, p.sys, p.types, p.s_idx,
+                         p.ir4, p.rpos, p.s_ir, p.s_r, static_cast<double*>(p.tables),
+                         p.bucket_off + nbk);
+    else
+      hipLaunchKernelGGL(k_gather_tables<float>, grid, block, 0, s, p.sys, p.types, p.s_idx,
+                         p.ir4, p.rpos, p.s_ir, p.s_r, static_cast<float*>(p.tables),
+                         p.bucket_off + nbk);
   }
   return hipGetLastError();
 }

@@ -24,6 +24,12 @@ theta = np.pi / 3                     # dens.py:16 (set by callers, not read)
 PRECISION = 1.0E-24                   # dens.py:18
 dtyp = np.float64                     # dens.py:20 (FP64 S(q) accumulation)
 
+# FFT precision of the GPU path: "fp32", "fp64" or "auto" (FP64 for grids of
+# at most 128^3 points, where it costs little and FP32 rounding is visible on
+# sparse densities; FP32 above).  S(q) is always summed in FP64.
+FFT_PRECISION = "auto"
+AUTO_FP64_MAX_POINTS = 128 ** 3
+
 
 def load_radii(filename):
     """``{name: (Nel, sigma_angstrom)}`` from ``Nel name radius_pm`` lines (dens.py:23-35)."""
@@ -50,6 +56,16 @@ def get_dplot(dmag):
     sf = torch.from_numpy(np.ascontiguousarray(dmag)).to(plan.device)
     sfplt, _, _ = plan.outputs(sf, [1.0, 1.0, 1.0], False)
     return sfplt.cpu().numpy()
+
+
+def _use_fft64(n):
+    if FFT_PRECISION == "fp64":
+        return True
+    if FFT_PRECISION == "fp32":
+        return False
+    if FFT_PRECISION != "auto":
+        raise ValueError("FFT_PRECISION must be 'auto', 'fp32' or 'fp64'")
+    return int(np.prod([int(x) for x in n])) <= AUTO_FP64_MAX_POINTS
 
 
 def _output_plan(n):
@@ -83,7 +99,7 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     s = engine.Setup(r, L, typ, rad, Sres, USE_BETTER_RESOLUTION, ucell)
     if PRINT_DETAILS:
         _print_details(s.L, Sres, s.N)
-    plan = engine.Plan.from_setup(s)
+    plan = engine.Plan.from_setup(s, fft64=_use_fft64(s.N))
     dev = plan.device
     coords = torch.from_numpy(np.ascontiguousarray(r)).to(dev)
     scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)

@@ -17,6 +17,7 @@ ERR_TYPE = 4
 ERR_UNSUPPORTED = 5
 F32 = 0
 F64 = 1
+FFT_FP64 = 1
 
 c_int32 = ctypes.c_int32
 c_int64 = ctypes.c_int64
@@ -35,7 +36,7 @@ class GridDesc(ctypes.Structure):
     """``mdsf_grid_desc``."""
     _fields_ = [("n", c_int32 * 3), ("nborder", c_int32), ("dr", c_double * 3),
                 ("ucell", c_double * 9), ("ntypes", c_int32),
-                ("types", ctypes.POINTER(TypeDesc)), ("natoms", c_int32)]
+                ("types", ctypes.POINTER(TypeDesc)), ("natoms", c_int32), ("flags", c_int32)]
 
 
 # name -> (restype, argtypes); every symbol include/mdsf.h declares
@@ -55,6 +56,11 @@ SIGNATURES = {
     "mdsf_deposit_fold": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
                                     ctypes.POINTER(c_double), c_int32, c_int64, c_int64,
                                     c_void_p, c_void_p, c_void_p]),
+    "mdsf_prepare_frame": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
+                                     ctypes.POINTER(c_double), c_int32, c_int64, c_int64,
+                                     c_void_p, c_void_p, c_void_p]),
+    "mdsf_deposit": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "mdsf_fft_pass": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "mdsf_accumulate_density": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mdsf_rfft3": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mdsf_sf_read": (c_int32, [c_void_p, c_void_p, c_void_p]),

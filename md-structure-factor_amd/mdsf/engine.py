@@ -85,7 +85,8 @@ class Setup:
 class Plan:
     """One system (grid, border, cell, types) bound to a libmdsf plan on a GPU."""
 
-    def __init__(self, n, nborder, dr, ucell, amp, two_sig2, half, counts, device=None):
+    def __init__(self, n, nborder, dr, ucell, amp, two_sig2, half, counts, device=None,
+                 fft64=False):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise RuntimeError("libmdsf needs a ROCm GPU (torch.cuda.is_available() is False)")
@@ -111,13 +112,16 @@ class Plan:
         desc.ntypes = ntypes
         desc.types = tarr
         desc.natoms = self.natoms
+        desc.flags = _lib.FFT_FP64 if fft64 else 0
+        self.fft64 = bool(fft64)
         self._handle = _lib.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.mdsf_plan_create(desc, self._handle))
 
     @classmethod
-    def from_setup(cls, s, device=None):
-        return cls(s.N, s.nborder, s.dr, s.ucell, s.amp, s.two_sig2, s.half, s.counts, device)
+    def from_setup(cls, s, device=None, fft64=False):
+        return cls(s.N, s.nborder, s.dr, s.ucell, s.amp, s.two_sig2, s.half, s.counts, device,
+                   fft64)
 
     # ----------------------------------------------------------------- utils
     def _stream(self):
@@ -164,9 +168,13 @@ class Plan:
             self._handle, self._ptr(coords), dtype, coords.shape[0], self._ptr(scale),
             _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]), self._stream()))
 
+    @property
+    def real_dtype(self):
+        return torch.float64 if self.fft64 else torch.float32
+
     def deposit_fold(self, coords, types, scale, box, arith, wrap, home=None):
-        """One frame (dens.py:283-311): folded FP32 density (Nx, Ny, Nz)."""
-        out = torch.empty(self.n, dtype=torch.float32, device=self.device)
+        """One frame (dens.py:283-311): folded density (Nx, Ny, Nz) in plan precision."""
+        out = torch.empty(self.n, dtype=self.real_dtype, device=self.device)
         dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
         _lib.check(self.lib.mdsf_deposit_fold(
             self._handle, self._ptr(coords), dtype, self._ptr(types), self._ptr(scale),
@@ -174,14 +182,30 @@ class Plan:
             self._ptr(home), self._stream()))
         return out
 
+    def prepare_frame(self, coords, types, scale, box, arith, wrap, occupancy=None, home=None):
+        """Rescale/wrap one frame in place and bin its atoms (first launches of a frame)."""
+        dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
+        _lib.check(self.lib.mdsf_prepare_frame(
+            self._handle, self._ptr(coords), dtype, self._ptr(types), self._ptr(scale),
+            _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]), self._ptr(occupancy),
+            self._ptr(home), self._stream()))
+
+    def deposit(self, out=None):
+        _lib.check(self.lib.mdsf_deposit(self._handle, self._ptr(out), self._stream()))
+
+    def fft_pass(self, which, density=None):
+        _lib.check(self.lib.mdsf_fft_pass(self._handle, int(which), self._ptr(density),
+                                          self._stream()))
+
     def accumulate_density(self, density):
-        d = density.to(device=self.device, dtype=torch.float32).contiguous()
+        d = density.to(device=self.device, dtype=self.real_dtype).contiguous()
         _lib.check(self.lib.mdsf_accumulate_density(self._handle, self._ptr(d), self._stream()))
         return d
 
     def rfft3(self, density):
-        d = density.to(device=self.device, dtype=torch.float32).contiguous()
-        out = torch.empty(self.half_shape + (2,), dtype=torch.float32, device=self.device)
+        """np.fft.rfftn of a density in the plan's precision (complex128 out)."""
+        d = density.to(device=self.device, dtype=self.real_dtype).contiguous()
+        out = torch.empty(self.half_shape + (2,), dtype=torch.float64, device=self.device)
         _lib.check(self.lib.mdsf_rfft3(self._handle, self._ptr(d), self._ptr(out), self._stream()))
         return torch.view_as_complex(out)
 

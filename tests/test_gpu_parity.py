@@ -29,18 +29,21 @@ def assert_sq_close(got, want, rel=SQ_REL, floor=SQ_FLOOR):
     return err
 
 
-def assert_density_close(got, want):
-    tol = 1e-5 * np.abs(want) + 2e-6 * np.max(np.abs(want))
+def assert_density_close(got, want, fp64=False):
+    if fp64:
+        tol = 1e-12 * np.abs(want) + 1e-14 * np.max(np.abs(want))
+    else:
+        tol = 1e-5 * np.abs(want) + 2e-6 * np.max(np.abs(want))
     bad = np.abs(got - want) > tol
     assert not bad.any(), (f"{bad.sum()} voxels off; max abs err "
                            f"{np.max(np.abs(got - want)):.3e}, max {np.max(want):.3e}")
 
 
-def _gpu(r, dims, typ, rad, sres, better, ucell):
+def _gpu(r, dims, typ, rad, sres, better, ucell, fp64=False):
     import torch
     from mdsf import engine
     s = engine.Setup(r, dims, typ, rad, sres, better, ucell)
-    plan = engine.Plan.from_setup(s)
+    plan = engine.Plan.from_setup(s, fft64=fp64)
     coords = torch.from_numpy(np.ascontiguousarray(r)).to(plan.device)
     types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(plan.device)
     scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(plan.device)
@@ -59,9 +62,11 @@ def _oracle_frames(r, dims, typ, rad, sres, better, ucell):
     return rr, o, out
 
 
-def _run_dropin(c, rad, tmp_path):
+def _run_dropin(c, rad, tmp_path, precision="auto"):
     import dens
-    saved = (dens.USE_BETTER_RESOLUTION, dens.RANDOM_NOISE, dens.PRINT_DETAILS)
+    saved = (dens.USE_BETTER_RESOLUTION, dens.RANDOM_NOISE, dens.PRINT_DETAILS,
+             dens.FFT_PRECISION)
+    dens.FFT_PRECISION = precision
     dens.USE_BETTER_RESOLUTION = bool(c["in_better"])
     dens.RANDOM_NOISE = int(c["in_noise"])
     dens.PRINT_DETAILS = False
@@ -73,11 +78,13 @@ def _run_dropin(c, rad, tmp_path):
         dens.compute_sf(r, np.array(c["in_dims"]), c["typ"], out, rad, c["in_ucell"], c["sres"])
         return r, dict(np.load(out + ".npz"))
     finally:
-        dens.USE_BETTER_RESOLUTION, dens.RANDOM_NOISE, dens.PRINT_DETAILS = saved
+        (dens.USE_BETTER_RESOLUTION, dens.RANDOM_NOISE, dens.PRINT_DETAILS,
+         dens.FFT_PRECISION) = saved
 
 
 @pytest.mark.parametrize("name", golden_cases())
 def test_dropin_matches_reference_goldens(name, rad, gpu, tmp_path):
+    """Default precision policy (FP64 FFT at these small sizes)."""
     c = load_golden(name)
     r, out = _run_dropin(c, rad, tmp_path)
     assert sorted(out) == sorted(["sf", "sfplt", "L", "N", "kgrid", "kgridplt"])
@@ -93,14 +100,26 @@ def test_dropin_matches_reference_goldens(name, rad, gpu, tmp_path):
     assert np.array_equal(out["kgridplt"][..., 3], out["sfplt"])
 
 
+@pytest.mark.parametrize("name", golden_cases())
+def test_dropin_fp32_fft_on_goldens(name, rad, gpu, tmp_path):
+    """FP32 FFT on the tiny sparse goldens: an FP32 transform (even numpy's
+    single-precision pocketfft) reaches 1.2e-5 .. 5e-5 here, so the bound is 1e-4."""
+    c = load_golden(name)
+    r, out = _run_dropin(c, rad, tmp_path, precision="fp32")
+    assert np.array_equal(r, c["out_r"])
+    assert np.array_equal(out["kgrid"], c["out_kgrid"])
+    assert_sq_close(out["sf"], c["out_sf"], rel=1e-4)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("name", ["orth_corners_f64", "mono120_f32", "na_wide",
                                   "aniso_sres_array", "mixed_f32_24"])
-def test_deposit_fold_and_home_cells(name, rad, gpu):
+def test_deposit_fold_and_home_cells(name, fp64, rad, gpu):
     import torch
     c = load_golden(name)
     ucell = c["in_ucell"]
     args = (c["in_coords"], c["in_dims"], c["typ"], rad, c["sres"], bool(c["in_better"]), ucell)
-    s, plan, coords, types, scale = _gpu(*args)
+    s, plan, coords, types, scale = _gpu(*args, fp64=fp64)
     _, o, ref = _oracle_frames(*args)
     for t in range(coords.shape[0]):
         home = torch.empty((s.natoms, 3), dtype=torch.int32, device=plan.device)
@@ -108,7 +127,7 @@ def test_deposit_fold_and_home_cells(name, rad, gpu):
                               s.wrap, home=home)
         plan.check_errors()
         assert np.array_equal(home.cpu().numpy(), ref[t][1]), "home cells differ"
-        assert_density_close(d.cpu().numpy().astype(np.float64), ref[t][0])
+        assert_density_close(d.cpu().numpy().astype(np.float64), ref[t][0], fp64)
 
 
 def _corner_case(rng, box, n_each, names, theta_deg=None, dtype=np.float64):
@@ -130,18 +149,19 @@ def _corner_case(rng, box, n_each, names, theta_deg=None, dtype=np.float64):
     return r, np.array([box]), typ, ucell
 
 
+@pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("theta", [None, 120.0, 90.0])
 @pytest.mark.parametrize("sres", [0.5, 0.3])
-def test_corner_fold_quirk(theta, sres, rad, gpu):
+def test_corner_fold_quirk(theta, sres, fp64, rad, gpu):
     rng = np.random.default_rng(11)
     box = np.array([7.0, 8.0, 9.0])
     r, dims, typ, ucell = _corner_case(rng, box, 2, ["C", "O", "H", "N"], theta)
     args = (r, dims, typ, rad, sres, True, ucell)
-    s, plan, coords, types, scale = _gpu(*args)
+    s, plan, coords, types, scale = _gpu(*args, fp64=fp64)
     _, o, ref = _oracle_frames(*args)
     d = plan.deposit_fold(coords[0], types, scale[0], [float(x) for x in s.L], s.arith, s.wrap)
     plan.check_errors()
-    assert_density_close(d.cpu().numpy().astype(np.float64), ref[0][0])
+    assert_density_close(d.cpu().numpy().astype(np.float64), ref[0][0], fp64)
 
 
 def test_occupancy_histogram_bit_exact(rad, gpu):
@@ -160,21 +180,21 @@ def test_occupancy_histogram_bit_exact(rad, gpu):
     assert np.array_equal(occ.cpu().numpy().astype(np.int64), want)
 
 
+@pytest.mark.parametrize("fft64", [False, True])
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (64, 64, 64), (96, 160, 192),
                                    (38, 14, 22), (30, 18, 50), (2, 4, 2), (256, 256, 256)])
-def test_rfft3_matches_numpy(shape, gpu):
+def test_rfft3_matches_numpy(shape, fft64, gpu):
     import torch
     from mdsf import engine
     rng = np.random.default_rng(7)
-    x = rng.random(shape).astype(np.float32)
-    plan = engine.Plan(shape, 0, [1.0] * 3, np.eye(3), [], [], [], [])
-    got = plan.rfft3(torch.from_numpy(x)).cpu().numpy().astype(np.complex128)
+    x = rng.random(shape).astype(np.float32) - 0.5
+    plan = engine.Plan(shape, 0, [1.0] * 3, np.eye(3), [], [], [], [], fft64=fft64)
+    got = plan.rfft3(torch.from_numpy(x)).cpu().numpy()
     want = np.fft.rfftn(x.astype(np.float64))
-    err = np.abs(got - want).max() / np.abs(want).max()
-    assert err < 2e-6, err
-    # relative error on the non-DC spectrum, against its own scale
+    # error against the rms spectrum amplitude (Parseval scale of a zero-mean input)
     scale = np.sqrt(np.mean(np.abs(want) ** 2))
-    assert np.abs(got - want).max() < 1e-5 * scale * np.sqrt(np.log2(x.size))
+    err = np.abs(got - want).max() / scale
+    assert err < (1e-12 if fft64 else 2e-6) * np.log2(x.size), err
 
 
 def test_accumulate_density_power_sum(gpu):
