@@ -306,7 +306,7 @@ __device__ void stage_xy(StageXY<ACC>& st, int a, int part, int c, const SysDev&
   ACC(*b)[NB][T] = d == 0 ? st.bx : st.by;
   int(*v)[T] = d == 0 ? st.xv : st.yv;
   int mask = 0;
-#pragma unroll
+#pragma unroll 2
   for (int i = 0; i < T; ++i) {
     int bits = 0;
 #pragma unroll
@@ -404,11 +404,11 @@ __device__ void accumulate_xy(const StageXY<ACC>& st, int nbatch, int wave, int 
     const int xv = st.xv[a][ci], yv = st.yv[a][cj];
     const ACC amp = st.amp[a], k2 = st.k2[a];
     ACC ws = 0, wl = 0, wh = 0;
-#pragma unroll
+#pragma unroll 1
     for (int sx = 0; sx < 3; ++sx) {
       if (!((xv >> sx) & 1)) continue;
       const ACC bx = st.bx[sx][a][ci];
-#pragma unroll
+#pragma unroll 1
       for (int sy = 0; sy < 3; ++sy) {
         if (!((yv >> sy) & 1)) continue;
         const ACC by = st.by[sy][a][cj];
@@ -425,86 +425,68 @@ __device__ void accumulate_xy(const StageXY<ACC>& st, int nbatch, int wave, int 
   }
 }
 
-template <int MODE, typename ACC>
-__global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev* __restrict__ td,
-                                                      const int* __restrict__ boff,
-                                                      const int4* __restrict__ s_ir,
-                                                      const double* __restrict__ s_r,
-                                                      const void* __restrict__ tables,
-                                                      ACC* __restrict__ rho) {
-  using Stage = typename std::conditional<MODE == MODE_DIAG, StageDiag<ACC>, StageXY<ACC>>::type;
-  __shared__ Stage st;
-  __shared__ int list[NB + kThreads];
-  __shared__ int wsum[kThreads / 64];
-  __shared__ int col_s[kThreads], col_pre[kThreads + 1], col_t[kThreads];
-  __shared__ int sh_half[kMaxTypes][3];
+// ---------------------------------------------------------------------------
+// Candidate enumeration shared by both accumulation paths.
+constexpr int kListCap = 320;  // trigger (<= 64) - 1 + one 256-candidate chunk, rounded up
 
+struct EnumShared {
+  int list[kListCap];  // candidate sorted index | corner << 31
+  int wsum[kThreads / 64];
+  int col_s[kThreads], col_pre[kThreads + 1], col_t[kThreads];
+  int half[kMaxTypes][3];
+};
+
+// block-wide exclusive scan of one int per thread; returns the total
+__device__ __forceinline__ int block_scan(EnumShared& sh, int v, int* excl) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh.wsum[wave] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    const int s = sh.wsum[w];
+    off += w < wave ? s : 0;
+    tot += s;
+  }
+  *excl = off + x - v;
+  __syncthreads();
+  return tot;
+}
+
+// drop the first n list entries (n <= count <= kListCap)
+__device__ __forceinline__ void list_shift(EnumShared& sh, int n, int count) {
   const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  // XCD-aware order: consecutive tiles (neighbours in z) share an XCD's L2.
-  const int nblk = gridDim.x, bid = blockIdx.x;
-  const int q8 = nblk / 8;
-  const int tile = bid < 8 * q8 ? (bid % 8) * q8 + bid / 8 : bid;
-  const int ntz = sys.ntiles[2], nty = sys.ntiles[1];
-  const int tz = tile % ntz, ty_ = (tile / ntz) % nty, tx = tile / (ntz * nty);
-  Tile tl;
-  tl.t0[0] = tx * T;
-  tl.t0[1] = ty_ * T;
-  tl.t0[2] = tz * T;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) tl.t1[d] = min(tl.t0[d] + T, sys.n[d]);
-  for (int i = tid; i < sys.ntypes * 3; i += kThreads) sh_half[i / 3][i % 3] = td[i / 3].half[i % 3];
+  const int rem = count - n;
+  __syncthreads();
+  const int v0 = tid < rem ? sh.list[n + tid] : 0;
+  const int v1 = tid + kThreads < rem ? sh.list[n + tid + kThreads] : 0;
+  __syncthreads();
+  if (tid < rem) sh.list[tid] = v0;
+  if (tid + kThreads < rem) sh.list[tid + kThreads] = v1;
+  __syncthreads();
+}
 
-  const int ci = (wave & 1) * 8 + (lane & 7);
-  const int cj = (wave >> 1) * 8 + (lane >> 3);
-  const ACC g00 = (ACC)sys.g00, g01x2 = (ACC)(2.0 * sys.g01), g11 = (ACC)sys.g11;
-  ACC acc[T];
-#pragma unroll
-  for (int k = 0; k < T; ++k) acc[k] = 0;
+// Streams every atom whose window reaches the tile into sh.list in a fixed
+// order; `process(count)` consumes a prefix of the list and returns its size.
+template <typename Process>
+__device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td,
+                               const int* __restrict__ boff, const int4* __restrict__ s_ir,
+                               const Tile& tl, EnumShared& sh, int trigger, Process&& process) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < sys.ntypes * 3; i += kThreads) sh.half[i / 3][i % 3] = td[i / 3].half[i % 3];
   int count = 0;
-
-  // block-wide exclusive scan of one int per thread; returns the total
-  auto block_scan = [&](int v, int* excl) -> int {
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) {
-      const int s = wsum[w];
-      off += w < wave ? s : 0;
-      tot += s;
-    }
-    *excl = off + x - v;
-    __syncthreads();
-    return tot;
-  };
-
-  auto process = [&](int nbatch) {
-    __syncthreads();
-    const int a = tid >> 2, part = tid & 3;
-    if (a < nbatch) {
-      if constexpr (MODE == MODE_DIAG) stage_diag<ACC>(st, a, part, list[a], sys, td, s_ir, tables, tl);
-      else stage_xy<ACC>(st, a, part, list[a], sys, td, s_ir, s_r, tables, tl);
-    }
-    __syncthreads();
-    if constexpr (MODE == MODE_DIAG) accumulate_diag<ACC>(st, nbatch, wave, ci, cj, acc);
-    else accumulate_xy<ACC>(st, nbatch, wave, ci, cj, acc, g00, g01x2, g11);
-  };
-
-  // total bucket columns over all types (uniform)
   int total_cols = 0;
   for (int t = 0; t < sys.ntypes; ++t) {
     TypeRanges r;
     type_ranges(td[t], sys, tl.t0, tl.t1, r);
     total_cols += r.ncols;
   }
-
   for (int cbase = 0; cbase < total_cols; cbase += kThreads) {
     // phase A: one bucket column per thread -> (start, length, type)
     int my_s = 0, my_len = 0, my_t = 0;
@@ -531,70 +513,433 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
       }
     }
     int excl;
-    const int ncand = block_scan(my_len, &excl);
-    col_s[tid] = my_s;
-    col_t[tid] = my_t;
-    col_pre[tid] = excl;
-    if (tid == 0) col_pre[kThreads] = ncand;
+    const int ncand = block_scan(sh, my_len, &excl);
+    sh.col_s[tid] = my_s;
+    sh.col_t[tid] = my_t;
+    sh.col_pre[tid] = excl;
+    if (tid == 0) sh.col_pre[kThreads] = ncand;
     __syncthreads();
     // phase B: stream the candidates, exact test, ordered compaction
     for (int g0 = 0; g0 < ncand; g0 += kThreads) {
       const int g = g0 + tid;
       bool hit = false;
-      int c = 0;
+      int entry = 0;
       if (g < ncand) {
-        int lo = 0, hi = kThreads;  // last r with col_pre[r] <= g
+        int lo = 0, hi = kThreads;  // last column r with col_pre[r] <= g
         while (hi - lo > 1) {
           const int mid = (lo + hi) >> 1;
-          if (col_pre[mid] <= g) lo = mid;
+          if (sh.col_pre[mid] <= g) lo = mid;
           else hi = mid;
         }
-        c = col_s[lo] + (g - col_pre[lo]);
-        const int t = col_t[lo];
+        const int c = sh.col_s[lo] + (g - sh.col_pre[lo]);
+        const int t = sh.col_t[lo];
         const int4 q = s_ir[c];
-        hit = overlap1(q.x, sh_half[t][0], tl.t0[0], tl.t1[0], sys.n[0]) &&
-              overlap1(q.y, sh_half[t][1], tl.t0[1], tl.t1[1], sys.n[1]) &&
-              overlap1(q.z, sh_half[t][2], tl.t0[2], tl.t1[2], sys.n[2]);
+        hit = overlap1(q.x, sh.half[t][0], tl.t0[0], tl.t1[0], sys.n[0]) &&
+              overlap1(q.y, sh.half[t][1], tl.t0[1], tl.t1[1], sys.n[1]) &&
+              overlap1(q.z, sh.half[t][2], tl.t0[2], tl.t1[2], sys.n[2]);
+        entry = c | ((q.w >> 16) ? (int)0x80000000u : 0);
       }
       int pos;
-      const int nhit = block_scan(hit ? 1 : 0, &pos);
-      if (hit) list[count + pos] = c;
+      const int nhit = block_scan(sh, hit ? 1 : 0, &pos);
+      if (hit) sh.list[count + pos] = entry;
       count += nhit;
       __syncthreads();
-      while (count >= NB) {
-        process(NB);
-        __syncthreads();
-        const int rem = count - NB;
-        const int v = tid < rem ? list[NB + tid] : 0;
-        __syncthreads();
-        if (tid < rem) list[tid] = v;
-        count = rem;
-        __syncthreads();
+      while (count >= trigger) {
+        const int n = process(count);
+        list_shift(sh, n, count);
+        count -= n;
       }
     }
     __syncthreads();
   }
-  if (count > 0) process(count);
+  while (count > 0) {
+    const int n = process(count);
+    list_shift(sh, n, count);
+    count -= n;
+  }
+}
+
+__device__ __forceinline__ void tile_coords(const SysDev& sys, Tile& tl) {
+  // XCD-aware order: consecutive tiles (neighbours in z) share an XCD's L2.
+  const int nblk = gridDim.x, bid = blockIdx.x;
+  const int q8 = nblk / 8;
+  const int tile = bid < 8 * q8 ? (bid % 8) * q8 + bid / 8 : bid;
+  const int ntz = sys.ntiles[2], nty = sys.ntiles[1];
+  const int tz = tile % ntz, ty_ = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  tl.t0[0] = tx * T;
+  tl.t0[1] = ty_ * T;
+  tl.t0[2] = tz * T;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) tl.t1[d] = min(tl.t0[d] + T, sys.n[d]);
+}
+
+// ---------------------------------------------------------------------------
+// VALU path (FP64 plans): lane = one (x, y) column, 16 z accumulators.
+template <int MODE, typename ACC>
+__global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev* __restrict__ td,
+                                                      const int* __restrict__ boff,
+                                                      const int4* __restrict__ s_ir,
+                                                      const double* __restrict__ s_r,
+                                                      const void* __restrict__ tables,
+                                                      ACC* __restrict__ rho) {
+  using Stage = typename std::conditional<MODE == MODE_DIAG, StageDiag<ACC>, StageXY<ACC>>::type;
+  __shared__ Stage st;
+  __shared__ EnumShared sh;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  Tile tl;
+  tile_coords(sys, tl);
+  const int ci = (wave & 1) * 8 + (lane & 7);
+  const int cj = (wave >> 1) * 8 + (lane >> 3);
+  const ACC g00 = (ACC)sys.g00, g01x2 = (ACC)(2.0 * sys.g01), g11 = (ACC)sys.g11;
+  ACC acc[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) acc[k] = 0;
+
+  auto process = [&](int count) -> int {
+    const int nbatch = min(count, NB);
+    __syncthreads();
+    const int a = tid >> 2, part = tid & 3;
+    if (a < nbatch) {
+      const int c = sh.list[a] & 0x7fffffff;
+      if constexpr (MODE == MODE_DIAG) stage_diag<ACC>(st, a, part, c, sys, td, s_ir, tables, tl);
+      else stage_xy<ACC>(st, a, part, c, sys, td, s_ir, s_r, tables, tl);
+    }
+    __syncthreads();
+    if constexpr (MODE == MODE_DIAG) accumulate_diag<ACC>(st, nbatch, wave, ci, cj, acc);
+    else accumulate_xy<ACC>(st, nbatch, wave, ci, cj, acc, g00, g01x2, g11);
+    return nbatch;
+  };
+  enumerate_tile(sys, td, boff, s_ir, tl, sh, NB, process);
 
   const int gx = tl.t0[0] + ci, gy = tl.t0[1] + cj;
   if (gx < sys.n[0] && gy < sys.n[1]) {
     ACC* dst = rho + ((size_t)gx * sys.n[1] + gy) * sys.n[2] + tl.t0[2];
     const int nz = tl.t1[2] - tl.t0[2];
-    if (nz == T && (sys.n[2] & 3) == 0) {
 #pragma unroll
-      for (int g = 0; g < T / 4; ++g) {
-        if constexpr (std::is_same<ACC, float>::value) {
-          reinterpret_cast<float4*>(dst)[g] =
-              make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+    for (int k = 0; k < T; ++k)
+      if (k < nz) dst[k] = acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA path (FP32 plans).  Each atom becomes one or more separable "virtual
+// atoms" (X[16], Y[16], Z[16] over the tile), and the tile is the sum of their
+// rank-1 products: D[(x, y), z] += sum_k X_k[x] Y_k[y] Z_k[z].  Wave w owns the
+// x rows 4w..4w+3, i.e. four 16(y) x 16(z) blocks, accumulated with
+// v_mfma_f32_16x16x4f32 (A[y][k] = X_k[x] Y_k[y], B[k][z] = Z_k[z], 4 virtual
+// atoms per instruction).  DIAG corner-quirk atoms expand into 4 separable
+// terms: (Xc, Yall, Zs), (Xn, Yc, Zs), (Xn, Yl, ZL), (Xn, Yh, ZH); XY atoms
+// stage image offsets and build A per lane (classes all / std / L / H).
+constexpr int NV = 64;  // virtual atom slots per batch
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct StageM {
+  __align__(16) float vx[NV][T];
+  __align__(16) float vy[NV][T];
+  __align__(16) float vz[NV][T];
+  int vxm[NV];
+};
+
+struct StageMXY {
+  __align__(16) float bx[3][NV][T];
+  __align__(16) float by[3][NV][T];
+  __align__(16) float vz[NV][T];
+  int xv[NV][T], yv[NV][T];
+  float amp[NV], k2[NV];
+  int vreal[NV], vcls[NV], vxm[NV];
+};
+
+// Per-cell values of atom `c`'s factor along axis d over the tile, split by
+// the side of the image (L / C / H) and, for z of a corner atom, the two
+// shifted slots of the corner quirk (SL: y on L, SH: y on H).
+struct AxisCtx {
+  const float* tb;
+  int kL, kC, kH, kSL, kSH, len, tn, t0, N, nb;
+};
+
+__device__ __forceinline__ AxisCtx axis_ctx(int d, const int4& q, const TypeDev& ty, int slot,
+                                            const SysDev& sys, const float* tables, const Tile& tl) {
+  AxisCtx x;
+  x.N = pick3(sys.n, d);
+  const int A = pick3(ty.half, d);
+  const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+  const int lo = ir - A;
+  x.len = 2 * A;
+  x.tb = atom_tables<float>(tables, ty, slot) +
+         (d == 0 ? 0 : (d == 1 ? 2 * ty.half[0] : 2 * ty.half[0] + 2 * ty.half[1]));
+  x.t0 = pick3(tl.t0, d);
+  x.tn = pick3(tl.t1, d) - x.t0;
+  x.kL = x.t0 - x.N - lo;
+  x.kC = x.t0 - lo;
+  x.kH = x.t0 + x.N - lo;
+  x.nb = sys.nb;
+  x.kSL = x.t0 + x.nb - lo;
+  x.kSH = x.t0 - x.nb - lo;
+  return x;
+}
+
+__device__ __forceinline__ float ax_val(const AxisCtx& x, int k, bool ok) {
+  return ok ? x.tb[k] : 0.f;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
+                                                           const int* __restrict__ boff,
+                                                           const int4* __restrict__ s_ir,
+                                                           const double* __restrict__ s_r,
+                                                           const float* __restrict__ tables,
+                                                           float* __restrict__ rho) {
+  using Stage = typename std::conditional<MODE == MODE_DIAG, StageM, StageMXY>::type;
+  constexpr int EXTRA = MODE == MODE_DIAG ? 3 : 2;  // extra virtual slots of a corner atom
+  __shared__ Stage st;
+  __shared__ EnumShared sh;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  Tile tl;
+  tile_coords(sys, tl);
+  const float g00 = (float)sys.g00, g01x2 = (float)(2.0 * sys.g01), g11 = (float)sys.g11;
+  v4f acc[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  auto process = [&](int count) -> int {
+    __syncthreads();
+    // how many real atoms fit into NV virtual slots (same result in every wave)
+    const int e = lane < count ? sh.list[lane] : 0;
+    const bool isc = lane < count && (e < 0);
+    const unsigned long long cm = __ballot(isc);
+    const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+    const int vupto = (lane + 1) + EXTRA * __popcll(cm & le);
+    const bool fits = lane < count && vupto <= NV;
+    const int nreal = __popcll(__ballot(fits));
+    const int vtot = nreal > 0 ? __shfl(vupto, nreal - 1, 64) : 0;
+    const int vbase_l = lane + EXTRA * __popcll(cm & (le >> 1));
+    // stage: thread (a, part) with a = tid >> 2
+    const int a = tid >> 2, part = tid & 3;
+    const int vb = __shfl(vbase_l, a & 63, 64);
+    if (a < nreal) {
+      const int c = sh.list[a] & 0x7fffffff;
+      const int4 q = s_ir[c];
+      const int t = q.w & 0xffff;
+      const bool corner = (q.w >> 16) != 0;
+      const TypeDev& ty = td[t];
+      const int slot = c - ty.type_start;
+      const bool valid = slot >= 0 && slot < ty.count;  // false only after a reported error
+      const int nv = corner ? EXTRA + 1 : 1;
+      if (!valid) {
+        for (int v = 0; v < nv; ++v) {
+          if (part == 0) {
+            st.vxm[vb + v] = 0;
+            if constexpr (MODE == MODE_DIAG) {
+              for (int i = 0; i < T; ++i) st.vx[vb + v][i] = 0.f;
+            } else {
+              for (int i = 0; i < T; ++i) st.xv[vb + v][i] = 0;
+            }
+          } else if (part == 1) {
+            if constexpr (MODE == MODE_DIAG) {
+              for (int i = 0; i < T; ++i) st.vy[vb + v][i] = 0.f;
+            } else {
+              for (int i = 0; i < T; ++i) st.yv[vb + v][i] = 0;
+              st.vreal[vb + v] = vb;
+              st.vcls[vb + v] = 0;
+            }
+          } else if (part == 2) {
+            for (int i = 0; i < T; ++i) st.vz[vb + v][i] = 0.f;
+          }
+        }
+      } else if constexpr (MODE == MODE_DIAG) {
+        if (part < 3) {
+          const AxisCtx x = axis_ctx(part, q, ty, slot, sys, tables, tl);
+          int mAll = 0, mC = 0, mN = 0;
+#pragma unroll 4
+          for (int i = 0; i < T; ++i) {
+            const bool in = i < x.tn;
+            const bool bL = in && x.kL + i >= 0 && x.kL + i < x.len;
+            const bool bC = in && x.kC + i >= 0 && x.kC + i < x.len;
+            const bool bH = in && x.kH + i >= 0 && x.kH + i < x.len;
+            const float vL = ax_val(x, x.kL + i, bL), vC = ax_val(x, x.kC + i, bC);
+            const float vH = ax_val(x, x.kH + i, bH);
+            const float all = vC + vL + vH;
+            if (part == 0) {
+              mAll |= (bL || bC || bH) ? 1 << i : 0;
+              mC |= bC ? 1 << i : 0;
+              mN |= (bL || bH) ? 1 << i : 0;
+              st.vx[vb][i] = corner ? vC : all;
+              if (corner) {
+                const float xn = vL + vH;
+                st.vx[vb + 1][i] = xn;
+                st.vx[vb + 2][i] = xn;
+                st.vx[vb + 3][i] = xn;
+              }
+            } else if (part == 1) {
+              st.vy[vb][i] = all;
+              if (corner) {
+                st.vy[vb + 1][i] = vC;
+                st.vy[vb + 2][i] = vL;
+                st.vy[vb + 3][i] = vH;
+              }
+            } else {
+              st.vz[vb][i] = all;
+              if (corner) {
+                const int u1 = x.t0 + i + x.nb, u2 = x.t0 + i - x.nb;
+                const bool s1 = in && u1 >= x.N && x.kSL + i >= 0 && x.kSL + i < x.len;
+                const bool s2 = in && u2 < 0 && x.kSH + i >= 0 && x.kSH + i < x.len;
+                st.vz[vb + 1][i] = all;
+                st.vz[vb + 2][i] = vC + vL + ax_val(x, x.kSL + i, s1);
+                st.vz[vb + 3][i] = vC + vH + ax_val(x, x.kSH + i, s2);
+              }
+            }
+          }
+          if (part == 0) {
+            st.vxm[vb] = corner ? mC : mAll;
+            if (corner) st.vxm[vb + 1] = st.vxm[vb + 2] = st.vxm[vb + 3] = mN;
+          }
+        }
+      } else {
+        if (part == 2) {
+          const AxisCtx x = axis_ctx(2, q, ty, slot, sys, tables, tl);
+#pragma unroll 4
+          for (int i = 0; i < T; ++i) {
+            const bool in = i < x.tn;
+            const bool bL = in && x.kL + i >= 0 && x.kL + i < x.len;
+            const bool bC = in && x.kC + i >= 0 && x.kC + i < x.len;
+            const bool bH = in && x.kH + i >= 0 && x.kH + i < x.len;
+            const float vL = ax_val(x, x.kL + i, bL), vC = ax_val(x, x.kC + i, bC);
+            const float vH = ax_val(x, x.kH + i, bH);
+            st.vz[vb][i] = vC + vL + vH;
+            if (corner) {
+              const int u1 = x.t0 + i + x.nb, u2 = x.t0 + i - x.nb;
+              const bool s1 = in && u1 >= x.N && x.kSL + i >= 0 && x.kSL + i < x.len;
+              const bool s2 = in && u2 < 0 && x.kSH + i >= 0 && x.kSH + i < x.len;
+              st.vz[vb + 1][i] = vC + vL + ax_val(x, x.kSL + i, s1);
+              st.vz[vb + 2][i] = vC + vH + ax_val(x, x.kSH + i, s2);
+            }
+          }
+        } else if (part == 3) {
+          // the real atom's data live in slot vb; classes 0 (all) or 1/2/3 (std/L/H)
+          st.amp[vb] = ty.amp_f;
+          st.k2[vb] = ty.c2;
+          for (int v = 0; v < nv; ++v) {
+            st.vreal[vb + v] = vb;
+            st.vcls[vb + v] = corner ? v + 1 : 0;
+          }
         } else {
-          reinterpret_cast<double2*>(dst)[2 * g] = make_double2(acc[4 * g], acc[4 * g + 1]);
-          reinterpret_cast<double2*>(dst)[2 * g + 1] = make_double2(acc[4 * g + 2], acc[4 * g + 3]);
+          const int d = part;
+          const int N = d == 0 ? sys.n[0] : sys.n[1];
+          const int A = d == 0 ? ty.half[0] : ty.half[1];
+          const int ir = d == 0 ? q.x : q.y;
+          const int lo = ir - A, len = 2 * A;
+          const double r = s_r[3 * (size_t)c + d];
+          const double dr = d == 0 ? sys.dr[0] : sys.dr[1];
+          const int t0 = d == 0 ? tl.t0[0] : tl.t0[1];
+          const int tn = (d == 0 ? tl.t1[0] : tl.t1[1]) - t0;
+          float(*bb)[NV][T] = d == 0 ? st.bx : st.by;
+          int(*vv)[T] = d == 0 ? st.xv : st.yv;
+          int mask = 0;
+#pragma unroll 2
+          for (int i = 0; i < T; ++i) {
+            int bits = 0;
+#pragma unroll
+            for (int sd = 0; sd < 3; ++sd) {
+              const int u = t0 + i + (sd - 1) * N;
+              const int k = u - lo;
+              const bool ok = i < tn && k >= 0 && k < len;
+              bb[sd][vb][i] = ok ? (float)(r - (double)u * dr) : 0.f;
+              bits |= ok ? (1 << sd) : 0;
+            }
+            vv[vb][i] = bits;
+            if (bits) mask |= 1 << i;
+          }
+          if (d == 0)
+            for (int v = 0; v < nv; ++v) st.vxm[vb + v] = mask;
         }
       }
-    } else {
+    }
+    // zero the padding slots of the last group of 4
+    const int vpad = (vtot + 3) & ~3;
+    for (int i = tid; i < (vpad - vtot) * T; i += kThreads) {
+      const int v = vtot + i / T, cc = i % T;
+      if constexpr (MODE == MODE_DIAG) {
+        st.vx[v][cc] = 0.f;
+        st.vy[v][cc] = 0.f;
+      } else {
+        st.xv[v][cc] = 0;
+        st.yv[v][cc] = 0;
+        if (cc == 0) {
+          st.vreal[v] = v;
+          st.vcls[v] = 0;
+          st.amp[v] = 0.f;
+          st.k2[v] = 0.f;
+        }
+      }
+      st.vz[v][cc] = 0.f;
+      if (cc == 0) st.vxm[v] = 0;
+    }
+    __syncthreads();
+    // MFMA accumulation over groups of 4 virtual atoms
+    const int kk = lane >> 4, jj = lane & 15;
+    for (int k0 = 0; k0 < vpad; k0 += 4) {
+      const int m = ((st.vxm[k0] | st.vxm[k0 + 1] | st.vxm[k0 + 2] | st.vxm[k0 + 3]) >> (4 * wave)) & 0xf;
+      const int mu = __builtin_amdgcn_readfirstlane(m);
+      if (!mu) continue;
+      const int k = k0 + kk;
+      const float zv = st.vz[k][jj];
+      if constexpr (MODE == MODE_DIAG) {
+        const float yv = st.vy[k][jj];
 #pragma unroll
-      for (int k = 0; k < T; ++k)
-        if (k < nz) dst[k] = acc[k];
+        for (int b = 0; b < 4; ++b) {
+          if ((mu >> b) & 1) {
+            const float av = st.vx[k][4 * wave + b] * yv;
+            acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, zv, acc[b], 0, 0, 0);
+          }
+        }
+      } else {
+        const int ra = st.vreal[k], cls = st.vcls[k];
+        const float amp = st.amp[ra], k2 = st.k2[ra];
+        const int yvb = st.yv[ra][jj];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if ((mu >> b) & 1) {
+            const int xc = 4 * wave + b;
+            const int xvb = st.xv[ra][xc];
+            float w = 0.f;
+#pragma unroll 1
+            for (int sx = 0; sx < 3; ++sx) {
+              if (!((xvb >> sx) & 1)) continue;
+              const float bx = st.bx[sx][ra][xc];
+#pragma unroll 1
+              for (int sy = 0; sy < 3; ++sy) {
+                if (!((yvb >> sy) & 1)) continue;
+                const bool take = cls == 0 || (cls == 1 && (sx == SIDE_C || sy == SIDE_C)) ||
+                                  (cls == 2 && sx != SIDE_C && sy == SIDE_L) ||
+                                  (cls == 3 && sx != SIDE_C && sy == SIDE_H);
+                if (!take) continue;
+                const float by = st.by[sy][ra][jj];
+                const float qf = fmaf(g00 * bx, bx, fmaf(g01x2 * bx, by, g11 * by * by));
+                w += amp * exp2f(-qf * k2);
+              }
+            }
+            acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, zv, acc[b], 0, 0, 0);
+          }
+        }
+      }
+    }
+    return nreal;
+  };
+  enumerate_tile(sys, td, boff, s_ir, tl, sh, NV, process);
+
+  // D[y][z] of block b: lane l holds y = 4 (l / 16) + r, z = l % 16
+  const int gz = tl.t0[2] + (lane & 15);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int gx = tl.t0[0] + 4 * wave + b;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gy = tl.t0[1] + 4 * (lane >> 4) + r;
+      if (gx < tl.t1[0] && gy < tl.t1[1] && gz < tl.t1[2])
+        rho[((size_t)gx * sys.n[1] + gy) * sys.n[2] + gz] = acc[b][r];
     }
   }
 }
@@ -604,12 +949,23 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
   const SysDev& sys = p.sys;
   const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
   if (ntiles == 0) return hipSuccess;
-  if (sys.mode == MODE_DIAG)
-    hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), dim3((unsigned)ntiles), dim3(kThreads), 0, s,
-                       sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.tables, out);
-  else
-    hipLaunchKernelGGL((k_deposit<MODE_XY, ACC>), dim3((unsigned)ntiles), dim3(kThreads), 0, s,
-                       sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.tables, out);
+  const dim3 grid((unsigned)ntiles), block(kThreads);
+  if constexpr (std::is_same<ACC, float>::value) {
+    const float* tb = static_cast<const float*>(p.tables);
+    if (sys.mode == MODE_DIAG)
+      hipLaunchKernelGGL(k_deposit_mfma<MODE_DIAG>, grid, block, 0, s, sys, p.types, p.bucket_off,
+                         p.s_ir, p.s_r, tb, out);
+    else
+      hipLaunchKernelGGL(k_deposit_mfma<MODE_XY>, grid, block, 0, s, sys, p.types, p.bucket_off,
+                         p.s_ir, p.s_r, tb, out);
+  } else {
+    if (sys.mode == MODE_DIAG)
+      hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), grid, block, 0, s, sys, p.types,
+                         p.bucket_off, p.s_ir, p.s_r, p.tables, out);
+    else
+      hipLaunchKernelGGL((k_deposit<MODE_XY, ACC>), grid, block, 0, s, sys, p.types,
+                         p.bucket_off, p.s_ir, p.s_r, p.tables, out);
+  }
   return hipGetLastError();
 }
 
