@@ -164,6 +164,38 @@ int mdsf_outputs(mdsf_plan* plan, const double* d_sf, const double box[3],
                  int box_f32, double* d_sfplt, double* d_kgrid,
                  double* d_kgridplt, mdsf_stream stream);
 
+/* ---- adjoint (dL/dS -> dL/dx) ------------------------------------------
+ * The reference has no gradient; these differentiate the function the
+ * reference computes (dens.py:283-318, including the corner fold of
+ * dens.py:107) for a loss L(S).  d_dsf is dL/dS in the reference layout,
+ * double (Nx, Ny, Nz/2+1).  Nothing per frame is stored by the forward pass:
+ * each frame's density and spectrum are recomputed. */
+
+/* dL/d(coords) for `nframes` frames laid out as for mdsf_accumulate_frames.
+ * The coordinates are rescaled and wrapped in place exactly as the forward
+ * pass does (pass a copy of the original coordinates); d_grad (nframes,
+ * natoms, 3) double receives dL/d(original coordinates) = scale * dL/dr.
+ * Atoms that fail the window check get a zero gradient and raise the plan's
+ * error flag (mdsf_check_errors). */
+int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype,
+                        int64_t nframes, const int32_t* d_types,
+                        const double* d_scale, const double box[3], int arith,
+                        int64_t wrap_lo, int64_t wrap_hi, const double* d_dsf,
+                        double* d_grad, mdsf_stream stream);
+
+/* dL/d(density) = 2 Re sum_q dsf(q) F(q) exp(+2 pi i q.r/N) for one periodic
+ * density (plan precision in and out; the two may be the same buffer):
+ * the transpose of density -> |rfftn(density)|^2 (dens.py:313-318). */
+int mdsf_density_adjoint(mdsf_plan* plan, const void* d_density,
+                         const double* d_dsf, void* d_grad_density,
+                         mdsf_stream stream);
+
+/* dL/dr (natoms, 3) double, for the wrapped coordinates of the frame last
+ * given to mdsf_prepare_frame / mdsf_deposit_fold, from dL/d(folded density)
+ * in plan precision: the transpose of deposition + fold (dens.py:283-311). */
+int mdsf_deposit_adjoint(mdsf_plan* plan, const void* d_grad_density,
+                         double* d_grad, mdsf_stream stream);
+
 /* Ewald-sphere sections of a kgridplt volume (plot2d.py:218-263): trilinear
  * interpolation of channel 3 at the points of the xy, xz and yz sections for
  * wavelength `wavelength` (Angstrom); NaN outside the grid

@@ -41,11 +41,12 @@ hipError_t dalloc(T** p, size_t count) {
 
 void free_plan(Plan* p) {
   if (!p) return;
-  void* ptrs[] = {p->types,     p->ir4,       p->key,        p->rpos,       p->bucket_count,
+  void* ptrs[] = {p->bucket_aux, p->types,     p->ir4,       p->key,        p->rpos,       p->bucket_count,
                   p->bucket_off, p->bucket_cur, p->s_idx,      p->s_ir,       p->s_r,
                   p->tables,    p->err,       p->rho,        p->spec_nyq,   p->sf_main,
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
-                  p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64};
+                  p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
+                  p->adj_w};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -58,6 +59,14 @@ int make_axis(FftAxis& ax, int n, hipStream_t s) {
   MDSF_HIP(dalloc(&ax.tw, (size_t)n));
   MDSF_HIP(dalloc(&ax.tw64, (size_t)n));
   MDSF_HIP(fft_make_twiddles(ax.tw, ax.tw64, n, s));
+  return MDSF_OK;
+}
+
+int ensure_adjoint(Plan& p) {
+  const size_t nxy = (size_t)p.sys.n[0] * p.sys.n[1];
+  const size_t m = (size_t)p.sys.n[2] / 2;
+  if (!p.adj_part) MDSF_HIP(dalloc(&p.adj_part, 3 * (size_t)std::max(p.adj_items, 1LL)));
+  if (!p.adj_w) MDSF_HIP(dalloc(&p.adj_w, nxy * (m + 1)));
   return MDSF_OK;
 }
 
@@ -174,6 +183,34 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   }
   s.nbuckets = bucket_base;
   p->table_floats = table_floats;
+  // adjoint work items (atom, x-chunk of <= 64 planes, chunk of (y, z)
+  // pairs sized for ~16k window voxels), heaviest windows numbered first so
+  // that their workgroups start first
+  {
+    int order[kMaxTypes];
+    for (int t = 0; t < desc->ntypes; ++t) order[t] = t;
+    auto vol = [&](int t) {
+      const TypeDev& y = p->types_host[t];
+      return (double)y.half[0] * y.half[1] * y.half[2];
+    };
+    std::stable_sort(order, order + desc->ntypes, [&](int a, int b) { return vol(a) > vol(b); });
+    long long items = 0;
+    for (int k = 0; k < desc->ntypes; ++k) {
+      TypeDev& y = p->types_host[order[k]];
+      const int wx = 2 * y.half[0];
+      const long long npairs = 4LL * y.half[1] * y.half[2];
+      y.adj_item_base = items;
+      y.adj_ppi = 64;
+      y.adj_nxc = y.adj_npc = 0;
+      if (wx == 0 || npairs == 0) continue;
+      const int xw = std::min(64, wx);
+      y.adj_nxc = (wx + 63) / 64;
+      y.adj_ppi = 64 * ((256 + xw - 1) / xw);
+      y.adj_npc = (int)((npairs + y.adj_ppi - 1) / y.adj_ppi);
+      items += (long long)y.count * y.adj_nxc * y.adj_npc;
+    }
+    p->adj_items = items;
+  }
   (void)hipGetDevice(&p->device);
 
   const size_t nat = (size_t)desc->natoms;
@@ -197,6 +234,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   TRY(dalloc(&p->bucket_count, (size_t)s.nbuckets + 1));
   TRY(dalloc(&p->bucket_off, (size_t)s.nbuckets + 1));
   TRY(dalloc(&p->bucket_cur, (size_t)s.nbuckets + 1));
+  TRY(dalloc(&p->bucket_aux, (size_t)s.nbuckets / 1024 + 2));
   TRY(hipMemset(p->bucket_count, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
   TRY(hipMemset(p->bucket_off, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
   TRY(dalloc(&p->s_idx, nat));
@@ -268,9 +306,10 @@ int mdsf_sf_reset(mdsf_plan* plan, mdsf_stream stream) {
   Plan* p = reinterpret_cast<Plan*>(plan);
   if (!p) return fail(MDSF_ERR_ARG, "null plan");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const size_t nxy = (size_t)p->sys.n[0] * p->sys.n[1];
-  MDSF_HIP(hipMemsetAsync(p->sf_main, 0, sizeof(double) * nxy * (p->sys.n[2] / 2), s));
-  MDSF_HIP(hipMemsetAsync(p->sf_nyq, 0, sizeof(double) * nxy, s));
+  const long long nxy = (long long)p->sys.n[0] * p->sys.n[1];
+  // a fill kernel rather than a memset node keeps this capturable in graphs
+  MDSF_HIP(launch_fill_zero(p->sf_main, nxy * (p->sys.n[2] / 2), s));
+  MDSF_HIP(launch_fill_zero(p->sf_nyq, nxy, s));
   return MDSF_OK;
 }
 
@@ -394,6 +433,63 @@ int mdsf_rfft3(mdsf_plan* plan, const void* d_density, double* d_spectrum, mdsf_
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   MDSF_HIP(fft_forward(*p, d_density, false, s));
   MDSF_HIP(fft_spectrum_read(*p, d_spectrum, s));
+  return MDSF_OK;
+}
+
+int mdsf_density_adjoint(mdsf_plan* plan, const void* d_density, const double* d_dsf,
+                         void* d_grad_density, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !d_density || !d_dsf || !d_grad_density) return fail(MDSF_ERR_ARG, "null argument");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  int rc = ensure_adjoint(*p);
+  if (rc != MDSF_OK) return rc;
+  const long long nxy = (long long)p->sys.n[0] * p->sys.n[1];
+  MDSF_HIP(launch_adjoint_weights(*p, d_dsf, s));
+  MDSF_HIP(fft_forward(*p, d_density, false, s));
+  MDSF_HIP(fft_inverse(*p, p->adj_w, p->adj_w + nxy * (p->sys.n[2] / 2), d_grad_density, s));
+  return MDSF_OK;
+}
+
+int mdsf_deposit_adjoint(mdsf_plan* plan, const void* d_grad_density, double* d_grad,
+                         mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !d_grad_density) return fail(MDSF_ERR_ARG, "null plan or density gradient");
+  if (p->sys.natoms > 0 && !d_grad) return fail(MDSF_ERR_ARG, "null coordinate gradient");
+  int rc = ensure_adjoint(*p);
+  if (rc != MDSF_OK) return rc;
+  MDSF_HIP(launch_deposit_adjoint(*p, d_grad_density, nullptr, d_grad,
+                                  reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nframes,
+                        const int32_t* d_types, const double* d_scale, const double box[3],
+                        int arith, int64_t wrap_lo, int64_t wrap_hi, const double* d_dsf,
+                        double* d_grad, mdsf_stream stream) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  int rc = check_frame_args(p, d_coords, dtype, d_types, d_scale, box, arith);
+  if (rc != MDSF_OK) return rc;
+  if (nframes < 0) return fail(MDSF_ERR_ARG, "negative frame count");
+  if (!d_dsf) return fail(MDSF_ERR_ARG, "null dL/dS");
+  if (p->sys.natoms > 0 && nframes > 0 && !d_grad) return fail(MDSF_ERR_ARG, "null gradient");
+  if ((rc = ensure_adjoint(*p)) != MDSF_OK) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const long long nxy = (long long)p->sys.n[0] * p->sys.n[1];
+  const double* w_nyq = p->adj_w + nxy * (p->sys.n[2] / 2);
+  MDSF_HIP(launch_adjoint_weights(*p, d_dsf, s));
+  const size_t esz = dtype == MDSF_F32 ? 4 : 8;
+  const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
+  for (int64_t t = 0; t < nframes; ++t) {
+    char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
+    MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
+                         nullptr, nullptr, s));
+    MDSF_HIP(launch_binning(*p, s));
+    MDSF_HIP(launch_deposit(*p, p->rho, s));
+    MDSF_HIP(fft_forward(*p, p->rho, false, s));
+    MDSF_HIP(fft_inverse(*p, p->adj_w, w_nyq, p->rho, s));
+    MDSF_HIP(launch_deposit_adjoint(*p, p->rho, d_scale + 3 * t,
+                                    d_grad + (size_t)t * p->sys.natoms * 3, s));
+  }
   return MDSF_OK;
 }
 

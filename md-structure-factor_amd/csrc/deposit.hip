@@ -434,6 +434,8 @@ struct EnumShared {
   int wsum[kThreads / 64];
   int col_s[kThreads], col_pre[kThreads + 1], col_t[kThreads];
   int half[kMaxTypes][3];
+  TypeRanges tr[kMaxTypes];
+  int col_base[kMaxTypes + 1];
 };
 
 // block-wide exclusive scan of one int per thread; returns the total
@@ -480,37 +482,39 @@ __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td
                                const Tile& tl, EnumShared& sh, int trigger, Process&& process) {
   const int tid = threadIdx.x;
   for (int i = tid; i < sys.ntypes * 3; i += kThreads) sh.half[i / 3][i % 3] = td[i / 3].half[i % 3];
-  int count = 0;
-  int total_cols = 0;
-  for (int t = 0; t < sys.ntypes; ++t) {
-    TypeRanges r;
-    type_ranges(td[t], sys, tl.t0, tl.t1, r);
-    total_cols += r.ncols;
+  for (int t = tid; t < sys.ntypes; t += kThreads) type_ranges(td[t], sys, tl.t0, tl.t1, sh.tr[t]);
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int t = 0; t < sys.ntypes; ++t) {
+      sh.col_base[t] = acc;
+      acc += sh.tr[t].ncols;
+    }
+    sh.col_base[sys.ntypes] = acc;
   }
+  __syncthreads();
+  int count = 0;
+  const int total_cols = sh.col_base[sys.ntypes];
   for (int cbase = 0; cbase < total_cols; cbase += kThreads) {
     // phase A: one bucket column per thread -> (start, length, type)
     int my_s = 0, my_len = 0, my_t = 0;
-    int cidx = cbase + tid;
+    const int cidx = cbase + tid;
     if (cidx < total_cols) {
-      for (int t = 0; t < sys.ntypes; ++t) {
-        TypeRanges r;
-        type_ranges(td[t], sys, tl.t0, tl.t1, r);
-        if (cidx < r.ncols) {
-          const int zr = cidx % r.rn[2];
-          const int byi = (cidx / r.rn[2]) % r.ny;
-          const int bxi = cidx / (r.rn[2] * r.ny);
-          const TypeDev& ty = td[t];
-          const int bx = range_item(r, 0, bxi), by = range_item(r, 1, byi);
-          const int col = ty.bucket_base + (bx * ty.nbk[1] + by) * ty.nbk[2];
-          const int zlo = zr == 0 ? r.lo[2][0] : r.lo[2][1];
-          const int zhi = zr == 0 ? r.hi[2][0] : r.hi[2][1];
-          my_s = boff[col + zlo];
-          my_len = boff[col + zhi + 1] - my_s;
-          my_t = t;
-          break;
-        }
-        cidx -= r.ncols;
-      }
+      int t = 0;
+      while (sh.col_base[t + 1] <= cidx) ++t;
+      const TypeRanges& r = sh.tr[t];
+      const int ci = cidx - sh.col_base[t];
+      const int zr = r.rn[2] == 2 ? (ci & 1) : 0;
+      const int cxy = r.rn[2] == 2 ? (ci >> 1) : ci;
+      const int bxi = cxy / r.ny, byi = cxy - bxi * r.ny;
+      const TypeDev& ty = td[t];
+      const int bx = range_item(r, 0, bxi), by = range_item(r, 1, byi);
+      const int col = ty.bucket_base + (bx * ty.nbk[1] + by) * ty.nbk[2];
+      const int zlo = zr == 0 ? r.lo[2][0] : r.lo[2][1];
+      const int zhi = zr == 0 ? r.hi[2][0] : r.hi[2][1];
+      my_s = boff[col + zlo];
+      my_len = boff[col + zhi + 1] - my_s;
+      my_t = t;
     }
     int excl;
     const int ncand = block_scan(sh, my_len, &excl);
@@ -695,6 +699,9 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
   constexpr int EXTRA = MODE == MODE_DIAG ? 3 : 2;  // extra virtual slots of a corner atom
   __shared__ Stage st;
   __shared__ EnumShared sh;
+  __shared__ struct {
+    int vbase[NV], entry[NV];
+  } bt;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   Tile tl;
@@ -715,46 +722,68 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
     const bool fits = lane < count && vupto <= NV;
     const int nreal = __popcll(__ballot(fits));
     const int vtot = nreal > 0 ? __shfl(vupto, nreal - 1, 64) : 0;
-    const int vbase_l = lane + EXTRA * __popcll(cm & (le >> 1));
-    // stage: thread (a, part) with a = tid >> 2
-    const int a = tid >> 2, part = tid & 3;
-    const int vb = __shfl(vbase_l, a & 63, 64);
-    if (a < nreal) {
-      const int c = sh.list[a] & 0x7fffffff;
-      const int4 q = s_ir[c];
-      const int t = q.w & 0xffff;
-      const bool corner = (q.w >> 16) != 0;
-      const TypeDev& ty = td[t];
-      const int slot = c - ty.type_start;
-      const bool valid = slot >= 0 && slot < ty.count;  // false only after a reported error
-      const int nv = corner ? EXTRA + 1 : 1;
-      if (!valid) {
-        for (int v = 0; v < nv; ++v) {
-          if (part == 0) {
-            st.vxm[vb + v] = 0;
+    const int vpad = (vtot + 3) & ~3;
+    if (wave == 0 && lane < nreal) {
+      bt.vbase[lane] = lane + EXTRA * __popcll(cm & (le >> 1));
+      bt.entry[lane] = e & 0x7fffffff;
+    }
+    __syncthreads();
+    // staging: wave d < 3 builds axis d for every (atom, cell) of the batch
+    if (wave < 3) {
+      const int d = wave;
+      for (int base = 0; base < nreal * T; base += 64) {
+        const int idx = base + lane;
+        const bool act = idx < nreal * T;
+        const int a = act ? idx >> 4 : 0, i = idx & 15;
+        const int c = bt.entry[a], vb = bt.vbase[a];
+        const int4 q = s_ir[c];
+        const int t = q.w & 0xffff;
+        const bool corner = (q.w >> 16) != 0;
+        const TypeDev& ty = td[t];
+        const int slot = c - ty.type_start;
+        const bool valid = act && slot >= 0 && slot < ty.count;  // false only after a reported error
+        const int nv = corner ? EXTRA + 1 : 1;
+        bool hC = false, hN = false;
+        if (act && !valid) {
+          for (int v = 0; v < nv; ++v) {
             if constexpr (MODE == MODE_DIAG) {
-              for (int i = 0; i < T; ++i) st.vx[vb + v][i] = 0.f;
+              if (d == 0) st.vx[vb + v][i] = 0.f;
+              if (d == 1) st.vy[vb + v][i] = 0.f;
             } else {
-              for (int i = 0; i < T; ++i) st.xv[vb + v][i] = 0;
+              if (d == 0) st.xv[vb + v][i] = 0;
+              if (d == 1) st.yv[vb + v][i] = 0;
             }
-          } else if (part == 1) {
-            if constexpr (MODE == MODE_DIAG) {
-              for (int i = 0; i < T; ++i) st.vy[vb + v][i] = 0.f;
-            } else {
-              for (int i = 0; i < T; ++i) st.yv[vb + v][i] = 0;
-              st.vreal[vb + v] = vb;
-              st.vcls[vb + v] = 0;
-            }
-          } else if (part == 2) {
-            for (int i = 0; i < T; ++i) st.vz[vb + v][i] = 0.f;
+            if (d == 2) st.vz[vb + v][i] = 0.f;
           }
-        }
-      } else if constexpr (MODE == MODE_DIAG) {
-        if (part < 3) {
-          const AxisCtx x = axis_ctx(part, q, ty, slot, sys, tables, tl);
-          int mAll = 0, mC = 0, mN = 0;
-#pragma unroll 4
-          for (int i = 0; i < T; ++i) {
+        } else if (valid) {
+          if (MODE == MODE_XY && d < 2) {
+            const int N = d == 0 ? sys.n[0] : sys.n[1];
+            const int A = d == 0 ? ty.half[0] : ty.half[1];
+            const int lo = (d == 0 ? q.x : q.y) - A, len = 2 * A;
+            const double r = s_r[3 * (size_t)c + d];
+            const double dr = d == 0 ? sys.dr[0] : sys.dr[1];
+            const int t0 = d == 0 ? tl.t0[0] : tl.t0[1];
+            const int tn = (d == 0 ? tl.t1[0] : tl.t1[1]) - t0;
+            int bits = 0;
+#pragma unroll
+            for (int sd = 0; sd < 3; ++sd) {
+              const int u = t0 + i + (sd - 1) * N;
+              const int k = u - lo;
+              const bool ok = i < tn && k >= 0 && k < len;
+              const float bv = ok ? (float)(r - (double)u * dr) : 0.f;
+              if constexpr (MODE == MODE_XY) {
+                if (d == 0) st.bx[sd][vb][i] = bv;
+                else st.by[sd][vb][i] = bv;
+              }
+              bits |= ok ? (1 << sd) : 0;
+            }
+            if constexpr (MODE == MODE_XY) {
+              if (d == 0) st.xv[vb][i] = bits;
+              else st.yv[vb][i] = bits;
+            }
+            hC = bits != 0;
+          } else {
+            const AxisCtx x = axis_ctx(d, q, ty, slot, sys, tables, tl);
             const bool in = i < x.tn;
             const bool bL = in && x.kL + i >= 0 && x.kL + i < x.len;
             const bool bC = in && x.kC + i >= 0 && x.kC + i < x.len;
@@ -762,120 +791,97 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
             const float vL = ax_val(x, x.kL + i, bL), vC = ax_val(x, x.kC + i, bC);
             const float vH = ax_val(x, x.kH + i, bH);
             const float all = vC + vL + vH;
-            if (part == 0) {
-              mAll |= (bL || bC || bH) ? 1 << i : 0;
-              mC |= bC ? 1 << i : 0;
-              mN |= (bL || bH) ? 1 << i : 0;
-              st.vx[vb][i] = corner ? vC : all;
-              if (corner) {
-                const float xn = vL + vH;
-                st.vx[vb + 1][i] = xn;
-                st.vx[vb + 2][i] = xn;
-                st.vx[vb + 3][i] = xn;
-              }
-            } else if (part == 1) {
-              st.vy[vb][i] = all;
-              if (corner) {
-                st.vy[vb + 1][i] = vC;
-                st.vy[vb + 2][i] = vL;
-                st.vy[vb + 3][i] = vH;
-              }
-            } else {
+            if (d == 2) {
               st.vz[vb][i] = all;
               if (corner) {
                 const int u1 = x.t0 + i + x.nb, u2 = x.t0 + i - x.nb;
                 const bool s1 = in && u1 >= x.N && x.kSL + i >= 0 && x.kSL + i < x.len;
                 const bool s2 = in && u2 < 0 && x.kSH + i >= 0 && x.kSH + i < x.len;
-                st.vz[vb + 1][i] = all;
-                st.vz[vb + 2][i] = vC + vL + ax_val(x, x.kSL + i, s1);
-                st.vz[vb + 3][i] = vC + vH + ax_val(x, x.kSH + i, s2);
+                const float zl = vC + vL + ax_val(x, x.kSL + i, s1);
+                const float zh = vC + vH + ax_val(x, x.kSH + i, s2);
+                if constexpr (MODE == MODE_DIAG) {
+                  st.vz[vb + 1][i] = all;
+                  st.vz[vb + 2][i] = zl;
+                  st.vz[vb + 3][i] = zh;
+                } else {
+                  st.vz[vb + 1][i] = zl;
+                  st.vz[vb + 2][i] = zh;
+                }
+              }
+            } else if constexpr (MODE == MODE_DIAG) {
+              if (d == 0) {
+                st.vx[vb][i] = corner ? vC : all;
+                if (corner) {
+                  const float xn = vL + vH;
+                  st.vx[vb + 1][i] = xn;
+                  st.vx[vb + 2][i] = xn;
+                  st.vx[vb + 3][i] = xn;
+                }
+                hC = corner ? bC : (bL || bC || bH);
+                hN = bL || bH;
+              } else {
+                st.vy[vb][i] = all;
+                if (corner) {
+                  st.vy[vb + 1][i] = vC;
+                  st.vy[vb + 2][i] = vL;
+                  st.vy[vb + 3][i] = vH;
+                }
               }
             }
           }
-          if (part == 0) {
-            st.vxm[vb] = corner ? mC : mAll;
-            if (corner) st.vxm[vb + 1] = st.vxm[vb + 2] = st.vxm[vb + 3] = mN;
-          }
         }
-      } else {
-        if (part == 2) {
-          const AxisCtx x = axis_ctx(2, q, ty, slot, sys, tables, tl);
-#pragma unroll 4
-          for (int i = 0; i < T; ++i) {
-            const bool in = i < x.tn;
-            const bool bL = in && x.kL + i >= 0 && x.kL + i < x.len;
-            const bool bC = in && x.kC + i >= 0 && x.kC + i < x.len;
-            const bool bH = in && x.kH + i >= 0 && x.kH + i < x.len;
-            const float vL = ax_val(x, x.kL + i, bL), vC = ax_val(x, x.kC + i, bC);
-            const float vH = ax_val(x, x.kH + i, bH);
-            st.vz[vb][i] = vC + vL + vH;
-            if (corner) {
-              const int u1 = x.t0 + i + x.nb, u2 = x.t0 + i - x.nb;
-              const bool s1 = in && u1 >= x.N && x.kSL + i >= 0 && x.kSL + i < x.len;
-              const bool s2 = in && u2 < 0 && x.kSH + i >= 0 && x.kSH + i < x.len;
-              st.vz[vb + 1][i] = vC + vL + ax_val(x, x.kSL + i, s1);
-              st.vz[vb + 2][i] = vC + vH + ax_val(x, x.kSH + i, s2);
+        // x-cell masks of the virtual atoms: 16 lanes per atom in the ballot
+        if (d == 0) {
+          const unsigned long long bC = __ballot(hC), bN = __ballot(hN);
+          if (act && i == 0) {
+            const int sh16 = lane & ~15;
+            const int mC = (int)((bC >> sh16) & 0xffff), mN = (int)((bN >> sh16) & 0xffff);
+            if (!valid) {
+              for (int v = 0; v < nv; ++v) st.vxm[vb + v] = 0;
+            } else if (MODE == MODE_DIAG && corner) {
+              st.vxm[vb] = mC;
+              st.vxm[vb + 1] = st.vxm[vb + 2] = st.vxm[vb + 3] = mN;
+            } else {
+              for (int v = 0; v < nv; ++v) st.vxm[vb + v] = mC;
             }
           }
-        } else if (part == 3) {
-          // the real atom's data live in slot vb; classes 0 (all) or 1/2/3 (std/L/H)
+        }
+      }
+    } else {
+      // wave 3: per-atom scalars (XY) and the zero padding of the last group
+      if constexpr (MODE == MODE_XY) {
+        for (int a = lane; a < nreal; a += 64) {
+          const int c = bt.entry[a], vb = bt.vbase[a];
+          const int4 q = s_ir[c];
+          const TypeDev& ty = td[q.w & 0xffff];
+          const bool corner = (q.w >> 16) != 0;
           st.amp[vb] = ty.amp_f;
           st.k2[vb] = ty.c2;
+          const int nv = corner ? EXTRA + 1 : 1;
           for (int v = 0; v < nv; ++v) {
             st.vreal[vb + v] = vb;
             st.vcls[vb + v] = corner ? v + 1 : 0;
           }
+        }
+      }
+      for (int j = lane; j < (vpad - vtot) * T; j += 64) {
+        const int v = vtot + j / T, cc = j % T;
+        if constexpr (MODE == MODE_DIAG) {
+          st.vx[v][cc] = 0.f;
+          st.vy[v][cc] = 0.f;
         } else {
-          const int d = part;
-          const int N = d == 0 ? sys.n[0] : sys.n[1];
-          const int A = d == 0 ? ty.half[0] : ty.half[1];
-          const int ir = d == 0 ? q.x : q.y;
-          const int lo = ir - A, len = 2 * A;
-          const double r = s_r[3 * (size_t)c + d];
-          const double dr = d == 0 ? sys.dr[0] : sys.dr[1];
-          const int t0 = d == 0 ? tl.t0[0] : tl.t0[1];
-          const int tn = (d == 0 ? tl.t1[0] : tl.t1[1]) - t0;
-          float(*bb)[NV][T] = d == 0 ? st.bx : st.by;
-          int(*vv)[T] = d == 0 ? st.xv : st.yv;
-          int mask = 0;
-#pragma unroll 2
-          for (int i = 0; i < T; ++i) {
-            int bits = 0;
-#pragma unroll
-            for (int sd = 0; sd < 3; ++sd) {
-              const int u = t0 + i + (sd - 1) * N;
-              const int k = u - lo;
-              const bool ok = i < tn && k >= 0 && k < len;
-              bb[sd][vb][i] = ok ? (float)(r - (double)u * dr) : 0.f;
-              bits |= ok ? (1 << sd) : 0;
-            }
-            vv[vb][i] = bits;
-            if (bits) mask |= 1 << i;
+          st.xv[v][cc] = 0;
+          st.yv[v][cc] = 0;
+          if (cc == 0) {
+            st.vreal[v] = v;
+            st.vcls[v] = 0;
+            st.amp[v] = 0.f;
+            st.k2[v] = 0.f;
           }
-          if (d == 0)
-            for (int v = 0; v < nv; ++v) st.vxm[vb + v] = mask;
         }
+        st.vz[v][cc] = 0.f;
+        if (cc == 0) st.vxm[v] = 0;
       }
-    }
-    // zero the padding slots of the last group of 4
-    const int vpad = (vtot + 3) & ~3;
-    for (int i = tid; i < (vpad - vtot) * T; i += kThreads) {
-      const int v = vtot + i / T, cc = i % T;
-      if constexpr (MODE == MODE_DIAG) {
-        st.vx[v][cc] = 0.f;
-        st.vy[v][cc] = 0.f;
-      } else {
-        st.xv[v][cc] = 0;
-        st.yv[v][cc] = 0;
-        if (cc == 0) {
-          st.vreal[v] = v;
-          st.vcls[v] = 0;
-          st.amp[v] = 0.f;
-          st.k2[v] = 0.f;
-        }
-      }
-      st.vz[v][cc] = 0.f;
-      if (cc == 0) st.vxm[v] = 0;
     }
     __syncthreads();
     // MFMA accumulation over groups of 4 virtual atoms

@@ -418,14 +418,14 @@ struct PencilArgs {
 
 enum Epilogue { EPI_STORE = 0, EPI_ACC = 1 };
 
+template <typename R>
+int tile_pencils(int n);
+
+// Generic tile: load P pencils into LDS, all stages in LDS, store / accumulate.
 template <bool INV, typename R, int EPI>
-__global__ __launch_bounds__(kFftThreads) void k_pencils(PencilArgs<R> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  R* lds = reinterpret_cast<R*>(lds_raw);
-  const bool second = (long long)blockIdx.x >= a.blocks0;
-  const PencilJob<R>& J = second ? a.job[1] : a.job[0];
-  const long long tile = second ? (long long)blockIdx.x - a.blocks0 : (long long)blockIdx.x;
-  const int n = a.n, P = J.P, pitch = lds_pitch(n);
+__device__ void generic_tile(const PencilJob<R>& J, long long tile, int n, const FftAxis& ax,
+                             R* lds) {
+  const int P = J.P, pitch = lds_pitch(n);
   R* re = lds;
   R* im = lds + P * pitch;
   const long long outer = tile / J.tiles_inner;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(kFftThreads) void k_pencils(PencilArgs<R> a) {
   }
   __syncthreads();
   {
-    const int cur = run_stages<INV, R>(lds, n, P, pitch, a.ax);
+    const int cur = run_stages<INV, R>(lds, n, P, pitch, ax);
     re = lds + 2 * cur * P * pitch;
     im = re + P * pitch;
   }
@@ -476,6 +476,139 @@ __global__ __launch_bounds__(kFftThreads) void k_pencils(PencilArgs<R> a) {
   }
 }
 
+template <bool INV, typename R, int EPI>
+__global__ __launch_bounds__(kFftThreads) void k_pencils(PencilArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  R* lds = reinterpret_cast<R*>(lds_raw);
+  const bool second = (long long)blockIdx.x >= a.blocks0;
+  const long long tile = second ? (long long)blockIdx.x - a.blocks0 : (long long)blockIdx.x;
+  generic_tile<INV, R, EPI>(second ? a.job[1] : a.job[0], tile, a.n, a.ax, lds);
+}
+
+// Register-staged tile for n = R1*R2*R3 (R3 = 1: two stages) with pencils
+// adjacent in memory: the first stage reads its inputs straight from global
+// memory (16 pencils = 128-byte rows), the last stage writes its outputs
+// straight to global memory (or folds |X|^2 into the FP64 accumulator), so
+// a two-stage transform makes a single round trip through LDS.
+template <typename R>
+constexpr int reg_tile_pencils(int stages) {
+  return (sizeof(R) == 4 ? 16 : 8) / (stages == 3 ? 2 : 1);
+}
+
+template <int RAD, bool INV, typename R>
+__device__ __forceinline__ void twiddle_in(Cx<R>* v, int k, int step, const void* tw) {
+#pragma unroll
+  for (int r = 1; r < RAD; ++r) {
+    if (k > 0) {
+      const Cx<R> w = ld_tw<R>(tw, k * r * step);
+      v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+    }
+  }
+}
+
+template <typename R, int R1, int R2, int R3, bool INV, int EPI>
+__device__ void reg_tile(const PencilJob<R>& J, long long tile, const FftAxis& ax, R* lds) {
+  constexpr int n = R1 * R2 * R3;
+  constexpr int P = reg_tile_pencils<R>(R3 == 1 ? 2 : 3);
+  constexpr int pitch = n + 1;
+  constexpr int plane = P * pitch;
+  const void* tw = sizeof(R) == 8 ? (const void*)ax.tw64 : (const void*)ax.tw;
+  const long long outer = tile / J.tiles_inner;
+  const int inner = (int)(tile - outer * J.tiles_inner);
+  const int p0 = inner * P;
+  const int pc = min(P, J.pencils - p0);
+  const long long base = outer * J.outer_stride + (long long)p0 * J.pstride;
+  R* re0 = lds;
+  R* im0 = lds + plane;
+  // stage 1 (span 1): global -> DFT_R1 -> LDS plane 0 at j*R1 + r
+  {
+    constexpr int nq = n / R1;
+    for (int b = threadIdx.x; b < P * nq; b += kFftThreads) {
+      const int p = b % P, j = b / P;
+      Cx<R> v[R1];
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        const long long g = base + p * J.pstride + (long long)(j + r * nq) * J.estride;
+        if (p < pc) {
+          const auto x = J.data[g];
+          const R sc = J.scale ? (R)J.scale[g] : (R)1;
+          v[r] = mk<R>(x.x * sc, x.y * sc);
+        } else {
+          v[r] = mk<R>(0, 0);
+        }
+      }
+      dft<R1, INV>(v);
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        re0[p * pitch + j * R1 + r] = v[r].x;
+        im0[p * pitch + j * R1 + r] = v[r].y;
+      }
+    }
+    __syncthreads();
+  }
+  const R* sre = re0;
+  const R* sim = im0;
+  if constexpr (R3 > 1) {
+    // middle stage (span R1): LDS plane 0 -> DFT_R2 -> LDS plane 1
+    R* re1 = lds + 2 * plane;
+    R* im1 = lds + 3 * plane;
+    constexpr int ns = R1, nq = n / R2;
+    for (int b = threadIdx.x; b < P * nq; b += kFftThreads) {
+      const int p = b % P, j = b / P, k = j % ns;
+      Cx<R> v[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) v[r] = mk<R>(re0[p * pitch + j + r * nq], im0[p * pitch + j + r * nq]);
+      twiddle_in<R2, INV>(v, k, n / (ns * R2), tw);
+      dft<R2, INV>(v);
+      const int d0 = (j / ns) * ns * R2 + k;
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        re1[p * pitch + d0 + r * ns] = v[r].x;
+        im1[p * pitch + d0 + r * ns] = v[r].y;
+      }
+    }
+    __syncthreads();
+    sre = re1;
+    sim = im1;
+  }
+  // last stage (span n / RL): LDS -> DFT_RL -> global, element k + r * ns
+  {
+    constexpr int RL = R3 > 1 ? R3 : R2;
+    constexpr int ns = n / RL, nq = n / RL;
+    for (int b = threadIdx.x; b < P * nq; b += kFftThreads) {
+      const int p = b % P, k = b / P;  // k < ns
+      Cx<R> v[RL];
+#pragma unroll
+      for (int r = 0; r < RL; ++r) v[r] = mk<R>(sre[p * pitch + k + r * nq], sim[p * pitch + k + r * nq]);
+      twiddle_in<RL, INV>(v, k, 1, tw);
+      dft<RL, INV>(v);
+      if (p < pc) {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const long long g = base + p * J.pstride + (long long)(k + r * ns) * J.estride;
+          if (EPI == EPI_STORE) {
+            J.data[g].x = v[r].x;
+            J.data[g].y = v[r].y;
+          } else {
+            J.acc[g] += (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename R, int R1, int R2, int R3, bool INV, int EPI>
+__global__ __launch_bounds__(kFftThreads) void k_pencils_reg(PencilArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  R* lds = reinterpret_cast<R*>(lds_raw);
+  if ((long long)blockIdx.x < a.blocks0) {
+    reg_tile<R, R1, R2, R3, INV, EPI>(a.job[0], (long long)blockIdx.x, a.ax, lds);
+  } else {
+    generic_tile<INV, R, EPI>(a.job[1], (long long)blockIdx.x - a.blocks0, a.n, a.ax, lds);
+  }
+}
+
 template <typename R>
 int tile_pencils(int n) {
   int P = (int)(8192 / sizeof(R)) / (n > 0 ? n : 1);
@@ -489,27 +622,89 @@ size_t tile_lds(int n, int P) {
   return (size_t)P * lds_pitch(n) * 4 * sizeof(R);
 }
 
-// job[0] runs over outer0 outer groups, job[1] over outer1.
+// LDS bytes one workgroup may allocate on the current device (queried once per device).
+inline size_t max_lds_per_block() {
+  static int cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (cached[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+      v = 65536;
+    cached[dev] = v;
+  }
+  return (size_t)cached[dev];
+}
+
+// Radix split of the register-staged kernel for n (R3 = 1: two stages), or 0.
+inline int reg_split(int n, int* r1, int* r2, int* r3) {
+  switch (n) {
+    case 16: *r1 = 4; *r2 = 4; *r3 = 1; return 1;
+    case 32: *r1 = 8; *r2 = 4; *r3 = 1; return 1;
+    case 64: *r1 = 8; *r2 = 8; *r3 = 1; return 1;
+    case 128: *r1 = 16; *r2 = 8; *r3 = 1; return 1;
+    case 256: *r1 = 16; *r2 = 16; *r3 = 1; return 1;
+    case 512: *r1 = 8; *r2 = 8; *r3 = 8; return 1;
+    case 1024: *r1 = 16; *r2 = 8; *r3 = 8; return 1;
+    default: return 0;
+  }
+}
+
+template <typename R, bool INV, int EPI>
+void launch_reg(int n, dim3 g, dim3 b, size_t lds, hipStream_t s, const PencilArgs<R>& a) {
+  switch (n) {
+    case 16: hipLaunchKernelGGL((k_pencils_reg<R, 4, 4, 1, INV, EPI>), g, b, lds, s, a); break;
+    case 32: hipLaunchKernelGGL((k_pencils_reg<R, 8, 4, 1, INV, EPI>), g, b, lds, s, a); break;
+    case 64: hipLaunchKernelGGL((k_pencils_reg<R, 8, 8, 1, INV, EPI>), g, b, lds, s, a); break;
+    case 128: hipLaunchKernelGGL((k_pencils_reg<R, 16, 8, 1, INV, EPI>), g, b, lds, s, a); break;
+    case 256: hipLaunchKernelGGL((k_pencils_reg<R, 16, 16, 1, INV, EPI>), g, b, lds, s, a); break;
+    case 512: hipLaunchKernelGGL((k_pencils_reg<R, 8, 8, 8, INV, EPI>), g, b, lds, s, a); break;
+    default: hipLaunchKernelGGL((k_pencils_reg<R, 16, 8, 8, INV, EPI>), g, b, lds, s, a); break;
+  }
+}
+
+// job[0] runs over outer0 outer groups, job[1] over outer1.  job[0] uses the
+// register-staged kernel when n has a supported split and its pencils are
+// adjacent in memory; job[1] (the Nyquist plane) always the generic tile.
 template <typename R>
 hipError_t run_pencils(PencilArgs<R> a, long long outer0, long long outer1, bool inv, int epi,
                        hipStream_t s) {
+  int r1, r2, r3;
+  bool reg = reg_split(a.n, &r1, &r2, &r3) && !a.job[0].pencil_major;
+  if (reg) {
+    const size_t need =
+        (size_t)reg_tile_pencils<R>(r3 == 1 ? 2 : 3) * (a.n + 1) * (r3 == 1 ? 2 : 4) * sizeof(R);
+    reg = need <= max_lds_per_block();
+  }
   long long blocks[2];
   const long long outer[2] = {outer0, outer1};
   size_t lds = 0;
   for (int j = 0; j < 2; ++j) {
     PencilJob<R>& J = a.job[j];
-    J.P = tile_pencils<R>(a.n);
+    const bool use_reg = reg && j == 0;
+    J.P = use_reg ? reg_tile_pencils<R>(r3 == 1 ? 2 : 3) : tile_pencils<R>(a.n);
     J.tiles_inner = (J.pencils + J.P - 1) / J.P;
     blocks[j] = J.pencils > 0 ? outer[j] * J.tiles_inner : 0;
-    if (blocks[j] > 0) lds = tile_lds<R>(a.n, J.P) > lds ? tile_lds<R>(a.n, J.P) : lds;
+    if (blocks[j] > 0) {
+      const size_t need = use_reg ? (size_t)J.P * (a.n + 1) * (r3 == 1 ? 2 : 4) * sizeof(R)
+                                  : tile_lds<R>(a.n, J.P);
+      lds = need > lds ? need : lds;
+    }
   }
   a.blocks0 = blocks[0];
   const long long total = blocks[0] + blocks[1];
   if (total == 0) return hipSuccess;
   const dim3 g((unsigned)total), b(kFftThreads);
-  if (inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<true, R, EPI_STORE>), g, b, lds, s, a);
-  else if (!inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<false, R, EPI_STORE>), g, b, lds, s, a);
-  else hipLaunchKernelGGL((k_pencils<false, R, EPI_ACC>), g, b, lds, s, a);
+  if (reg) {
+    if (inv && epi == EPI_STORE) launch_reg<R, true, EPI_STORE>(a.n, g, b, lds, s, a);
+    else if (!inv && epi == EPI_STORE) launch_reg<R, false, EPI_STORE>(a.n, g, b, lds, s, a);
+    else launch_reg<R, false, EPI_ACC>(a.n, g, b, lds, s, a);
+  } else {
+    if (inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<true, R, EPI_STORE>), g, b, lds, s, a);
+    else if (!inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<false, R, EPI_STORE>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_pencils<false, R, EPI_ACC>), g, b, lds, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -602,24 +797,24 @@ hipError_t forward_impl(Plan& p, const void* density, bool accumulate, hipStream
 // bin first multiplied by the FP64 weight w (internal layout):
 //   out(r) = Re sum_{kx, ky, kz in [0, Nz/2]} w(q) X(q) e^{+i q.r}.
 template <typename R>
-hipError_t inverse_impl(Plan& p, const double* w_main, const double* w_nyq, float* out,
+hipError_t inverse_impl(Plan& p, const double* w_main, const double* w_nyq, void* out,
                         hipStream_t s) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
   hipError_t e = pass_x<R>(p, true, false, w_main, w_nyq, s);
   if (e != hipSuccess) return e;
   e = pass_y<R>(p, true, s);
   if (e != hipSuccess) return e;
-  C2RArgs<R, float> a;
+  C2RArgs<R, R> a;
   a.main = spec_main<R>(p);
   a.nyq = spec_nyq<R>(p);
-  a.out = out;
+  a.out = static_cast<R*>(out);
   a.rows = (long long)nx * ny;
   a.m = m;
   a.P = tile_pencils<R>(m);
   a.ax = p.ax_zc;
   a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
   const long long blocks = (a.rows + a.P - 1) / a.P;
-  hipLaunchKernelGGL((k_c2r_z<R, float>), dim3((unsigned)blocks), dim3(kFftThreads),
+  hipLaunchKernelGGL((k_c2r_z<R, R>), dim3((unsigned)blocks), dim3(kFftThreads),
                      tile_lds<R>(m, a.P), s, a);
   return hipGetLastError();
 }
@@ -674,7 +869,7 @@ hipError_t fft_forward(Plan& p, const void* density, bool accumulate, hipStream_
                  : forward_impl<float>(p, density, accumulate, s);
 }
 
-hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, float* out,
+hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, void* out,
                        hipStream_t s) {
   return p.fft64 ? inverse_impl<double>(p, w_main, w_nyq, out, s)
                  : inverse_impl<float>(p, w_main, w_nyq, out, s);

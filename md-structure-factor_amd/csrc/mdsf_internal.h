@@ -35,6 +35,12 @@ struct TypeDev {
   int bsz[3];        // bucket edge per axis
   int nbk[3];        // buckets per axis
   int bucket_base;   // first bucket id of this type
+  // adjoint work items: per atom adj_nxc x-chunks times adj_npc chunks of
+  // adj_ppi (y, z) pairs, numbered from adj_item_base
+  int adj_nxc;
+  int adj_ppi;
+  int adj_npc;
+  long long adj_item_base;
 };
 
 // Per-system constants passed by value to kernels.
@@ -74,6 +80,7 @@ struct Plan {
   int* bucket_count = nullptr;
   int* bucket_off = nullptr;
   int* bucket_cur = nullptr;
+  int* bucket_aux = nullptr;     // per-1024-block totals of the bucket scan
   int* s_idx = nullptr;
   int4* s_ir = nullptr;
   double* s_r = nullptr;
@@ -90,6 +97,10 @@ struct Plan {
   FftAxis ax_x, ax_y, ax_zc;     // zc: complex length Nz/2
   float2* tw_r2c = nullptr;      // W_Nz^k, k < Nz
   double2* tw_r2c64 = nullptr;
+  // adjoint (allocated on first use)
+  long long adj_items = 0;
+  double* adj_part = nullptr;    // 3 FP64 partial sums per work item
+  double* adj_w = nullptr;       // 2 dL/dS split into main (Nx*Ny*Nz/2) + Nyquist (Nx*Ny)
 };
 
 // error handling -------------------------------------------------------------
@@ -115,14 +126,23 @@ hipError_t fft_make_twiddles(float2* tw, double2* tw64, int n, hipStream_t s);
 // density (FP32, Nx*Ny*Nz) -> spectrum; with `accumulate` the x pass adds
 // |F|^2 into sf_main / sf_nyq instead of storing F.
 hipError_t fft_forward(Plan& p, const void* density, bool accumulate, hipStream_t s);
-// out = Re sum_q w(q) F(q) e^{+i q.r} over the stored spectrum (consumes it).
-hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, float* out,
+// out = Re sum_q w(q) F(q) e^{+i q.r} over the stored spectrum (consumes it);
+// out has the plan precision and may be the plan's density buffer.
+hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, void* out,
                        hipStream_t s);
 // one forward pass: 0 = z (R2C from `density`), 1 = y, 2 = x with |F|^2 accumulation
 hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s);
 // stored spectrum -> complex128 (Nx, Ny, Nz/2+1)
 hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s);
 bool fft_factor(int n, int* radix, int* nstages);
+
+hipError_t launch_fill_zero(double* p, long long n, hipStream_t s);
+// adj_w <- 2 dsf split into main / Nyquist parts
+hipError_t launch_adjoint_weights(Plan& p, const double* dsf, hipStream_t s);
+// grad[atom] = scale3 * dL/dr for the last prepared frame from dL/drho (grho,
+// plan precision); scale3 may be null (unit scale)
+hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale3, double* grad,
+                                  hipStream_t s);
 
 hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s);
 hipError_t launch_outputs(const int n[3], const double* sf, const double box[3], int box_f32,

@@ -96,35 +96,67 @@ __global__ void k_prep(T* __restrict__ coords, SysDev sys, const TypeDev* __rest
   if (occ) atomicAdd(&occ[((size_t)h[0] * sys.n[1] + h[1]) * sys.n[2] + h[2]], 1u);
 }
 
-// Exclusive scan of the bucket counts (one workgroup), also seeds the fill
-// cursors and clears the counts for the next frame.
+// Exclusive scan of the bucket counts in two coalesced passes: per-block
+// scans (k_scan_blocks), then block offsets (k_scan_add).  The counts are
+// cleared for the next frame and the fill cursors seeded.
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_scan(int* __restrict__ bcount,
-                                                       int* __restrict__ boff,
-                                                       int* __restrict__ bcur, int nbk) {
-  __shared__ int part[kScanThreads];
-  const int tid = threadIdx.x;
-  const int per = (nbk + kScanThreads - 1) / kScanThreads;
-  const int lo = min(nbk, tid * per), hi = min(nbk, lo + per);
-  int s = 0;
-  for (int i = lo; i < hi; ++i) s += bcount[i];
-  part[tid] = s;
+
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int* total) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
   __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    const int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    const int s = wsum[w];
+    off += w < wave ? s : 0;
+    tot += s;
   }
-  int run = part[tid] - s;
-  for (int i = lo; i < hi; ++i) {
-    const int c = bcount[i];
-    boff[i] = run;
-    bcur[i] = run;
+  *total = tot;
+  return off + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int* __restrict__ bcount,
+                                                              int* __restrict__ boff,
+                                                              int* __restrict__ aux, int nbk) {
+  __shared__ int wsum[kScanThreads / 64];
+  const int i = blockIdx.x * kScanThreads + threadIdx.x;
+  const int v = i < nbk ? bcount[i] : 0;
+  int total;
+  const int ex = block_excl_scan(v, wsum, &total);
+  if (i < nbk) {
+    boff[i] = ex;
     bcount[i] = 0;
-    run += c;
   }
-  if (tid == kScanThreads - 1) boff[nbk] = part[tid];
+  if (threadIdx.x == 0) aux[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_add(int* __restrict__ boff,
+                                                           int* __restrict__ bcur,
+                                                           const int* __restrict__ aux, int nbk,
+                                                           int nblocks) {
+  __shared__ int base_sh;
+  if (threadIdx.x < 64) {
+    int s = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += 64) s += aux[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (threadIdx.x == 0) base_sh = s;
+  }
+  __syncthreads();
+  const int base = base_sh;
+  const int i = blockIdx.x * kScanThreads + threadIdx.x;
+  if (i < nbk) {
+    const int v = boff[i] + base;
+    boff[i] = v;
+    bcur[i] = v;
+  }
+  if (blockIdx.x == nblocks - 1 && threadIdx.x == 0) boff[nbk] = base + aux[blockIdx.x];
 }
 
 __global__ void k_fill(const int* __restrict__ key, int* __restrict__ bcur,
@@ -138,80 +170,103 @@ __global__ void k_fill(const int* __restrict__ key, int* __restrict__ bcur,
 }
 
 // Restores atom-index order inside each bucket so that the gather visits
-// atoms in a canonical order (bitwise-reproducible sums).
-__global__ void k_bucket_sort(const int* __restrict__ boff, int* __restrict__ s_idx, int nbk) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbk) return;
-  const int lo = boff[b], hi = boff[b + 1];
-  for (int i = lo + 1; i < hi; ++i) {
-    const int v = s_idx[i];
-    int j = i - 1;
-    while (j >= lo && s_idx[j] > v) {
-      s_idx[j + 1] = s_idx[j];
-      --j;
+// atoms in a canonical order (bitwise-reproducible sums).  A workgroup sorts
+// 256 consecutive buckets; their entries are contiguous and are sorted in LDS
+// when they fit, in place in global memory otherwise.
+constexpr int kSortThreads = 256;
+constexpr int kSortCap = 8192;
+__global__ __launch_bounds__(kSortThreads) void k_bucket_sort(const int* __restrict__ boff,
+                                                              int* __restrict__ s_idx, int nbk) {
+  __shared__ int buf[kSortCap];
+  const int b0 = blockIdx.x * kSortThreads;
+  const int b = b0 + threadIdx.x;
+  const int bend = min(b0 + kSortThreads, nbk);
+  const int base = boff[b0], top = boff[bend];
+  const int n = top - base;
+  const int lo = b < nbk ? boff[b] : top, hi = b < nbk ? boff[b + 1] : top;
+  if (n <= kSortCap) {
+    for (int i = threadIdx.x; i < n; i += kSortThreads) buf[i] = s_idx[base + i];
+    __syncthreads();
+    for (int i = lo + 1; i < hi; ++i) {
+      const int v = buf[i - base];
+      int j = i - 1;
+      while (j >= lo && buf[j - base] > v) {
+        buf[j + 1 - base] = buf[j - base];
+        --j;
+      }
+      buf[j + 1 - base] = v;
     }
-    s_idx[j + 1] = v;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kSortThreads) s_idx[base + i] = buf[i];
+  } else {
+    for (int i = lo + 1; i < hi; ++i) {
+      const int v = s_idx[i];
+      int j = i - 1;
+      while (j >= lo && s_idx[j] > v) {
+        s_idx[j + 1] = s_idx[j];
+        --j;
+      }
+      s_idx[j + 1] = v;
+    }
   }
 }
 
-// Gathers atoms into sorted order and writes their 1-D factor tables:
+// Gathers atom data into sorted order (valid sorted positions only).
+__global__ void k_gather(const int* __restrict__ s_idx, const int4* __restrict__ ir4,
+                         const double* __restrict__ rpos, int4* __restrict__ s_ir,
+                         double* __restrict__ s_r, const int* __restrict__ nvalid) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= *nvalid) return;
+  const int i = s_idx[s];
+  s_ir[s] = ir4[i];
+  s_r[3 * (size_t)s + 0] = rpos[3 * (size_t)i + 0];
+  s_r[3 * (size_t)s + 1] = rpos[3 * (size_t)i + 1];
+  s_r[3 * (size_t)s + 2] = rpos[3 * (size_t)i + 2];
+}
+
+// Per-atom 1-D factor tables, one thread per entry:
 //   x: amp * exp(-g00 bx^2 / 2s^2), y: exp(-g11 by^2 / 2s^2), z: exp(-g22 bz^2 / 2s^2)
 // with b = r - (ir - A + k) dr in FP64 exactly as buf = r - grid (dens.py:252-256,299),
 // k in [0, 2A).  Values are rounded once to the plan's precision (FP32 or FP64).
-constexpr int kTabThreads = 64;
 template <typename TT>
-__global__ __launch_bounds__(256) void k_gather_tables(SysDev sys, const TypeDev* __restrict__ td,
-                                                       const int* __restrict__ s_idx,
-                                                       const int4* __restrict__ ir4,
-                                                       const double* __restrict__ rpos,
-                                                       int4* __restrict__ s_ir,
-                                                       double* __restrict__ s_r,
-                                                       TT* __restrict__ tables,
-                                                       const int* __restrict__ nvalid) {
+__global__ void k_tables(SysDev sys, const TypeDev* __restrict__ td, const int4* __restrict__ s_ir,
+                         const double* __restrict__ s_r, TT* __restrict__ tables,
+                         long long total) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * (blockDim.x / kTabThreads) + threadIdx.x / kTabThreads;
-  const int lane = threadIdx.x % kTabThreads;
-  if (s >= *nvalid) return;
-  const int i = s_idx[s];
-  const int4 q = ir4[i];
-  const double r[3] = {rpos[3 * (size_t)i], rpos[3 * (size_t)i + 1], rpos[3 * (size_t)i + 2]};
-  if (lane == 0) {
-    s_ir[s] = q;
-    s_r[3 * (size_t)s + 0] = r[0];
-    s_r[3 * (size_t)s + 1] = r[1];
-    s_r[3 * (size_t)s + 2] = r[2];
-  }
-  const int t = q.w & 0xffff;
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int t = 0;
+  while (t + 1 < sys.ntypes && td[t + 1].table_base <= g) ++t;
   const TypeDev& ty = td[t];
-  // sorted positions match the per-type table slots only when every atom was
-  // valid; after a range / type error (reported to the host) write nothing
-  const int slot = s - ty.type_start;
-  if (slot < 0 || slot >= ty.count) return;
-  TT* tab = tables + ty.table_base + (long long)slot * ty.stride;
-  const int ir[3] = {q.x, q.y, q.z};
-  const double g[3] = {sys.g00, sys.g11, sys.g22};
-  const int len0 = 2 * ty.half[0], len1 = 2 * ty.half[1], len2 = 2 * ty.half[2];
-  const int total = len0 + len1 + len2;
-  for (int e = lane; e < total; e += kTabThreads) {
-    int d, k;
-    if (e < len0) {
-      d = 0;
-      k = e;
-    } else if (e < len0 + len1) {
-      d = 1;
-      k = e - len0;
-    } else {
-      d = 2;
-      k = e - len0 - len1;
-    }
-    const int u = ir[d] - ty.half[d] + k;
-    const double gridv = (double)u * sys.dr[d];
-    const double b = r[d] - gridv;
-    const double qv = (g[d] * (b * b)) / ty.two_sig2;
-    double v = exp(-qv);
-    if (d == 0) v = ty.amp * v;
-    tab[e] = (TT)v;
+  const long long local = g - ty.table_base;
+  const int slot = (int)(local / ty.stride);
+  const int e = (int)(local - (long long)slot * ty.stride);
+  const int s = ty.type_start + slot;
+  const int4 q = s_ir[s];
+  const int len0 = 2 * ty.half[0], len1 = 2 * ty.half[1];
+  int d, k;
+  if (e < len0) {
+    d = 0;
+    k = e;
+  } else if (e < len0 + len1) {
+    d = 1;
+    k = e - len0;
+  } else {
+    d = 2;
+    k = e - len0 - len1;
   }
+  const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+  const int A = d == 0 ? ty.half[0] : (d == 1 ? ty.half[1] : ty.half[2]);
+  const double dr = d == 0 ? sys.dr[0] : (d == 1 ? sys.dr[1] : sys.dr[2]);
+  const double gm = d == 0 ? sys.g00 : (d == 1 ? sys.g11 : sys.g22);
+  const double r = s_r[3 * (size_t)s + d];
+  const int u = ir - A + k;
+  const double gridv = (double)u * dr;
+  const double b = r - gridv;
+  const double qv = (gm * (b * b)) / ty.two_sig2;
+  double v = exp(-qv);
+  if (d == 0) v = ty.amp * v;
+  tables[g] = (TT)v;
 }
 
 }  // namespace
@@ -243,23 +298,29 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
 hipError_t launch_binning(Plan& p, hipStream_t s) {
   const int n = p.sys.natoms;
   const int nbk = p.sys.nbuckets;
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, p.bucket_count, p.bucket_off,
-                     p.bucket_cur, nbk);
+  const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
+                     p.bucket_off, p.bucket_aux, nbk);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off, p.bucket_cur,
+                     p.bucket_aux, nbk, nblk);
   if (n > 0) {
     hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
                        p.s_idx, n);
-    hipLaunchKernelGGL(k_bucket_sort, dim3((nbk + 255) / 256), dim3(256), 0, s, p.bucket_off,
-                       p.s_idx, nbk);
-    const int per_block = 256 / kTabThreads;
-    const dim3 grid((n + per_block - 1) / per_block), block(256);
-    if (p.fft64)
-      hipLaunchKernelGGL(k_gather_tables<double>, grid, block, 0, s, p.sys, p.types, p.s_idx,
-                         p.ir4, p.rpos, p.s_ir, p.s_r, static_cast<double*>(p.tables),
-                         p.bucket_off + nbk);
-    else
-      hipLaunchKernelGGL(k_gather_tables<float>, grid, block, 0, s, p.sys, p.types, p.s_idx,
-                         p.ir4, p.rpos, p.s_ir, p.s_r, static_cast<float*>(p.tables),
-                         p.bucket_off + nbk);
+    hipLaunchKernelGGL(k_bucket_sort, dim3((nbk + kSortThreads - 1) / kSortThreads),
+                       dim3(kSortThreads), 0, s, p.bucket_off, p.s_idx, nbk);
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.s_idx, p.ir4, p.rpos,
+                       p.s_ir, p.s_r, p.bucket_off + nbk);
+    const long long total = p.table_floats;
+    if (total > 0) {
+      const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+      if (p.fft64)
+        hipLaunchKernelGGL(k_tables<double>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
+                           static_cast<double*>(p.tables), total);
+      else
+        hipLaunchKernelGGL(k_tables<float>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
+                           static_cast<float*>(p.tables), total);
+    }
   }
   return hipGetLastError();
 }

@@ -68,6 +68,11 @@ SIGNATURES = {
                                c_void_p, c_void_p, c_void_p]),
     "mdsf_ewald_slices": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_double, c_void_p,
                                     c_void_p, c_void_p, c_void_p]),
+    "mdsf_adjoint_frames": (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
+                                      ctypes.POINTER(c_double), c_int32, c_int64, c_int64,
+                                      c_void_p, c_void_p, c_void_p]),
+    "mdsf_density_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mdsf_deposit_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 

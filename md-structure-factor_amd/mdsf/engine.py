@@ -209,6 +209,42 @@ class Plan:
         _lib.check(self.lib.mdsf_rfft3(self._handle, self._ptr(d), self._ptr(out), self._stream()))
         return torch.view_as_complex(out)
 
+    # -------------------------------------------------------------- adjoint
+    def adjoint_frames(self, coords, types, scale, box, arith, wrap, dsf, grad=None):
+        """dL/d(coords) for device ``coords`` (F, natoms, 3) given ``dsf`` = dL/dS
+        (Nx, Ny, Nz/2+1) FP64.  ``coords`` is rescaled and wrapped in place (pass a
+        copy); returns FP64 (F, natoms, 3).  Each frame is recomputed."""
+        dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
+        g = dsf.to(device=self.device, dtype=torch.float64).contiguous()
+        if tuple(g.shape) != self.half_shape:
+            raise ValueError(f"dL/dS must have shape {self.half_shape}")
+        if grad is None:
+            grad = torch.empty(tuple(coords.shape), dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.mdsf_adjoint_frames(
+            self._handle, self._ptr(coords), dtype, coords.shape[0], self._ptr(types),
+            self._ptr(scale), _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]),
+            self._ptr(g), self._ptr(grad), self._stream()))
+        return grad
+
+    def density_adjoint(self, density, dsf):
+        """dL/d(density) for L(|rfftn(density)|^2) given dL/dS (plan precision)."""
+        d = density.to(device=self.device, dtype=self.real_dtype).contiguous()
+        g = dsf.to(device=self.device, dtype=torch.float64).contiguous()
+        if tuple(g.shape) != self.half_shape:
+            raise ValueError(f"dL/dS must have shape {self.half_shape}")
+        out = torch.empty(self.n, dtype=self.real_dtype, device=self.device)
+        _lib.check(self.lib.mdsf_density_adjoint(self._handle, self._ptr(d), self._ptr(g),
+                                                 self._ptr(out), self._stream()))
+        return out
+
+    def deposit_adjoint(self, grad_density):
+        """dL/dr (natoms, 3) FP64 of the last prepared frame from dL/d(folded density)."""
+        gd = grad_density.to(device=self.device, dtype=self.real_dtype).contiguous()
+        out = torch.empty((self.natoms, 3), dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.mdsf_deposit_adjoint(self._handle, self._ptr(gd), self._ptr(out),
+                                                 self._stream()))
+        return out
+
     def sf(self):
         out = torch.empty(self.half_shape, dtype=torch.float64, device=self.device)
         _lib.check(self.lib.mdsf_sf_read(self._handle, self._ptr(out), self._stream()))

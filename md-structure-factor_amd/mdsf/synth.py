@@ -7,6 +7,9 @@
                104 x 90 x 80 box lined with C/O/Na headgroups, C/H tails
                elsewhere, 50,000 atoms; frames = base + N(0, 0.2) noise.
 * ``uniform``: uniform-random atoms of given types in a cubic box.
+* ``adjoint``: 20,000 C/O/H atoms uniform in a 60 box, 16 frames of
+               base + N(0, 0.2) noise, plus a target copy of the base
+               perturbed by N(0, 0.5) (grid 128^3 at Sres = 60/128).
 """
 import numpy as np
 
@@ -97,3 +100,26 @@ def uniform(seed, natoms, box, names, nframes, dtype=np.float32):
     coords = rng.uniform(0.0, box, size=(nframes, natoms, 3)).astype(dtype)
     dims = np.full((nframes, 3), float(box))
     return coords, dims, typ
+
+
+ADJ_BOX = 60.0
+
+
+def adjoint(seed=4004, target_seed=4005, natoms=20000, nframes=16, dtype=np.float64):
+    """Returns (coords (F, natoms, 3), target coords (F, natoms, 3), dims (F, 3), typ)."""
+    rng = np.random.default_rng(seed)
+    names = ["C", "O", "H"]
+    typ = [names[i] for i in rng.integers(0, 3, size=natoms)]
+    base = rng.uniform(0.0, ADJ_BOX, size=(natoms, 3))
+    coords = np.stack([np.mod(base + rng.normal(0.0, 0.2, size=base.shape), ADJ_BOX)
+                       for _ in range(nframes)]).astype(dtype)
+    trng = np.random.default_rng(target_seed)
+    tbase = np.mod(base + trng.normal(0.0, 0.5, size=base.shape), ADJ_BOX)
+    target = np.stack([np.mod(tbase + trng.normal(0.0, 0.2, size=base.shape), ADJ_BOX)
+                       for _ in range(nframes)]).astype(dtype)
+    dims = np.full((nframes, 3), ADJ_BOX)
+    return coords, target, dims, typ
+
+
+def adjoint_sres():
+    return ADJ_BOX / 128.0
