@@ -33,6 +33,9 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 METRIC = "frames/s for S(q) at 256^3, 50k atoms"
+# HBM bytes per deposition launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by the same counters on a plain copy); None until measured
+DEPOSIT_TRAFFIC = None
 
 
 def log(msg):
@@ -108,6 +111,49 @@ def cpu_baseline(coords0, dims, typ, rad, sres, budget_atoms=None):
     }
 
 
+def adjoint_leg(rad, iters):
+    """fwd+bwd frames/s on the adjoint workload: 20k C/O/H atoms, 60 A box, 128^3,
+    16 frames, L = sum (S - S_target)^2; backward recomputes every frame."""
+    import torch
+    from mdsf import synth
+    from mdsf.autograd import SFModel
+    coords, target, dims, typ = synth.adjoint()
+    nframes = coords.shape[0]
+    model = SFModel(coords, dims, typ, rad, synth.adjoint_sres())
+    dev = model.plan.device
+    with torch.no_grad():
+        st = model(torch.tensor(target, device=dev))
+    x = torch.tensor(coords, device=dev, requires_grad=True)
+
+    def fwd_bwd():
+        x.grad = None
+        loss = torch.sum((model(x) - st) ** 2)
+        loss.backward()
+
+    def fwd():
+        with torch.no_grad():
+            model(x)
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) / iters
+    fwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        fwd()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / iters
+    return {"workload": "adjoint_128^3_20k_atoms_16_frames", "frames": nframes,
+            "fwd_bwd_frames_per_s": nframes / t_fb, "fwd_frames_per_s": nframes / t_f,
+            "ms_per_fwd_bwd": t_fb * 1e3, "iters": iters, "dtype": "f32",
+            "grad_norm": float(x.grad.norm())}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +161,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fft", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+bwd leg")
+    ap.add_argument("--adj-iters", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the S(q) path runs on a single MI355X here "
@@ -142,39 +190,31 @@ def main():
     box = [float(x) for x in s.L]
     stream = torch.cuda.current_stream(dev)
     stages = ["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"]
-
-    def run(frames, record):
-        evs = []
-        for t in frames:
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
-            if record:
-                e[0].record(stream)
-            plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap)
-            if record:
-                e[1].record(stream)
-            plan.deposit()
-            if record:
-                e[2].record(stream)
-            plan.fft_pass(0)
-            if record:
-                e[3].record(stream)
-            plan.fft_pass(1)
-            if record:
-                e[4].record(stream)
-            plan.fft_pass(2)
-            if record:
-                e[5].record(stream)
-                evs.append(e)
-        return evs
+    steps = [lambda t: plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap),
+             lambda t: plan.deposit(),
+             lambda t: plan.fft_pass(0),
+             lambda t: plan.fft_pass(1),
+             lambda t: plan.fft_pass(2)]
+    # events are created up front; each stage is bracketed on the stream the
+    # kernels run on
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+           for _ in range(args.steps)]
 
     log(f"warmup {args.warmup} frames")
-    run(range(args.warmup), False)
+    for t in range(args.warmup):
+        for st in steps:
+            st(t)
     plan.reset()
     torch.cuda.synchronize()
     log(f"timing {args.steps} frames")
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
-    evs = run(range(args.steps), True)
+    for t in range(args.steps):
+        e = evs[t]
+        e[0].record(stream)
+        for i, st in enumerate(steps):
+            st(t)
+            e[i + 1].record(stream)
     end.record(stream)
     torch.cuda.synchronize()
     plan.check_errors()
@@ -192,15 +232,19 @@ def main():
                          "bytes": fb[key], "avg_ms": round(per_stage[st], 4)}
     updates = voxel_updates(s)
     dep_tflops = 2.0 * updates / (per_stage["deposit"] * 1e-3) / 1e12
-    dominant = max(per_stage, key=per_stage.get)
+    # dominant kernel: the single launch with the largest average time
+    # ("prepare" is several small kernels)
+    kernels = {k: v for k, v in per_stage.items() if k != "prepare"}
+    dominant = max(kernels, key=kernels.get)
     if dominant == "deposit":
-        roofline = {"bound": "mfma", "kernel": "k_deposit (VALU FP32 FMA gather)",
+        roofline = {"bound": "mfma", "kernel": "k_deposit_mfma (deposition + fold, FP32 MFMA)",
                     "achieved": round(dep_tflops, 3), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(dep_tflops / FP32_PEAK_TFLOPS, 4),
-                    "traffic": None,
-                    "basis": f"{updates} voxel updates/frame x 2 flop (one FMA each)"}
+                    "traffic": DEPOSIT_TRAFFIC,
+                    "basis": f"{updates} voxel updates/frame x 2 flop (one FMA each) / avg "
+                             f"deposit time {per_stage['deposit'] * 1e3:.1f} us"}
     else:
-        key = {"fft_z": "z", "fft_y": "y", "fft_x_acc": "x"}.get(dominant, "x")
+        key = {"fft_z": "z", "fft_y": "y", "fft_x_acc": "x"}[dominant]
         roofline = {"bound": "hbm", "kernel": f"FFT {key} pass",
                     "achieved": fft_roof[key]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": fft_roof[key]["frac"], "traffic": None}
@@ -228,6 +272,8 @@ def main():
                     "tflops_fma": round(dep_tflops, 3)},
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
     }
+    if not args.no_adjoint:
+        out["fwd_bwd"] = adjoint_leg(rad, args.adj_iters)
     if not args.no_cpu:
         log("cpu baseline (oracle on a stratified sample of frame 0)")
         c0, d0, t0 = synth.hii(seed=2002, nframes=1)

@@ -684,6 +684,28 @@ __device__ __forceinline__ AxisCtx axis_ctx(int d, const int4& q, const TypeDev&
   return x;
 }
 
+// Same, from the per-atom table offset and the type's half-widths held in LDS.
+__device__ __forceinline__ AxisCtx axis_ctx_lds(int d, const int4& q, const int* half,
+                                                const float* atom_tb, const SysDev& sys,
+                                                const Tile& tl) {
+  AxisCtx x;
+  x.N = pick3(sys.n, d);
+  const int A = half[d];
+  const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+  const int lo = ir - A;
+  x.len = 2 * A;
+  x.tb = atom_tb + (d == 0 ? 0 : (d == 1 ? 2 * half[0] : 2 * half[0] + 2 * half[1]));
+  x.t0 = pick3(tl.t0, d);
+  x.tn = pick3(tl.t1, d) - x.t0;
+  x.kL = x.t0 - x.N - lo;
+  x.kC = x.t0 - lo;
+  x.kH = x.t0 + x.N - lo;
+  x.nb = sys.nb;
+  x.kSL = x.t0 + x.nb - lo;
+  x.kSH = x.t0 - x.nb - lo;
+  return x;
+}
+
 __device__ __forceinline__ float ax_val(const AxisCtx& x, int k, bool ok) {
   return ok ? x.tb[k] : 0.f;
 }
@@ -701,6 +723,8 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
   __shared__ EnumShared sh;
   __shared__ struct {
     int vbase[NV], entry[NV];
+    int4 q[NV];          // home cell, type | corner << 16
+    long long tb[NV];    // float offset of the atom's factor tables, -1 if invalid
   } bt;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -724,24 +748,33 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
     const int vtot = nreal > 0 ? __shfl(vupto, nreal - 1, 64) : 0;
     const int vpad = (vtot + 3) & ~3;
     if (wave == 0 && lane < nreal) {
+      // per-atom lookups once per batch, so that the staging loop below only
+      // issues independent table loads
+      const int c = e & 0x7fffffff;
+      const int4 q = s_ir[c];
+      const TypeDev& ty = td[q.w & 0xffff];
+      const int slot = c - ty.type_start;
       bt.vbase[lane] = lane + EXTRA * __popcll(cm & (le >> 1));
-      bt.entry[lane] = e & 0x7fffffff;
+      bt.entry[lane] = c;
+      bt.q[lane] = q;
+      bt.tb[lane] = (slot >= 0 && slot < ty.count) ? ty.table_base + (long long)slot * ty.stride
+                                                   : -1LL;  // invalid only after a reported error
     }
     __syncthreads();
     // staging: wave d < 3 builds axis d for every (atom, cell) of the batch
     if (wave < 3) {
       const int d = wave;
+#pragma unroll 2
       for (int base = 0; base < nreal * T; base += 64) {
         const int idx = base + lane;
         const bool act = idx < nreal * T;
         const int a = act ? idx >> 4 : 0, i = idx & 15;
         const int c = bt.entry[a], vb = bt.vbase[a];
-        const int4 q = s_ir[c];
+        const int4 q = bt.q[a];
         const int t = q.w & 0xffff;
         const bool corner = (q.w >> 16) != 0;
-        const TypeDev& ty = td[t];
-        const int slot = c - ty.type_start;
-        const bool valid = act && slot >= 0 && slot < ty.count;  // false only after a reported error
+        const long long tbo = bt.tb[a];
+        const bool valid = act && tbo >= 0;
         const int nv = corner ? EXTRA + 1 : 1;
         bool hC = false, hN = false;
         if (act && !valid) {
@@ -758,7 +791,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
         } else if (valid) {
           if (MODE == MODE_XY && d < 2) {
             const int N = d == 0 ? sys.n[0] : sys.n[1];
-            const int A = d == 0 ? ty.half[0] : ty.half[1];
+            const int A = sh.half[t][d];
             const int lo = (d == 0 ? q.x : q.y) - A, len = 2 * A;
             const double r = s_r[3 * (size_t)c + d];
             const double dr = d == 0 ? sys.dr[0] : sys.dr[1];
@@ -783,7 +816,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
             }
             hC = bits != 0;
           } else {
-            const AxisCtx x = axis_ctx(d, q, ty, slot, sys, tables, tl);
+            const AxisCtx x = axis_ctx_lds(d, q, sh.half[t], tables + tbo, sys, tl);
             const bool in = i < x.tn;
             const bool bL = in && x.kL + i >= 0 && x.kL + i < x.len;
             const bool bC = in && x.kC + i >= 0 && x.kC + i < x.len;
@@ -851,8 +884,8 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
       // wave 3: per-atom scalars (XY) and the zero padding of the last group
       if constexpr (MODE == MODE_XY) {
         for (int a = lane; a < nreal; a += 64) {
-          const int c = bt.entry[a], vb = bt.vbase[a];
-          const int4 q = s_ir[c];
+          const int vb = bt.vbase[a];
+          const int4 q = bt.q[a];
           const TypeDev& ty = td[q.w & 0xffff];
           const bool corner = (q.w >> 16) != 0;
           st.amp[vb] = ty.amp_f;

@@ -90,7 +90,7 @@ def test_adjoint_frames_match_autograd(theta, fp64, rad, gpu):
     rng = np.random.default_rng(2)
     x = torch.tensor(coords, device=model.plan.device, requires_grad=True)
     S = model(x)
-    w = torch.tensor(rng.normal(size=tuple(S.shape)), device=S.device) / S.max()
+    w = torch.tensor(rng.normal(size=tuple(S.shape)), device=S.device) / S.detach().max()
     torch.sum(w * S).backward()
     got = x.grad.cpu().numpy()
     # the caller's coordinates are untouched (the adjoint works on copies)
