@@ -111,6 +111,16 @@ def cpu_baseline(coords0, dims, typ, rad, sres, budget_atoms=None):
     }
 
 
+def copy_calibration(nbytes=256 << 20):
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": 3}), flush=True)
+
+
 def adjoint_leg(rad, iters):
     """fwd+bwd frames/s on the adjoint workload: 20k C/O/H atoms, 60 A box, 128^3,
     16 frames, L = sum (S - S_target)^2; backward recomputes every frame."""
@@ -163,7 +173,13 @@ def main():
     ap.add_argument("--fft", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+bwd leg")
     ap.add_argument("--adj-iters", type=int, default=5)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 3 times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the S(q) path runs on a single MI355X here "
               "(frames would shard across GPUs with one FP64 all-reduce of S(q)); "

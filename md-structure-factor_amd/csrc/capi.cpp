@@ -245,8 +245,8 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     TRY(dalloc(&tb, (size_t)table_floats));
     p->tables = tb;
   } else {
-    float* tb = nullptr;
-    TRY(dalloc(&tb, (size_t)table_floats));
+    float* tb = nullptr;  // unused: the FP32 path evaluates its factors in the gather
+    TRY(dalloc(&tb, (size_t)1));
     p->tables = tb;
   }
   TRY(dalloc(&p->err, (size_t)1));

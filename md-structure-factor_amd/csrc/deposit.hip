@@ -655,81 +655,49 @@ struct StageMXY {
   int vreal[NV], vcls[NV], vxm[NV];
 };
 
-// Per-cell values of atom `c`'s factor along axis d over the tile, split by
-// the side of the image (L / C / H) and, for z of a corner atom, the two
-// shifted slots of the corner quirk (SL: y on L, SH: y on H).
-struct AxisCtx {
-  const float* tb;
-  int kL, kC, kH, kSL, kSH, len, tn, t0, N, nb;
-};
-
-__device__ __forceinline__ AxisCtx axis_ctx(int d, const int4& q, const TypeDev& ty, int slot,
-                                            const SysDev& sys, const float* tables, const Tile& tl) {
-  AxisCtx x;
-  x.N = pick3(sys.n, d);
-  const int A = pick3(ty.half, d);
-  const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
-  const int lo = ir - A;
-  x.len = 2 * A;
-  x.tb = atom_tables<float>(tables, ty, slot) +
-         (d == 0 ? 0 : (d == 1 ? 2 * ty.half[0] : 2 * ty.half[0] + 2 * ty.half[1]));
-  x.t0 = pick3(tl.t0, d);
-  x.tn = pick3(tl.t1, d) - x.t0;
-  x.kL = x.t0 - x.N - lo;
-  x.kC = x.t0 - lo;
-  x.kH = x.t0 + x.N - lo;
-  x.nb = sys.nb;
-  x.kSL = x.t0 + x.nb - lo;
-  x.kSH = x.t0 - x.nb - lo;
-  return x;
-}
-
-// Same, from the per-atom table offset and the type's half-widths held in LDS.
-__device__ __forceinline__ AxisCtx axis_ctx_lds(int d, const int4& q, const int* half,
-                                                const float* atom_tb, const SysDev& sys,
-                                                const Tile& tl) {
-  AxisCtx x;
-  x.N = pick3(sys.n, d);
-  const int A = half[d];
-  const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
-  const int lo = ir - A;
-  x.len = 2 * A;
-  x.tb = atom_tb + (d == 0 ? 0 : (d == 1 ? 2 * half[0] : 2 * half[0] + 2 * half[1]));
-  x.t0 = pick3(tl.t0, d);
-  x.tn = pick3(tl.t1, d) - x.t0;
-  x.kL = x.t0 - x.N - lo;
-  x.kC = x.t0 - lo;
-  x.kH = x.t0 + x.N - lo;
-  x.nb = sys.nb;
-  x.kSL = x.t0 + x.nb - lo;
-  x.kSH = x.t0 - x.nb - lo;
-  return x;
-}
-
-__device__ __forceinline__ float ax_val(const AxisCtx& x, int k, bool ok) {
-  return ok ? x.tb[k] : 0.f;
-}
-
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const int* __restrict__ boff,
                                                            const int4* __restrict__ s_ir,
                                                            const double* __restrict__ s_r,
-                                                           const float* __restrict__ tables,
                                                            float* __restrict__ rho) {
   using Stage = typename std::conditional<MODE == MODE_DIAG, StageM, StageMXY>::type;
   constexpr int EXTRA = MODE == MODE_DIAG ? 3 : 2;  // extra virtual slots of a corner atom
   __shared__ Stage st;
   __shared__ EnumShared sh;
   __shared__ struct {
+    double r[NV][3];     // wrapped position (FP64)
     int vbase[NV], entry[NV];
     int4 q[NV];          // home cell, type | corner << 16
-    long long tb[NV];    // float offset of the atom's factor tables, -1 if invalid
+    int ok[NV];          // 0 only after a reported error
   } bt;
+  // per-type constants: g_dd / (2 sigma^2) per axis and Nel / sigma^3
+  __shared__ double tgk[kMaxTypes][3];
+  __shared__ float tamp[kMaxTypes];
+  // grid size, spacing and tile extent per axis
+  __shared__ struct {
+    double dr[3];
+    int n[3], t0[3], t1[3];
+  } axs;
+  for (int i = threadIdx.x; i < sys.ntypes; i += kThreads) {
+    tgk[i][0] = sys.g00 / td[i].two_sig2;
+    tgk[i][1] = sys.g11 / td[i].two_sig2;
+    tgk[i][2] = sys.g22 / td[i].two_sig2;
+    tamp[i] = td[i].amp_f;
+  }
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   Tile tl;
   tile_coords(sys, tl);
+  if (tid == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      axs.dr[d] = sys.dr[d];
+      axs.n[d] = sys.n[d];
+      axs.t0[d] = tl.t0[d];
+      axs.t1[d] = tl.t1[d];
+    }
+  }
   const float g00 = (float)sys.g00, g01x2 = (float)(2.0 * sys.g01), g11 = (float)sys.g11;
   v4f acc[4];
 #pragma unroll
@@ -757,24 +725,29 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
       bt.vbase[lane] = lane + EXTRA * __popcll(cm & (le >> 1));
       bt.entry[lane] = c;
       bt.q[lane] = q;
-      bt.tb[lane] = (slot >= 0 && slot < ty.count) ? ty.table_base + (long long)slot * ty.stride
-                                                   : -1LL;  // invalid only after a reported error
+      bt.ok[lane] = slot >= 0 && slot < ty.count;  // false only after a reported error
+      bt.r[lane][0] = s_r[3 * (size_t)c + 0];
+      bt.r[lane][1] = s_r[3 * (size_t)c + 1];
+      bt.r[lane][2] = s_r[3 * (size_t)c + 2];
     }
     __syncthreads();
     // staging: wave d < 3 builds axis d for every (atom, cell) of the batch
     if (wave < 3) {
       const int d = wave;
-#pragma unroll 2
+      // this wave's axis (from LDS: a runtime index into the tile's arrays
+      // would otherwise go through private memory)
+      const int Nd = axs.n[d], t0d = axs.t0[d], tnd = axs.t1[d] - axs.t0[d];
+      const double drd = axs.dr[d];
+#pragma unroll 1
       for (int base = 0; base < nreal * T; base += 64) {
         const int idx = base + lane;
         const bool act = idx < nreal * T;
         const int a = act ? idx >> 4 : 0, i = idx & 15;
-        const int c = bt.entry[a], vb = bt.vbase[a];
+        const int vb = bt.vbase[a];
         const int4 q = bt.q[a];
         const int t = q.w & 0xffff;
         const bool corner = (q.w >> 16) != 0;
-        const long long tbo = bt.tb[a];
-        const bool valid = act && tbo >= 0;
+        const bool valid = act && bt.ok[a];
         const int nv = corner ? EXTRA + 1 : 1;
         bool hC = false, hN = false;
         if (act && !valid) {
@@ -793,7 +766,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
             const int N = d == 0 ? sys.n[0] : sys.n[1];
             const int A = sh.half[t][d];
             const int lo = (d == 0 ? q.x : q.y) - A, len = 2 * A;
-            const double r = s_r[3 * (size_t)c + d];
+            const double r = bt.r[a][d];
             const double dr = d == 0 ? sys.dr[0] : sys.dr[1];
             const int t0 = d == 0 ? tl.t0[0] : tl.t0[1];
             const int tn = (d == 0 ? tl.t1[0] : tl.t1[1]) - t0;
@@ -816,22 +789,32 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
             }
             hC = bits != 0;
           } else {
-            const AxisCtx x = axis_ctx_lds(d, q, sh.half[t], tables + tbo, sys, tl);
-            const bool in = i < x.tn;
-            const bool bL = in && x.kL + i >= 0 && x.kL + i < x.len;
-            const bool bC = in && x.kC + i >= 0 && x.kC + i < x.len;
-            const bool bH = in && x.kH + i >= 0 && x.kH + i < x.len;
-            const float vL = ax_val(x, x.kL + i, bL), vC = ax_val(x, x.kC + i, bC);
-            const float vH = ax_val(x, x.kH + i, bH);
+            // 1-D factor of the image at grid coordinate u (window offset
+            // k = u - lo): amp^[d==0] exp(-g_dd b^2 / 2 s^2), b = r - u dr,
+            // the exponent in FP64 (dens.py:252-256, 299-308)
+            const int A = sh.half[t][d];
+            const int lo = (d == 0 ? q.x : (d == 1 ? q.y : q.z)) - A, len = 2 * A;
+            const double r = bt.r[a][d], gk = tgk[t][d];
+            const float am = d == 0 ? tamp[t] : 1.f;
+            const bool in = i < tnd;
+            const int uC = t0d + i;
+            auto inwin = [&](int u) { return in && u - lo >= 0 && u - lo < len; };
+            auto val = [&](int u, bool ok) -> float {
+              if (!ok) return 0.f;
+              const double b = r - (double)u * drd;
+              return am * expf(-(float)(gk * (b * b)));
+            };
+            const bool bL = inwin(uC - Nd), bC = inwin(uC), bH = inwin(uC + Nd);
+            const float vL = val(uC - Nd, bL), vC = val(uC, bC), vH = val(uC + Nd, bH);
             const float all = vC + vL + vH;
             if (d == 2) {
               st.vz[vb][i] = all;
               if (corner) {
-                const int u1 = x.t0 + i + x.nb, u2 = x.t0 + i - x.nb;
-                const bool s1 = in && u1 >= x.N && x.kSL + i >= 0 && x.kSL + i < x.len;
-                const bool s2 = in && u2 < 0 && x.kSH + i >= 0 && x.kSH + i < x.len;
-                const float zl = vC + vL + ax_val(x, x.kSL + i, s1);
-                const float zh = vC + vH + ax_val(x, x.kSH + i, s2);
+                const int u1 = uC + sys.nb, u2 = uC - sys.nb;
+                const bool s1 = u1 >= Nd && inwin(u1);
+                const bool s2 = u2 < 0 && inwin(u2);
+                const float zl = vC + vL + val(u1, s1);
+                const float zh = vC + vH + val(u2, s2);
                 if constexpr (MODE == MODE_DIAG) {
                   st.vz[vb + 1][i] = all;
                   st.vz[vb + 2][i] = zl;
@@ -990,13 +973,12 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
   if (ntiles == 0) return hipSuccess;
   const dim3 grid((unsigned)ntiles), block(kThreads);
   if constexpr (std::is_same<ACC, float>::value) {
-    const float* tb = static_cast<const float*>(p.tables);
     if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL(k_deposit_mfma<MODE_DIAG>, grid, block, 0, s, sys, p.types, p.bucket_off,
-                         p.s_ir, p.s_r, tb, out);
+                         p.s_ir, p.s_r, out);
     else
       hipLaunchKernelGGL(k_deposit_mfma<MODE_XY>, grid, block, 0, s, sys, p.types, p.bucket_off,
-                         p.s_ir, p.s_r, tb, out);
+                         p.s_ir, p.s_r, out);
   } else {
     if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), grid, block, 0, s, sys, p.types,

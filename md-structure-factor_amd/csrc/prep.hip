@@ -311,7 +311,9 @@ hipError_t launch_binning(Plan& p, hipStream_t s) {
                        dim3(kSortThreads), 0, s, p.bucket_off, p.s_idx, nbk);
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.s_idx, p.ir4, p.rpos,
                        p.s_ir, p.s_r, p.bucket_off + nbk);
-    const long long total = p.table_floats;
+    // factor tables feed the FP64 gather; the FP32 MFMA path evaluates its
+    // factors in place
+    const long long total = p.fft64 ? p.table_floats : 0;
     if (total > 0) {
       const dim3 grid((unsigned)((total + 255) / 256)), block(256);
       if (p.fft64)
