@@ -164,6 +164,38 @@ def adjoint_leg(rad, iters):
             "grad_norm": float(x.grad.norm())}
 
 
+def sq_rel_err(got, want, floor=1e-6):
+    """max / median relative error of S(q) on voxels above floor * max (and the voxel share)."""
+    import torch
+    m = torch.max(want)
+    mask = want > floor * m
+    err = torch.abs(got[mask] - want[mask]) / want[mask]
+    return {"max": float(err.max()), "median": float(err.median()),
+            "voxels": int(mask.sum()), "of": int(want.numel())}
+
+
+def precision_leg(coords, s, nframes=2):
+    """FP32 plan (the measured path) against the FP64 plan on the first frames of
+    the workload; the FP64 plan is pinned to the FP64 numpy oracle by the tests."""
+    import torch
+    from mdsf import engine
+    dev = torch.device("cuda", 0)
+    out = {}
+    for fp64 in (False, True):
+        plan = engine.Plan.from_setup(s, fft64=fp64)
+        c = torch.from_numpy(np.ascontiguousarray(coords[:nframes])).to(dev)
+        types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
+        scale = torch.from_numpy(np.ascontiguousarray(s.scale[:nframes], dtype=np.float64)).to(dev)
+        plan.accumulate_frames(c, types, scale, [float(x) for x in s.L], s.arith, s.wrap)
+        plan.check_errors()
+        out[fp64] = plan.sf()
+        plan.close()
+    res = sq_rel_err(out[False], out[True])
+    res.update({"reference": "fp64 plan (FP64 deposition, FFT and sum)", "frames": nframes,
+                "floor": "1e-6 * max S"})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fft", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+bwd leg")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the fp32-vs-fp64 S(q) error report")
     ap.add_argument("--adj-iters", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 3 times (the reference "
@@ -288,7 +322,11 @@ def main():
                     "tflops_fma": round(dep_tflops, 3)},
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
     }
+    if not args.no_precision:
+        log("precision: fp32 plan vs fp64 plan on 2 frames")
+        out["precision"] = precision_leg(coords, s)
     if not args.no_adjoint:
+        log("fwd+bwd leg (adjoint workload)")
         out["fwd_bwd"] = adjoint_leg(rad, args.adj_iters)
     if not args.no_cpu:
         log("cpu baseline (oracle on a stratified sample of frame 0)")

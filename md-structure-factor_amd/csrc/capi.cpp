@@ -46,7 +46,7 @@ void free_plan(Plan* p) {
                   p->tables,    p->err,       p->rho,        p->spec_nyq,   p->sf_main,
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
                   p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
-                  p->adj_w};
+                  p->adj_w,     p->slices};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -142,7 +142,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   s.natoms = desc->natoms;
   p->fft64 = (desc->flags & MDSF_FFT_FP64) ? 1 : 0;
 
-  long long table_floats = 0;
+  long long table_floats = 0, slice_floats = 0;
   int start = 0, bucket_base = 0;
   long long counted = 0;
   for (int t = 0; t < desc->ntypes; ++t) {
@@ -167,6 +167,18 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       ty.bsz[d] = bucket_edge(in.half[d]);
       ty.nbk[d] = (desc->n[d] + ty.bsz[d] - 1) / ty.bsz[d];
     }
+    // tile slices: a cyclic window of 2A cells covers at most ceil(2A/16)+1
+    // tiles, one more when the last tile along the axis is partial
+    int sstride = 0;
+    for (int d = 0; d < 3; ++d) {
+      const int nt = (desc->n[d] + kTile - 1) / kTile;
+      const int extra = (desc->n[d] % kTile) ? 2 : 1;
+      ty.nsl[d] = std::min(nt, (2 * in.half[d] + kTile - 1) / kTile + extra);
+      sstride += kTile * ty.nsl[d];
+    }
+    ty.slice_stride = sstride;
+    ty.slice_base = slice_floats;
+    slice_floats += (long long)in.count * sstride;
     ty.count = in.count;
     ty.type_start = start;
     ty.stride = stride;
@@ -183,6 +195,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   }
   s.nbuckets = bucket_base;
   p->table_floats = table_floats;
+  p->slice_floats = (!p->fft64 && s.mode == MODE_DIAG) ? slice_floats : 0;
   // adjoint work items (atom, x-chunk of <= 64 planes, chunk of (y, z)
   // pairs sized for ~16k window voxels), heaviest windows numbered first so
   // that their workgroups start first
@@ -245,10 +258,11 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     TRY(dalloc(&tb, (size_t)table_floats));
     p->tables = tb;
   } else {
-    float* tb = nullptr;  // unused: the FP32 path evaluates its factors in the gather
+    float* tb = nullptr;  // unused: the FP32 path stages tile slices instead
     TRY(dalloc(&tb, (size_t)1));
     p->tables = tb;
   }
+  TRY(dalloc(&p->slices, (size_t)p->slice_floats));
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));
   if (p->fft64) {

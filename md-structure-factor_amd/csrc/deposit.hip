@@ -660,6 +660,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
                                                            const int* __restrict__ boff,
                                                            const int4* __restrict__ s_ir,
                                                            const double* __restrict__ s_r,
+                                                           const float* __restrict__ slices,
                                                            float* __restrict__ rho) {
   using Stage = typename std::conditional<MODE == MODE_DIAG, StageM, StageMXY>::type;
   constexpr int EXTRA = MODE == MODE_DIAG ? 3 : 2;  // extra virtual slots of a corner atom
@@ -670,6 +671,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
     int vbase[NV], entry[NV];
     int4 q[NV];          // home cell, type | corner << 16
     int ok[NV];          // 0 only after a reported error
+    long long so[MODE == MODE_DIAG ? NV : 1][3];  // tile slice offset per axis, -1: none
   } bt;
   // per-type constants: g_dd / (2 sigma^2) per axis and Nel / sigma^3
   __shared__ double tgk[kMaxTypes][3];
@@ -713,6 +715,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
     const int vupto = (lane + 1) + EXTRA * __popcll(cm & le);
     const bool fits = lane < count && vupto <= NV;
     const int nreal = __popcll(__ballot(fits));
+    const int ncorner = __popcll(__ballot(fits && isc));
     const int vtot = nreal > 0 ? __shfl(vupto, nreal - 1, 64) : 0;
     const int vpad = (vtot + 3) & ~3;
     if (wave == 0 && lane < nreal) {
@@ -729,10 +732,45 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
       bt.r[lane][0] = s_r[3 * (size_t)c + 0];
       bt.r[lane][1] = s_r[3 * (size_t)c + 1];
       bt.r[lane][2] = s_r[3 * (size_t)c + 2];
+      if constexpr (MODE == MODE_DIAG) {
+        // this tile's slice per axis: slice j = (tile - T0) mod ntiles, T0 the
+        // tile of the window start (same numbering as k_slices)
+        const bool ok = slot >= 0 && slot < ty.count;
+        long long off = ty.slice_base + (long long)slot * ty.slice_stride;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+          const int N = sys.n[d], nt = sys.ntiles[d];
+          int j = tl.t0[d] / T - pmod(ir - ty.half[d], N) / T;
+          if (j < 0) j += nt;
+          bt.so[lane][d] = (ok && j < ty.nsl[d]) ? off + (long long)T * j : -1LL;
+          off += (long long)T * ty.nsl[d];
+        }
+      }
     }
     __syncthreads();
-    // staging: wave d < 3 builds axis d for every (atom, cell) of the batch
-    if (wave < 3) {
+    if constexpr (MODE == MODE_DIAG) {
+      // atoms off the corner regions: copy their three tile slices (float4 per
+      // lane, all waves); the last group's padding is zeroed alongside
+      for (int it = tid; it < nreal * 12; it += kThreads) {
+        const int a = it / 12, r12 = it - 12 * (it / 12), d = r12 >> 2, q4 = r12 & 3;
+        if ((bt.q[a].w >> 16) != 0) continue;  // corner atom: evaluated below
+        const long long off = bt.so[a][d];
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (off >= 0) v = *reinterpret_cast<const float4*>(slices + off + 4 * q4);
+        float* row = d == 0 ? st.vx[bt.vbase[a]] : (d == 1 ? st.vy[bt.vbase[a]] : st.vz[bt.vbase[a]]);
+        *reinterpret_cast<float4*>(row + 4 * q4) = v;
+      }
+      for (int j = tid; j < (vpad - vtot) * T; j += kThreads) {
+        const int v = vtot + j / T, cc = j % T;
+        st.vx[v][cc] = 0.f;
+        st.vy[v][cc] = 0.f;
+        st.vz[v][cc] = 0.f;
+      }
+    }
+    // staging by evaluation: wave d < 3 builds axis d for every (atom, cell)
+    // of the batch (DIAG: corner atoms only, and only when the batch has one)
+    if (wave < 3 && (MODE != MODE_DIAG || ncorner > 0)) {
       const int d = wave;
       // this wave's axis (from LDS: a runtime index into the tile's arrays
       // would otherwise go through private memory)
@@ -750,7 +788,9 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
         const bool valid = act && bt.ok[a];
         const int nv = corner ? EXTRA + 1 : 1;
         bool hC = false, hN = false;
-        if (act && !valid) {
+        if (MODE == MODE_DIAG && !corner) {
+          // staged from the slices above
+        } else if (act && !valid) {
           for (int v = 0; v < nv; ++v) {
             if constexpr (MODE == MODE_DIAG) {
               if (d == 0) st.vx[vb + v][i] = 0.f;
@@ -863,7 +903,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
           }
         }
       }
-    } else {
+    } else if (wave == 3 && MODE == MODE_XY) {
       // wave 3: per-atom scalars (XY) and the zero padding of the last group
       if constexpr (MODE == MODE_XY) {
         for (int a = lane; a < nreal; a += 64) {
@@ -903,8 +943,16 @@ __global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const Typ
     // MFMA accumulation over groups of 4 virtual atoms
     const int kk = lane >> 4, jj = lane & 15;
     for (int k0 = 0; k0 < vpad; k0 += 4) {
-      const int m = ((st.vxm[k0] | st.vxm[k0 + 1] | st.vxm[k0 + 2] | st.vxm[k0 + 3]) >> (4 * wave)) & 0xf;
-      const int mu = __builtin_amdgcn_readfirstlane(m);
+      int mu;
+      if constexpr (MODE == MODE_DIAG) {
+        // x rows 4w..4w+3 where any of the 4 virtual atoms is non-zero
+        const float xr = st.vx[k0 + kk][4 * wave + (jj & 3)];
+        const unsigned long long bl = __ballot(jj < 4 && xr != 0.f);
+        mu = (int)((bl | (bl >> 16) | (bl >> 32) | (bl >> 48)) & 0xf);
+      } else {
+        const int m = ((st.vxm[k0] | st.vxm[k0 + 1] | st.vxm[k0 + 2] | st.vxm[k0 + 3]) >> (4 * wave)) & 0xf;
+        mu = __builtin_amdgcn_readfirstlane(m);
+      }
       if (!mu) continue;
       const int k = k0 + kk;
       const float zv = st.vz[k][jj];
@@ -975,10 +1023,10 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
   if constexpr (std::is_same<ACC, float>::value) {
     if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL(k_deposit_mfma<MODE_DIAG>, grid, block, 0, s, sys, p.types, p.bucket_off,
-                         p.s_ir, p.s_r, out);
+                         p.s_ir, p.s_r, p.slices, out);
     else
       hipLaunchKernelGGL(k_deposit_mfma<MODE_XY>, grid, block, 0, s, sys, p.types, p.bucket_off,
-                         p.s_ir, p.s_r, out);
+                         p.s_ir, p.s_r, p.slices, out);
   } else {
     if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), grid, block, 0, s, sys, p.types,

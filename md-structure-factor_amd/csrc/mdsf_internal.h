@@ -35,6 +35,11 @@ struct TypeDev {
   int bsz[3];        // bucket edge per axis
   int nbk[3];        // buckets per axis
   int bucket_base;   // first bucket id of this type
+  // tile slices (FP32 DIAG deposition): per atom and axis nsl[d] slices of
+  // kTile floats, the image-summed 1-D factor over one tile along that axis
+  int nsl[3];
+  int slice_stride;        // floats per atom: 16 * (nsl[0] + nsl[1] + nsl[2])
+  long long slice_base;    // float offset of this type's slices
   // adjoint work items: per atom adj_nxc x-chunks times adj_npc chunks of
   // adj_ppi (y, z) pairs, numbered from adj_item_base
   int adj_nxc;
@@ -84,7 +89,9 @@ struct Plan {
   int* s_idx = nullptr;
   int4* s_ir = nullptr;
   double* s_r = nullptr;
-  void* tables = nullptr;         // per-atom factor tables, FP32 or FP64 (fft64)
+  void* tables = nullptr;         // per-atom factor tables (FP64 plans)
+  float* slices = nullptr;        // per-atom tile slices (FP32 plans, orthogonal-axis cells)
+  long long slice_floats = 0;
   int* err = nullptr;
   // grids
   void* rho = nullptr;           // Nx*Ny*Nz folded density (FP32, or FP64 when fft64)

@@ -10,6 +10,8 @@
 * ``adjoint``: 20,000 C/O/H atoms uniform in a 60 box, 16 frames of
                base + N(0, 0.2) noise, plus a target copy of the base
                perturbed by N(0, 0.5) (grid 128^3 at Sres = 60/128).
+* ``large``  : 1,000,000 H/C/N/O atoms uniform in a 200 box, fresh frames
+               (grid 512^3 at Sres = 200/512).
 """
 import numpy as np
 
@@ -123,3 +125,20 @@ def adjoint(seed=4004, target_seed=4005, natoms=20000, nframes=16, dtype=np.floa
 
 def adjoint_sres():
     return ADJ_BOX / 128.0
+
+
+LARGE_BOX = 200.0
+
+
+def large(seed=5005, natoms=1_000_000, nframes=1, dtype=np.float64):
+    """1M uniform-random H/C/N/O atoms in a 200 box, fresh frames (grid 512^3 at Sres = 200/512)."""
+    rng = np.random.default_rng(seed)
+    names = ["H", "C", "N", "O"]
+    typ = [names[i] for i in rng.integers(0, 4, size=natoms)]
+    coords = rng.uniform(0.0, LARGE_BOX, size=(nframes, natoms, 3)).astype(dtype)
+    dims = np.full((nframes, 3), LARGE_BOX)
+    return coords, dims, typ
+
+
+def large_sres():
+    return LARGE_BOX / 512.0
