@@ -655,8 +655,11 @@ struct StageMXY {
   int vreal[NV], vcls[NV], vxm[NV];
 };
 
+// The gather is latency-bound (batch staging, candidate streaming): the
+// second bound asks for 5 waves per SIMD, which the register budget allows
+// (no spills) and the 29 KB LDS footprint matches (5 blocks per CU).
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
+__global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const int* __restrict__ boff,
                                                            const int4* __restrict__ s_ir,
                                                            const double* __restrict__ s_r,

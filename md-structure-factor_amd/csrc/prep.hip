@@ -269,18 +269,21 @@ __global__ void k_tables(SysDev sys, const TypeDev* __restrict__ td, const int4*
   tables[g] = (TT)v;
 }
 
-// Tile slices for the FP32 deposition, one thread per (sorted atom, axis):
-// slice j holds cell c = 16 T + i of tile T = (T0 + j) mod ntiles (T0 = tile
-// of the window start lo mod N); its value is the sum over the periodic
-// images u = c + sN (s = -1, 0, 1) inside the window [lo, lo + 2A) of
-// amp^[d==0] exp(-g_dd (r - u dr)^2 / 2 s^2): the exponent in FP64, exp in
-// FP32 (dens.py:252-256, 299-308).  Cells past N are zero.
+// Tile slices for the FP32 deposition.  16 lanes per (sorted atom, axis),
+// lane i owning cell i of every slice, so that each slice row is written as
+// one contiguous 64-byte segment.  Slice j holds cell c = 16 T + i of tile
+// T = (T0 + j) mod ntiles (T0 = tile of the window start lo mod N); its value
+// is the sum over the periodic images u = c + sN (s = -1, 0, 1) inside the
+// window [lo, lo + 2A) of amp^[d==0] exp(-g_dd (r - u dr)^2 / 2 s^2): the
+// exponent in FP64, exp in FP32 (dens.py:252-256, 299-308).  Cells past N
+// are zero.
 __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4* __restrict__ s_ir,
                          const double* __restrict__ s_r, const int* __restrict__ nvalid,
                          float* __restrict__ slices) {
 #pragma clang fp contract(off)
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = g / 3, d = g - 3 * (g / 3);
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int pair = (int)(g / kTile), i = (int)(g - (long long)pair * kTile);
+  const int s = pair / 3, d = pair - 3 * (pair / 3);
   if (s >= *nvalid) return;
   const int4 q = s_ir[s];
   const int t = q.w & 0xffff;
@@ -300,30 +303,22 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
   const double r = s_r[3 * (size_t)s + d];
   const int lo = ir - A, hi = ir + A;
   const int off_d = d == 0 ? 0 : (d == 1 ? kTile * ty.nsl[0] : kTile * (ty.nsl[0] + ty.nsl[1]));
-  float4* out = reinterpret_cast<float4*>(slices + ty.slice_base +
-                                          (long long)slot * ty.slice_stride + off_d);
+  float* out = slices + ty.slice_base + (long long)slot * ty.slice_stride + off_d + i;
   int T = pos_mod(lo, N) / kTile;
   for (int j = 0; j < nsl; ++j) {
-    for (int i4 = 0; i4 < kTile / 4; ++i4) {
-      float v[4];
+    const int c = T * kTile + i;
+    float acc = 0.f;
+    if (c < N) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int c = T * kTile + 4 * i4 + e;
-        float acc = 0.f;
-        if (c < N) {
-#pragma unroll
-          for (int sd = -1; sd <= 1; ++sd) {
-            const int u = c + sd * N;
-            if (u >= lo && u < hi) {
-              const double b = r - (double)u * dr;
-              acc += expf(-(float)(gk * (b * b)));
-            }
-          }
+      for (int sd = -1; sd <= 1; ++sd) {
+        const int u = c + sd * N;
+        if (u >= lo && u < hi) {
+          const double b = r - (double)u * dr;
+          acc += expf(-(float)(gk * (b * b)));
         }
-        v[e] = am * acc;
       }
-      out[j * (kTile / 4) + i4] = make_float4(v[0], v[1], v[2], v[3]);
     }
+    out[j * kTile] = am * acc;
     T = T + 1 == nt ? 0 : T + 1;
   }
 }
@@ -371,8 +366,8 @@ hipError_t launch_binning(Plan& p, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.s_idx, p.ir4, p.rpos,
                        p.s_ir, p.s_r, p.bucket_off + nbk);
     if (p.slice_floats > 0)
-      hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * n + 255) / 256)), dim3(256), 0, s,
-                         p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices);
+      hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0,
+                         s, p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices);
     // factor tables feed the FP64 gather
     const long long total = p.fft64 ? p.table_floats : 0;
     if (total > 0) {
