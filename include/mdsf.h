@@ -41,6 +41,8 @@ extern "C" {
 
 /* mdsf_grid_desc.flags */
 #define MDSF_FFT_FP64 1       /* deposition, density and FFT in FP64 (default FP32) */
+#define MDSF_AUTOTUNE 2       /* time the FFT tile / kernel choices for this grid at
+                                 plan creation (cached per grid shape and precision) */
 
 typedef struct mdsf_plan mdsf_plan;
 typedef void* mdsf_stream; /* a hipStream_t */
@@ -195,6 +197,40 @@ int mdsf_density_adjoint(mdsf_plan* plan, const void* d_density,
  * in plan precision: the transpose of deposition + fold (dens.py:283-311). */
 int mdsf_deposit_adjoint(mdsf_plan* plan, const void* d_grad_density,
                          double* d_grad, mdsf_stream stream);
+
+/* ---- batches of independent systems ---------------------------------------
+ * No counterpart in the reference, whose driver runs one system per call
+ * (main_gromacs.py:209-212).  A batch owns one plan per system (created with
+ * MDSF_AUTOTUNE); mdsf_batch_run runs every system's frame loop -- S(q)
+ * reset, then per frame the work of mdsf_accumulate_frames -- from one
+ * instantiated HIP graph, captured on the first run after the frames change
+ * and replayed afterwards.  Unlike mdsf_accumulate_frames the batch reads the
+ * coordinates without rescaling / wrapping them in place, so every replay
+ * sees the same input.  The device buffers passed to mdsf_batch_set_frames
+ * must stay valid while the batch may run. */
+typedef struct mdsf_batch mdsf_batch;
+
+int mdsf_batch_create(const mdsf_grid_desc* descs, int32_t nsys,
+                      mdsf_batch** out);
+int mdsf_batch_destroy(mdsf_batch* batch);
+
+/* The plan of system `sys` (for mdsf_sf_read, mdsf_outputs, ...); NULL if out
+ * of range. */
+mdsf_plan* mdsf_batch_plan(mdsf_batch* batch, int32_t sys);
+
+/* Frames of system `sys`, with the meaning of mdsf_accumulate_frames'
+ * arguments. */
+int mdsf_batch_set_frames(mdsf_batch* batch, int32_t sys,
+                          const void* d_coords, int dtype, int64_t nframes,
+                          const int32_t* d_types, const double* d_scale,
+                          const double box[3], int arith, int64_t wrap_lo,
+                          int64_t wrap_hi);
+
+/* Enqueue the whole batch on `stream` (one graph launch). */
+int mdsf_batch_run(mdsf_batch* batch, mdsf_stream stream);
+
+/* mdsf_check_errors over every system (synchronises the stream). */
+int mdsf_batch_check_errors(mdsf_batch* batch, mdsf_stream stream);
 
 /* Ewald-sphere sections of a kgridplt volume (plot2d.py:218-263): trilinear
  * interpolation of channel 3 at the points of the xy, xz and yz sections for

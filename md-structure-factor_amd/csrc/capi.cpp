@@ -25,12 +25,10 @@ int hip_fail(hipError_t e, const char* what) {
 
 namespace {
 
-int bucket_edge(int half) {
-  if (half < 24) return 16;
-  if (half < 48) return 32;
-  if (half < 96) return 64;
-  return 128;
-}
+// Bucket edge (cells) of a type's binning grid: wide enough that a tile's
+// candidate columns stay few, small enough that a bucket holds few atoms
+// (each bucket is sorted by one thread to fix the gather order).
+int bucket_edge(int half) { return half < 24 ? 16 : 32; }
 
 template <typename T>
 hipError_t dalloc(T** p, size_t count) {
@@ -46,7 +44,7 @@ void free_plan(Plan* p) {
                   p->tables,    p->err,       p->rho,        p->spec_nyq,   p->sf_main,
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
                   p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
-                  p->adj_w,     p->slices};
+                  p->adj_w,     p->slices,    p->s_tmp};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -179,6 +177,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     ty.slice_stride = sstride;
     ty.slice_base = slice_floats;
     slice_floats += (long long)in.count * sstride;
+    ty.corner_stride = kTile * (2 * ty.nsl[0] + 3 * ty.nsl[1] + 2 * ty.nsl[2]);
     ty.count = in.count;
     ty.type_start = start;
     ty.stride = stride;
@@ -195,6 +194,11 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   }
   s.nbuckets = bucket_base;
   p->table_floats = table_floats;
+  for (int t = 0; t < desc->ntypes; ++t) {  // corner components follow the regular slices
+    TypeDev& ty = p->types_host[t];
+    ty.corner_base = slice_floats;
+    slice_floats += (long long)ty.count * ty.corner_stride;
+  }
   p->slice_floats = (!p->fft64 && s.mode == MODE_DIAG) ? slice_floats : 0;
   // adjoint work items (atom, x-chunk of <= 64 planes, chunk of (y, z)
   // pairs sized for ~16k window voxels), heaviest windows numbered first so
@@ -251,6 +255,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   TRY(hipMemset(p->bucket_count, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
   TRY(hipMemset(p->bucket_off, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
   TRY(dalloc(&p->s_idx, nat));
+  TRY(dalloc(&p->s_tmp, nat));
   TRY(dalloc(&p->s_ir, nat));
   TRY(dalloc(&p->s_r, 3 * nat));
   if (p->fft64) {
@@ -303,6 +308,16 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     rc = hip_fail(e, "plan set-up");
     free_plan(p);
     return rc;
+  }
+  if (desc->flags & MDSF_AUTOTUNE) {
+    if ((e = fft_autotune(*p, nullptr)) != hipSuccess ||
+        (e = hipMemset(p->sf_main, 0, sizeof(double) * nx * ny * m)) != hipSuccess ||
+        (e = hipMemset(p->sf_nyq, 0, sizeof(double) * nx * ny)) != hipSuccess ||
+        (e = hipDeviceSynchronize()) != hipSuccess) {
+      rc = hip_fail(e, "FFT autotuning");
+      free_plan(p);
+      return rc;
+    }
   }
   *out = reinterpret_cast<mdsf_plan*>(p);
   return MDSF_OK;

@@ -675,6 +675,9 @@ __global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const 
     int4 q[NV];          // home cell, type | corner << 16
     int ok[NV];          // 0 only after a reported error
     long long so[MODE == MODE_DIAG ? NV : 1][3];  // tile slice offset per axis, -1: none
+    long long co[MODE == MODE_DIAG ? NV : 1][3];  // corner component 0 of that slice, -1: none
+    int cs[MODE == MODE_DIAG ? NV : 1][3];        // corner component stride (floats)
+    int clist[NV];                                // batch indices of the corner atoms
   } bt;
   // per-type constants: g_dd / (2 sigma^2) per axis and Nel / sigma^3
   __shared__ double tgk[kMaxTypes][3];
@@ -740,15 +743,21 @@ __global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const 
         // tile of the window start (same numbering as k_slices)
         const bool ok = slot >= 0 && slot < ty.count;
         long long off = ty.slice_base + (long long)slot * ty.slice_stride;
+        long long coff = ty.corner_base + (long long)slot * ty.corner_stride;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
           const int N = sys.n[d], nt = sys.ntiles[d];
           int j = tl.t0[d] / T - pmod(ir - ty.half[d], N) / T;
           if (j < 0) j += nt;
-          bt.so[lane][d] = (ok && j < ty.nsl[d]) ? off + (long long)T * j : -1LL;
+          const bool has = ok && j < ty.nsl[d];
+          bt.so[lane][d] = has ? off + (long long)T * j : -1LL;
+          bt.co[lane][d] = has ? coff + (long long)T * j : -1LL;
+          bt.cs[lane][d] = T * ty.nsl[d];
           off += (long long)T * ty.nsl[d];
+          coff += (long long)(d == 1 ? 3 : 2) * T * ty.nsl[d];
         }
+        if (isc) bt.clist[__popcll(cm & ((1ull << lane) - 1))] = lane;
       }
     }
     __syncthreads();
@@ -770,10 +779,28 @@ __global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const 
         st.vy[v][cc] = 0.f;
         st.vz[v][cc] = 0.f;
       }
+      // corner atoms: 4 virtual atoms (Xc, Yall, Zs), (Xn, Yc, Zs),
+      // (Xn, Yl, ZL), (Xn, Yh, ZH) from the regular slices and the corner
+      // components (x {C, L+H}, y {C, L, H}, z {ZL, ZH})
+      for (int it = tid; it < ncorner * 48; it += kThreads) {
+        const int ca = it / 48, r48 = it - 48 * ca;
+        const int v = r48 / 12, r12 = r48 - 12 * v, d = r12 >> 2, q4 = r12 & 3;
+        const int a = bt.clist[ca];
+        const long long cb = bt.co[a][d];
+        long long off;
+        if (d == 0) off = cb >= 0 ? cb + (v == 0 ? 0 : bt.cs[a][0]) : -1LL;
+        else if (d == 1) off = v == 0 ? bt.so[a][1] : (cb >= 0 ? cb + (long long)(v - 1) * bt.cs[a][1] : -1LL);
+        else off = v < 2 ? bt.so[a][2] : (cb >= 0 ? cb + (long long)(v - 2) * bt.cs[a][2] : -1LL);
+        float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (off >= 0) val = *reinterpret_cast<const float4*>(slices + off + 4 * q4);
+        const int vb = bt.vbase[a] + v;
+        float* row = d == 0 ? st.vx[vb] : (d == 1 ? st.vy[vb] : st.vz[vb]);
+        *reinterpret_cast<float4*>(row + 4 * q4) = val;
+      }
     }
-    // staging by evaluation: wave d < 3 builds axis d for every (atom, cell)
-    // of the batch (DIAG: corner atoms only, and only when the batch has one)
-    if (wave < 3 && (MODE != MODE_DIAG || ncorner > 0)) {
+    // staging by evaluation (xy-coupled cells): wave d < 3 builds axis d for
+    // every (atom, cell) of the batch
+    if (MODE == MODE_XY && wave < 3) {
       const int d = wave;
       // this wave's axis (from LDS: a runtime index into the tile's arrays
       // would otherwise go through private memory)

@@ -15,6 +15,9 @@
 // P x pitch values (pitch odd) twice (ping-pong between stages), and threads
 // are mapped pencil-fastest, so every Stockham read and write is
 // conflict-free for any radix and span.
+#include <map>
+#include <mutex>
+
 #include "mdsf_internal.h"
 
 namespace mdsf {
@@ -667,11 +670,14 @@ void launch_reg(int n, dim3 g, dim3 b, size_t lds, hipStream_t s, const PencilAr
 // job[0] runs over outer0 outer groups, job[1] over outer1.  job[0] uses the
 // register-staged kernel when n has a supported split and its pencils are
 // adjacent in memory; job[1] (the Nyquist plane) always the generic tile.
+// `choice`: 0 = default (register-staged when available), kTuneReg = the
+// register-staged kernel, 1..16 = the generic LDS tile with that many pencils.
 template <typename R>
 hipError_t run_pencils(PencilArgs<R> a, long long outer0, long long outer1, bool inv, int epi,
-                       hipStream_t s) {
+                       hipStream_t s, int choice = 0) {
   int r1, r2, r3;
-  bool reg = reg_split(a.n, &r1, &r2, &r3) && !a.job[0].pencil_major;
+  bool reg = (choice == 0 || choice == kTuneReg) && reg_split(a.n, &r1, &r2, &r3) &&
+             !a.job[0].pencil_major;
   if (reg) {
     const size_t need =
         (size_t)reg_tile_pencils<R>(r3 == 1 ? 2 : 3) * (a.n + 1) * (r3 == 1 ? 2 : 4) * sizeof(R);
@@ -683,7 +689,9 @@ hipError_t run_pencils(PencilArgs<R> a, long long outer0, long long outer1, bool
   for (int j = 0; j < 2; ++j) {
     PencilJob<R>& J = a.job[j];
     const bool use_reg = reg && j == 0;
-    J.P = use_reg ? reg_tile_pencils<R>(r3 == 1 ? 2 : 3) : tile_pencils<R>(a.n);
+    const bool forced = j == 0 && choice >= 1 && choice <= 16 &&
+                        tile_lds<R>(a.n, choice) <= max_lds_per_block();
+    J.P = use_reg ? reg_tile_pencils<R>(r3 == 1 ? 2 : 3) : (forced ? choice : tile_pencils<R>(a.n));
     J.tiles_inner = (J.pencils + J.P - 1) / J.P;
     blocks[j] = J.pencils > 0 ? outer[j] * J.tiles_inner : 0;
     if (blocks[j] > 0) {
@@ -726,7 +734,9 @@ hipError_t pass_z(Plan& p, const void* density, hipStream_t s) {
   a.nyq = spec_nyq<R>(p);
   a.rows = (long long)nx * ny;
   a.m = m;
-  a.P = tile_pencils<R>(m);
+  a.P = (p.tune_z >= 1 && p.tune_z <= 16 && tile_lds<R>(m, p.tune_z) <= max_lds_per_block())
+            ? p.tune_z
+            : tile_pencils<R>(m);
   a.ax = p.ax_zc;
   a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
   const long long blocks = (a.rows + a.P - 1) / a.P;
@@ -754,7 +764,7 @@ hipError_t pass_y(Plan& p, bool inv, hipStream_t s) {
   K.pstride = ny;
   K.pencils = nx;
   K.pencil_major = 1;
-  return run_pencils(a, nx, 1, inv, EPI_STORE, s);
+  return run_pencils(a, nx, 1, inv, EPI_STORE, s, p.tune_y);
 }
 
 // x pass: main (pencils = (y, kz), elements = x) + nyq (pencils = y).  Forward
@@ -781,7 +791,7 @@ hipError_t pass_x(Plan& p, bool inv, bool accumulate, const double* w_main, cons
   K.estride = ny;
   K.pstride = 1;
   K.pencils = ny;
-  return run_pencils(a, 1, 1, inv, accumulate ? EPI_ACC : EPI_STORE, s);
+  return run_pencils(a, 1, 1, inv, accumulate ? EPI_ACC : EPI_STORE, s, p.tune_x);
 }
 
 template <typename R>
@@ -867,6 +877,116 @@ hipError_t fft_make_twiddles(float2* tw, double2* tw64, int n, hipStream_t s) {
 hipError_t fft_forward(Plan& p, const void* density, bool accumulate, hipStream_t s) {
   return p.fft64 ? forward_impl<double>(p, density, accumulate, s)
                  : forward_impl<float>(p, density, accumulate, s);
+}
+
+namespace {
+
+struct TuneKey {
+  int n[3];
+  int fft64;
+  bool operator<(const TuneKey& o) const {
+    for (int d = 0; d < 3; ++d)
+      if (n[d] != o.n[d]) return n[d] < o.n[d];
+    return fft64 < o.fft64;
+  }
+};
+struct TuneVal {
+  int z, y, x;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune_cache;
+
+// Best of 3 timed launches (after one warm-up) of `run` on stream s, in ms.
+template <typename F>
+float time_best(F&& run, hipStream_t s, hipEvent_t e0, hipEvent_t e1, hipError_t* err) {
+  float best = 1e30f;
+  for (int it = 0; it < 4; ++it) {
+    (void)hipEventRecord(e0, s);
+    const hipError_t e = run();
+    (void)hipEventRecord(e1, s);
+    if (e != hipSuccess) {
+      *err = e;
+      return best;
+    }
+    if ((*err = hipEventSynchronize(e1)) != hipSuccess) return best;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (it > 0 && ms < best) best = ms;
+  }
+  return best;
+}
+
+template <typename R>
+hipError_t autotune_impl(Plan& p, hipStream_t s) {
+  const int m = p.sys.n[2] / 2;
+  hipEvent_t e0, e1;
+  hipError_t err = hipEventCreate(&e0);
+  if (err != hipSuccess) return err;
+  if ((err = hipEventCreate(&e1)) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return err;
+  }
+  const int cand_p[4] = {2, 4, 8, 16};
+  // z pass: rows per tile
+  float best = 1e30f;
+  int pick = 0;
+  for (int P : cand_p) {
+    if (tile_lds<R>(m, P) > max_lds_per_block()) continue;
+    p.tune_z = P;
+    const float t = time_best([&] { return pass_z<R>(p, p.rho, s); }, s, e0, e1, &err);
+    if (err != hipSuccess) break;
+    if (t < best) {
+      best = t;
+      pick = P;
+    }
+  }
+  p.tune_z = pick;
+  // y and x passes: the register-staged kernel or the generic tile
+  const int cand_c[5] = {kTuneReg, 2, 4, 8, 16};
+  for (int axis = 0; axis < 2 && err == hipSuccess; ++axis) {
+    int& slot = axis == 0 ? p.tune_y : p.tune_x;
+    best = 1e30f;
+    pick = 0;
+    for (int c : cand_c) {
+      slot = c;
+      const float t = time_best(
+          [&] { return axis == 0 ? pass_y<R>(p, false, s) : pass_x<R>(p, false, true, nullptr, nullptr, s); },
+          s, e0, e1, &err);
+      if (err != hipSuccess) break;
+      if (t < best) {
+        best = t;
+        pick = c;
+      }
+    }
+    slot = pick;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return err;
+}
+
+}  // namespace
+
+// Picks the z tile height and the y / x pencil kernels for this grid by timing
+// the candidates on the plan's own buffers (results cached per grid shape and
+// precision).  Clobbers the density, spectrum and S(q) accumulators.
+hipError_t fft_autotune(Plan& p, hipStream_t s) {
+  const TuneKey key{{p.sys.n[0], p.sys.n[1], p.sys.n[2]}, p.fft64};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      p.tune_z = it->second.z;
+      p.tune_y = it->second.y;
+      p.tune_x = it->second.x;
+      return hipSuccess;
+    }
+  }
+  const hipError_t e = p.fft64 ? autotune_impl<double>(p, s) : autotune_impl<float>(p, s);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = TuneVal{p.tune_z, p.tune_y, p.tune_x};
+  return hipSuccess;
 }
 
 hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, void* out,

@@ -14,6 +14,7 @@ constexpr int kMaxTypes = 64;
 constexpr int kTile = 16;       // deposition tile edge along x, y and z
 constexpr int kBatch = 32;      // atoms staged per deposition batch
 constexpr int kMaxStages = 16;  // FFT stages per axis
+constexpr int kTuneReg = 100;   // FFT pencil choice: register-staged kernel
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
 enum Mode { MODE_DIAG = 0, MODE_XY = 1 };
@@ -40,6 +41,11 @@ struct TypeDev {
   int nsl[3];
   int slice_stride;        // floats per atom: 16 * (nsl[0] + nsl[1] + nsl[2])
   long long slice_base;    // float offset of this type's slices
+  // corner components (atoms whose window leaves the cell on all three axes):
+  // x {C, L+H}, y {C, L, H}, z {z slot for y on L, z slot for y on H}, each
+  // nsl[d] slices per component
+  int corner_stride;       // floats per atom: 16 * (2 nsl[0] + 3 nsl[1] + 2 nsl[2])
+  long long corner_base;   // float offset of this type's corner components
   // adjoint work items: per atom adj_nxc x-chunks times adj_npc chunks of
   // adj_ppi (y, z) pairs, numbered from adj_item_base
   int adj_nxc;
@@ -86,12 +92,13 @@ struct Plan {
   int* bucket_off = nullptr;
   int* bucket_cur = nullptr;
   int* bucket_aux = nullptr;     // per-1024-block totals of the bucket scan
-  int* s_idx = nullptr;
+  int* s_tmp = nullptr;          // bucket fill (arbitrary order within a bucket)
+  int* s_idx = nullptr;          // atom indices sorted by (type, bucket, index)
   int4* s_ir = nullptr;
   double* s_r = nullptr;
   void* tables = nullptr;         // per-atom factor tables (FP64 plans)
   float* slices = nullptr;        // per-atom tile slices (FP32 plans, orthogonal-axis cells)
-  long long slice_floats = 0;
+  long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
   // grids
   void* rho = nullptr;           // Nx*Ny*Nz folded density (FP32, or FP64 when fft64)
@@ -102,6 +109,9 @@ struct Plan {
   // FFT
   int fft64 = 0;                 // deposition, density and FFT in FP64 (else FP32)
   FftAxis ax_x, ax_y, ax_zc;     // zc: complex length Nz/2
+  // FFT launch choices (0 = default): z rows per tile; y / x pencil kernel
+  // (kTuneReg = register-staged, 1..16 = generic LDS tile with that many pencils)
+  int tune_z = 0, tune_y = 0, tune_x = 0;
   float2* tw_r2c = nullptr;      // W_Nz^k, k < Nz
   double2* tw_r2c64 = nullptr;
   // adjoint (allocated on first use)
@@ -125,7 +135,7 @@ int hip_fail(hipError_t e, const char* what);
 hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_t* types,
                        const double* scale3, const double box[3], int64_t wrap_lo,
                        int64_t wrap_hi, uint32_t* occ, int32_t* home, hipStream_t s,
-                       bool coords_only = false);
+                       bool coords_only = false, bool write_back = true);
 hipError_t launch_binning(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);
 
@@ -142,6 +152,7 @@ hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s);
 // stored spectrum -> complex128 (Nx, Ny, Nz/2+1)
 hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s);
 bool fft_factor(int n, int* radix, int* nstages);
+hipError_t fft_autotune(Plan& p, hipStream_t s);
 
 hipError_t launch_fill_zero(double* p, long long n, hipStream_t s);
 // adj_w <- 2 dsf split into main / Nyquist parts

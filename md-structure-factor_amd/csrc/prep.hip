@@ -27,7 +27,8 @@ __global__ void k_prep(T* __restrict__ coords, SysDev sys, const TypeDev* __rest
                        double L0, double L1, double L2, long long wlo, long long whi,
                        int4* __restrict__ ir4, int* __restrict__ key, double* __restrict__ rpos,
                        int* __restrict__ bcount, unsigned* __restrict__ occ,
-                       int* __restrict__ home, int* __restrict__ err, int coords_only) {
+                       int* __restrict__ home, int* __restrict__ err, int coords_only,
+                       int write_back) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= sys.natoms) return;
   T x[3];
@@ -49,9 +50,11 @@ __global__ void k_prep(T* __restrict__ coords, SysDev sys, const TypeDev* __rest
       if (!(v > (AR)0)) x[d] = (T)(v + box[d]);
     }
   }
-  coords[3 * (size_t)i + 0] = x[0];
-  coords[3 * (size_t)i + 1] = x[1];
-  coords[3 * (size_t)i + 2] = x[2];
+  if (write_back) {
+    coords[3 * (size_t)i + 0] = x[0];
+    coords[3 * (size_t)i + 1] = x[1];
+    coords[3 * (size_t)i + 2] = x[2];
+  }
   if (coords_only) return;
 
   const int t = types[i];
@@ -160,55 +163,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(int* __restrict__ bof
 }
 
 __global__ void k_fill(const int* __restrict__ key, int* __restrict__ bcur,
-                       int* __restrict__ s_idx, int natoms) {
+                       int* __restrict__ s_tmp, int natoms) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natoms) return;
   const int b = key[i];
   if (b < 0) return;
   const int pos = atomicAdd(&bcur[b], 1);
-  s_idx[pos] = i;
+  s_tmp[pos] = i;
 }
 
-// Restores atom-index order inside each bucket so that the gather visits
-// atoms in a canonical order (bitwise-reproducible sums).  A workgroup sorts
-// 256 consecutive buckets; their entries are contiguous and are sorted in LDS
-// when they fit, in place in global memory otherwise.
-constexpr int kSortThreads = 256;
-constexpr int kSortCap = 8192;
-__global__ __launch_bounds__(kSortThreads) void k_bucket_sort(const int* __restrict__ boff,
-                                                              int* __restrict__ s_idx, int nbk) {
-  __shared__ int buf[kSortCap];
-  const int b0 = blockIdx.x * kSortThreads;
-  const int b = b0 + threadIdx.x;
-  const int bend = min(b0 + kSortThreads, nbk);
-  const int base = boff[b0], top = boff[bend];
-  const int n = top - base;
-  const int lo = b < nbk ? boff[b] : top, hi = b < nbk ? boff[b + 1] : top;
-  if (n <= kSortCap) {
-    for (int i = threadIdx.x; i < n; i += kSortThreads) buf[i] = s_idx[base + i];
-    __syncthreads();
-    for (int i = lo + 1; i < hi; ++i) {
-      const int v = buf[i - base];
-      int j = i - 1;
-      while (j >= lo && buf[j - base] > v) {
-        buf[j + 1 - base] = buf[j - base];
-        --j;
-      }
-      buf[j + 1 - base] = v;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += kSortThreads) s_idx[base + i] = buf[i];
-  } else {
-    for (int i = lo + 1; i < hi; ++i) {
-      const int v = s_idx[i];
-      int j = i - 1;
-      while (j >= lo && s_idx[j] > v) {
-        s_idx[j + 1] = s_idx[j];
-        --j;
-      }
-      s_idx[j + 1] = v;
-    }
-  }
+// Restores atom-index order inside each bucket, so that the gather visits
+// atoms in a canonical order (bitwise-reproducible sums): the entry at
+// position p of bucket b moves to boff[b] + (number of entries of b with a
+// smaller atom index).  One thread per entry; the entries of a bucket are
+// read by the threads of that bucket together (mostly the same wavefront).
+__global__ void k_rank_sort(const int* __restrict__ key, const int* __restrict__ boff,
+                            const int* __restrict__ s_tmp, int* __restrict__ s_idx,
+                            const int* __restrict__ nvalid) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= *nvalid) return;
+  const int i = s_tmp[p];
+  const int b = key[i];
+  const int lo = boff[b], hi = boff[b + 1];
+  int r = 0;
+#pragma unroll 4
+  for (int q = lo; q < hi; ++q) r += s_tmp[q] < i ? 1 : 0;
+  s_idx[lo + r] = i;
 }
 
 // Gathers atom data into sorted order (valid sorted positions only).
@@ -304,21 +284,37 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
   const int lo = ir - A, hi = ir + A;
   const int off_d = d == 0 ? 0 : (d == 1 ? kTile * ty.nsl[0] : kTile * (ty.nsl[0] + ty.nsl[1]));
   float* out = slices + ty.slice_base + (long long)slot * ty.slice_stride + off_d + i;
+  const bool corner = (q.w >> 16) != 0;
+  const int coff_d = d == 0 ? 0 : (d == 1 ? 2 * kTile * ty.nsl[0]
+                                          : 2 * kTile * ty.nsl[0] + 3 * kTile * ty.nsl[1]);
+  float* cout = slices + ty.corner_base + (long long)slot * ty.corner_stride + coff_d + i;
+  auto f = [&](int u) -> float {  // factor of the image at grid coordinate u, 0 outside the window
+    if (u < lo || u >= hi) return 0.f;
+    const double b = r - (double)u * dr;
+    return am * expf(-(float)(gk * (b * b)));
+  };
   int T = pos_mod(lo, N) / kTile;
   for (int j = 0; j < nsl; ++j) {
     const int c = T * kTile + i;
-    float acc = 0.f;
-    if (c < N) {
-#pragma unroll
-      for (int sd = -1; sd <= 1; ++sd) {
-        const int u = c + sd * N;
-        if (u >= lo && u < hi) {
-          const double b = r - (double)u * dr;
-          acc += expf(-(float)(gk * (b * b)));
-        }
+    const bool in = c < N;
+    const float vL = in ? f(c - N) : 0.f, vC = in ? f(c) : 0.f, vH = in ? f(c + N) : 0.f;
+    out[j * kTile] = vC + vL + vH;
+    if (corner) {
+      // the components the corner virtual atoms take (dens.py:101-107): on z
+      // the slots where the reference's corner copy lands for y on L / on H
+      if (d == 0) {
+        cout[j * kTile] = vC;
+        cout[(nsl + j) * kTile] = vL + vH;
+      } else if (d == 1) {
+        cout[j * kTile] = vC;
+        cout[(nsl + j) * kTile] = vL;
+        cout[(2 * nsl + j) * kTile] = vH;
+      } else {
+        const int u1 = c + sys.nb, u2 = c - sys.nb;
+        cout[j * kTile] = vC + vL + (in && u1 >= N ? f(u1) : 0.f);
+        cout[(nsl + j) * kTile] = vC + vH + (in && u2 < 0 ? f(u2) : 0.f);
       }
     }
-    out[j * kTile] = am * acc;
     T = T + 1 == nt ? 0 : T + 1;
   }
 }
@@ -328,23 +324,24 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
 hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_t* types,
                        const double* scale3, const double box[3], int64_t wrap_lo,
                        int64_t wrap_hi, uint32_t* occ, int32_t* home, hipStream_t s,
-                       bool coords_only) {
+                       bool coords_only, bool write_back) {
   const int n = p.sys.natoms;
   const int co = coords_only ? 1 : 0;
+  const int wb = write_back ? 1 : 0;
   if (n == 0) return hipSuccess;
   const dim3 grid((n + 255) / 256), block(256);
   if (dtype == MDSF_F32 && arith == MDSF_F32) {
     hipLaunchKernelGGL((k_prep<float, float>), grid, block, 0, s, (float*)coords, p.sys, p.types,
                        types, scale3, box[0], box[1], box[2], (long long)wrap_lo,
-                       (long long)wrap_hi, p.ir4, p.key, p.rpos, p.bucket_count, occ, home, p.err, co);
+                       (long long)wrap_hi, p.ir4, p.key, p.rpos, p.bucket_count, occ, home, p.err, co, wb);
   } else if (dtype == MDSF_F32) {
     hipLaunchKernelGGL((k_prep<float, double>), grid, block, 0, s, (float*)coords, p.sys, p.types,
                        types, scale3, box[0], box[1], box[2], (long long)wrap_lo,
-                       (long long)wrap_hi, p.ir4, p.key, p.rpos, p.bucket_count, occ, home, p.err, co);
+                       (long long)wrap_hi, p.ir4, p.key, p.rpos, p.bucket_count, occ, home, p.err, co, wb);
   } else {
     hipLaunchKernelGGL((k_prep<double, double>), grid, block, 0, s, (double*)coords, p.sys,
                        p.types, types, scale3, box[0], box[1], box[2], (long long)wrap_lo,
-                       (long long)wrap_hi, p.ir4, p.key, p.rpos, p.bucket_count, occ, home, p.err, co);
+                       (long long)wrap_hi, p.ir4, p.key, p.rpos, p.bucket_count, occ, home, p.err, co, wb);
   }
   return hipGetLastError();
 }
@@ -360,9 +357,9 @@ hipError_t launch_binning(Plan& p, hipStream_t s) {
                      p.bucket_aux, nbk, nblk);
   if (n > 0) {
     hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
-                       p.s_idx, n);
-    hipLaunchKernelGGL(k_bucket_sort, dim3((nbk + kSortThreads - 1) / kSortThreads),
-                       dim3(kSortThreads), 0, s, p.bucket_off, p.s_idx, nbk);
+                       p.s_tmp, n);
+    hipLaunchKernelGGL(k_rank_sort, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_off,
+                       p.s_tmp, p.s_idx, p.bucket_off + nbk);
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.s_idx, p.ir4, p.rpos,
                        p.s_ir, p.s_r, p.bucket_off + nbk);
     if (p.slice_floats > 0)

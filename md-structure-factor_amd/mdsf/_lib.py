@@ -18,6 +18,7 @@ ERR_UNSUPPORTED = 5
 F32 = 0
 F64 = 1
 FFT_FP64 = 1
+AUTOTUNE = 2
 
 c_int32 = ctypes.c_int32
 c_int64 = ctypes.c_int64
@@ -73,6 +74,15 @@ SIGNATURES = {
                                       c_void_p, c_void_p, c_void_p]),
     "mdsf_density_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "mdsf_deposit_adjoint": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mdsf_batch_create": (c_int32, [ctypes.POINTER(GridDesc), c_int32,
+                                    ctypes.POINTER(c_void_p)]),
+    "mdsf_batch_destroy": (c_int32, [c_void_p]),
+    "mdsf_batch_plan": (c_void_p, [c_void_p, c_int32]),
+    "mdsf_batch_set_frames": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_int64, c_void_p,
+                                        c_void_p, ctypes.POINTER(c_double), c_int32, c_int64,
+                                        c_int64]),
+    "mdsf_batch_run": (c_int32, [c_void_p, c_void_p]),
+    "mdsf_batch_check_errors": (c_int32, [c_void_p, c_void_p]),
 }
 
 

@@ -15,7 +15,8 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 INCLUDE = os.path.join(os.path.dirname(PKG_ROOT), "include")
 BUILD_DIR = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(HERE, "libmdsf.so")
-SOURCES = ["capi.cpp", "prep.hip", "deposit.hip", "fft.hip", "post.hip", "adjoint.hip"]
+SOURCES = ["capi.cpp", "batch.cpp", "prep.hip", "deposit.hip", "fft.hip", "post.hip",
+           "adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MDSF_OFFLOAD_ARCH", "gfx950")

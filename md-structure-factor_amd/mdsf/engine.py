@@ -82,46 +82,76 @@ class Setup:
                     "(dens.py:86-108) needs Nborder <= N; use a finer Sres")
 
 
+def _device(device):
+    """``None`` (current device), an index or a ``torch.device`` -> torch.device('cuda', i)."""
+    if device is None:
+        return torch.device("cuda", torch.cuda.current_device())
+    if isinstance(device, torch.device):
+        return torch.device("cuda", device.index if device.index is not None
+                            else torch.cuda.current_device())
+    return torch.device("cuda", int(device))
+
+
+def grid_desc(n, nborder, dr, ucell, amp, two_sig2, half, counts, flags=0, desc=None):
+    """Fill an ``mdsf_grid_desc``; returns (desc, type array) -- keep both alive."""
+    ntypes = len(amp)
+    tarr = (_lib.TypeDesc * max(ntypes, 1))()
+    for t in range(ntypes):
+        tarr[t].amp = float(amp[t])
+        tarr[t].two_sig2 = float(two_sig2[t])
+        for d in range(3):
+            tarr[t].half[d] = int(half[t][d])
+        tarr[t].count = int(counts[t])
+    desc = _lib.GridDesc() if desc is None else desc
+    for d in range(3):
+        desc.n[d] = int(n[d])
+        desc.dr[d] = float(dr[d])
+    desc.nborder = int(nborder)
+    u = np.asarray(ucell, dtype=np.float64).reshape(9)
+    for i in range(9):
+        desc.ucell[i] = float(u[i])
+    desc.ntypes = ntypes
+    desc.types = tarr
+    desc.natoms = int(np.sum(counts))
+    desc.flags = int(flags)
+    return desc, tarr
+
+
+def setup_desc(s, flags=0, desc=None):
+    return grid_desc(s.N, s.nborder, s.dr, s.ucell, s.amp, s.two_sig2, s.half, s.counts, flags,
+                     desc)
+
+
 class Plan:
-    """One system (grid, border, cell, types) bound to a libmdsf plan on a GPU."""
+    """One system (grid, border, cell, types) bound to a libmdsf plan on a GPU.
+
+    ``fft64`` selects FP64 deposition and FFT; ``autotune`` times the FFT tile
+    and kernel choices for this grid when the plan is created.
+    """
 
     def __init__(self, n, nborder, dr, ucell, amp, two_sig2, half, counts, device=None,
-                 fft64=False):
+                 fft64=False, autotune=False, _handle=None):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise RuntimeError("libmdsf needs a ROCm GPU (torch.cuda.is_available() is False)")
-        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self.device = _device(device)
         self.n = [int(x) for x in n]
         self.natoms = int(np.sum(counts))
-        ntypes = len(amp)
-        tarr = (_lib.TypeDesc * max(ntypes, 1))()
-        for t in range(ntypes):
-            tarr[t].amp = float(amp[t])
-            tarr[t].two_sig2 = float(two_sig2[t])
-            for d in range(3):
-                tarr[t].half[d] = int(half[t][d])
-            tarr[t].count = int(counts[t])
-        desc = _lib.GridDesc()
-        for d in range(3):
-            desc.n[d] = self.n[d]
-            desc.dr[d] = float(dr[d])
-        desc.nborder = int(nborder)
-        u = np.asarray(ucell, dtype=np.float64).reshape(9)
-        for i in range(9):
-            desc.ucell[i] = float(u[i])
-        desc.ntypes = ntypes
-        desc.types = tarr
-        desc.natoms = self.natoms
-        desc.flags = _lib.FFT_FP64 if fft64 else 0
         self.fft64 = bool(fft64)
+        self._owned = _handle is None
+        if _handle is not None:
+            self._handle = _lib.c_void_p(_handle)
+            return
+        flags = (_lib.FFT_FP64 if fft64 else 0) | (_lib.AUTOTUNE if autotune else 0)
+        desc, self._tarr = grid_desc(n, nborder, dr, ucell, amp, two_sig2, half, counts, flags)
         self._handle = _lib.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.mdsf_plan_create(desc, self._handle))
 
     @classmethod
-    def from_setup(cls, s, device=None, fft64=False):
+    def from_setup(cls, s, device=None, fft64=False, autotune=False):
         return cls(s.N, s.nborder, s.dr, s.ucell, s.amp, s.two_sig2, s.half, s.counts, device,
-                   fft64)
+                   fft64, autotune)
 
     # ----------------------------------------------------------------- utils
     def _stream(self):
@@ -133,7 +163,8 @@ class Plan:
 
     def close(self):
         if getattr(self, "_handle", None) is not None and self._handle.value:
-            self.lib.mdsf_plan_destroy(self._handle)
+            if getattr(self, "_owned", True):
+                self.lib.mdsf_plan_destroy(self._handle)
             self._handle = _lib.c_void_p()
 
     def __del__(self):
@@ -261,6 +292,69 @@ class Plan:
             self._handle, self._ptr(sf), _lib.doubles(box), 1 if box_f32 else 0,
             self._ptr(sfplt), self._ptr(kgrid), self._ptr(kgridplt), self._stream()))
         return sfplt, kgrid, kgridplt
+
+
+class Batch:
+    """Independent systems (each a ``Setup``) run together from one HIP graph.
+
+    Every system gets an autotuned FP32 plan.  ``set_frames`` binds device
+    coordinates (never modified by the batch); ``run`` enqueues all systems'
+    frame loops as one graph launch (captured on the first run after the
+    frames change); ``sf(i)`` reads system i's S(q).
+    """
+
+    def __init__(self, setups, device=None):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("libmdsf needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = _device(device)
+        self.setups = list(setups)
+        descs = (_lib.GridDesc * len(self.setups))()
+        keep = []
+        for i, s in enumerate(self.setups):
+            _, tarr = setup_desc(s, 0, descs[i])
+            keep.append(tarr)
+        self._handle = _lib.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.mdsf_batch_create(descs, len(self.setups), self._handle))
+        self.plans = [Plan(s.N, 0, s.dr, s.ucell, [], [], [], s.counts, self.device,
+                           _handle=self.lib.mdsf_batch_plan(self._handle, i))
+                      for i, s in enumerate(self.setups)]
+        self._bound = [None] * len(self.setups)
+
+    def _stream(self):
+        return _lib.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def set_frames(self, i, coords, types, scale):
+        s = self.setups[i]
+        dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
+        self._bound[i] = (coords, types, scale)   # keep the device buffers alive
+        _lib.check(self.lib.mdsf_batch_set_frames(
+            self._handle, i, _lib.c_void_p(coords.data_ptr()), dtype, coords.shape[0],
+            _lib.c_void_p(types.data_ptr()), _lib.c_void_p(scale.data_ptr()),
+            _lib.doubles([float(x) for x in s.L]), s.arith, int(s.wrap[0]), int(s.wrap[1])))
+
+    def run(self):
+        _lib.check(self.lib.mdsf_batch_run(self._handle, self._stream()))
+
+    def check_errors(self):
+        _lib.check(self.lib.mdsf_batch_check_errors(self._handle, self._stream()))
+
+    def sf(self, i):
+        return self.plans[i].sf()
+
+    def close(self):
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            for p in self.plans:
+                p._handle = _lib.c_void_p()
+            self.lib.mdsf_batch_destroy(self._handle)
+            self._handle = _lib.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ewald_slices(kgridplt, wavelength):

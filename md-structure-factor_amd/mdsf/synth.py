@@ -12,6 +12,8 @@
                perturbed by N(0, 0.5) (grid 128^3 at Sres = 60/128).
 * ``large``  : 1,000,000 H/C/N/O atoms uniform in a 200 box, fresh frames
                (grid 512^3 at Sres = 200/512).
+* ``batch``  : 32 independent cubic systems, grids {64..192}^3, boxes 30-80,
+               5k-60k H/C/O/Na atoms, fresh frames.
 """
 import numpy as np
 
@@ -142,3 +144,25 @@ def large(seed=5005, natoms=1_000_000, nframes=1, dtype=np.float64):
 
 def large_sres():
     return LARGE_BOX / 512.0
+
+
+BATCH_GRIDS = (64, 96, 128, 160, 192)
+
+
+def batch(seed=3003, nsys=32, nframes=50, grids=BATCH_GRIDS, box=(30.0, 80.0),
+          natoms=(5000, 60000), names=("H", "C", "O", "Na"), dtype=np.float32):
+    """Independent cubic systems of uneven size: per system a grid n from
+    ``grids``, a box L ~ U[box], natoms ~ U{natoms}, types uniform over
+    ``names``, fresh uniform frames.  Returns a list of (coords, dims, typ, sres)
+    with sres = L / (n + 1/2), so that int(L / sres) = n exactly."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nsys):
+        n = int(rng.choice(grids))
+        L = float(rng.uniform(*box))
+        na = int(rng.integers(natoms[0], natoms[1] + 1))
+        typ = [names[i] for i in rng.integers(0, len(names), size=na)]
+        coords = rng.uniform(0.0, L, size=(nframes, na, 3)).astype(dtype)
+        dims = np.full((nframes, 3), L)
+        out.append((coords, dims, typ, L / (n + 0.5)))
+    return out
