@@ -429,17 +429,20 @@ __device__ void accumulate_xy(const StageXY<ACC>& st, int nbatch, int wave, int 
 // Candidate enumeration shared by both accumulation paths.
 constexpr int kListCap = 320;  // trigger (<= 64) - 1 + one 256-candidate chunk, rounded up
 
+// MT: the most atom types the block handles (LDS is sized for it)
+template <int MT>
 struct EnumShared {
   int list[kListCap];  // candidate sorted index | corner << 31
   int wsum[kThreads / 64];
   int col_s[kThreads], col_pre[kThreads + 1], col_t[kThreads];
-  int half[kMaxTypes][3];
-  TypeRanges tr[kMaxTypes];
-  int col_base[kMaxTypes + 1];
+  int half[MT][3];
+  TypeRanges tr[MT];
+  int col_base[MT + 1];
 };
 
 // block-wide exclusive scan of one int per thread; returns the total
-__device__ __forceinline__ int block_scan(EnumShared& sh, int v, int* excl) {
+template <class SH>
+__device__ __forceinline__ int block_scan(SH& sh, int v, int* excl) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   int x = v;
 #pragma unroll
@@ -462,7 +465,8 @@ __device__ __forceinline__ int block_scan(EnumShared& sh, int v, int* excl) {
 }
 
 // drop the first n list entries (n <= count <= kListCap)
-__device__ __forceinline__ void list_shift(EnumShared& sh, int n, int count) {
+template <class SH>
+__device__ __forceinline__ void list_shift(SH& sh, int n, int count) {
   const int tid = threadIdx.x;
   const int rem = count - n;
   __syncthreads();
@@ -476,10 +480,10 @@ __device__ __forceinline__ void list_shift(EnumShared& sh, int n, int count) {
 
 // Streams every atom whose window reaches the tile into sh.list in a fixed
 // order; `process(count)` consumes a prefix of the list and returns its size.
-template <typename Process>
+template <class SH, typename Process>
 __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td,
                                const int* __restrict__ boff, const int4* __restrict__ s_ir,
-                               const Tile& tl, EnumShared& sh, int trigger, Process&& process) {
+                               const Tile& tl, SH& sh, int trigger, Process&& process) {
   const int tid = threadIdx.x;
   for (int i = tid; i < sys.ntypes * 3; i += kThreads) sh.half[i / 3][i % 3] = td[i / 3].half[i % 3];
   for (int t = tid; t < sys.ntypes; t += kThreads) type_ranges(td[t], sys, tl.t0, tl.t1, sh.tr[t]);
@@ -588,7 +592,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
                                                       ACC* __restrict__ rho) {
   using Stage = typename std::conditional<MODE == MODE_DIAG, StageDiag<ACC>, StageXY<ACC>>::type;
   __shared__ Stage st;
-  __shared__ EnumShared sh;
+  __shared__ EnumShared<kMaxTypes> sh;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   Tile tl;
@@ -643,7 +647,6 @@ struct StageM {
   __align__(16) float vx[NV][T];
   __align__(16) float vy[NV][T];
   __align__(16) float vz[NV][T];
-  int vxm[NV];
 };
 
 struct StageMXY {
@@ -656,10 +659,11 @@ struct StageMXY {
 };
 
 // The gather is latency-bound (batch staging, candidate streaming): the
-// second bound asks for 5 waves per SIMD, which the register budget allows
-// (no spills) and the 29 KB LDS footprint matches (5 blocks per CU).
-template <int MODE>
-__global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
+// orthogonal-cell kernel asks for 6 waves per SIMD, which its register budget
+// allows without spills and its LDS footprint (< 26.6 KB) matches.
+template <int MODE, int MT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == MODE_DIAG ? 6 : 1, 8)))
+void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const int* __restrict__ boff,
                                                            const int4* __restrict__ s_ir,
                                                            const double* __restrict__ s_r,
@@ -668,9 +672,11 @@ __global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const 
   using Stage = typename std::conditional<MODE == MODE_DIAG, StageM, StageMXY>::type;
   constexpr int EXTRA = MODE == MODE_DIAG ? 3 : 2;  // extra virtual slots of a corner atom
   __shared__ Stage st;
-  __shared__ EnumShared sh;
+  __shared__ EnumShared<MT> sh;
+  constexpr int NXY = MODE == MODE_XY ? NV : 1;         // arrays only the xy path reads
+  constexpr int TXY = MODE == MODE_XY ? MT : 1;
   __shared__ struct {
-    double r[NV][3];     // wrapped position (FP64)
+    double r[NXY][3];    // wrapped position (FP64), xy-coupled cells
     int vbase[NV], entry[NV];
     int4 q[NV];          // home cell, type | corner << 16
     int ok[NV];          // 0 only after a reported error
@@ -680,18 +686,20 @@ __global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const 
     int clist[NV];                                // batch indices of the corner atoms
   } bt;
   // per-type constants: g_dd / (2 sigma^2) per axis and Nel / sigma^3
-  __shared__ double tgk[kMaxTypes][3];
-  __shared__ float tamp[kMaxTypes];
+  __shared__ double tgk[TXY][3];
+  __shared__ float tamp[TXY];
   // grid size, spacing and tile extent per axis
   __shared__ struct {
     double dr[3];
     int n[3], t0[3], t1[3];
   } axs;
-  for (int i = threadIdx.x; i < sys.ntypes; i += kThreads) {
-    tgk[i][0] = sys.g00 / td[i].two_sig2;
-    tgk[i][1] = sys.g11 / td[i].two_sig2;
-    tgk[i][2] = sys.g22 / td[i].two_sig2;
-    tamp[i] = td[i].amp_f;
+  if constexpr (MODE == MODE_XY) {
+    for (int i = threadIdx.x; i < sys.ntypes; i += kThreads) {
+      tgk[i][0] = sys.g00 / td[i].two_sig2;
+      tgk[i][1] = sys.g11 / td[i].two_sig2;
+      tgk[i][2] = sys.g22 / td[i].two_sig2;
+      tamp[i] = td[i].amp_f;
+    }
   }
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -735,9 +743,11 @@ __global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const 
       bt.entry[lane] = c;
       bt.q[lane] = q;
       bt.ok[lane] = slot >= 0 && slot < ty.count;  // false only after a reported error
-      bt.r[lane][0] = s_r[3 * (size_t)c + 0];
-      bt.r[lane][1] = s_r[3 * (size_t)c + 1];
-      bt.r[lane][2] = s_r[3 * (size_t)c + 2];
+      if constexpr (MODE == MODE_XY) {
+        bt.r[lane][0] = s_r[3 * (size_t)c + 0];
+        bt.r[lane][1] = s_r[3 * (size_t)c + 1];
+        bt.r[lane][2] = s_r[3 * (size_t)c + 2];
+      }
       if constexpr (MODE == MODE_DIAG) {
         // this tile's slice per axis: slice j = (tile - T0) mod ntiles, T0 the
         // tile of the window start (same numbering as k_slices)
@@ -800,7 +810,7 @@ __global__ __launch_bounds__(kThreads, 5) void k_deposit_mfma(SysDev sys, const 
     }
     // staging by evaluation (xy-coupled cells): wave d < 3 builds axis d for
     // every (atom, cell) of the batch
-    if (MODE == MODE_XY && wave < 3) {
+    if constexpr (MODE == MODE_XY) if (wave < 3) {
       const int d = wave;
       // this wave's axis (from LDS: a runtime index into the tile's arrays
       // would otherwise go through private memory)
@@ -1051,12 +1061,20 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
   if (ntiles == 0) return hipSuccess;
   const dim3 grid((unsigned)ntiles), block(kThreads);
   if constexpr (std::is_same<ACC, float>::value) {
-    if (sys.mode == MODE_DIAG)
-      hipLaunchKernelGGL(k_deposit_mfma<MODE_DIAG>, grid, block, 0, s, sys, p.types, p.bucket_off,
-                         p.s_ir, p.s_r, p.slices, out);
+    // blocks size their per-type LDS for 16 types when the system has no more
+    const bool few = sys.ntypes <= 16;
+    if (sys.mode == MODE_DIAG && few)
+      hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, 16>), grid, block, 0, s, sys, p.types,
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
+    else if (sys.mode == MODE_DIAG)
+      hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, kMaxTypes>), grid, block, 0, s, sys, p.types,
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
+    else if (few)
+      hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, 16>), grid, block, 0, s, sys, p.types,
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
     else
-      hipLaunchKernelGGL(k_deposit_mfma<MODE_XY>, grid, block, 0, s, sys, p.types, p.bucket_off,
-                         p.s_ir, p.s_r, p.slices, out);
+      hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, kMaxTypes>), grid, block, 0, s, sys, p.types,
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
   } else {
     if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), grid, block, 0, s, sys, p.types,
