@@ -255,6 +255,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the fp32-vs-fp64 S(q) error report")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the default FFT launch choices instead of timing them at plan creation")
     ap.add_argument("--batch-frames", type=int, default=50)
     ap.add_argument("--adj-iters", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -283,7 +285,7 @@ def main():
     sres = synth.hii_sres()
     s = engine.Setup(coords, dims, typ, rad, sres, False, np.eye(3))
     assert list(s.N) == [256, 256, 256], s.N
-    plan = engine.Plan.from_setup(s, fft64=(args.fft == "fp64"))
+    plan = engine.Plan.from_setup(s, fft64=(args.fft == "fp64"), autotune=not args.no_autotune)
     d_coords = torch.from_numpy(coords).to(dev)
     d_types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
     d_scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)

@@ -32,6 +32,24 @@ def _random_rotations(rng, n):
     ], 1)
 
 
+def water_files(workdir, seed=1001, nwater=900, box=30.0, nframes=10):
+    """Write the water workload as GROMACS files (nm): ``water.gro`` (topology,
+    first frame) and ``water.trr`` (all frames, single precision).  Returns
+    the two paths."""
+    import os
+    from . import trajio
+    coords, dims, typ = water(seed, nwater, box, nframes)
+    nm = coords / np.float32(10)
+    cell = np.diag([box / 10.0] * 3)
+    gro = os.path.join(workdir, "water.gro")
+    trr = os.path.join(workdir, "water.trr")
+    resids = [k // 3 + 1 for k in range(len(typ))]
+    trajio.write_gro(gro, nm[:1], cell, typ, resnames=["SOL"] * len(typ), resids=resids,
+                     title="water workload")
+    trajio.write_trr(trr, nm, cell, times=np.arange(nframes, dtype=float))
+    return gro, trr
+
+
 def water(seed=1001, nwater=900, box=30.0, nframes=10):
     """Returns (coords float32 (F, 3*nwater, 3), dims float32 (F, 3), typ list)."""
     rng = np.random.default_rng(seed)

@@ -224,10 +224,20 @@ def test_determinism_bitwise(rad, gpu):
 
 
 def test_water_64_end_to_end(rad, gpu, tmp_path):
-    """The CPU-runnable water workload (64^3, 2,700 atoms), 2 frames, vs the oracle."""
+    """The CPU-runnable water workload (64^3, 2,700 atoms), 2 frames, through
+    .gro/.trr files and load_traj.process_gro_mdtraj as the driver does
+    (main_gromacs.py:197-212), vs the oracle."""
     import dens
+    import load_traj
     from mdsf import synth
-    coords, dims, typ = synth.water(seed=1001, nwater=900, box=30.0, nframes=2)
+    gro, trr = synth.water_files(str(tmp_path), seed=1001, nwater=900, box=30.0, nframes=2)
+    load_traj.process_gro_mdtraj(gro, trr, str(tmp_path / "traj"))
+    traj = np.load(str(tmp_path / "traj.npz"))
+    coords, dims, typ = traj["coords"], traj["dims"], traj["typ"]
+    ref_coords, ref_dims, _ = synth.water(seed=1001, nwater=900, box=30.0, nframes=2)
+    assert coords.dtype == np.float32 and np.array_equal(coords, ref_coords)
+    assert np.array_equal(dims, ref_dims)
+    typ = [str(x) for x in typ]
     ro = np.array(coords)
     want = O.compute_sf(ro, np.array(dims), typ, None, rad, np.eye(3), 0.46875)
     r = np.array(coords)
