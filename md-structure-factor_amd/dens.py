@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from mdsf import engine
+from mdsf import engine, npzio
 
 USE_BETTER_RESOLUTION = True          # dens.py:10
 PRINT_DETAILS = True                  # dens.py:11
@@ -29,6 +29,10 @@ dtyp = np.float64                     # dens.py:20 (FP64 S(q) accumulation)
 # sparse densities; FP32 above).  S(q) is always summed in FP64.
 FFT_PRECISION = "auto"
 AUTO_FP64_MAX_POINTS = 128 ** 3
+
+# Threads deflating the .npz members (None: up to 16, one per core); the
+# archive is what np.savez_compressed writes, compressed in parallel.
+NPZ_THREADS = None
 
 
 def load_radii(filename):
@@ -117,6 +121,7 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     sf_dev = plan.sf()
     sfplt, kgrid, kgridplt = plan.outputs(sf_dev, box, s.box_f32)
     sf = sf_dev.cpu().numpy()
-    np.savez_compressed(out_filename, sf=sf, sfplt=sfplt.cpu().numpy(), L=s.L, N=s.N,
-                        kgrid=kgrid.cpu().numpy(), kgridplt=kgridplt.cpu().numpy())
+    npzio.savez_compressed(out_filename, threads=NPZ_THREADS, sf=sf, sfplt=sfplt.cpu().numpy(),
+                           L=s.L, N=s.N, kgrid=kgrid.cpu().numpy(),
+                           kgridplt=kgridplt.cpu().numpy())
     plan.close()
