@@ -244,6 +244,33 @@ def batch_leg(rad, nframes):
     return out
 
 
+def dropin_leg(rad, nframes):
+    """Wall time of the drop-in dens.compute_sf on the 256^3 workload (host
+    arrays in, .npz out), by phase: set-up, upload, frame loop, outputs, write."""
+    import tempfile
+
+    import dens
+    from mdsf import synth
+    coords, dims, typ = synth.hii(seed=2002, nframes=nframes)
+    dens.PRINT_DETAILS = False
+    saved = dens.USE_BETTER_RESOLUTION
+    dens.USE_BETTER_RESOLUTION = False
+    try:
+        with tempfile.TemporaryDirectory() as d:
+            t0 = time.perf_counter()
+            dens.compute_sf(coords, dims, typ, os.path.join(d, "sf"), rad, np.eye(3),
+                            synth.hii_sres())
+            total = time.perf_counter() - t0
+            size = os.path.getsize(os.path.join(d, "sf.npz"))
+    finally:
+        dens.USE_BETTER_RESOLUTION = saved
+    out = {"workload": f"compute_sf hii_phase_256^3_50k_atoms {nframes} frames",
+           "total_s": total, "npz_MB": size / 1e6,
+           "frames_per_s_in_frame_loop": nframes / dens.LAST_TIMINGS["frames"]}
+    out.update({k + "_s": v for k, v in dens.LAST_TIMINGS.items()})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,6 +285,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default FFT launch choices instead of timing them at plan creation")
     ap.add_argument("--batch-frames", type=int, default=50)
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in compute_sf leg")
+    ap.add_argument("--dropin-frames", type=int, default=200)
     ap.add_argument("--adj-iters", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 3 times (the reference "
@@ -380,6 +409,9 @@ def main():
     if not args.no_batch:
         log(f"batched leg: 32 systems x {args.batch_frames} frames")
         out["batched"] = batch_leg(rad, args.batch_frames)
+    if not args.no_dropin:
+        log("drop-in compute_sf leg (host arrays in, .npz out)")
+        out["dropin"] = dropin_leg(rad, args.dropin_frames)
     if not args.no_adjoint:
         log("fwd+bwd leg (adjoint workload)")
         out["fwd_bwd"] = adjoint_leg(rad, args.adj_iters)

@@ -34,6 +34,10 @@ AUTO_FP64_MAX_POINTS = 128 ** 3
 # archive is what np.savez_compressed writes, compressed in parallel.
 NPZ_THREADS = None
 
+# Wall-clock seconds of the phases of the last compute_sf call: "setup",
+# "upload", "frames", "outputs", "write" (informational).
+LAST_TIMINGS = {}
+
 
 def load_radii(filename):
     """``{name: (Nel, sigma_angstrom)}`` from ``Nel name radius_pm`` lines (dens.py:23-35)."""
@@ -97,7 +101,10 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     frames of |rfftn(rho)|^2), ``sfplt``, ``L``, ``N``, ``kgrid``, ``kgridplt``.
     ``ucell`` defaults to the identity and ``Sres`` to 1 Angstrom.
     """
+    import time
+
     import torch
+    t0 = time.perf_counter()
     if ucell is None:
         ucell = np.eye(3)
     s = engine.Setup(r, L, typ, rad, Sres, USE_BETTER_RESOLUTION, ucell)
@@ -105,9 +112,12 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
         _print_details(s.L, Sres, s.N)
     plan = engine.Plan.from_setup(s, fft64=_use_fft64(s.N))
     dev = plan.device
+    t1 = time.perf_counter()
     coords = torch.from_numpy(np.ascontiguousarray(r)).to(dev)
     scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
     box = [float(x) for x in s.L]
+    torch.cuda.synchronize(dev)
+    t2 = time.perf_counter()
     if RANDOM_NOISE > 0:
         plan.rescale_wrap(coords, scale, box, s.arith, s.wrap)
         for _ in range(s.nframes):
@@ -116,12 +126,18 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     else:
         types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
         plan.accumulate_frames(coords, types, scale, box, s.arith, s.wrap)
-    r[...] = coords.cpu().numpy()
     plan.check_errors()
+    t3 = time.perf_counter()
+    r[...] = coords.cpu().numpy()
     sf_dev = plan.sf()
     sfplt, kgrid, kgridplt = plan.outputs(sf_dev, box, s.box_f32)
-    sf = sf_dev.cpu().numpy()
-    npzio.savez_compressed(out_filename, threads=NPZ_THREADS, sf=sf, sfplt=sfplt.cpu().numpy(),
-                           L=s.L, N=s.N, kgrid=kgrid.cpu().numpy(),
-                           kgridplt=kgridplt.cpu().numpy())
+    out = dict(sf=sf_dev.cpu().numpy(), sfplt=sfplt.cpu().numpy(), L=s.L, N=s.N,
+               kgrid=kgrid.cpu().numpy(), kgridplt=kgridplt.cpu().numpy())
     plan.close()
+    del sfplt, kgrid, kgridplt, sf_dev
+    t4 = time.perf_counter()
+    npzio.savez_compressed(out_filename, threads=NPZ_THREADS, **out)
+    t5 = time.perf_counter()
+    LAST_TIMINGS.clear()
+    LAST_TIMINGS.update(setup=t1 - t0, upload=t2 - t1, frames=t3 - t2, outputs=t4 - t3,
+                        write=t5 - t4)
