@@ -427,7 +427,10 @@ __device__ void accumulate_xy(const StageXY<ACC>& st, int nbatch, int wave, int 
 
 // ---------------------------------------------------------------------------
 // Candidate enumeration shared by both accumulation paths.
-constexpr int kListCap = 320;  // trigger (<= 64) - 1 + one 256-candidate chunk, rounded up
+// Candidate ring buffer: at most trigger (<= 64) - 1 + one 256-candidate chunk
+// entries are live; a power of two makes the wrap a mask.
+constexpr int kListCap = 512;
+constexpr int kListMask = kListCap - 1;
 
 // MT: the most atom types the block handles (LDS is sized for it)
 template <int MT>
@@ -464,22 +467,9 @@ __device__ __forceinline__ int block_scan(SH& sh, int v, int* excl) {
   return tot;
 }
 
-// drop the first n list entries (n <= count <= kListCap)
-template <class SH>
-__device__ __forceinline__ void list_shift(SH& sh, int n, int count) {
-  const int tid = threadIdx.x;
-  const int rem = count - n;
-  __syncthreads();
-  const int v0 = tid < rem ? sh.list[n + tid] : 0;
-  const int v1 = tid + kThreads < rem ? sh.list[n + tid + kThreads] : 0;
-  __syncthreads();
-  if (tid < rem) sh.list[tid] = v0;
-  if (tid + kThreads < rem) sh.list[tid + kThreads] = v1;
-  __syncthreads();
-}
-
-// Streams every atom whose window reaches the tile into sh.list in a fixed
-// order; `process(count)` consumes a prefix of the list and returns its size.
+// Streams every atom whose window reaches the tile into the ring sh.list in a
+// fixed order; `process(count, head)` consumes entries head, head+1, ...
+// (mod kListCap) and returns how many it took.
 template <class SH, typename Process>
 __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td,
                                const int* __restrict__ boff, const int4* __restrict__ s_ir,
@@ -497,7 +487,7 @@ __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td
     sh.col_base[sys.ntypes] = acc;
   }
   __syncthreads();
-  int count = 0;
+  int count = 0, head = 0;
   const int total_cols = sh.col_base[sys.ntypes];
   for (int cbase = 0; cbase < total_cols; cbase += kThreads) {
     // phase A: one bucket column per thread -> (start, length, type)
@@ -549,20 +539,20 @@ __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td
       }
       int pos;
       const int nhit = block_scan(sh, hit ? 1 : 0, &pos);
-      if (hit) sh.list[count + pos] = entry;
+      if (hit) sh.list[(head + count + pos) & kListMask] = entry;
       count += nhit;
       __syncthreads();
       while (count >= trigger) {
-        const int n = process(count);
-        list_shift(sh, n, count);
+        const int n = process(count, head);
+        head = (head + n) & kListMask;
         count -= n;
       }
     }
     __syncthreads();
   }
   while (count > 0) {
-    const int n = process(count);
-    list_shift(sh, n, count);
+    const int n = process(count, head);
+    head = (head + n) & kListMask;
     count -= n;
   }
 }
@@ -604,12 +594,12 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
 #pragma unroll
   for (int k = 0; k < T; ++k) acc[k] = 0;
 
-  auto process = [&](int count) -> int {
+  auto process = [&](int count, int head) -> int {
     const int nbatch = min(count, NB);
     __syncthreads();
     const int a = tid >> 2, part = tid & 3;
     if (a < nbatch) {
-      const int c = sh.list[a] & 0x7fffffff;
+      const int c = sh.list[(head + a) & kListMask] & 0x7fffffff;
       if constexpr (MODE == MODE_DIAG) stage_diag<ACC>(st, a, part, c, sys, td, s_ir, tables, tl);
       else stage_xy<ACC>(st, a, part, c, sys, td, s_ir, s_r, tables, tl);
     }
@@ -693,6 +683,25 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
     double dr[3];
     int n[3], t0[3], t1[3];
   } axs;
+  // per-type slice layout (orthogonal cells), read by the batch set-up
+  constexpr int TD = MODE == MODE_DIAG ? MT : 1;
+  __shared__ struct {
+    long long slice_base[TD], corner_base[TD];
+    int slice_stride[TD], corner_stride[TD], type_start[TD], count[TD], nsl[TD][3];
+  } tyl;
+  if constexpr (MODE == MODE_DIAG) {
+    for (int i = threadIdx.x; i < sys.ntypes; i += kThreads) {
+      tyl.slice_base[i] = td[i].slice_base;
+      tyl.corner_base[i] = td[i].corner_base;
+      tyl.slice_stride[i] = td[i].slice_stride;
+      tyl.corner_stride[i] = td[i].corner_stride;
+      tyl.type_start[i] = td[i].type_start;
+      tyl.count[i] = td[i].count;
+      tyl.nsl[i][0] = td[i].nsl[0];
+      tyl.nsl[i][1] = td[i].nsl[1];
+      tyl.nsl[i][2] = td[i].nsl[2];
+    }
+  }
   if constexpr (MODE == MODE_XY) {
     for (int i = threadIdx.x; i < sys.ntypes; i += kThreads) {
       tgk[i][0] = sys.g00 / td[i].two_sig2;
@@ -719,10 +728,12 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
 #pragma unroll
   for (int b = 0; b < 4; ++b) acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  auto process = [&](int count) -> int {
+  auto process = [&](int count, int head) -> int {
+    // separates the previous batch's MFMA reads of the staging area from
+    // this batch's writes
     __syncthreads();
     // how many real atoms fit into NV virtual slots (same result in every wave)
-    const int e = lane < count ? sh.list[lane] : 0;
+    const int e = lane < count ? sh.list[(head + lane) & kListMask] : 0;
     const bool isc = lane < count && (e < 0);
     const unsigned long long cm = __ballot(isc);
     const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
@@ -737,35 +748,40 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
       // issues independent table loads
       const int c = e & 0x7fffffff;
       const int4 q = s_ir[c];
-      const TypeDev& ty = td[q.w & 0xffff];
-      const int slot = c - ty.type_start;
       bt.vbase[lane] = lane + EXTRA * __popcll(cm & (le >> 1));
-      bt.entry[lane] = c;
       bt.q[lane] = q;
-      bt.ok[lane] = slot >= 0 && slot < ty.count;  // false only after a reported error
       if constexpr (MODE == MODE_XY) {
+        const TypeDev& ty = td[q.w & 0xffff];
+        const int slot = c - ty.type_start;
+        bt.entry[lane] = c;
+        bt.ok[lane] = slot >= 0 && slot < ty.count;  // false only after a reported error
         bt.r[lane][0] = s_r[3 * (size_t)c + 0];
         bt.r[lane][1] = s_r[3 * (size_t)c + 1];
         bt.r[lane][2] = s_r[3 * (size_t)c + 2];
       }
       if constexpr (MODE == MODE_DIAG) {
         // this tile's slice per axis: slice j = (tile - T0) mod ntiles, T0 the
-        // tile of the window start (same numbering as k_slices)
-        const bool ok = slot >= 0 && slot < ty.count;
-        long long off = ty.slice_base + (long long)slot * ty.slice_stride;
-        long long coff = ty.corner_base + (long long)slot * ty.corner_stride;
+        // tile of the window start (same numbering as k_slices); the window
+        // start lies in [-Nb, N + Nb), Nb <= N, so one wrap suffices
+        const int t = q.w & 0xffff;
+        const int slot = c - tyl.type_start[t];
+        const bool ok = slot >= 0 && slot < tyl.count[t];  // false only after a reported error
+        long long off = tyl.slice_base[t] + (long long)slot * tyl.slice_stride[t];
+        long long coff = tyl.corner_base[t] + (long long)slot * tyl.corner_stride[t];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
-          const int N = sys.n[d], nt = sys.ntiles[d];
-          int j = tl.t0[d] / T - pmod(ir - ty.half[d], N) / T;
+          const int N = sys.n[d], nt = sys.ntiles[d], ns = tyl.nsl[t][d];
+          int lo = ir - sh.half[t][d];
+          lo = lo < 0 ? lo + N : (lo >= N ? lo - N : lo);
+          int j = tl.t0[d] / T - lo / T;
           if (j < 0) j += nt;
-          const bool has = ok && j < ty.nsl[d];
+          const bool has = ok && j < ns;
           bt.so[lane][d] = has ? off + (long long)T * j : -1LL;
           bt.co[lane][d] = has ? coff + (long long)T * j : -1LL;
-          bt.cs[lane][d] = T * ty.nsl[d];
-          off += (long long)T * ty.nsl[d];
-          coff += (long long)(d == 1 ? 3 : 2) * T * ty.nsl[d];
+          bt.cs[lane][d] = T * ns;
+          off += (long long)T * ns;
+          coff += (long long)(d == 1 ? 3 : 2) * T * ns;
         }
         if (isc) bt.clist[__popcll(cm & ((1ull << lane) - 1))] = lane;
       }
