@@ -321,37 +321,65 @@ def main():
     box = [float(x) for x in s.L]
     stream = torch.cuda.current_stream(dev)
     stages = ["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"]
-    steps = [lambda t: plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap),
-             lambda t: plan.deposit(),
-             lambda t: plan.fft_pass(0),
-             lambda t: plan.fft_pass(1),
-             lambda t: plan.fft_pass(2)]
-    # events are created up front; each stage is bracketed on the stream the
-    # kernels run on
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
-           for _ in range(args.steps)]
+    # the frame pipeline of mdsf_accumulate_frames, driven stage by stage:
+    # prepare + deposit on the main stream, the FFT passes on a second stream;
+    # preparing frame t + 1 overlaps the FFT of frame t, and depositing frame
+    # t + 1 waits for that FFT to release the density buffer
+    fft_stream = torch.cuda.Stream(dev)
+    ev_dep = torch.cuda.Event()
+    ev_fft = torch.cuda.Event()
+
+    def frame(t, e=None):
+        with torch.cuda.stream(stream):
+            if e is not None:
+                e[0].record(stream)
+            plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap)
+            if e is not None:
+                e[1].record(stream)
+            stream.wait_event(ev_fft)
+            if e is not None:
+                e[2].record(stream)
+            plan.deposit()
+            if e is not None:
+                e[3].record(stream)
+            ev_dep.record(stream)
+        fft_stream.wait_event(ev_dep)
+        with torch.cuda.stream(fft_stream):
+            for i in range(3):
+                if e is not None:
+                    e[4 + 2 * i].record(fft_stream)
+                plan.fft_pass(i)
+                if e is not None:
+                    e[5 + 2 * i].record(fft_stream)
+            ev_fft.record(fft_stream)
+
+    # per frame: (prepare start, end), (deposit start, end), (z, y, x start / end)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(10)] for _ in range(args.steps)]
+    pairs = {"prepare": (0, 1), "deposit": (2, 3), "fft_z": (4, 5), "fft_y": (6, 7),
+             "fft_x_acc": (8, 9)}
 
     log(f"warmup {args.warmup} frames")
+    ev_fft.record(fft_stream)
     for t in range(args.warmup):
-        for st in steps:
-            st(t)
+        frame(t)
+    stream.wait_event(ev_fft)
+    torch.cuda.synchronize()
     plan.reset()
     torch.cuda.synchronize()
     log(f"timing {args.steps} frames")
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
+    fft_stream.wait_stream(stream)
+    ev_fft.record(fft_stream)
     for t in range(args.steps):
-        e = evs[t]
-        e[0].record(stream)
-        for i, st in enumerate(steps):
-            st(t)
-            e[i + 1].record(stream)
+        frame(t, evs[t])
+    stream.wait_event(ev_fft)
     end.record(stream)
     torch.cuda.synchronize()
     plan.check_errors()
     total_ms = start.elapsed_time(end)
-    per_stage = {st: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
-                 for i, st in enumerate(stages)}
+    per_stage = {st: float(np.mean([e[i].elapsed_time(e[j]) for e in evs]))
+                 for st, (i, j) in pairs.items()}
     log("per-stage avg ms: " + json.dumps({k: round(v, 4) for k, v in per_stage.items()}))
 
     n = [int(x) for x in s.N]

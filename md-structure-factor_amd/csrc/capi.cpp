@@ -39,6 +39,9 @@ hipError_t dalloc(T** p, size_t count) {
 
 void free_plan(Plan* p) {
   if (!p) return;
+  for (hipEvent_t e : {p->ev_fork, p->ev_dep, p->ev_fft})
+    if (e) (void)hipEventDestroy(e);
+  if (p->fft_stream) (void)hipStreamDestroy(p->fft_stream);
   void* ptrs[] = {p->bucket_aux, p->types,     p->ir4,       p->key,        p->rpos,       p->bucket_count,
                   p->bucket_off, p->bucket_cur, p->s_idx,      p->s_ir,       p->s_r,
                   p->tables,    p->err,       p->rho,        p->spec_nyq,   p->sf_main,
@@ -309,6 +312,14 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     free_plan(p);
     return rc;
   }
+  if ((e = hipStreamCreateWithFlags(&p->fft_stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_dep, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_fft, hipEventDisableTiming)) != hipSuccess) {
+    rc = hip_fail(e, "frame pipeline stream / events");
+    free_plan(p);
+    return rc;
+  }
   if (desc->flags & MDSF_AUTOTUNE) {
     if ((e = fft_autotune(*p, nullptr)) != hipSuccess ||
         (e = hipMemset(p->sf_main, 0, sizeof(double) * nx * ny * m)) != hipSuccess ||
@@ -367,18 +378,33 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   if (rc != MDSF_OK) return rc;
   if (nframes < 0) return fail(MDSF_ERR_ARG, "negative frame count");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (nframes == 0) return MDSF_OK;
   const size_t esz = dtype == MDSF_F32 ? 4 : 8;
   const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
+  // Two-stage pipeline: frame preparation and deposition on `s`, the FFT
+  // passes (which read and write only the density / spectrum buffer and the
+  // S(q) accumulators) on fft_stream.  Preparing frame t + 1 overlaps the FFT
+  // of frame t; depositing frame t + 1 waits until that FFT has released the
+  // density buffer.  The S(q) sum stays in frame order on one stream.
+  hipStream_t f = p->fft_stream;
+  MDSF_HIP(hipEventRecord(p->ev_fork, s));
+  MDSF_HIP(hipStreamWaitEvent(f, p->ev_fork, 0));
+  MDSF_HIP(hipEventRecord(p->ev_fft, f));
   for (int64_t t = 0; t < nframes; ++t) {
     char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
     int32_t* home = (t == nframes - 1) ? d_home_cells : nullptr;
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
     MDSF_HIP(launch_binning(*p, s));
+    MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft, 0));
     MDSF_HIP(launch_deposit(*p, p->rho, s));
-    rc = run_fft_accumulate(*p, p->rho, s);
+    MDSF_HIP(hipEventRecord(p->ev_dep, s));
+    MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep, 0));
+    rc = run_fft_accumulate(*p, p->rho, f);
     if (rc != MDSF_OK) return rc;
+    MDSF_HIP(hipEventRecord(p->ev_fft, f));
   }
+  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft, 0));
   return MDSF_OK;
 }
 

@@ -114,6 +114,10 @@ struct Plan {
   int tune_z = 0, tune_y = 0, tune_x = 0;
   float2* tw_r2c = nullptr;      // W_Nz^k, k < Nz
   double2* tw_r2c64 = nullptr;
+  // frame pipeline: the FFT passes of frame t run on fft_stream while the
+  // caller's stream prepares frame t + 1 (events: fork, density ready, FFT done)
+  hipStream_t fft_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_dep = nullptr, ev_fft = nullptr;
   // adjoint (allocated on first use)
   long long adj_items = 0;
   double* adj_part = nullptr;    // 3 FP64 partial sums per work item
