@@ -322,21 +322,24 @@ def main():
     stream = torch.cuda.current_stream(dev)
     stages = ["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"]
     # the frame pipeline of mdsf_accumulate_frames, driven stage by stage:
-    # prepare + deposit on the main stream, the FFT passes on a second stream;
-    # preparing frame t + 1 overlaps the FFT of frame t, and depositing frame
-    # t + 1 waits for that FFT to release the density buffer
+    # prepare + deposit on the main stream, the FFT passes on a second stream,
+    # frames alternating between the plan's two density buffers; frame t + 1
+    # is prepared and deposited while frame t is transformed, and depositing
+    # frame t waits for the FFT of frame t - 2 to release its buffer
     fft_stream = torch.cuda.Stream(dev)
     ev_dep = torch.cuda.Event()
-    ev_fft = torch.cuda.Event()
+    ev_fft = [torch.cuda.Event(), torch.cuda.Event()]
 
     def frame(t, e=None):
+        b = t & 1
         with torch.cuda.stream(stream):
             if e is not None:
                 e[0].record(stream)
             plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap)
             if e is not None:
                 e[1].record(stream)
-            stream.wait_event(ev_fft)
+            plan.set_frame_buffer(b)
+            stream.wait_event(ev_fft[b])
             if e is not None:
                 e[2].record(stream)
             plan.deposit()
@@ -351,7 +354,7 @@ def main():
                 plan.fft_pass(i)
                 if e is not None:
                     e[5 + 2 * i].record(fft_stream)
-            ev_fft.record(fft_stream)
+            ev_fft[b].record(fft_stream)
 
     # per frame: (prepare start, end), (deposit start, end), (z, y, x start / end)
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(10)] for _ in range(args.steps)]
@@ -359,10 +362,11 @@ def main():
              "fft_x_acc": (8, 9)}
 
     log(f"warmup {args.warmup} frames")
-    ev_fft.record(fft_stream)
+    for ev in ev_fft:
+        ev.record(fft_stream)
     for t in range(args.warmup):
         frame(t)
-    stream.wait_event(ev_fft)
+    stream.wait_stream(fft_stream)
     torch.cuda.synchronize()
     plan.reset()
     torch.cuda.synchronize()
@@ -370,10 +374,11 @@ def main():
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     fft_stream.wait_stream(stream)
-    ev_fft.record(fft_stream)
+    for ev in ev_fft:
+        ev.record(fft_stream)
     for t in range(args.steps):
         frame(t, evs[t])
-    stream.wait_event(ev_fft)
+    stream.wait_stream(fft_stream)
     end.record(stream)
     torch.cuda.synchronize()
     plan.check_errors()

@@ -140,6 +140,14 @@ int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t
 int mdsf_deposit(mdsf_plan* plan, void* d_density, mdsf_stream stream);
 int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_stream stream);
 
+/* The plan keeps two density / spectrum buffers so that mdsf_accumulate_frames
+ * can transform one frame while depositing the next.  This selects the buffer
+ * (0 or 1) that mdsf_deposit, mdsf_fft_pass and the other single-frame entry
+ * points use from now on; a caller pipelining the stages itself alternates
+ * them.  No counterpart in the reference (its frame loop is sequential,
+ * dens.py:277-321). */
+int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which);
+
 /* |rfftn(density)|^2 summed into the accumulator for a caller-supplied
  * periodic density in plan precision (the RANDOM_NOISE branch,
  * dens.py:279-280, 313-318). */

@@ -39,12 +39,13 @@ hipError_t dalloc(T** p, size_t count) {
 
 void free_plan(Plan* p) {
   if (!p) return;
-  for (hipEvent_t e : {p->ev_fork, p->ev_dep, p->ev_fft})
+  for (hipEvent_t e : {p->ev_fork, p->ev_dep, p->ev_fft[0], p->ev_fft[1]})
     if (e) (void)hipEventDestroy(e);
   if (p->fft_stream) (void)hipStreamDestroy(p->fft_stream);
   void* ptrs[] = {p->bucket_aux, p->types,     p->ir4,       p->key,        p->rpos,       p->bucket_count,
                   p->bucket_off, p->bucket_cur, p->s_idx,      p->s_ir,       p->s_r,
-                  p->tables,    p->err,       p->rho,        p->spec_nyq,   p->sf_main,
+                  p->tables,    p->err,       p->rho_buf[0], p->nyq_buf[0], p->sf_main,
+                  p->rho_buf[1], p->nyq_buf[1],
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
                   p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
                   p->adj_w,     p->slices,    p->s_tmp};
@@ -273,22 +274,24 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   TRY(dalloc(&p->slices, (size_t)p->slice_floats));
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));
-  if (p->fft64) {
-    double* rho = nullptr;
-    double2* nyq = nullptr;
-    TRY(dalloc(&rho, nx * ny * nz));
-    TRY(dalloc(&nyq, nx * ny));
-    p->rho = rho;
-    p->spec_nyq = nyq;
-  } else {
-    float* rho = nullptr;
-    float2* nyq = nullptr;
-    TRY(dalloc(&rho, nx * ny * nz));
-    TRY(dalloc(&nyq, nx * ny));
-    p->rho = rho;
-    p->spec_nyq = nyq;
+  for (int b = 0; b < 2; ++b) {
+    if (p->fft64) {
+      double* rho = nullptr;
+      double2* nyq = nullptr;
+      TRY(dalloc(&rho, nx * ny * nz));
+      TRY(dalloc(&nyq, nx * ny));
+      p->rho_buf[b] = rho;
+      p->nyq_buf[b] = nyq;
+    } else {
+      float* rho = nullptr;
+      float2* nyq = nullptr;
+      TRY(dalloc(&rho, nx * ny * nz));
+      TRY(dalloc(&nyq, nx * ny));
+      p->rho_buf[b] = rho;
+      p->nyq_buf[b] = nyq;
+    }
   }
-  p->spec_main = p->rho;  // the z pass writes the half spectrum over the density
+  p->use_buffer(0);  // the z pass writes the half spectrum over the density
   TRY(dalloc(&p->sf_main, nx * ny * m));
   TRY(dalloc(&p->sf_nyq, nx * ny));
   TRY(hipMemset(p->sf_main, 0, sizeof(double) * nx * ny * m));
@@ -315,7 +318,8 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   if ((e = hipStreamCreateWithFlags(&p->fft_stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_dep, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_fft, hipEventDisableTiming)) != hipSuccess) {
+      (e = hipEventCreateWithFlags(&p->ev_fft[0], hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_fft[1], hipEventDisableTiming)) != hipSuccess) {
     rc = hip_fail(e, "frame pipeline stream / events");
     free_plan(p);
     return rc;
@@ -382,29 +386,33 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   const size_t esz = dtype == MDSF_F32 ? 4 : 8;
   const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
   // Two-stage pipeline: frame preparation and deposition on `s`, the FFT
-  // passes (which read and write only the density / spectrum buffer and the
-  // S(q) accumulators) on fft_stream.  Preparing frame t + 1 overlaps the FFT
-  // of frame t; depositing frame t + 1 waits until that FFT has released the
-  // density buffer.  The S(q) sum stays in frame order on one stream.
+  // passes (which touch only a density / spectrum buffer and the S(q)
+  // accumulators) on fft_stream, frames alternating between the two density
+  // buffers.  Frame t + 1 is prepared and deposited while frame t is
+  // transformed; depositing frame t waits until the FFT of frame t - 2 has
+  // released its buffer.  The S(q) sum stays in frame order on one stream.
   hipStream_t f = p->fft_stream;
   MDSF_HIP(hipEventRecord(p->ev_fork, s));
   MDSF_HIP(hipStreamWaitEvent(f, p->ev_fork, 0));
-  MDSF_HIP(hipEventRecord(p->ev_fft, f));
+  MDSF_HIP(hipEventRecord(p->ev_fft[0], f));
+  MDSF_HIP(hipEventRecord(p->ev_fft[1], f));
   for (int64_t t = 0; t < nframes; ++t) {
+    const int b = (int)(t & 1);
     char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
     int32_t* home = (t == nframes - 1) ? d_home_cells : nullptr;
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
     MDSF_HIP(launch_binning(*p, s));
-    MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft, 0));
+    p->use_buffer(b);
+    MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[b], 0));
     MDSF_HIP(launch_deposit(*p, p->rho, s));
     MDSF_HIP(hipEventRecord(p->ev_dep, s));
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep, 0));
     rc = run_fft_accumulate(*p, p->rho, f);
     if (rc != MDSF_OK) return rc;
-    MDSF_HIP(hipEventRecord(p->ev_fft, f));
+    MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }
-  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft, 0));
+  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) & 1], 0));
   return MDSF_OK;
 }
 
@@ -456,6 +464,14 @@ int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t
   MDSF_HIP(launch_prep(*p, d_coords, dtype, arith, d_types, d_scale, box, wrap_lo, wrap_hi,
                        d_occupancy, d_home_cells, s));
   MDSF_HIP(launch_binning(*p, s));
+  return MDSF_OK;
+}
+
+int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  if (which != 0 && which != 1) return fail(MDSF_ERR_ARG, "frame buffer must be 0 or 1");
+  p->use_buffer(which);
   return MDSF_OK;
 }
 

@@ -100,10 +100,18 @@ struct Plan {
   float* slices = nullptr;        // per-atom tile slices (FP32 plans, orthogonal-axis cells)
   long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
-  // grids
+  // grids: two density / spectrum buffers so that one frame's FFT can run
+  // while the next frame is deposited; rho / spec_* point at the current one
+  void* rho_buf[2] = {nullptr, nullptr};
+  void* nyq_buf[2] = {nullptr, nullptr};
   void* rho = nullptr;           // Nx*Ny*Nz folded density (FP32, or FP64 when fft64)
   void* spec_main = nullptr;     // Nx*Ny*(Nz/2) complex, in place over rho
   void* spec_nyq = nullptr;      // Nx*Ny complex: the kz = Nz/2 plane
+  void use_buffer(int b) {
+    rho = rho_buf[b & 1];
+    spec_main = rho;
+    spec_nyq = nyq_buf[b & 1];
+  }
   double* sf_main = nullptr;     // Nx*Ny*(Nz/2) FP64 accumulator
   double* sf_nyq = nullptr;      // Nx*Ny FP64 accumulator
   // FFT
@@ -117,7 +125,7 @@ struct Plan {
   // frame pipeline: the FFT passes of frame t run on fft_stream while the
   // caller's stream prepares frame t + 1 (events: fork, density ready, FFT done)
   hipStream_t fft_stream = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_dep = nullptr, ev_fft = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_dep = nullptr, ev_fft[2] = {nullptr, nullptr};
   // adjoint (allocated on first use)
   long long adj_items = 0;
   double* adj_part = nullptr;    // 3 FP64 partial sums per work item

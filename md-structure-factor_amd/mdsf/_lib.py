@@ -61,6 +61,7 @@ SIGNATURES = {
                                      ctypes.POINTER(c_double), c_int32, c_int64, c_int64,
                                      c_void_p, c_void_p, c_void_p]),
     "mdsf_deposit": (c_int32, [c_void_p, c_void_p, c_void_p]),
+    "mdsf_set_frame_buffer": (c_int32, [c_void_p, c_int32]),
     "mdsf_fft_pass": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "mdsf_accumulate_density": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mdsf_rfft3": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),

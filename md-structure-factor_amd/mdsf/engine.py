@@ -221,6 +221,10 @@ class Plan:
             _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]), self._ptr(occupancy),
             self._ptr(home), self._stream()))
 
+    def set_frame_buffer(self, which):
+        """Select density / spectrum buffer 0 or 1 for the single-frame entry points."""
+        _lib.check(self.lib.mdsf_set_frame_buffer(self._handle, int(which)))
+
     def deposit(self, out=None):
         _lib.check(self.lib.mdsf_deposit(self._handle, self._ptr(out), self._stream()))
 
