@@ -123,6 +123,29 @@ def copy_calibration(nbytes=256 << 20):
     print(json.dumps({"copy_bytes": nbytes, "reps": 3}), flush=True)
 
 
+def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream):
+    """Each stage alone: frames run stage by stage on one stream, HIP events
+    around every stage (the kernels' own durations, without the overlap of the
+    pipelined loop)."""
+    import torch
+    names = ["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"]
+    steps = [lambda t: plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap),
+             lambda t: plan.deposit(), lambda t: plan.fft_pass(0), lambda t: plan.fft_pass(1),
+             lambda t: plan.fft_pass(2)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(nframes)]
+    plan.set_frame_buffer(0)
+    torch.cuda.synchronize()
+    for t in range(nframes):
+        evs[t][0].record(stream)
+        for i, st in enumerate(steps):
+            st(t)
+            evs[t][i + 1].record(stream)
+    torch.cuda.synchronize()
+    plan.check_errors()
+    return {n: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
+            for i, n in enumerate(names)}
+
+
 def adjoint_leg(rad, iters):
     """fwd+bwd frames/s on the adjoint workload: 20k C/O/H atoms, 60 A box, 128^3,
     16 frames, L = sum (S - S_target)^2; backward recomputes every frame."""
@@ -320,72 +343,29 @@ def main():
     d_scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
     box = [float(x) for x in s.L]
     stream = torch.cuda.current_stream(dev)
-    stages = ["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"]
-    # the frame pipeline of mdsf_accumulate_frames, driven stage by stage:
-    # prepare + deposit on the main stream, the FFT passes on a second stream,
-    # frames alternating between the plan's two density buffers; frame t + 1
-    # is prepared and deposited while frame t is transformed, and depositing
-    # frame t waits for the FFT of frame t - 2 to release its buffer
-    fft_stream = torch.cuda.Stream(dev)
-    ev_dep = torch.cuda.Event()
-    ev_fft = [torch.cuda.Event(), torch.cuda.Event()]
-
-    def frame(t, e=None):
-        b = t & 1
-        with torch.cuda.stream(stream):
-            if e is not None:
-                e[0].record(stream)
-            plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap)
-            if e is not None:
-                e[1].record(stream)
-            plan.set_frame_buffer(b)
-            stream.wait_event(ev_fft[b])
-            if e is not None:
-                e[2].record(stream)
-            plan.deposit()
-            if e is not None:
-                e[3].record(stream)
-            ev_dep.record(stream)
-        fft_stream.wait_event(ev_dep)
-        with torch.cuda.stream(fft_stream):
-            for i in range(3):
-                if e is not None:
-                    e[4 + 2 * i].record(fft_stream)
-                plan.fft_pass(i)
-                if e is not None:
-                    e[5 + 2 * i].record(fft_stream)
-            ev_fft[b].record(fft_stream)
-
-    # per frame: (prepare start, end), (deposit start, end), (z, y, x start / end)
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(10)] for _ in range(args.steps)]
-    pairs = {"prepare": (0, 1), "deposit": (2, 3), "fft_z": (4, 5), "fft_y": (6, 7),
-             "fft_x_acc": (8, 9)}
-
+    # the timed region is the library's own frame loop (mdsf_accumulate_frames):
+    # frame t + 1 prepared on this stream while frame t is deposited and frame
+    # t - 1 transformed on the plan's two other streams; the plan records HIP
+    # events around every stage on the stream it runs on
+    plan.set_stage_timing(True)
     log(f"warmup {args.warmup} frames")
-    for ev in ev_fft:
-        ev.record(fft_stream)
-    for t in range(args.warmup):
-        frame(t)
-    stream.wait_stream(fft_stream)
-    torch.cuda.synchronize()
+    plan.accumulate_frames(d_coords[:args.warmup], d_types, d_scale[:args.warmup], box, s.arith,
+                           s.wrap)
     plan.reset()
     torch.cuda.synchronize()
     log(f"timing {args.steps} frames")
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
-    fft_stream.wait_stream(stream)
-    for ev in ev_fft:
-        ev.record(fft_stream)
-    for t in range(args.steps):
-        frame(t, evs[t])
-    stream.wait_stream(fft_stream)
+    plan.accumulate_frames(d_coords[:args.steps], d_types, d_scale[:args.steps], box, s.arith,
+                           s.wrap)
     end.record(stream)
     torch.cuda.synchronize()
     plan.check_errors()
     total_ms = start.elapsed_time(end)
-    per_stage = {st: float(np.mean([e[i].elapsed_time(e[j]) for e in evs]))
-                 for st, (i, j) in pairs.items()}
-    log("per-stage avg ms: " + json.dumps({k: round(v, 4) for k, v in per_stage.items()}))
+    per_stage = plan.stage_times()
+    log("per-stage avg ms (pipelined): " + json.dumps({k: round(v, 4) for k, v in per_stage.items()}))
+    iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, min(args.steps, 30), stream)
+    log("per-stage avg ms (isolated): " + json.dumps({k: round(v, 4) for k, v in iso.items()}))
 
     n = [int(x) for x in s.N]
     fb = fft_bytes(n)
@@ -396,8 +376,8 @@ def main():
                          "bytes": fb[key], "avg_ms": round(per_stage[st], 4)}
     updates = voxel_updates(s)
     dep_tflops = 2.0 * updates / (per_stage["deposit"] * 1e-3) / 1e12
-    # dominant kernel: the single launch with the largest average time
-    # ("prepare" is several small kernels)
+    # dominant kernel: the single launch with the largest average time in the
+    # timed (pipelined) region ("prepare" is several small kernels)
     kernels = {k: v for k, v in per_stage.items() if k != "prepare"}
     dominant = max(kernels, key=kernels.get)
     if dominant == "deposit":
@@ -435,6 +415,13 @@ def main():
                     "updates_per_s": round(updates / (per_stage["deposit"] * 1e-3), 1),
                     "tflops_fma": round(dep_tflops, 3)},
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
+        "stage_ms_isolated": {k: round(v, 4) for k, v in iso.items()},
+        "roofline_isolated": {
+            "deposit_tflops": round(2.0 * updates / (iso["deposit"] * 1e-3) / 1e12, 3),
+            "deposit_frac": round(2.0 * updates / (iso["deposit"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+            "fft_gbs": {k: round(fb[k] / (iso[st] * 1e-3) / 1e9, 1)
+                        for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"))},
+            "note": "the same kernels run one at a time on one stream"},
     }
     if not args.no_precision:
         log("precision: fp32 plan vs fp64 plan on 2 frames")

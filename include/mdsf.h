@@ -140,13 +140,23 @@ int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t
 int mdsf_deposit(mdsf_plan* plan, void* d_density, mdsf_stream stream);
 int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_stream stream);
 
-/* The plan keeps two density / spectrum buffers so that mdsf_accumulate_frames
- * can transform one frame while depositing the next.  This selects the buffer
- * (0 or 1) that mdsf_deposit, mdsf_fft_pass and the other single-frame entry
- * points use from now on; a caller pipelining the stages itself alternates
- * them.  No counterpart in the reference (its frame loop is sequential,
- * dens.py:277-321). */
+/* The plan keeps two frame slots (per-frame atom state and density /
+ * spectrum buffer) so that mdsf_accumulate_frames can prepare one frame,
+ * deposit the previous one and transform the one before on three streams.
+ * This selects the slot (0 or 1) that mdsf_prepare_frame, mdsf_deposit,
+ * mdsf_fft_pass and the other single-frame entry points use from now on; a
+ * caller pipelining the stages itself alternates them.  No counterpart in the
+ * reference (its frame loop is sequential, dens.py:277-321). */
 int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which);
+
+/* Stage timing of mdsf_accumulate_frames (off by default): when enabled, the
+ * frame loop records HIP events around each frame's preparation, deposition
+ * and three FFT passes on the stream each runs on; mdsf_stage_times then
+ * waits for the last call's events and returns the average milliseconds per
+ * frame of {prepare, deposit, z pass, y pass, x pass + |F|^2 sum} in
+ * avg_ms[5].  No counterpart in the reference. */
+int mdsf_set_stage_timing(mdsf_plan* plan, int32_t enable);
+int mdsf_stage_times(mdsf_plan* plan, double* avg_ms);
 
 /* |rfftn(density)|^2 summed into the accumulator for a caller-supplied
  * periodic density in plan precision (the RANDOM_NOISE branch,

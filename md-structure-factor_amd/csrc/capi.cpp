@@ -39,16 +39,23 @@ hipError_t dalloc(T** p, size_t count) {
 
 void free_plan(Plan* p) {
   if (!p) return;
-  for (hipEvent_t e : {p->ev_fork, p->ev_dep, p->ev_fft[0], p->ev_fft[1]})
+  for (hipEvent_t e : {p->ev_fork, p->ev_prep[0], p->ev_prep[1], p->ev_dep[0], p->ev_dep[1],
+                       p->ev_fft[0], p->ev_fft[1]})
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
+  if (p->dep_stream) (void)hipStreamDestroy(p->dep_stream);
   if (p->fft_stream) (void)hipStreamDestroy(p->fft_stream);
-  void* ptrs[] = {p->bucket_aux, p->types,     p->ir4,       p->key,        p->rpos,       p->bucket_count,
-                  p->bucket_off, p->bucket_cur, p->s_idx,      p->s_ir,       p->s_r,
-                  p->tables,    p->err,       p->rho_buf[0], p->nyq_buf[0], p->sf_main,
+  for (FrameState& f : p->slot) {
+    void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
+                  f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices};
+    for (void* q : fs)
+      if (q) (void)hipFree(q);
+  }
+  void* ptrs[] = {p->types,     p->err,       p->rho_buf[0], p->nyq_buf[0], p->sf_main,
                   p->rho_buf[1], p->nyq_buf[1],
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
                   p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
-                  p->adj_w,     p->slices,    p->s_tmp};
+                  p->adj_w};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -249,29 +256,31 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   } while (0)
   TRY(dalloc(&p->types, (size_t)kMaxTypes));
   TRY(hipMemcpy(p->types, p->types_host, sizeof(TypeDev) * kMaxTypes, hipMemcpyHostToDevice));
-  TRY(dalloc(&p->ir4, nat));
-  TRY(dalloc(&p->key, nat));
-  TRY(dalloc(&p->rpos, 3 * nat));
-  TRY(dalloc(&p->bucket_count, (size_t)s.nbuckets + 1));
-  TRY(dalloc(&p->bucket_off, (size_t)s.nbuckets + 1));
-  TRY(dalloc(&p->bucket_cur, (size_t)s.nbuckets + 1));
-  TRY(dalloc(&p->bucket_aux, (size_t)s.nbuckets / 1024 + 2));
-  TRY(hipMemset(p->bucket_count, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
-  TRY(hipMemset(p->bucket_off, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
-  TRY(dalloc(&p->s_idx, nat));
-  TRY(dalloc(&p->s_tmp, nat));
-  TRY(dalloc(&p->s_ir, nat));
-  TRY(dalloc(&p->s_r, 3 * nat));
-  if (p->fft64) {
-    double* tb = nullptr;
-    TRY(dalloc(&tb, (size_t)table_floats));
-    p->tables = tb;
-  } else {
-    float* tb = nullptr;  // unused: the FP32 path stages tile slices instead
-    TRY(dalloc(&tb, (size_t)1));
-    p->tables = tb;
+  for (FrameState& f : p->slot) {
+    TRY(dalloc(&f.ir4, nat));
+    TRY(dalloc(&f.key, nat));
+    TRY(dalloc(&f.rpos, 3 * nat));
+    TRY(dalloc(&f.bucket_count, (size_t)s.nbuckets + 1));
+    TRY(dalloc(&f.bucket_off, (size_t)s.nbuckets + 1));
+    TRY(dalloc(&f.bucket_cur, (size_t)s.nbuckets + 1));
+    TRY(dalloc(&f.bucket_aux, (size_t)s.nbuckets / 1024 + 2));
+    TRY(hipMemset(f.bucket_count, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
+    TRY(hipMemset(f.bucket_off, 0, sizeof(int) * ((size_t)s.nbuckets + 1)));
+    TRY(dalloc(&f.s_idx, nat));
+    TRY(dalloc(&f.s_tmp, nat));
+    TRY(dalloc(&f.s_ir, nat));
+    TRY(dalloc(&f.s_r, 3 * nat));
+    if (p->fft64) {
+      double* tb = nullptr;
+      TRY(dalloc(&tb, (size_t)table_floats));
+      f.tables = tb;
+    } else {
+      float* tb = nullptr;  // unused: the FP32 path stages tile slices instead
+      TRY(dalloc(&tb, (size_t)1));
+      f.tables = tb;
+    }
+    TRY(dalloc(&f.slices, (size_t)p->slice_floats));
   }
-  TRY(dalloc(&p->slices, (size_t)p->slice_floats));
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));
   for (int b = 0; b < 2; ++b) {
@@ -291,7 +300,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       p->nyq_buf[b] = nyq;
     }
   }
-  p->use_buffer(0);  // the z pass writes the half spectrum over the density
+  p->use_slot(0);  // (the z pass writes the half spectrum over the density)
   TRY(dalloc(&p->sf_main, nx * ny * m));
   TRY(dalloc(&p->sf_nyq, nx * ny));
   TRY(hipMemset(p->sf_main, 0, sizeof(double) * nx * ny * m));
@@ -315,9 +324,13 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     free_plan(p);
     return rc;
   }
-  if ((e = hipStreamCreateWithFlags(&p->fft_stream, hipStreamNonBlocking)) != hipSuccess ||
+  if ((e = hipStreamCreateWithFlags(&p->dep_stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&p->fft_stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_dep, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_prep[0], hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_prep[1], hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_dep[0], hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&p->ev_dep[1], hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_fft[0], hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_fft[1], hipEventDisableTiming)) != hipSuccess) {
     rc = hip_fail(e, "frame pipeline stream / events");
@@ -385,31 +398,56 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   if (nframes == 0) return MDSF_OK;
   const size_t esz = dtype == MDSF_F32 ? 4 : 8;
   const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
-  // Two-stage pipeline: frame preparation and deposition on `s`, the FFT
-  // passes (which touch only a density / spectrum buffer and the S(q)
-  // accumulators) on fft_stream, frames alternating between the two density
-  // buffers.  Frame t + 1 is prepared and deposited while frame t is
-  // transformed; depositing frame t waits until the FFT of frame t - 2 has
-  // released its buffer.  The S(q) sum stays in frame order on one stream.
-  hipStream_t f = p->fft_stream;
+  // Three-stage pipeline over the two frame slots (atom state + density
+  // buffer): frame t is prepared on `s`, deposited on dep_stream and
+  // transformed on fft_stream, so that while frame t is deposited frame t + 1
+  // is prepared and frame t - 1 transformed.  A slot is reused two frames
+  // later: preparing frame t waits until frame t - 2's deposition has read
+  // its atom state, depositing it until frame t - 2's FFT has released its
+  // density buffer.  The S(q) sum stays in frame order on one stream.
+  hipStream_t d = p->dep_stream, f = p->fft_stream;
+  if (p->timing) {
+    while ((int64_t)p->tev.size() < 10 * nframes) {
+      hipEvent_t e;
+      MDSF_HIP(hipEventCreate(&e));
+      p->tev.push_back(e);
+    }
+    p->timed_frames = nframes;
+  }
+  auto mark = [&](int64_t t, int k, hipStream_t st) -> hipError_t {
+    return p->timing ? hipEventRecord(p->tev[10 * t + k], st) : hipSuccess;
+  };
   MDSF_HIP(hipEventRecord(p->ev_fork, s));
+  MDSF_HIP(hipStreamWaitEvent(d, p->ev_fork, 0));
   MDSF_HIP(hipStreamWaitEvent(f, p->ev_fork, 0));
-  MDSF_HIP(hipEventRecord(p->ev_fft[0], f));
-  MDSF_HIP(hipEventRecord(p->ev_fft[1], f));
+  for (int b = 0; b < 2; ++b) {
+    MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
+    MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
+  }
   for (int64_t t = 0; t < nframes; ++t) {
     const int b = (int)(t & 1);
     char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
     int32_t* home = (t == nframes - 1) ? d_home_cells : nullptr;
+    p->use_slot(b);
+    MDSF_HIP(hipStreamWaitEvent(s, p->ev_dep[b], 0));
+    MDSF_HIP(mark(t, 0, s));
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
     MDSF_HIP(launch_binning(*p, s));
-    p->use_buffer(b);
-    MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[b], 0));
-    MDSF_HIP(launch_deposit(*p, p->rho, s));
-    MDSF_HIP(hipEventRecord(p->ev_dep, s));
-    MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep, 0));
-    rc = run_fft_accumulate(*p, p->rho, f);
-    if (rc != MDSF_OK) return rc;
+    MDSF_HIP(mark(t, 1, s));
+    MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
+    MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));
+    MDSF_HIP(hipStreamWaitEvent(d, p->ev_fft[b], 0));
+    MDSF_HIP(mark(t, 2, d));
+    MDSF_HIP(launch_deposit(*p, p->rho, d));
+    MDSF_HIP(mark(t, 3, d));
+    MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
+    MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
+    for (int pass = 0; pass < 3; ++pass) {
+      MDSF_HIP(mark(t, 4 + 2 * pass, f));
+      MDSF_HIP(fft_pass(*p, pass, p->rho, f));
+      MDSF_HIP(mark(t, 5 + 2 * pass, f));
+    }
     MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }
   MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) & 1], 0));
@@ -467,11 +505,38 @@ int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t
   return MDSF_OK;
 }
 
+int mdsf_set_stage_timing(mdsf_plan* plan, int32_t enable) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p) return fail(MDSF_ERR_ARG, "null plan");
+  p->timing = enable != 0;
+  p->timed_frames = 0;
+  return MDSF_OK;
+}
+
+int mdsf_stage_times(mdsf_plan* plan, double* avg_ms) {
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  if (!p || !avg_ms) return fail(MDSF_ERR_ARG, "null argument");
+  const int64_t n = p->timed_frames;
+  for (int k = 0; k < 5; ++k) avg_ms[k] = 0.0;
+  if (n == 0) return fail(MDSF_ERR_ARG, "no timed mdsf_accumulate_frames call");
+  MDSF_HIP(hipEventSynchronize(p->tev[10 * (n - 1) + 9]));
+  MDSF_HIP(hipEventSynchronize(p->tev[10 * (n - 1) + 3]));
+  for (int64_t t = 0; t < n; ++t) {
+    for (int k = 0; k < 5; ++k) {
+      float ms = 0.f;
+      MDSF_HIP(hipEventElapsedTime(&ms, p->tev[10 * t + 2 * k], p->tev[10 * t + 2 * k + 1]));
+      avg_ms[k] += ms;
+    }
+  }
+  for (int k = 0; k < 5; ++k) avg_ms[k] /= (double)n;
+  return MDSF_OK;
+}
+
 int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which) {
   Plan* p = reinterpret_cast<Plan*>(plan);
   if (!p) return fail(MDSF_ERR_ARG, "null plan");
   if (which != 0 && which != 1) return fail(MDSF_ERR_ARG, "frame buffer must be 0 or 1");
-  p->use_buffer(which);
+  p->use_slot(which);
   return MDSF_OK;
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/mdsf.h"
 
@@ -77,6 +78,25 @@ struct FftAxis {
   double2* tw64 = nullptr;
 };
 
+// Per-frame atom state (binning, sorted atoms, factor slices / tables): one
+// per frame slot, so that one frame can be prepared while the previous one is
+// deposited.
+struct FrameState {
+  int4* ir4 = nullptr;
+  int* key = nullptr;
+  double* rpos = nullptr;
+  int* bucket_count = nullptr;
+  int* bucket_off = nullptr;
+  int* bucket_cur = nullptr;
+  int* bucket_aux = nullptr;
+  int* s_tmp = nullptr;
+  int* s_idx = nullptr;
+  int4* s_ir = nullptr;
+  double* s_r = nullptr;
+  void* tables = nullptr;
+  float* slices = nullptr;
+};
+
 struct Plan {
   SysDev sys{};
   TypeDev types_host[kMaxTypes]{};
@@ -107,7 +127,24 @@ struct Plan {
   void* rho = nullptr;           // Nx*Ny*Nz folded density (FP32, or FP64 when fft64)
   void* spec_main = nullptr;     // Nx*Ny*(Nz/2) complex, in place over rho
   void* spec_nyq = nullptr;      // Nx*Ny complex: the kz = Nz/2 plane
-  void use_buffer(int b) {
+  FrameState slot[2];
+  // Frame slot b: its atom state and density / spectrum buffer become the
+  // ones the launches below use (kernel arguments are taken at launch).
+  void use_slot(int b) {
+    const FrameState& f = slot[b & 1];
+    ir4 = f.ir4;
+    key = f.key;
+    rpos = f.rpos;
+    bucket_count = f.bucket_count;
+    bucket_off = f.bucket_off;
+    bucket_cur = f.bucket_cur;
+    bucket_aux = f.bucket_aux;
+    s_tmp = f.s_tmp;
+    s_idx = f.s_idx;
+    s_ir = f.s_ir;
+    s_r = f.s_r;
+    tables = f.tables;
+    slices = f.slices;
     rho = rho_buf[b & 1];
     spec_main = rho;
     spec_nyq = nyq_buf[b & 1];
@@ -122,10 +159,17 @@ struct Plan {
   int tune_z = 0, tune_y = 0, tune_x = 0;
   float2* tw_r2c = nullptr;      // W_Nz^k, k < Nz
   double2* tw_r2c64 = nullptr;
-  // frame pipeline: the FFT passes of frame t run on fft_stream while the
-  // caller's stream prepares frame t + 1 (events: fork, density ready, FFT done)
-  hipStream_t fft_stream = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_dep = nullptr, ev_fft[2] = {nullptr, nullptr};
+  // frame pipeline: frame t + 1 is prepared on the caller's stream while frame
+  // t is deposited on dep_stream and frame t - 1 transformed on fft_stream
+  hipStream_t dep_stream = nullptr, fft_stream = nullptr;
+  // optional stage timing of mdsf_accumulate_frames: 10 timing events per
+  // frame (prepare, deposit, z, y, x: start / end, each on its stream)
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  int64_t timed_frames = 0;
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_prep[2] = {nullptr, nullptr}, ev_dep[2] = {nullptr, nullptr},
+             ev_fft[2] = {nullptr, nullptr};
   // adjoint (allocated on first use)
   long long adj_items = 0;
   double* adj_part = nullptr;    // 3 FP64 partial sums per work item

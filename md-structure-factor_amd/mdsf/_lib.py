@@ -62,6 +62,8 @@ SIGNATURES = {
                                      c_void_p, c_void_p, c_void_p]),
     "mdsf_deposit": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mdsf_set_frame_buffer": (c_int32, [c_void_p, c_int32]),
+    "mdsf_set_stage_timing": (c_int32, [c_void_p, c_int32]),
+    "mdsf_stage_times": (c_int32, [c_void_p, ctypes.POINTER(c_double)]),
     "mdsf_fft_pass": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "mdsf_accumulate_density": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mdsf_rfft3": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -221,6 +221,16 @@ class Plan:
             _lib.doubles(box), arith, int(wrap[0]), int(wrap[1]), self._ptr(occupancy),
             self._ptr(home), self._stream()))
 
+    def set_stage_timing(self, enable=True):
+        _lib.check(self.lib.mdsf_set_stage_timing(self._handle, 1 if enable else 0))
+
+    def stage_times(self):
+        """Average ms per frame of the last timed accumulate_frames call:
+        {prepare, deposit, fft_z, fft_y, fft_x_acc}."""
+        out = (_lib.c_double * 5)()
+        _lib.check(self.lib.mdsf_stage_times(self._handle, out))
+        return dict(zip(["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"], list(out)))
+
     def set_frame_buffer(self, which):
         """Select density / spectrum buffer 0 or 1 for the single-frame entry points."""
         _lib.check(self.lib.mdsf_set_frame_buffer(self._handle, int(which)))
