@@ -35,9 +35,8 @@ FP32_PEAK_TFLOPS = 157.3
 METRIC = "frames/s for S(q) at 256^3, 50k atoms"
 # HBM bytes per deposition launch on this workload, from two rocprofv3 --pmc
 # runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by the same
-# counters on a 256 MiB device copy (FETCH x 2.000, WRITE x 1.000):
-# 37,123 KiB read + 66,007 KiB written
-DEPOSIT_TRAFFIC = 143610958
+# counters on a 256 MiB device copy (FETCH x 2.000, WRITE x 1.000)
+DEPOSIT_TRAFFIC = 140864814
 
 
 def log(msg):
@@ -305,6 +304,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the fp32-vs-fp64 S(q) error report")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-isolated", action="store_true",
+                    help="skip the one-stage-at-a-time timing pass")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default FFT launch choices instead of timing them at plan creation")
     ap.add_argument("--batch-frames", type=int, default=50)
@@ -364,8 +365,10 @@ def main():
     total_ms = start.elapsed_time(end)
     per_stage = plan.stage_times()
     log("per-stage avg ms (pipelined): " + json.dumps({k: round(v, 4) for k, v in per_stage.items()}))
-    iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, min(args.steps, 30), stream)
-    log("per-stage avg ms (isolated): " + json.dumps({k: round(v, 4) for k, v in iso.items()}))
+    iso = None
+    if not args.no_isolated:
+        iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, min(args.steps, 30), stream)
+        log("per-stage avg ms (isolated): " + json.dumps({k: round(v, 4) for k, v in iso.items()}))
 
     n = [int(x) for x in s.N]
     fb = fft_bytes(n)
@@ -415,14 +418,15 @@ def main():
                     "updates_per_s": round(updates / (per_stage["deposit"] * 1e-3), 1),
                     "tflops_fma": round(dep_tflops, 3)},
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
-        "stage_ms_isolated": {k: round(v, 4) for k, v in iso.items()},
-        "roofline_isolated": {
+    }
+    if iso is not None:
+        out["stage_ms_isolated"] = {k: round(v, 4) for k, v in iso.items()}
+        out["roofline_isolated"] = {
             "deposit_tflops": round(2.0 * updates / (iso["deposit"] * 1e-3) / 1e12, 3),
             "deposit_frac": round(2.0 * updates / (iso["deposit"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
             "fft_gbs": {k: round(fb[k] / (iso[st] * 1e-3) / 1e9, 1)
                         for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"))},
-            "note": "the same kernels run one at a time on one stream"},
-    }
+            "note": "the same kernels run one at a time on one stream"}
     if not args.no_precision:
         log("precision: fp32 plan vs fp64 plan on 2 frames")
         out["precision"] = precision_leg(coords, s)
