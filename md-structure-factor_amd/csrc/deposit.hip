@@ -1001,10 +1001,9 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
     for (int k0 = 0; k0 < vpad; k0 += 4) {
       int mu;
       if constexpr (MODE == MODE_DIAG) {
-        // x rows 4w..4w+3 where any of the 4 virtual atoms is non-zero
-        const float xr = st.vx[k0 + kk][4 * wave + (jj & 3)];
-        const unsigned long long bl = __ballot(jj < 4 && xr != 0.f);
-        mu = (int)((bl | (bl >> 16) | (bl >> 32) | (bl >> 48)) & 0xf);
+        // every x row: testing rows for support (LDS read + ballot per group)
+        // costs more than the MFMAs it would skip
+        mu = 0xf;
       } else {
         const int m = ((st.vxm[k0] | st.vxm[k0 + 1] | st.vxm[k0 + 2] | st.vxm[k0 + 3]) >> (4 * wave)) & 0xf;
         mu = __builtin_amdgcn_readfirstlane(m);
