@@ -47,6 +47,14 @@ __device__ __forceinline__ float rl<float>(float v, int l) { return rl_f(v, l); 
 template <>
 __device__ __forceinline__ double rl<double>(double v, int l) { return rl_d(v, l); }
 
+// exp(-q) of a non-negative FP64 exponent, evaluated in the plan precision.
+template <typename R>
+__device__ __forceinline__ double pair_exp(double q);
+template <>
+__device__ __forceinline__ double pair_exp<float>(double q) { return (double)expf(-(float)q); }
+template <>
+__device__ __forceinline__ double pair_exp<double>(double q) { return exp(-q); }
+
 // exp of the xy cross term: exp2 with a pre-scaled slope in FP32, exp in FP64.
 __device__ __forceinline__ float cross_exp(float k, float by) { return exp2f(k * by); }
 __device__ __forceinline__ double cross_exp(double k, double by) { return exp(k * by); }
@@ -57,7 +65,14 @@ __global__ __launch_bounds__(kAdjThreads) void k_deposit_adjoint(
     const double* __restrict__ s_r, const int* __restrict__ nvalid, const R* __restrict__ grho,
     double* __restrict__ part, long long nitems) {
   const int lane = threadIdx.x & 63;
-  const long long item = (long long)blockIdx.x * (kAdjThreads / 64) + (threadIdx.x >> 6);
+  // XCD-aware order: workgroups go round-robin over the 8 XCDs, so block b
+  // takes the b/8-th group of XCD (b mod 8)'s contiguous share of the items.
+  // Items follow the atoms' (type, home bucket) order, so each XCD's L2 sees
+  // one spatial band of the gradient grid at a time.
+  const int nblk = gridDim.x, bid = blockIdx.x;
+  const int q8 = nblk / 8;
+  const int blk = bid < 8 * q8 ? (bid % 8) * q8 + bid / 8 : bid;
+  const long long item = (long long)blk * (kAdjThreads / 64) + (threadIdx.x >> 6);
   if (item >= nitems) return;  // uniform per wavefront
   int t = -1;
   for (int u = 0; u < sys.ntypes; ++u) {
@@ -113,8 +128,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_deposit_adjoint(
         const int uy = q.y - Ay + j, uz = q.z - Az + k;
         const double by = ry - (double)uy * sys.dr[1];
         const double bz = rz - (double)uz * sys.dr[2];
-        const double ey = exp(-(sys.g11 * (by * by)) / ts2);
-        const double ez = exp(-(sys.g22 * (bz * bz)) / ts2);
+        // pair factors: the exponent in FP64, exp in the plan precision
+        const double ey = pair_exp<R>((sys.g11 * (by * by)) / ts2);
+        const double ez = pair_exp<R>((sys.g22 * (bz * bz)) / ts2);
         const int sy = side_of(uy, n1), sz = side_of(uz, n2);
         const int yd = wrap_idx(uy, n1), zs = wrap_idx(uz, n2);
         // corner images (all three sides off the core) take the reference's
@@ -126,6 +142,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_deposit_adjoint(
         const long long ostd = (long long)yd * n2 + zs, oalt = (long long)yd * n2 + za;
         const R byr = (R)by;
         R a0 = (R)0, a1 = (R)0;
+#pragma unroll 8
         for (int i = 0; i < xw; ++i) {
           const int xdi = rl_i(xd, i);
           const int xni = rl_i(xnc, i);
@@ -188,6 +205,35 @@ __global__ void k_adjoint_reduce(SysDev sys, const TypeDev* __restrict__ td,
   grad[3 * (size_t)a + 2] = c2 * g2;
 }
 
+// Tiled adjoint: grad[atom] = scale * (sum of the atom's (tile) slots in slot order).
+__global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td,
+                                      const int4* __restrict__ s_ir, const int* __restrict__ s_idx,
+                                      const int* __restrict__ nvalid,
+                                      const double* __restrict__ tpart,
+                                      const double* __restrict__ scale3, double* __restrict__ grad) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= *nvalid) return;
+  const int t = s_ir[s].w & 0xffff;
+  if (t >= sys.ntypes) return;
+  const TypeDev& ty = td[t];
+  const int slot = s - ty.type_start;
+  if (slot < 0 || slot >= ty.count) return;
+  const int per = ty.nsl[0] * ty.nsl[1] * ty.nsl[2];
+  const double* src = tpart + 3 * (ty.adjt_base + (long long)slot * per);
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  for (int it = 0; it < per; ++it) {
+    g0 += src[3 * it + 0];
+    g1 += src[3 * it + 1];
+    g2 += src[3 * it + 2];
+  }
+  const double c0 = scale3 ? scale3[0] : 1.0, c1 = scale3 ? scale3[1] : 1.0,
+               c2 = scale3 ? scale3[2] : 1.0;
+  const int a = s_idx[s];
+  grad[3 * (size_t)a + 0] = c0 * g0;
+  grad[3 * (size_t)a + 1] = c1 * g1;
+  grad[3 * (size_t)a + 2] = c2 * g2;
+}
+
 __global__ void k_fill_zero(double* __restrict__ p, long long n) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -224,12 +270,21 @@ hipError_t launch_adjoint_weights(Plan& p, const double* dsf, hipStream_t s) {
 }
 
 hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale3, double* grad,
-                                  hipStream_t s) {
+                                  hipStream_t s, bool derivs_ready) {
   const int n = p.sys.natoms;
   if (n == 0) return hipSuccess;
   hipError_t e = launch_fill_zero(grad, 3LL * n, s);
   if (e != hipSuccess) return e;
   const int* nvalid = p.bucket_off + p.sys.nbuckets;
+  if (p.adj_tslots > 0 && p.adj_tpart && p.dslices) {
+    // FP32 orthogonal cells: MFMA transpose of the tiled deposition
+    if (!derivs_ready && (e = launch_slice_derivs(p, s)) != hipSuccess) return e;
+    if ((e = launch_fill_zero(p.adj_tpart, 3 * p.adj_tslots, s)) != hipSuccess) return e;
+    if ((e = launch_adjoint_tiles(p, static_cast<const float*>(grho), s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adjoint_tile_reduce, dim3((n + 255) / 256), dim3(256), 0, s, p.sys,
+                       p.types, p.s_ir, p.s_idx, nvalid, p.adj_tpart, scale3, grad);
+    return hipGetLastError();
+  }
   if (p.adj_items > 0) {
     const dim3 grid((unsigned)((p.adj_items + 3) / 4)), block(kAdjThreads);
     if (p.fft64) {

@@ -4,7 +4,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 
 #include "mdsf_internal.h"
 
@@ -37,17 +40,44 @@ hipError_t dalloc(T** p, size_t count) {
   return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
 }
 
+// The frame pipeline's two streams, one pair per device shared by every plan
+// and kept until the process ends.  HIP gives a process few hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default) and hands them to streams in turn, so
+// private streams per plan would soon share queues, and a deposition could
+// no longer overlap an FFT queued behind it.  Each frame loop starts from and
+// joins back into the caller's stream through its own plan's events, so
+// plans sharing the streams stay correctly ordered.
+std::mutex g_stream_mu;
+std::map<int, std::pair<hipStream_t, hipStream_t>> g_streams;
+
+hipError_t pipeline_streams(int device, hipStream_t* dep, hipStream_t* fft) {
+  std::lock_guard<std::mutex> lock(g_stream_mu);
+  auto it = g_streams.find(device);
+  if (it == g_streams.end()) {
+    hipStream_t a = nullptr, b = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      if (a) (void)hipStreamDestroy(a);
+      return e;
+    }
+    it = g_streams.emplace(device, std::make_pair(a, b)).first;
+  }
+  *dep = it->second.first;
+  *fft = it->second.second;
+  return hipSuccess;
+}
+
 void free_plan(Plan* p) {
   if (!p) return;
   for (hipEvent_t e : {p->ev_fork, p->ev_prep[0], p->ev_prep[1], p->ev_dep[0], p->ev_dep[1],
                        p->ev_fft[0], p->ev_fft[1]})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
-  if (p->dep_stream) (void)hipStreamDestroy(p->dep_stream);
-  if (p->fft_stream) (void)hipStreamDestroy(p->fft_stream);
   for (FrameState& f : p->slot) {
     void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
-                  f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices};
+                  f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices,
+                  f.dslices};
     for (void* q : fs)
       if (q) (void)hipFree(q);
   }
@@ -55,7 +85,7 @@ void free_plan(Plan* p) {
                   p->rho_buf[1], p->nyq_buf[1],
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
                   p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
-                  p->adj_w};
+                  p->adj_w,     p->adj_tpart};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -76,6 +106,13 @@ int ensure_adjoint(Plan& p) {
   const size_t m = (size_t)p.sys.n[2] / 2;
   if (!p.adj_part) MDSF_HIP(dalloc(&p.adj_part, 3 * (size_t)std::max(p.adj_items, 1LL)));
   if (!p.adj_w) MDSF_HIP(dalloc(&p.adj_w, nxy * (m + 1)));
+  if (p.adj_tslots > 0 && !p.adj_tpart) {
+    // tiled adjoint: slice derivatives per frame slot and the result slots
+    for (FrameState& f : p.slot)
+      if (!f.dslices) MDSF_HIP(dalloc(&f.dslices, (size_t)p.slice_floats));
+    MDSF_HIP(dalloc(&p.adj_tpart, 3 * (size_t)p.adj_tslots));
+    p.use_slot(p.cur_slot);
+  }
   return MDSF_OK;
 }
 
@@ -211,6 +248,15 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     slice_floats += (long long)ty.count * ty.corner_stride;
   }
   p->slice_floats = (!p->fft64 && s.mode == MODE_DIAG) ? slice_floats : 0;
+  {  // tiled adjoint result slots: per atom one per tile slice triple
+    long long slots = 0;
+    for (int t = 0; t < desc->ntypes; ++t) {
+      TypeDev& ty = p->types_host[t];
+      ty.adjt_base = slots;
+      slots += (long long)ty.count * ty.nsl[0] * ty.nsl[1] * ty.nsl[2];
+    }
+    p->adj_tslots = p->slice_floats > 0 ? slots : 0;
+  }
   // adjoint work items (atom, x-chunk of <= 64 planes, chunk of (y, z)
   // pairs sized for ~16k window voxels), heaviest windows numbered first so
   // that their workgroups start first
@@ -324,8 +370,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     free_plan(p);
     return rc;
   }
-  if ((e = hipStreamCreateWithFlags(&p->dep_stream, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipStreamCreateWithFlags(&p->fft_stream, hipStreamNonBlocking)) != hipSuccess ||
+  if ((e = pipeline_streams(p->device, &p->dep_stream, &p->fft_stream)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_prep[0], hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&p->ev_prep[1], hipEventDisableTiming)) != hipSuccess ||
@@ -613,19 +658,40 @@ int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nfra
   const long long nxy = (long long)p->sys.n[0] * p->sys.n[1];
   const double* w_nyq = p->adj_w + nxy * (p->sys.n[2] / 2);
   MDSF_HIP(launch_adjoint_weights(*p, d_dsf, s));
+  if (nframes == 0) return MDSF_OK;
   const size_t esz = dtype == MDSF_F32 ? 4 : 8;
   const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
+  // Same two-slot pipeline as the forward frame loop: frame t is prepared on
+  // `s`, deposited on dep_stream, and transformed forward, transformed back
+  // with the weights and gathered on fft_stream.  Slot b is reused by frame
+  // t + 2 only after frame t's gather (the last reader of its atom state and
+  // density buffer) has finished.
+  hipStream_t d = p->dep_stream, f = p->fft_stream;
+  MDSF_HIP(hipEventRecord(p->ev_fork, s));
+  MDSF_HIP(hipStreamWaitEvent(d, p->ev_fork, 0));
+  MDSF_HIP(hipStreamWaitEvent(f, p->ev_fork, 0));
+  for (int b = 0; b < 2; ++b) MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   for (int64_t t = 0; t < nframes; ++t) {
+    const int b = (int)(t & 1);
     char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
+    p->use_slot(b);
+    MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[b], 0));
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          nullptr, nullptr, s));
-    MDSF_HIP(launch_binning(*p, s));
-    MDSF_HIP(launch_deposit(*p, p->rho, s));
-    MDSF_HIP(fft_forward(*p, p->rho, false, s));
-    MDSF_HIP(fft_inverse(*p, p->adj_w, w_nyq, p->rho, s));
+    MDSF_HIP(launch_binning(*p, s, /*derivs=*/true));
+    MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
+    MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));
+    MDSF_HIP(launch_deposit(*p, p->rho, d));
+    MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
+    MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
+    MDSF_HIP(fft_forward(*p, p->rho, false, f));
+    MDSF_HIP(fft_inverse(*p, p->adj_w, w_nyq, p->rho, f));
     MDSF_HIP(launch_deposit_adjoint(*p, p->rho, d_scale + 3 * t,
-                                    d_grad + (size_t)t * p->sys.natoms * 3, s));
+                                    d_grad + (size_t)t * p->sys.natoms * 3, f,
+                                    /*derivs_ready=*/true));
+    MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }
+  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) & 1], 0));
   return MDSF_OK;
 }
 

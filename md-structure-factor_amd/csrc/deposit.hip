@@ -1069,6 +1069,248 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the MFMA deposition (FP32 orthogonal cells).  With G = dL/drho
+// on the folded grid and a virtual atom's tile slices X, Y, Z and their
+// derivatives in the atom position dX, dY, dZ, the atom's gradient from one
+// tile is
+//   ( sum G dX Y Z,  sum G X dY Z,  sum G X Y dZ )   over the tile's cells,
+// the exact transpose of the forward rank-1 sum (corner atoms: the sum over
+// their 4 virtual atoms).  The tile of G stays in registers as the A operand
+// (wave w: x rows 4w..4w+3; lane l: y = l % 16, z = 4 (l / 16) + c for
+// c < 4), and per 16 virtual atoms and x row two chains of
+// v_mfma_f32_16x16x4f32 contract z with Z and with dZ:
+//   U[y][k] = sum_z G[x][y][z] Z_k[z],   V[y][k] = sum_z G[x][y][z] dZ_k[z].
+// Each lane then accumulates W1 += dX_k[x] U, W2 += X_k[x] U, W3 += X_k[x] V
+// over the wave's rows, contracts y with Y, dY, Y, and the lanes and waves
+// are summed in a fixed order.  A real atom's result for the tile goes to
+// its own slot (atom, tile slice j0, j1, j2): no atomics, and a second kernel
+// adds the slots in a fixed order.
+struct StageAdj {
+  __align__(16) float vx[NV][T];
+  __align__(16) float vy[NV][T];
+  __align__(16) float vz[NV][T];
+  __align__(16) float dx[NV][T];
+  __align__(16) float dy[NV][T];
+  __align__(16) float dz[NV][T];
+  float red[kThreads / 64][NV][3];  // per-wave sums of each virtual atom
+};
+
+template <int MT>
+__global__ __launch_bounds__(kThreads) void k_adjoint_mfma(
+    SysDev sys, const TypeDev* __restrict__ td, const int* __restrict__ boff,
+    const int4* __restrict__ s_ir, const float* __restrict__ slices,
+    const float* __restrict__ dslices, const float* __restrict__ grho,
+    double* __restrict__ tpart) {
+  constexpr int EXTRA = 3;  // extra virtual slots of a corner atom
+  __shared__ StageAdj st;
+  __shared__ EnumShared<MT> sh;
+  __shared__ struct {
+    int vbase[NV], nv[NV];
+    int4 q[NV];
+    long long so[NV][3];  // tile slice offset per axis, -1: none
+    long long co[NV][3];  // corner component 0 of that slice, -1: none
+    int cs[NV][3];        // corner component stride (floats)
+    int clist[NV];        // batch indices of the corner atoms
+    long long pidx[NV];   // result slot, -1: none
+  } bt;
+  __shared__ struct {
+    long long slice_base[MT], corner_base[MT], part_base[MT];
+    int slice_stride[MT], corner_stride[MT], type_start[MT], count[MT], nsl[MT][3];
+  } tyl;
+  for (int i = threadIdx.x; i < sys.ntypes; i += kThreads) {
+    tyl.slice_base[i] = td[i].slice_base;
+    tyl.corner_base[i] = td[i].corner_base;
+    tyl.part_base[i] = td[i].adjt_base;
+    tyl.slice_stride[i] = td[i].slice_stride;
+    tyl.corner_stride[i] = td[i].corner_stride;
+    tyl.type_start[i] = td[i].type_start;
+    tyl.count[i] = td[i].count;
+    tyl.nsl[i][0] = td[i].nsl[0];
+    tyl.nsl[i][1] = td[i].nsl[1];
+    tyl.nsl[i][2] = td[i].nsl[2];
+  }
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int n16 = lane & 15, q4l = lane >> 4;
+  Tile tl;
+  tile_coords(sys, tl);
+  // this wave's share of the tile, zero outside the grid
+  float g[4][4];
+  {
+    const int gy = tl.t0[1] + n16;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int gx = tl.t0[0] + 4 * wave + b;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int gz = tl.t0[2] + 4 * q4l + c;
+        g[b][c] = (gx < tl.t1[0] && gy < tl.t1[1] && gz < tl.t1[2])
+                      ? grho[((size_t)gx * sys.n[1] + gy) * sys.n[2] + gz]
+                      : 0.f;
+      }
+    }
+  }
+
+  auto process = [&](int count, int head) -> int {
+    // separates the previous batch's reads of the staging area and of bt
+    // from this batch's writes
+    __syncthreads();
+    const int e = lane < count ? sh.list[(head + lane) & kListMask] : 0;
+    const bool isc = lane < count && (e < 0);
+    const unsigned long long cm = __ballot(isc);
+    const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+    const int vupto = (lane + 1) + EXTRA * __popcll(cm & le);
+    const bool fits = lane < count && vupto <= NV;
+    const int nreal = __popcll(__ballot(fits));
+    const int ncorner = __popcll(__ballot(fits && isc));
+    const int vtot = nreal > 0 ? __shfl(vupto, nreal - 1, 64) : 0;
+    const int vpad = (vtot + 15) & ~15;
+    if (wave == 0 && lane < nreal) {
+      const int c = e & 0x7fffffff;
+      const int4 q = s_ir[c];
+      bt.vbase[lane] = lane + EXTRA * __popcll(cm & (le >> 1));
+      bt.nv[lane] = isc ? EXTRA + 1 : 1;
+      bt.q[lane] = q;
+      const int t0 = q.w & 0xffff;
+      const bool tv = t0 < sys.ntypes;
+      const int t = tv ? t0 : 0;
+      const int slot = c - tyl.type_start[t];
+      const bool ok = tv && slot >= 0 && slot < tyl.count[t];  // false only after a reported error
+      long long off = tyl.slice_base[t] + (long long)slot * tyl.slice_stride[t];
+      long long coff = tyl.corner_base[t] + (long long)slot * tyl.corner_stride[t];
+      long long jj = 0;
+      bool all = ok;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        // same slice numbering as the forward: j = (tile - T0) mod ntiles
+        const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+        const int N = sys.n[d], nt = sys.ntiles[d], ns = tyl.nsl[t][d];
+        int lo = ir - sh.half[t][d];
+        lo = lo < 0 ? lo + N : (lo >= N ? lo - N : lo);
+        int j = tl.t0[d] / T - lo / T;
+        if (j < 0) j += nt;
+        const bool has = ok && j < ns;
+        bt.so[lane][d] = has ? off + (long long)T * j : -1LL;
+        bt.co[lane][d] = has ? coff + (long long)T * j : -1LL;
+        bt.cs[lane][d] = T * ns;
+        off += (long long)T * ns;
+        coff += (long long)(d == 1 ? 3 : 2) * T * ns;
+        jj = jj * ns + j;
+        all = all && has;
+      }
+      const long long nslp = (long long)tyl.nsl[t][0] * tyl.nsl[t][1] * tyl.nsl[t][2];
+      bt.pidx[lane] = all ? tyl.part_base[t] + (long long)slot * nslp + jj : -1LL;
+      if (isc) bt.clist[__popcll(cm & ((1ull << lane) - 1))] = lane;
+    }
+    __syncthreads();
+    // atoms off the corner regions: values and derivatives of their three
+    // tile slices (float4 per thread)
+    for (int it = tid; it < nreal * 12; it += kThreads) {
+      const int a = it / 12, r12 = it - 12 * (it / 12), d = r12 >> 2, qq = r12 & 3;
+      if ((bt.q[a].w >> 16) != 0) continue;  // corner atom: below
+      const long long off = bt.so[a][d];
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f), w = v;
+      if (off >= 0) {
+        v = *reinterpret_cast<const float4*>(slices + off + 4 * qq);
+        w = *reinterpret_cast<const float4*>(dslices + off + 4 * qq);
+      }
+      const int vb = bt.vbase[a];
+      float* row = d == 0 ? st.vx[vb] : (d == 1 ? st.vy[vb] : st.vz[vb]);
+      float* drow = d == 0 ? st.dx[vb] : (d == 1 ? st.dy[vb] : st.dz[vb]);
+      *reinterpret_cast<float4*>(row + 4 * qq) = v;
+      *reinterpret_cast<float4*>(drow + 4 * qq) = w;
+    }
+    for (int j = tid; j < (vpad - vtot) * T; j += kThreads) {
+      const int v = vtot + j / T, cc = j % T;
+      st.vx[v][cc] = st.vy[v][cc] = st.vz[v][cc] = 0.f;
+      st.dx[v][cc] = st.dy[v][cc] = st.dz[v][cc] = 0.f;
+    }
+    // corner atoms: (Xc, Yall, Zs), (Xn, Yc, Zs), (Xn, Yl, ZL), (Xn, Yh, ZH)
+    for (int it = tid; it < ncorner * 48; it += kThreads) {
+      const int ca = it / 48, r48 = it - 48 * ca;
+      const int v = r48 / 12, r12 = r48 - 12 * v, d = r12 >> 2, qq = r12 & 3;
+      const int a = bt.clist[ca];
+      const long long cb = bt.co[a][d];
+      long long off;
+      if (d == 0) off = cb >= 0 ? cb + (v == 0 ? 0 : bt.cs[a][0]) : -1LL;
+      else if (d == 1) off = v == 0 ? bt.so[a][1] : (cb >= 0 ? cb + (long long)(v - 1) * bt.cs[a][1] : -1LL);
+      else off = v < 2 ? bt.so[a][2] : (cb >= 0 ? cb + (long long)(v - 2) * bt.cs[a][2] : -1LL);
+      float4 val = make_float4(0.f, 0.f, 0.f, 0.f), dval = val;
+      if (off >= 0) {
+        val = *reinterpret_cast<const float4*>(slices + off + 4 * qq);
+        dval = *reinterpret_cast<const float4*>(dslices + off + 4 * qq);
+      }
+      const int vb = bt.vbase[a] + v;
+      float* row = d == 0 ? st.vx[vb] : (d == 1 ? st.vy[vb] : st.vz[vb]);
+      float* drow = d == 0 ? st.dx[vb] : (d == 1 ? st.dy[vb] : st.dz[vb]);
+      *reinterpret_cast<float4*>(row + 4 * qq) = val;
+      *reinterpret_cast<float4*>(drow + 4 * qq) = dval;
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < vpad; k0 += 16) {
+      const int k = k0 + n16;
+      const float4 zf = *reinterpret_cast<const float4*>(&st.vz[k][4 * q4l]);
+      const float4 dzf = *reinterpret_cast<const float4*>(&st.dz[k][4 * q4l]);
+      v4f w1 = v4f{0.f, 0.f, 0.f, 0.f}, w2 = w1, w3 = w1;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        v4f u = v4f{0.f, 0.f, 0.f, 0.f}, vv = u;
+        u = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][0], zf.x, u, 0, 0, 0);
+        vv = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][0], dzf.x, vv, 0, 0, 0);
+        u = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][1], zf.y, u, 0, 0, 0);
+        vv = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][1], dzf.y, vv, 0, 0, 0);
+        u = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][2], zf.z, u, 0, 0, 0);
+        vv = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][2], dzf.z, vv, 0, 0, 0);
+        u = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][3], zf.w, u, 0, 0, 0);
+        vv = __builtin_amdgcn_mfma_f32_16x16x4f32(g[b][3], dzf.w, vv, 0, 0, 0);
+        const float ax = st.vx[k][4 * wave + b], adx = st.dx[k][4 * wave + b];
+        w1 += adx * u;
+        w2 += ax * u;
+        w3 += ax * vv;
+      }
+      // U, V rows held by this lane: y = 4 (lane / 16) + r
+      const float4 yf = *reinterpret_cast<const float4*>(&st.vy[k][4 * q4l]);
+      const float4 dyf = *reinterpret_cast<const float4*>(&st.dy[k][4 * q4l]);
+      float sx = yf.x * w1[0] + yf.y * w1[1] + yf.z * w1[2] + yf.w * w1[3];
+      float sy = dyf.x * w2[0] + dyf.y * w2[1] + dyf.z * w2[2] + dyf.w * w2[3];
+      float sz = yf.x * w3[0] + yf.y * w3[1] + yf.z * w3[2] + yf.w * w3[3];
+      sx += __shfl_xor(sx, 16, 64);
+      sy += __shfl_xor(sy, 16, 64);
+      sz += __shfl_xor(sz, 16, 64);
+      sx += __shfl_xor(sx, 32, 64);
+      sy += __shfl_xor(sy, 32, 64);
+      sz += __shfl_xor(sz, 32, 64);
+      if (q4l == 0) {
+        st.red[wave][k][0] = sx;
+        st.red[wave][k][1] = sy;
+        st.red[wave][k][2] = sz;
+      }
+    }
+    __syncthreads();
+    // per real atom: its virtual atoms over the waves, in a fixed order
+    if (tid < nreal) {
+      const long long pi = bt.pidx[tid];
+      if (pi >= 0) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        const int vb = bt.vbase[tid], nv = bt.nv[tid];
+        for (int v = vb; v < vb + nv; ++v)
+#pragma unroll
+          for (int w = 0; w < kThreads / 64; ++w) {
+            ax += (double)st.red[w][v][0];
+            ay += (double)st.red[w][v][1];
+            az += (double)st.red[w][v][2];
+          }
+        tpart[3 * pi + 0] = ax;
+        tpart[3 * pi + 1] = ay;
+        tpart[3 * pi + 2] = az;
+      }
+    }
+    return nreal;
+  };
+  enumerate_tile(sys, td, boff, s_ir, tl, sh, NV, process);
+}
+
 template <typename ACC>
 hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
   const SysDev& sys = p.sys;
@@ -1106,6 +1348,20 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s) {
   if (p.fft64) return launch_impl<double>(p, static_cast<double*>(out), s);
   return launch_impl<float>(p, static_cast<float*>(out), s);
+}
+
+hipError_t launch_adjoint_tiles(Plan& p, const float* grho, hipStream_t s) {
+  const SysDev& sys = p.sys;
+  const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
+  if (ntiles == 0) return hipSuccess;
+  const dim3 grid((unsigned)ntiles), block(kThreads);
+  if (sys.ntypes <= 16)
+    hipLaunchKernelGGL(k_adjoint_mfma<16>, grid, block, 0, s, sys, p.types, p.bucket_off, p.s_ir,
+                       p.slices, p.dslices, grho, p.adj_tpart);
+  else
+    hipLaunchKernelGGL(k_adjoint_mfma<kMaxTypes>, grid, block, 0, s, sys, p.types, p.bucket_off,
+                       p.s_ir, p.slices, p.dslices, grho, p.adj_tpart);
+  return hipGetLastError();
 }
 
 }  // namespace mdsf

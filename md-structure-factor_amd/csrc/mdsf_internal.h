@@ -53,6 +53,9 @@ struct TypeDev {
   int adj_ppi;
   int adj_npc;
   long long adj_item_base;
+  // tiled adjoint (FP32 DIAG): one result slot of 3 FP64 per (atom, tile
+  // slice triple j0, j1, j2), nsl[0] nsl[1] nsl[2] per atom, from adjt_base
+  long long adjt_base;
 };
 
 // Per-system constants passed by value to kernels.
@@ -95,6 +98,7 @@ struct FrameState {
   double* s_r = nullptr;
   void* tables = nullptr;
   float* slices = nullptr;
+  float* dslices = nullptr;  // derivatives of the slices in the atom position (adjoint)
 };
 
 struct Plan {
@@ -118,6 +122,7 @@ struct Plan {
   double* s_r = nullptr;
   void* tables = nullptr;         // per-atom factor tables (FP64 plans)
   float* slices = nullptr;        // per-atom tile slices (FP32 plans, orthogonal-axis cells)
+  float* dslices = nullptr;       // their derivatives (adjoint only)
   long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
   // grids: two density / spectrum buffers so that one frame's FFT can run
@@ -130,7 +135,9 @@ struct Plan {
   FrameState slot[2];
   // Frame slot b: its atom state and density / spectrum buffer become the
   // ones the launches below use (kernel arguments are taken at launch).
+  int cur_slot = 0;
   void use_slot(int b) {
+    cur_slot = b & 1;
     const FrameState& f = slot[b & 1];
     ir4 = f.ir4;
     key = f.key;
@@ -145,6 +152,7 @@ struct Plan {
     s_r = f.s_r;
     tables = f.tables;
     slices = f.slices;
+    dslices = f.dslices;
     rho = rho_buf[b & 1];
     spec_main = rho;
     spec_nyq = nyq_buf[b & 1];
@@ -161,6 +169,7 @@ struct Plan {
   double2* tw_r2c64 = nullptr;
   // frame pipeline: frame t + 1 is prepared on the caller's stream while frame
   // t is deposited on dep_stream and frame t - 1 transformed on fft_stream
+  // (both shared by all plans on the device, never destroyed)
   hipStream_t dep_stream = nullptr, fft_stream = nullptr;
   // optional stage timing of mdsf_accumulate_frames: 10 timing events per
   // frame (prepare, deposit, z, y, x: start / end, each on its stream)
@@ -173,6 +182,8 @@ struct Plan {
   // adjoint (allocated on first use)
   long long adj_items = 0;
   double* adj_part = nullptr;    // 3 FP64 partial sums per work item
+  long long adj_tslots = 0;      // tiled adjoint result slots
+  double* adj_tpart = nullptr;   // 3 FP64 per slot
   double* adj_w = nullptr;       // 2 dL/dS split into main (Nx*Ny*Nz/2) + Nyquist (Nx*Ny)
 };
 
@@ -192,8 +203,16 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
                        const double* scale3, const double box[3], int64_t wrap_lo,
                        int64_t wrap_hi, uint32_t* occ, int32_t* home, hipStream_t s,
                        bool coords_only = false, bool write_back = true);
-hipError_t launch_binning(Plan& p, hipStream_t s);
+// sorts the prepared atoms and builds their tile slices; with `derivs` (and
+// dslices allocated) also the slices' derivatives for the tiled adjoint
+hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false);
+// only the slice derivatives of the last sorted frame
+hipError_t launch_slice_derivs(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);
+// tiled adjoint: adj_tpart slots <- per (atom, tile) gradient sums from
+// dL/drho (FP32 DIAG plans; slots of tiles an atom does not reach are left
+// untouched)
+hipError_t launch_adjoint_tiles(Plan& p, const float* grho, hipStream_t s);
 
 hipError_t fft_make_twiddles(float2* tw, double2* tw64, int n, hipStream_t s);
 // density (FP32, Nx*Ny*Nz) -> spectrum; with `accumulate` the x pass adds
@@ -214,9 +233,10 @@ hipError_t launch_fill_zero(double* p, long long n, hipStream_t s);
 // adj_w <- 2 dsf split into main / Nyquist parts
 hipError_t launch_adjoint_weights(Plan& p, const double* dsf, hipStream_t s);
 // grad[atom] = scale3 * dL/dr for the last prepared frame from dL/drho (grho,
-// plan precision); scale3 may be null (unit scale)
+// plan precision); scale3 may be null (unit scale).  `derivs_ready`: the
+// binning of this frame already built the slice derivatives.
 hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale3, double* grad,
-                                  hipStream_t s);
+                                  hipStream_t s, bool derivs_ready = false);
 
 hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s);
 hipError_t launch_outputs(const int n[3], const double* sf, const double box[3], int box_f32,

@@ -259,7 +259,7 @@ __global__ void k_tables(SysDev sys, const TypeDev* __restrict__ td, const int4*
 // are zero.
 __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4* __restrict__ s_ir,
                          const double* __restrict__ s_r, const int* __restrict__ nvalid,
-                         float* __restrict__ slices) {
+                         float* __restrict__ slices, float* __restrict__ dslices, int mode) {
 #pragma clang fp contract(off)
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const int pair = (int)(g / kTile), i = (int)(g - (long long)pair * kTile);
@@ -283,36 +283,51 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
   const double r = s_r[3 * (size_t)s + d];
   const int lo = ir - A, hi = ir + A;
   const int off_d = d == 0 ? 0 : (d == 1 ? kTile * ty.nsl[0] : kTile * (ty.nsl[0] + ty.nsl[1]));
-  float* out = slices + ty.slice_base + (long long)slot * ty.slice_stride + off_d + i;
+  const long long aoff = ty.slice_base + (long long)slot * ty.slice_stride + off_d + i;
   const bool corner = (q.w >> 16) != 0;
   const int coff_d = d == 0 ? 0 : (d == 1 ? 2 * kTile * ty.nsl[0]
                                           : 2 * kTile * ty.nsl[0] + 3 * kTile * ty.nsl[1]);
-  float* cout = slices + ty.corner_base + (long long)slot * ty.corner_stride + coff_d + i;
-  auto f = [&](int u) -> float {  // factor of the image at grid coordinate u, 0 outside the window
+  const long long acoff = ty.corner_base + (long long)slot * ty.corner_stride + coff_d + i;
+  // mode 0: slices, 1: slices and derivatives, 2: derivatives only
+  const bool wv = mode != 2, wd = mode != 0;
+  // factor of the image at grid coordinate u, 0 outside the window, and its
+  // derivative in r: am exp(-gk b^2) and am exp(-gk b^2) (-2 gk b), b = r - u dr
+  auto f = [&](int u, float& df) -> float {
+    df = 0.f;
     if (u < lo || u >= hi) return 0.f;
     const double b = r - (double)u * dr;
-    return am * expf(-(float)(gk * (b * b)));
+    const float v = am * expf(-(float)(gk * (b * b)));
+    df = v * (float)(-2.0 * gk * b);
+    return v;
+  };
+  auto put = [&](long long o, float v, float dv) {
+    if (wv) slices[o] = v;
+    if (wd) dslices[o] = dv;
   };
   int T = pos_mod(lo, N) / kTile;
   for (int j = 0; j < nsl; ++j) {
     const int c = T * kTile + i;
     const bool in = c < N;
-    const float vL = in ? f(c - N) : 0.f, vC = in ? f(c) : 0.f, vH = in ? f(c + N) : 0.f;
-    out[j * kTile] = vC + vL + vH;
+    float dL = 0.f, dC = 0.f, dH = 0.f;
+    const float vL = in ? f(c - N, dL) : 0.f, vC = in ? f(c, dC) : 0.f, vH = in ? f(c + N, dH) : 0.f;
+    put(aoff + j * kTile, vC + vL + vH, dC + dL + dH);
     if (corner) {
       // the components the corner virtual atoms take (dens.py:101-107): on z
       // the slots where the reference's corner copy lands for y on L / on H
       if (d == 0) {
-        cout[j * kTile] = vC;
-        cout[(nsl + j) * kTile] = vL + vH;
+        put(acoff + j * kTile, vC, dC);
+        put(acoff + (nsl + j) * kTile, vL + vH, dL + dH);
       } else if (d == 1) {
-        cout[j * kTile] = vC;
-        cout[(nsl + j) * kTile] = vL;
-        cout[(2 * nsl + j) * kTile] = vH;
+        put(acoff + j * kTile, vC, dC);
+        put(acoff + (nsl + j) * kTile, vL, dL);
+        put(acoff + (2 * nsl + j) * kTile, vH, dH);
       } else {
         const int u1 = c + sys.nb, u2 = c - sys.nb;
-        cout[j * kTile] = vC + vL + (in && u1 >= N ? f(u1) : 0.f);
-        cout[(nsl + j) * kTile] = vC + vH + (in && u2 < 0 ? f(u2) : 0.f);
+        float d1 = 0.f, d2 = 0.f;
+        const float v1 = in && u1 >= N ? f(u1, d1) : 0.f;
+        const float v2 = in && u2 < 0 ? f(u2, d2) : 0.f;
+        put(acoff + j * kTile, vC + vL + v1, dC + dL + d1);
+        put(acoff + (nsl + j) * kTile, vC + vH + v2, dC + dH + d2);
       }
     }
     T = T + 1 == nt ? 0 : T + 1;
@@ -346,7 +361,16 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
   return hipGetLastError();
 }
 
-hipError_t launch_binning(Plan& p, hipStream_t s) {
+hipError_t launch_slice_derivs(Plan& p, hipStream_t s) {
+  const int n = p.sys.natoms;
+  if (n == 0 || p.slice_floats == 0 || !p.dslices) return hipSuccess;
+  hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0, s,
+                     p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + p.sys.nbuckets, p.slices,
+                     p.dslices, 2);
+  return hipGetLastError();
+}
+
+hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
   const int n = p.sys.natoms;
   const int nbk = p.sys.nbuckets;
   const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
@@ -364,7 +388,8 @@ hipError_t launch_binning(Plan& p, hipStream_t s) {
                        p.s_ir, p.s_r, p.bucket_off + nbk);
     if (p.slice_floats > 0)
       hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0,
-                         s, p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices);
+                         s, p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices,
+                         p.dslices, (derivs && p.dslices) ? 1 : 0);
     // factor tables feed the FP64 gather
     const long long total = p.fft64 ? p.table_floats : 0;
     if (total > 0) {
