@@ -293,6 +293,25 @@ def dropin_leg(rad, nframes):
     return out
 
 
+def cylindrical_leg(plan, box, ucell):
+    """The r-theta-z average of PLOT_RAD_NEW (plot2d.py:559-655) on the 256^3
+    kgridplt of the timed frames (GPU outputs, then engine.rz_average)."""
+    import torch
+    from mdsf import engine
+    _, _, kp = plan.outputs(plan.sf(), box, False)
+    out = engine.rz_average(kp, ucell)          # first call: bins, library load
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = engine.rz_average(kp, ucell)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    samples = int(np.sum([max(int(engine.THETA_BINS_PER_INV_ANG * 2.0 * np.pi * r), 1)
+                          for r in out["rarr"]])) * len(out["zar"])
+    return {"workload": "rz_average of the hii 256^3 kgridplt (400 r bins x 255 z)",
+            "ms": dt * 1e3, "interpolations": samples,
+            "interpolations_per_s": samples / dt, "ucell": "orthogonal (the workload's cell)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,6 +331,8 @@ def main():
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in compute_sf leg")
     ap.add_argument("--dropin-frames", type=int, default=200)
     ap.add_argument("--adj-iters", type=int, default=5)
+    ap.add_argument("--no-cylindrical", action="store_true",
+                    help="skip the cylindrical-average (PLOT_RAD_NEW) leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy 3 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -427,6 +448,9 @@ def main():
             "fft_gbs": {k: round(fb[k] / (iso[st] * 1e-3) / 1e9, 1)
                         for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"))},
             "note": "the same kernels run one at a time on one stream"}
+    if not args.no_cylindrical:
+        log("cylindrical average of the 256^3 kgridplt")
+        out["cylindrical"] = cylindrical_leg(plan, box, np.eye(3))
     if not args.no_precision:
         log("precision: fp32 plan vs fp64 plan on 2 frames")
         out["precision"] = precision_leg(coords, s)

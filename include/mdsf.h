@@ -259,6 +259,25 @@ int mdsf_ewald_slices(const double* d_kgridplt, int32_t nx, int32_t ny,
                       int32_t nz, double wavelength, double* d_xy, double* d_xz,
                       double* d_yz, mdsf_stream stream);
 
+/* Cylindrical average over theta, the r-theta-z interpolation of
+ * PLOT_RAD_NEW (plot2d.py:595-627, run by main_gromacs.py:231 through
+ * Plot_Ewald_triclinic, plot2d.py:1010):
+ *   d_oa[ir * nzb + iz] = mean over k < d_ntheta[ir] of S at
+ *   (r cos t_k, r sin t_k, z) @ binv,  t_k = k * 2 pi / d_ntheta[ir],
+ *   r = d_rarr[ir], z = d_zar[iz], S the trilinear interpolant of channel 3
+ *   of kgridplt (RegularGridInterpolator, linear); NaN when a sample leaves
+ *   the grid.  binv: row-major 3x3 (the inverse of the reciprocal basis).
+ *   Requires d_ntheta[ir] >= 1. */
+int mdsf_rz_average(const double* d_kgridplt, int32_t nx, int32_t ny, int32_t nz,
+                    const double* d_rarr, const int32_t* d_ntheta, int32_t nr,
+                    const double* d_zar, int32_t nzb, const double binv[9], double* d_oa,
+                    mdsf_stream stream);
+
+/* The same interpolant at npts points d_pts (npts, 3) -> d_out (npts), NaN
+ * outside the grid (the plain-grid sections of plot2d.py:800-840). */
+int mdsf_interp_points(const double* d_kgridplt, int32_t nx, int32_t ny, int32_t nz,
+                       const double* d_pts, int64_t npts, double* d_out, mdsf_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

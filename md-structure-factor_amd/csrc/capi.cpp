@@ -722,4 +722,30 @@ int mdsf_ewald_slices(const double* d_kgridplt, int32_t nx, int32_t ny, int32_t 
   return MDSF_OK;
 }
 
+int mdsf_rz_average(const double* d_kgridplt, int32_t nx, int32_t ny, int32_t nz,
+                    const double* d_rarr, const int32_t* d_ntheta, int32_t nr,
+                    const double* d_zar, int32_t nzb, const double binv[9], double* d_oa,
+                    mdsf_stream stream) {
+  if (!d_kgridplt || !binv) return fail(MDSF_ERR_ARG, "null kgridplt or binv");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(MDSF_ERR_ARG, "each axis needs at least 2 points");
+  if (nr < 0 || nzb < 0) return fail(MDSF_ERR_ARG, "negative bin count");
+  if ((long long)nr * nzb == 0) return MDSF_OK;
+  if (!d_rarr || !d_ntheta || !d_zar || !d_oa) return fail(MDSF_ERR_ARG, "null bin array");
+  MDSF_HIP(launch_rz_average(d_kgridplt, nx, ny, nz, d_rarr, d_ntheta, nr, d_zar, nzb, binv, d_oa,
+                             reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_interp_points(const double* d_kgridplt, int32_t nx, int32_t ny, int32_t nz,
+                       const double* d_pts, int64_t npts, double* d_out, mdsf_stream stream) {
+  if (!d_kgridplt) return fail(MDSF_ERR_ARG, "null kgridplt");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(MDSF_ERR_ARG, "each axis needs at least 2 points");
+  if (npts < 0) return fail(MDSF_ERR_ARG, "negative point count");
+  if (npts == 0) return MDSF_OK;
+  if (!d_pts || !d_out) return fail(MDSF_ERR_ARG, "null points or output");
+  MDSF_HIP(launch_interp_points(d_kgridplt, nx, ny, nz, d_pts, npts, d_out,
+                                reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
 }  // extern "C"

@@ -243,5 +243,10 @@ hipError_t launch_outputs(const int n[3], const double* sf, const double box[3],
                           double* sfplt, double* kgrid, double* kgridplt, hipStream_t s);
 hipError_t launch_ewald(const double* kp, int nx, int ny, int nz, double wl, double* xy,
                         double* xz, double* yz, hipStream_t s);
+hipError_t launch_rz_average(const double* kp, int nx, int ny, int nz, const double* rarr,
+                             const int* nth, int nr, const double* zar, int nzb,
+                             const double binv[9], double* oa, hipStream_t s);
+hipError_t launch_interp_points(const double* kp, int nx, int ny, int nz, const double* pts,
+                                long long n, double* out, hipStream_t s);
 
 }  // namespace mdsf

@@ -97,18 +97,21 @@ __global__ void k_outputs(const double* __restrict__ sf, int nx, int ny, int nz,
 // --- Ewald sections: RegularGridInterpolator(method="linear", bounds_error=False)
 __device__ int find_cell(const double* __restrict__ kp, long long stride, int n, double x,
                          bool* oob) {
-  // grid values g[i] = kp[i * stride]; i = searchsorted(g, x, 'left') - 1, clipped.
+  // grid values g[i] = kp[i * stride]; i = searchsorted(g, x, 'left') - 1,
+  // clipped to [0, n - 2], i.e. g[i] < x <= g[i + 1] inside the grid.  The
+  // k-grid axes are evenly spaced, so the cell is guessed from the spacing
+  // and then moved until the comparisons against the stored values hold
+  // (the same cell a binary search finds, in one or two loads).
   const double g0 = kp[0], gl = kp[(long long)(n - 1) * stride];
-  if (x < g0 || x > gl || x != x) *oob = true;
-  int lo = 0, hi = n;  // first index with g >= x
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (kp[(long long)mid * stride] < x) lo = mid + 1;
-    else hi = mid;
+  if (x < g0 || x > gl || x != x) {
+    *oob = true;
+    return 0;
   }
-  int i = lo - 1;
-  if (i < 0) i = 0;
-  if (i > n - 2) i = n - 2;
+  const double h = (gl - g0) / (double)(n - 1);
+  int i = h > 0.0 ? (int)floor((x - g0) / h) : 0;
+  i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+  while (i > 0 && kp[(long long)i * stride] >= x) --i;
+  while (i < n - 2 && kp[(long long)(i + 1) * stride] < x) ++i;
   return i;
 }
 
@@ -172,6 +175,52 @@ __global__ void k_ewald(const double* __restrict__ kp, int nx, int ny, int nz, d
   }
 }
 
+// --- cylindrical (r, z) average: the r-theta-z interpolation of PLOT_RAD_NEW
+// (plot2d.py:595-627).  oa[ir][iz] = mean over k < nth[ir] of the
+// interpolant at (r cos t, r sin t, z) @ binv, t = k (2 pi / nth[ir]),
+// r = rarr[ir], z = zar[iz]; NaN when any sample leaves the grid (the
+// reference's np.average propagates the interpolator's NaN fill).  The
+// theta sum runs in sample order, as numpy's reduction over that axis does.
+struct Mat3 {
+  double m[9];  // row-major
+};
+
+__global__ void k_rz_average(const double* __restrict__ kp, int nx, int ny, int nz,
+                             const double* __restrict__ rarr, const int* __restrict__ nth, int nr,
+                             const double* __restrict__ zar, int nzb, Mat3 b,
+                             double* __restrict__ oa) {
+#pragma clang fp contract(off)
+  const long long total = (long long)nr * nzb;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ir = (int)(i / nzb), iz = (int)(i - (long long)ir * nzb);
+    const double r = rarr[ir], z = zar[iz];
+    const int nt = nth[ir];
+    const double step = (2.0 * 3.141592653589793) / (double)nt;  // np.linspace(0, 2 pi, nt, endpoint=False)
+    double sum = 0.0;
+    for (int k = 0; k < nt; ++k) {
+      const double t = (double)k * step;
+      const double x = r * cos(t), y = r * sin(t);
+      // (x, y, z) @ binv
+      const double u = x * b.m[0] + y * b.m[3] + z * b.m[6];
+      const double v = x * b.m[1] + y * b.m[4] + z * b.m[7];
+      const double w = x * b.m[2] + y * b.m[5] + z * b.m[8];
+      sum = sum + interp3(kp, nx, ny, nz, u, v, w);
+    }
+    oa[i] = sum / (double)nt;
+  }
+}
+
+// Interpolant at arbitrary points (n, 3): the RegularGridInterpolator calls
+// on plain grids (plot2d.py:588, 800-840).
+__global__ void k_interp_points(const double* __restrict__ kp, int nx, int ny, int nz,
+                                const double* __restrict__ pts, long long n,
+                                double* __restrict__ out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = interp3(kp, nx, ny, nz, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+}
+
 unsigned grid_for(long long total) {
   long long b = (total + 255) / 256;
   if (b > 65536) b = 65536;
@@ -202,6 +251,24 @@ hipError_t launch_ewald(const double* kp, int nx, int ny, int nz, double wl, dou
   const long long total = (long long)nx * ny + (long long)nx * nz + (long long)ny * nz;
   hipLaunchKernelGGL(k_ewald, dim3(grid_for(total)), dim3(256), 0, s, kp, nx, ny, nz, K, xy, xz,
                      yz);
+  return hipGetLastError();
+}
+
+hipError_t launch_rz_average(const double* kp, int nx, int ny, int nz, const double* rarr,
+                             const int* nth, int nr, const double* zar, int nzb,
+                             const double binv[9], double* oa, hipStream_t s) {
+  Mat3 b;
+  for (int k = 0; k < 9; ++k) b.m[k] = binv[k];
+  hipLaunchKernelGGL(k_rz_average, dim3(grid_for((long long)nr * nzb)), dim3(256), 0, s, kp, nx, ny,
+                     nz, rarr, nth, nr, zar, nzb, b, oa);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_points(const double* kp, int nx, int ny, int nz, const double* pts,
+                                long long n, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_interp_points, dim3(grid_for(n)), dim3(256), 0, s, kp, nx, ny, nz, pts, n,
+                     out);
   return hipGetLastError();
 }
 

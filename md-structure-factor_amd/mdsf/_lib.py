@@ -72,6 +72,11 @@ SIGNATURES = {
                                c_void_p, c_void_p, c_void_p]),
     "mdsf_ewald_slices": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_double, c_void_p,
                                     c_void_p, c_void_p, c_void_p]),
+    "mdsf_rz_average": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p,
+                                  c_int32, c_void_p, c_int32, ctypes.POINTER(c_double), c_void_p,
+                                  c_void_p]),
+    "mdsf_interp_points": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_int64,
+                                     c_void_p, c_void_p]),
     "mdsf_adjoint_frames": (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                                       ctypes.POINTER(c_double), c_int32, c_int64, c_int64,
                                       c_void_p, c_void_p, c_void_p]),

@@ -384,3 +384,117 @@ def ewald_slices(kgridplt, wavelength):
                                      _lib.c_void_p(xy.data_ptr()), _lib.c_void_p(xz.data_ptr()),
                                      _lib.c_void_p(yz.data_ptr()), s))
     return xy, xz, yz
+
+
+# --- cylindrical (r, z) average: PLOT_RAD_NEW's numbers (plot2d.py:559-650) ---
+THETA_BINS_PER_INV_ANG = 20.0   # plot2d.py:590
+MIN_THETA_BINS = 1              # plot2d.py:591
+RBINS = 400                     # plot2d.py:592
+
+
+def reciprocal_inverse(ucell):
+    """``b_inv`` (plot2d.py:595-603): inverse of the stacked reciprocal vectors
+    b_i = (a_j x a_k) / (a_i . (a_j x a_k)), no factor 2 pi."""
+    u = np.asarray(ucell, dtype=np.float64).reshape(3, 3)
+    a1, a2, a3 = u[0], u[1], u[2]
+    b1 = np.cross(a2, a3) / np.dot(a1, np.cross(a2, a3))
+    b2 = np.cross(a3, a1) / np.dot(a2, np.cross(a3, a1))
+    b3 = np.cross(a1, a2) / np.dot(a3, np.cross(a1, a2))
+    return np.linalg.inv(np.vstack((b1, b2, b3)))
+
+
+def rz_bins(X, Y, Z, ucell, rbins=RBINS):
+    """r and z bin centres and theta samples per r bin (plot2d.py:605-623)."""
+    u = np.asarray(ucell, dtype=np.float64).reshape(3, 3)
+    zbins = Z.shape[0]
+    xr = (X[-1] - X[0]) * u[0][0]
+    yr = (Y[-1] - Y[0]) * u[1][1]
+    rmax = min(xr, yr) / 2.0
+    rmax *= 0.95
+    rarr = np.linspace(0.0, rmax, rbins)
+    zar = np.linspace(Z[0], Z[-1], zbins)
+    circ = 2. * np.pi * rarr
+    nth = np.array([max(int(THETA_BINS_PER_INV_ANG * c), MIN_THETA_BINS) for c in circ],
+                   dtype=np.int32)
+    return rarr, zar, nth
+
+
+def alkane_intensity(rfin, z, final, inner=1.4, outer=1.57, angle=120):
+    """``normalize_alkanes`` (plot2d.py:874-918) on the device: mean intensity of
+    the (r, z) points with inner < |(r, z)| < outer whose angle atan(z / r)
+    (degrees) falls in the 90-bin histogram bins [start, nbins - start),
+    start = int((angle / 2) / (180 / nbins))."""
+    nbins = 90
+    dev = final.device
+    bins = torch.from_numpy(np.linspace(-90, 90, nbins)).to(dev)
+    R = torch.from_numpy(np.asarray(rfin, dtype=np.float64)).to(dev)[:, None]
+    Zt = torch.from_numpy(np.asarray(z, dtype=np.float64)).to(dev)[None, :]
+    nrm = torch.sqrt(R * R + Zt * Zt)
+    ring = (inner < nrm) & (nrm < outer)
+    ang = (180 / np.pi) * torch.atan(Zt / R)                  # r = 0: +-90 degrees
+    idx = torch.bucketize(ang, bins, right=True) - 1          # np.digitize(...) - 1
+    start = int((angle / 2) / (180 / nbins))
+    end = nbins - start
+    sel = ring & (idx >= start) & (idx < end)
+    total = torch.sum(torch.where(sel, final, torch.zeros_like(final)))
+    count = torch.sum(sel)
+    return total / count
+
+
+def rz_average(kgridplt, ucell, format=False, rbins=RBINS):
+    """PLOT_RAD_NEW's numbers (plot2d.py:575-650) for a device kgridplt (nx, ny, nz, 4):
+    the theta average of S over cylinders about q_z (HIP kernel), NaN bins set to
+    the smallest finite value, normalised by the mean (``format=False``) or by the
+    alkane-ring intensity (``format=True``), and mirrored to negative r.
+
+    Returns a dict of ``oa`` (rbins, nz) and ``final`` (2 rbins - 1, nz) device
+    tensors and the ``rarr``, ``zar``, ``rfin``, ``zfin`` (1, nz) axes (numpy)."""
+    lib = _lib.load()
+    kp = kgridplt.to(dtype=torch.float64).contiguous()
+    nx, ny, nz = (int(x) for x in kp.shape[:3])
+    X = kp[:, 0, 0, 0].cpu().numpy()
+    Y = kp[0, :, 0, 1].cpu().numpy()
+    Z = kp[0, 0, :, 2].cpu().numpy()
+    for ax in (X, Y, Z):
+        if not np.all(np.diff(ax) > 0):
+            raise ValueError("kgridplt axes must be strictly ascending")
+    rarr, zar, nth = rz_bins(X, Y, Z, ucell, rbins)
+    binv = np.ascontiguousarray(reciprocal_inverse(ucell), dtype=np.float64)
+    dev = kp.device
+    d_r = torch.from_numpy(rarr).to(dev)
+    d_z = torch.from_numpy(zar).to(dev)
+    d_n = torch.from_numpy(nth).to(dev)
+    oa = torch.empty((len(rarr), len(zar)), dtype=torch.float64, device=dev)
+    s = _lib.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(lib.mdsf_rz_average(_lib.c_void_p(kp.data_ptr()), nx, ny, nz,
+                                   _lib.c_void_p(d_r.data_ptr()), _lib.c_void_p(d_n.data_ptr()),
+                                   len(rarr), _lib.c_void_p(d_z.data_ptr()), len(zar),
+                                   binv.ctypes.data_as(_lib.ctypes.POINTER(_lib.c_double)),
+                                   _lib.c_void_p(oa.data_ptr()), s))
+    nan = torch.isnan(oa)
+    finite = oa[~nan]
+    mn = finite.min() if finite.numel() else torch.tensor(float("nan"), dtype=oa.dtype, device=dev)
+    oa = torch.where(nan, mn, oa)                                     # plot2d.py:629-630
+    if not format:
+        oa = oa / torch.mean(oa)                                      # plot2d.py:632-634
+    final = torch.cat([torch.flip(oa, [0]), oa[1:]], 0)              # plot2d.py:637
+    rfin = np.append(-rarr[::-1], rarr[1:])                          # plot2d.py:638
+    zfin = zar[None, :].copy()                                        # plot2d.py:639
+    if format:
+        final = final / alkane_intensity(rfin, zfin[0], final, 1.4, 1.57, 120)  # plot2d.py:653-655
+    return {"oa": oa, "final": final, "rarr": rarr, "zar": zar, "rfin": rfin, "zfin": zfin}
+
+
+def interp_points(kgridplt, pts):
+    """Trilinear interpolant of kgridplt's S at device points (n, 3), NaN outside
+    (RegularGridInterpolator(..., bounds_error=False), plot2d.py:588)."""
+    lib = _lib.load()
+    kp = kgridplt.to(dtype=torch.float64).contiguous()
+    nx, ny, nz = (int(x) for x in kp.shape[:3])
+    p = pts.to(device=kp.device, dtype=torch.float64).reshape(-1, 3).contiguous()
+    out = torch.empty(p.shape[0], dtype=torch.float64, device=kp.device)
+    s = _lib.c_void_p(torch.cuda.current_stream(kp.device).cuda_stream)
+    _lib.check(lib.mdsf_interp_points(_lib.c_void_p(kp.data_ptr()), nx, ny, nz,
+                                      _lib.c_void_p(p.data_ptr()), int(p.shape[0]),
+                                      _lib.c_void_p(out.data_ptr()), s))
+    return out.reshape(pts.shape[:-1])

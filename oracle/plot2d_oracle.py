@@ -1,0 +1,106 @@
+"""CPU oracle for the cylindrical (r, z) average: PLOT_RAD_NEW's numbers.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/`` may import this module; the product
+(``md-structure-factor_amd/``) never calls it.
+
+Restates, as plain numpy / scipy, the numeric part of joeyelk/MD-Structure-
+Factor ``plot2d.py``:
+
+* ``rz_average`` follows ``PLOT_RAD_NEW`` (``plot2d.py:559-655``): the
+  ``RegularGridInterpolator`` over the kgridplt axes (``:575-588``, the same
+  scipy routine the reference calls), the reciprocal basis and its inverse
+  (``:595-603``), the r / z bins (``:605-615``), the per-r theta sampling and
+  average (``:619-627``), the NaN fill and mean normalisation (``:629-634``),
+  the mirror to negative r (``:637-639``) and, with ``format``, the alkane
+  normalisation (``:653-655``); everything after that is plotting.
+* ``normalize_alkanes`` follows ``plot2d.py:874-918`` loop for loop.
+
+How it is pinned (``tests/test_plot2d_oracle.py``): a constant S (every bin
+equals the constant, ``final`` is all ones); an S linear in the grid
+coordinates, which the trilinear interpolant reproduces exactly, so that the
+theta average has the closed form a + z (b_inv c)_z on a monoclinic cell (a
+transposed ``b_inv`` or a wrong bin grid fails it); the interpolator against a
+hand-computed trilinear value; ``normalize_alkanes`` on a field whose ring
+mean is known.
+"""
+import numpy as np
+from scipy.interpolate import RegularGridInterpolator
+
+THETA_BINS_PER_INV_ANG = 20.    # plot2d.py:590
+MIN_THETA_BINS = 1              # plot2d.py:591
+RBINS = 400                     # plot2d.py:592
+
+
+def reciprocal_inverse(ucell):
+    """plot2d.py:595-603."""
+    a1, a2, a3 = ucell[0], ucell[1], ucell[2]
+    b1 = (np.cross(a2, a3)) / (np.dot(a1, np.cross(a2, a3)))
+    b2 = (np.cross(a3, a1)) / (np.dot(a2, np.cross(a3, a1)))
+    b3 = (np.cross(a1, a2)) / (np.dot(a3, np.cross(a1, a2)))
+    return np.linalg.inv(np.vstack((b1, b2, b3)))
+
+
+def normalize_alkanes(R, Z, Raw_Intensity, inner, outer, angle):
+    """plot2d.py:874-918 (the print dropped)."""
+    nbins = 90
+    bins = np.linspace(-90, 90, nbins)
+    angles = []
+    intensity = []
+    for i in range(R.shape[0]):
+        for j in range(Z.shape[0]):
+            if inner < np.linalg.norm([R[i], Z[j]]) < outer:
+                with np.errstate(divide="ignore"):
+                    angles.append((180/np.pi)*np.arctan(Z[j]/R[i]))
+                intensity.append(Raw_Intensity[i, j])
+    inds = np.digitize(angles, bins)
+    I = np.zeros([nbins])
+    counts = np.zeros([nbins])
+    for i in range(len(inds)):
+        I[inds[i] - 1] += intensity[i]
+        counts[inds[i] - 1] += 1
+    bin_range = 180 / nbins
+    start = int((angle/2) / bin_range)
+    end = nbins - start
+    total_intensity = np.sum(I[start:end])
+    return total_intensity / np.sum(counts[start:end])
+
+
+def rz_average(D, ucell, format=False, rbins=RBINS):
+    """PLOT_RAD_NEW numbers for kgridplt D (nx, ny, nz, 4): dict of oa, final,
+    rarr, zar, rfin, zfin (plot2d.py:575-655)."""
+    ucell = np.asarray(ucell, dtype=np.float64)
+    X = D[:, 0, 0, 0]
+    Y = D[0, :, 0, 1]
+    Z = D[0, 0, :, 2]
+    SF = D[..., 3]
+    ES = RegularGridInterpolator((X, Y, Z), SF, bounds_error=False)
+    b_inv = reciprocal_inverse(ucell)
+    ZBINS = Z.shape[0]
+    XR = (X[-1] - X[0])*ucell[0][0]
+    YR = (Y[-1] - Y[0])*ucell[1][1]
+    Rmax = min(XR, YR) / 2.0
+    Rmax *= 0.95
+    rarr, rspace = np.linspace(0.0, Rmax, rbins, retstep=True)
+    zar = np.linspace(Z[0], Z[-1], ZBINS)
+    oa = np.zeros((rarr.shape[0], zar.shape[0]))
+    circ = 2.*np.pi*rarr
+    for ir in range(rarr.shape[0]):
+        NTHETABINS = max(int(THETA_BINS_PER_INV_ANG*circ[ir]), MIN_THETA_BINS)
+        thetas = np.linspace(0.0, np.pi*2.0, NTHETABINS, endpoint=False)
+        t, r, z = np.meshgrid(thetas, rarr[ir], zar)
+        xar = r*np.cos(t)
+        yar = r*np.sin(t)
+        pts = np.vstack((xar.ravel(), yar.ravel(), z.ravel())).T
+        MCpts = np.matmul(pts, b_inv)
+        oa[ir, :] = np.average(ES(MCpts).reshape(r.shape), axis=1)
+    mn = np.nanmin(oa)
+    oa = np.where(np.isnan(oa), mn, oa)
+    if not format:
+        rad_avg = np.average(oa)
+        oa /= rad_avg
+    final = np.append(oa[::-1, :], oa[1:], axis=0)
+    rfin = np.append(-rarr[::-1], rarr[1:])
+    zfin = np.append(z[:, 0, :], z[1:, 0, :], axis=0)
+    if format:
+        final = final / normalize_alkanes(rfin, zfin[0], final, 1.4, 1.57, 120)
+    return {"oa": oa, "final": final, "rarr": rarr, "zar": zar, "rfin": rfin, "zfin": zfin}
