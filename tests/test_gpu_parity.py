@@ -315,3 +315,34 @@ def test_ewald_slices_match_scipy(rad, gpu):
         assert np.array_equal(np.isnan(g), np.isnan(want))
         ok = ~np.isnan(want)
         np.testing.assert_allclose(g[ok], want[ok], rtol=1e-12, atol=0)
+
+
+def test_gentraj_features_end_to_end(rad, gpu, tmp_path):
+    """A seeded pore + layer system in a 60-degree monoclinic cell (the
+    generator after gentraj.py), written as .gro/.trr, loaded by
+    load_traj.process_gro_mdtraj, moved to the monoclinic frame as
+    main_gromacs.py:204-207 does, then compute_sf vs the oracle."""
+    import dens
+    import load_traj
+    from mdsf import gentraj
+    pts, cell = gentraj.generate(seed=11, npoints=400, nframes=2, box=(2.5, 2.5, 2.5),
+                                 angles=(90.0, 90.0, 60.0), pores=True, radius=0.4, rows=2,
+                                 layers=True, dbwl=0.5, layer_width=0.1)
+    prefix = str(tmp_path / "g")
+    gentraj.write(prefix, pts, cell, name="C")
+    load_traj.process_gro_mdtraj(prefix + ".gro", prefix + ".trr", str(tmp_path / "traj"))
+    traj = np.load(str(tmp_path / "traj.npz"))
+    coords, dims, typ = np.array(traj["coords"]), traj["dims"], [str(x) for x in traj["typ"]]
+    theta = math.radians(60.0)
+    coords[..., 1] = coords[..., 1] / np.sin(theta)
+    coords[..., 0] = coords[..., 0] - coords[..., 1] * np.cos(theta)
+    ucell = np.array([[1, 0, 0], [np.cos(theta), np.sin(theta), 0], [0, 0, 1]])
+    ro = np.array(coords)
+    want = O.compute_sf(ro, np.array(dims), typ, None, rad, ucell, 0.5)
+    r = np.array(coords)
+    dens.PRINT_DETAILS = False
+    dens.compute_sf(r, dims, typ, str(tmp_path / "s"), rad, ucell, 0.5)
+    got = dict(np.load(str(tmp_path / "s.npz")))
+    assert np.array_equal(r, ro)
+    assert np.array_equal(got["N"], want["N"])
+    assert_sq_close(got["sf"], want["sf"])
