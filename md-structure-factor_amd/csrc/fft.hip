@@ -337,6 +337,81 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z(R2CArgs<R> a) {
   }
 }
 
+// Register-staged z pass (FP32, m = R1 * R2, two stages).  Stage 1 reads its
+// R1 inputs per butterfly straight from global memory (row-contiguous across
+// lanes) into registers; stage 2 runs in place in LDS (each thread reads and
+// writes the same R2 slots); the split into the half spectrum then reads the
+// pairs (k, m - k) from LDS and stores whole rows contiguously.  P rows per
+// block give every thread one stage-1 butterfly, and the LDS footprint is one
+// plane pair (no ping-pong), so the pass shares CUs well with the deposition.
+template <typename R, int R1, int R2>
+__global__ __launch_bounds__(kFftThreads) void k_r2c_z_reg(R2CArgs<R> a) {
+  constexpr int m = R1 * R2;
+  constexpr int nq1 = m / R1;             // stage-1 butterflies per row
+  constexpr int P = kFftThreads / nq1;    // rows per block
+  constexpr int pitch = m + 1;
+  constexpr int ns = m / R2;              // stage-2 butterflies per row
+  __shared__ R re[P * pitch];
+  __shared__ R im[P * pitch];
+  const long long r0 = (long long)blockIdx.x * P;
+  const int pc = (int)min((long long)P, a.rows - r0);
+  const auto* src = reinterpret_cast<const typename V2<R>::T*>(a.in) + r0 * m;
+  const void* tw = sizeof(R) == 8 ? (const void*)a.ax.tw64 : (const void*)a.ax.tw;
+  {
+    const int j = threadIdx.x % nq1, p = threadIdx.x / nq1;
+    Cx<R> v[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      if (p < pc) {
+        const auto x = src[p * m + j + r * nq1];
+        v[r] = mk<R>(x.x, x.y);
+      } else {
+        v[r] = mk<R>(0, 0);
+      }
+    }
+    dft<R1, false>(v);
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      re[p * pitch + j * R1 + r] = v[r].x;
+      im[p * pitch + j * R1 + r] = v[r].y;
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < P * ns; b += kFftThreads) {
+    const int k = b % ns, p = b / ns;
+    Cx<R> v[R2];
+#pragma unroll
+    for (int r = 0; r < R2; ++r) v[r] = mk<R>(re[p * pitch + k + r * ns], im[p * pitch + k + r * ns]);
+    twiddle_in<R2, false>(v, k, 1, tw);
+    dft<R2, false>(v);
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+      re[p * pitch + k + r * ns] = v[r].x;
+      im[p * pitch + k + r * ns] = v[r].y;
+    }
+  }
+  __syncthreads();
+  // split: X_k = A_k + W_nz^k B_k, A = (Z_k + conj Z_{m-k})/2, B = (Z_k - conj Z_{m-k})/(2i)
+  auto* dst = a.main + r0 * m;
+  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
+    const int p = idx / m, k = idx - p * m;
+    const int kk = (k == 0) ? 0 : m - k;
+    const Cx<R> z = mk<R>(re[p * pitch + k], im[p * pitch + k]);
+    const Cx<R> zc = mk<R>(re[p * pitch + kk], -im[p * pitch + kk]);
+    const Cx<R> A = cscale((R)0.5, cadd(z, zc));
+    const Cx<R> d = cscale((R)0.5, csub(z, zc));
+    const Cx<R> B = mk<R>(d.y, -d.x);  // d / i
+    const Cx<R> X = cadd(A, cmul(B, ld_tw<R>(a.tw2, k)));
+    dst[idx].x = X.x;
+    dst[idx].y = X.y;
+  }
+  for (int p = threadIdx.x; p < pc; p += kFftThreads) {
+    // X_m = A_0 - B_0 = Re z0 - Im z0 (real)
+    a.nyq[r0 + p].x = re[p * pitch] - im[p * pitch];
+    a.nyq[r0 + p].y = (R)0;
+  }
+}
+
 // --- adjoint z pass: half spectrum -> real rows ----------------------------------
 // Real adjoint of the R2C along z:  x_n = Re sum_{k=0..m} X_k e^{+2 pi i k n / nz}.
 // With X'_0 = Re X_0, X'_m = Re X_m and X'_k = X_k / 2 inside, this is the
@@ -725,6 +800,17 @@ typename V2<R>::T* spec_nyq(Plan& p) {
   return static_cast<typename V2<R>::T*>(p.spec_nyq);
 }
 
+// Rows per block of the register-staged z pass for half-length m (FP32), 0
+// when m has no two-stage split.
+inline int z_reg_rows(int m) {
+  switch (m) {
+    case 64: return kFftThreads / (64 / 8);
+    case 128: return kFftThreads / (128 / 16);
+    case 256: return kFftThreads / (256 / 16);
+    default: return 0;
+  }
+}
+
 template <typename R>
 hipError_t pass_z(Plan& p, const void* density, hipStream_t s) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
@@ -739,6 +825,19 @@ hipError_t pass_z(Plan& p, const void* density, hipStream_t s) {
             : tile_pencils<R>(m);
   a.ax = p.ax_zc;
   a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
+  if constexpr (sizeof(R) == 4) {
+    if (z_reg_rows(m) > 0 && (p.tune_z == 0 || p.tune_z == kTuneReg)) {
+      // register-staged kernel: the default where m has a two-stage split
+      const int P = z_reg_rows(m);
+      const dim3 g((unsigned)((a.rows + P - 1) / P)), b(kFftThreads);
+      switch (m) {
+        case 64: hipLaunchKernelGGL((k_r2c_z_reg<R, 8, 8>), g, b, 0, s, a); break;
+        case 128: hipLaunchKernelGGL((k_r2c_z_reg<R, 16, 8>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_r2c_z_reg<R, 16, 16>), g, b, 0, s, a); break;
+      }
+      return hipGetLastError();
+    }
+  }
   const long long blocks = (a.rows + a.P - 1) / a.P;
   hipLaunchKernelGGL((k_r2c_z<R>), dim3((unsigned)blocks), dim3(kFftThreads),
                      tile_lds<R>(m, a.P), s, a);
@@ -926,12 +1025,14 @@ hipError_t autotune_impl(Plan& p, hipStream_t s) {
     (void)hipEventDestroy(e0);
     return err;
   }
-  const int cand_p[4] = {2, 4, 8, 16};
-  // z pass: rows per tile
+  const int cand_p[5] = {kTuneReg, 2, 4, 8, 16};
+  // z pass: the register-staged kernel (FP32, m with a two-stage split) or
+  // the generic tile with that many rows
   float best = 1e30f;
   int pick = 0;
   for (int P : cand_p) {
-    if (tile_lds<R>(m, P) > max_lds_per_block()) continue;
+    if (P == kTuneReg && !(sizeof(R) == 4 && z_reg_rows(m) > 0)) continue;
+    if (P != kTuneReg && tile_lds<R>(m, P) > max_lds_per_block()) continue;
     p.tune_z = P;
     const float t = time_best([&] { return pass_z<R>(p, p.rho, s); }, s, e0, e1, &err);
     if (err != hipSuccess) break;
