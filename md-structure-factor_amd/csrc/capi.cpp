@@ -248,6 +248,10 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     slice_floats += (long long)ty.count * ty.corner_stride;
   }
   p->slice_floats = (!p->fft64 && s.mode == MODE_DIAG) ? slice_floats : 0;
+  if (p->slice_floats >= (1LL << 31)) {  // the deposition addresses slices with 32-bit offsets
+    free_plan(p);
+    return fail(MDSF_ERR_UNSUPPORTED, "too many atoms for the FP32 tile-slice layout (2^31 floats)");
+  }
   {  // tiled adjoint result slots: per atom one per tile slice triple
     long long slots = 0;
     for (int t = 0; t < desc->ntypes; ++t) {

@@ -633,10 +633,15 @@ constexpr int NV = 64;  // virtual atom slots per batch
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Row NV is kept zero: the padding entry of the per-wave lists.
 struct StageM {
-  __align__(16) float vx[NV][T];
-  __align__(16) float vy[NV][T];
-  __align__(16) float vz[NV][T];
+  __align__(16) float vx[NV + 1][T];
+  __align__(16) float vy[NV + 1][T];
+  __align__(16) float vz[NV + 1][T];
+  // per virtual atom and wave w: nonzero X on x rows 4w..4w+3 (read as one int)
+  __align__(4) unsigned char vq[NV][4];
+  // per wave: the virtual atoms it accumulates, padded to a multiple of 4 with NV
+  unsigned char wl[kThreads / 64][NV + 4];
 };
 
 struct StageMXY {
@@ -652,7 +657,7 @@ struct StageMXY {
 // orthogonal-cell kernel asks for 6 waves per SIMD, which its register budget
 // allows without spills and its LDS footprint (< 26.6 KB) matches.
 template <int MODE, int MT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == MODE_DIAG ? 6 : 1, 8)))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == MODE_DIAG ? 7 : 1, 8)))
 void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const int* __restrict__ boff,
                                                            const int4* __restrict__ s_ir,
@@ -667,13 +672,14 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
   constexpr int TXY = MODE == MODE_XY ? MT : 1;
   __shared__ struct {
     double r[NXY][3];    // wrapped position (FP64), xy-coupled cells
-    int vbase[NV], entry[NV];
+    int vbase[NV], entry[NXY];
     int4 q[NV];          // home cell, type | corner << 16
-    int ok[NV];          // 0 only after a reported error
-    long long so[MODE == MODE_DIAG ? NV : 1][3];  // tile slice offset per axis, -1: none
-    long long co[MODE == MODE_DIAG ? NV : 1][3];  // corner component 0 of that slice, -1: none
-    int cs[MODE == MODE_DIAG ? NV : 1][3];        // corner component stride (floats)
-    int clist[NV];                                // batch indices of the corner atoms
+    int ok[NXY];         // 0 only after a reported error
+    // tile slice offset per axis and corner component 0 of that slice (floats
+    // into the slice buffer, which the plan keeps below 2^31 floats), -1: none
+    int so[MODE == MODE_DIAG ? NV : 1][3];
+    int co[MODE == MODE_DIAG ? NV : 1][3];
+    int clist[NV];       // batch indices of the corner atoms
   } bt;
   // per-type constants: g_dd / (2 sigma^2) per axis and Nel / sigma^3
   __shared__ double tgk[TXY][3];
@@ -727,6 +733,10 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
   v4f acc[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
+  if constexpr (MODE == MODE_DIAG) {
+    // (ordered before the first batch by the enumeration's barriers)
+    if (tid < T) st.vx[NV][tid] = st.vy[NV][tid] = st.vz[NV][tid] = 0.f;
+  }
 
   auto process = [&](int count, int head) -> int {
     // separates the previous batch's MFMA reads of the staging area from
@@ -777,9 +787,8 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
           int j = tl.t0[d] / T - lo / T;
           if (j < 0) j += nt;
           const bool has = ok && j < ns;
-          bt.so[lane][d] = has ? off + (long long)T * j : -1LL;
-          bt.co[lane][d] = has ? coff + (long long)T * j : -1LL;
-          bt.cs[lane][d] = T * ns;
+          bt.so[lane][d] = has ? (int)(off + (long long)T * j) : -1;
+          bt.co[lane][d] = has ? (int)(coff + (long long)T * j) : -1;
           off += (long long)T * ns;
           coff += (long long)(d == 1 ? 3 : 2) * T * ns;
         }
@@ -793,17 +802,20 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
       for (int it = tid; it < nreal * 12; it += kThreads) {
         const int a = it / 12, r12 = it - 12 * (it / 12), d = r12 >> 2, q4 = r12 & 3;
         if ((bt.q[a].w >> 16) != 0) continue;  // corner atom: evaluated below
-        const long long off = bt.so[a][d];
+        const int off = bt.so[a][d];
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (off >= 0) v = *reinterpret_cast<const float4*>(slices + off + 4 * q4);
         float* row = d == 0 ? st.vx[bt.vbase[a]] : (d == 1 ? st.vy[bt.vbase[a]] : st.vz[bt.vbase[a]]);
         *reinterpret_cast<float4*>(row + 4 * q4) = v;
+        if (d == 0)
+          st.vq[bt.vbase[a]][q4] = (v.x != 0.f || v.y != 0.f || v.z != 0.f || v.w != 0.f) ? 1 : 0;
       }
       for (int j = tid; j < (vpad - vtot) * T; j += kThreads) {
         const int v = vtot + j / T, cc = j % T;
         st.vx[v][cc] = 0.f;
         st.vy[v][cc] = 0.f;
         st.vz[v][cc] = 0.f;
+        if (cc < 4) st.vq[v][cc] = 0;
       }
       // corner atoms: 4 virtual atoms (Xc, Yall, Zs), (Xn, Yc, Zs),
       // (Xn, Yl, ZL), (Xn, Yh, ZH) from the regular slices and the corner
@@ -812,16 +824,19 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
         const int ca = it / 48, r48 = it - 48 * ca;
         const int v = r48 / 12, r12 = r48 - 12 * v, d = r12 >> 2, q4 = r12 & 3;
         const int a = bt.clist[ca];
-        const long long cb = bt.co[a][d];
-        long long off;
-        if (d == 0) off = cb >= 0 ? cb + (v == 0 ? 0 : bt.cs[a][0]) : -1LL;
-        else if (d == 1) off = v == 0 ? bt.so[a][1] : (cb >= 0 ? cb + (long long)(v - 1) * bt.cs[a][1] : -1LL);
-        else off = v < 2 ? bt.so[a][2] : (cb >= 0 ? cb + (long long)(v - 2) * bt.cs[a][2] : -1LL);
+        const int cb = bt.co[a][d];
+        const int cs = T * tyl.nsl[bt.q[a].w & 0xffff][d];  // corner component stride
+        int off;
+        if (d == 0) off = cb >= 0 ? cb + (v == 0 ? 0 : cs) : -1;
+        else if (d == 1) off = v == 0 ? bt.so[a][1] : (cb >= 0 ? cb + (v - 1) * cs : -1);
+        else off = v < 2 ? bt.so[a][2] : (cb >= 0 ? cb + (v - 2) * cs : -1);
         float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
         if (off >= 0) val = *reinterpret_cast<const float4*>(slices + off + 4 * q4);
         const int vb = bt.vbase[a] + v;
         float* row = d == 0 ? st.vx[vb] : (d == 1 ? st.vy[vb] : st.vz[vb]);
         *reinterpret_cast<float4*>(row + 4 * q4) = val;
+        if (d == 0)
+          st.vq[vb][q4] = (val.x != 0.f || val.y != 0.f || val.z != 0.f || val.w != 0.f) ? 1 : 0;
       }
     }
     // staging by evaluation (xy-coupled cells): wave d < 3 builds axis d for
@@ -998,18 +1013,33 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
     __syncthreads();
     // MFMA accumulation over groups of 4 virtual atoms
     const int kk = lane >> 4, jj = lane & 15;
-    for (int k0 = 0; k0 < vpad; k0 += 4) {
+    int kend = vpad;
+    if constexpr (MODE == MODE_DIAG) {
+      // this wave's list: the virtual atoms with nonzero X on its 4 x rows
+      // (about 2/3 of them), padded with the zero row to whole groups of 4;
+      // the list is written and read by this wave only
+      const bool on = lane < vtot && st.vq[lane][wave] != 0;
+      const unsigned long long bm = __ballot(on);
+      const int pos = __popcll(bm & ((1ull << lane) - 1));
+      if (on) st.wl[wave][pos] = (unsigned char)lane;
+      const int cnt = __popcll(bm);
+      kend = (cnt + 3) & ~3;
+      if (lane >= cnt && lane < kend) st.wl[wave][lane] = (unsigned char)NV;
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int k0 = 0; k0 < kend; k0 += 4) {
       int mu;
       if constexpr (MODE == MODE_DIAG) {
-        // every x row: testing rows for support (LDS read + ballot per group)
-        // costs more than the MFMAs it would skip
+        // every x row of the wave's block: testing single rows (LDS read +
+        // ballot per group) costs more than the MFMAs it would skip
         mu = 0xf;
       } else {
         const int m = ((st.vxm[k0] | st.vxm[k0 + 1] | st.vxm[k0 + 2] | st.vxm[k0 + 3]) >> (4 * wave)) & 0xf;
         mu = __builtin_amdgcn_readfirstlane(m);
       }
       if (!mu) continue;
-      const int k = k0 + kk;
+      int k = k0 + kk;
+      if constexpr (MODE == MODE_DIAG) k = st.wl[wave][k0 + kk];
       const float zv = st.vz[k][jj];
       if constexpr (MODE == MODE_DIAG) {
         const float yv = st.vy[k][jj];
