@@ -1127,7 +1127,7 @@ struct StageAdj {
 };
 
 template <int MT>
-__global__ __launch_bounds__(kThreads) void k_adjoint_mfma(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adjoint_mfma(
     SysDev sys, const TypeDev* __restrict__ td, const int* __restrict__ boff,
     const int4* __restrict__ s_ir, const float* __restrict__ slices,
     const float* __restrict__ dslices, const float* __restrict__ grho,
@@ -1138,9 +1138,10 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_mfma(
   __shared__ struct {
     int vbase[NV], nv[NV];
     int4 q[NV];
-    long long so[NV][3];  // tile slice offset per axis, -1: none
-    long long co[NV][3];  // corner component 0 of that slice, -1: none
-    int cs[NV][3];        // corner component stride (floats)
+    // tile slice offset per axis and corner component 0 of that slice
+    // (floats, below 2^31), -1: none
+    int so[NV][3];
+    int co[NV][3];
     int clist[NV];        // batch indices of the corner atoms
     long long pidx[NV];   // result slot, -1: none
   } bt;
@@ -1221,9 +1222,8 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_mfma(
         int j = tl.t0[d] / T - lo / T;
         if (j < 0) j += nt;
         const bool has = ok && j < ns;
-        bt.so[lane][d] = has ? off + (long long)T * j : -1LL;
-        bt.co[lane][d] = has ? coff + (long long)T * j : -1LL;
-        bt.cs[lane][d] = T * ns;
+        bt.so[lane][d] = has ? (int)(off + (long long)T * j) : -1;
+        bt.co[lane][d] = has ? (int)(coff + (long long)T * j) : -1;
         off += (long long)T * ns;
         coff += (long long)(d == 1 ? 3 : 2) * T * ns;
         jj = jj * ns + j;
@@ -1239,7 +1239,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_mfma(
     for (int it = tid; it < nreal * 12; it += kThreads) {
       const int a = it / 12, r12 = it - 12 * (it / 12), d = r12 >> 2, qq = r12 & 3;
       if ((bt.q[a].w >> 16) != 0) continue;  // corner atom: below
-      const long long off = bt.so[a][d];
+      const int off = bt.so[a][d];
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f), w = v;
       if (off >= 0) {
         v = *reinterpret_cast<const float4*>(slices + off + 4 * qq);
@@ -1261,11 +1261,13 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_mfma(
       const int ca = it / 48, r48 = it - 48 * ca;
       const int v = r48 / 12, r12 = r48 - 12 * v, d = r12 >> 2, qq = r12 & 3;
       const int a = bt.clist[ca];
-      const long long cb = bt.co[a][d];
-      long long off;
-      if (d == 0) off = cb >= 0 ? cb + (v == 0 ? 0 : bt.cs[a][0]) : -1LL;
-      else if (d == 1) off = v == 0 ? bt.so[a][1] : (cb >= 0 ? cb + (long long)(v - 1) * bt.cs[a][1] : -1LL);
-      else off = v < 2 ? bt.so[a][2] : (cb >= 0 ? cb + (long long)(v - 2) * bt.cs[a][2] : -1LL);
+      const int cb = bt.co[a][d];
+      const int tq = bt.q[a].w & 0xffff;
+      const int cs = T * tyl.nsl[tq < sys.ntypes ? tq : 0][d];  // corner component stride
+      int off;
+      if (d == 0) off = cb >= 0 ? cb + (v == 0 ? 0 : cs) : -1;
+      else if (d == 1) off = v == 0 ? bt.so[a][1] : (cb >= 0 ? cb + (v - 1) * cs : -1);
+      else off = v < 2 ? bt.so[a][2] : (cb >= 0 ? cb + (v - 2) * cs : -1);
       float4 val = make_float4(0.f, 0.f, 0.f, 0.f), dval = val;
       if (off >= 0) {
         val = *reinterpret_cast<const float4*>(slices + off + 4 * qq);
