@@ -205,33 +205,56 @@ __global__ void k_adjoint_reduce(SysDev sys, const TypeDev* __restrict__ td,
   grad[3 * (size_t)a + 2] = c2 * g2;
 }
 
-// Tiled adjoint: grad[atom] = scale * (sum of the atom's (tile) slots in slot order).
+// Tiled adjoint: grad[atom] = scale * (sum of the atom's tile slots).  16
+// lanes per atom read its slots coalesced (lane i takes slots i, i + 16, ...
+// in order) and a fixed xor-shuffle tree adds the 16 partial sums, so the
+// result is deterministic.
+constexpr int kRedLanes = 16;
+
 __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td,
                                       const int4* __restrict__ s_ir, const int* __restrict__ s_idx,
                                       const int* __restrict__ nvalid,
                                       const double* __restrict__ tpart,
                                       const double* __restrict__ scale3, double* __restrict__ grad) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *nvalid) return;
-  const int t = s_ir[s].w & 0xffff;
-  if (t >= sys.ntypes) return;
-  const TypeDev& ty = td[t];
-  const int slot = s - ty.type_start;
-  if (slot < 0 || slot >= ty.count) return;
-  const int per = ty.nsl[0] * ty.nsl[1] * ty.nsl[2];
-  const double* src = tpart + 3 * (ty.adjt_base + (long long)slot * per);
-  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-  for (int it = 0; it < per; ++it) {
-    g0 += src[3 * it + 0];
-    g1 += src[3 * it + 1];
-    g2 += src[3 * it + 2];
+  const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int s = (int)(g / kRedLanes), li = (int)(g % kRedLanes);
+  // every lane of a 16-lane group takes the same branch, so the shuffles
+  // below run with the whole group
+  bool act = s < *nvalid;
+  int t = 0, slot = 0;
+  if (act) {
+    t = s_ir[s].w & 0xffff;
+    act = t < sys.ntypes;
   }
-  const double c0 = scale3 ? scale3[0] : 1.0, c1 = scale3 ? scale3[1] : 1.0,
-               c2 = scale3 ? scale3[2] : 1.0;
-  const int a = s_idx[s];
-  grad[3 * (size_t)a + 0] = c0 * g0;
-  grad[3 * (size_t)a + 1] = c1 * g1;
-  grad[3 * (size_t)a + 2] = c2 * g2;
+  if (act) {
+    slot = s - td[t].type_start;
+    act = slot >= 0 && slot < td[t].count;
+  }
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  if (act) {
+    const TypeDev& ty = td[t];
+    const int per = ty.nsl[0] * ty.nsl[1] * ty.nsl[2];
+    const double* src = tpart + 3 * (ty.adjt_base + (long long)slot * per);
+    for (int it = li; it < per; it += kRedLanes) {
+      g0 += src[3 * it + 0];
+      g1 += src[3 * it + 1];
+      g2 += src[3 * it + 2];
+    }
+  }
+#pragma unroll
+  for (int o = kRedLanes / 2; o > 0; o >>= 1) {
+    g0 += __shfl_xor(g0, o, kRedLanes);
+    g1 += __shfl_xor(g1, o, kRedLanes);
+    g2 += __shfl_xor(g2, o, kRedLanes);
+  }
+  if (act && li == 0) {
+    const double c0 = scale3 ? scale3[0] : 1.0, c1 = scale3 ? scale3[1] : 1.0,
+                 c2 = scale3 ? scale3[2] : 1.0;
+    const int a = s_idx[s];
+    grad[3 * (size_t)a + 0] = c0 * g0;
+    grad[3 * (size_t)a + 1] = c1 * g1;
+    grad[3 * (size_t)a + 2] = c2 * g2;
+  }
 }
 
 __global__ void k_fill_zero(double* __restrict__ p, long long n) {
@@ -281,8 +304,9 @@ hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale
     if (!derivs_ready && (e = launch_slice_derivs(p, s)) != hipSuccess) return e;
     if ((e = launch_fill_zero(p.adj_tpart, 3 * p.adj_tslots, s)) != hipSuccess) return e;
     if ((e = launch_adjoint_tiles(p, static_cast<const float*>(grho), s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adjoint_tile_reduce, dim3((n + 255) / 256), dim3(256), 0, s, p.sys,
-                       p.types, p.s_ir, p.s_idx, nvalid, p.adj_tpart, scale3, grad);
+    hipLaunchKernelGGL(k_adjoint_tile_reduce,
+                       dim3((unsigned)(((long long)n * kRedLanes + 255) / 256)), dim3(256), 0, s,
+                       p.sys, p.types, p.s_ir, p.s_idx, nvalid, p.adj_tpart, scale3, grad);
     return hipGetLastError();
   }
   if (p.adj_items > 0) {
