@@ -1060,20 +1060,32 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
             const int xc = 4 * wave + b;
             const int xvb = st.xv[ra][xc];
             float w = 0.f;
-#pragma unroll 1
-            for (int sx = 0; sx < 3; ++sx) {
-              if (!((xvb >> sx) & 1)) continue;
-              const float bx = st.bx[sx][ra][xc];
-#pragma unroll 1
-              for (int sy = 0; sy < 3; ++sy) {
-                if (!((yvb >> sy) & 1)) continue;
-                const bool take = cls == 0 || (cls == 1 && (sx == SIDE_C || sy == SIDE_C)) ||
-                                  (cls == 2 && sx != SIDE_C && sy == SIDE_L) ||
-                                  (cls == 3 && sx != SIDE_C && sy == SIDE_H);
-                if (!take) continue;
-                const float by = st.by[sy][ra][jj];
+            if (cls == 0 && ((xvb & (xvb - 1)) | (yvb & (yvb - 1))) == 0) {
+              // at most one image per axis (every cell but wrap-arounds of
+              // windows wider than the grid): one exponential, the same
+              // value the loop below would sum
+              if (xvb != 0 && yvb != 0) {
+                const float bx = st.bx[__builtin_ctz(xvb)][ra][xc];
+                const float by = st.by[__builtin_ctz(yvb)][ra][jj];
                 const float qf = fmaf(g00 * bx, bx, fmaf(g01x2 * bx, by, g11 * by * by));
-                w += amp * exp2f(-qf * k2);
+                w = 0.f + amp * exp2f(-qf * k2);
+              }
+            } else {
+#pragma unroll 1
+              for (int sx = 0; sx < 3; ++sx) {
+                if (!((xvb >> sx) & 1)) continue;
+                const float bx = st.bx[sx][ra][xc];
+#pragma unroll 1
+                for (int sy = 0; sy < 3; ++sy) {
+                  if (!((yvb >> sy) & 1)) continue;
+                  const bool take = cls == 0 || (cls == 1 && (sx == SIDE_C || sy == SIDE_C)) ||
+                                    (cls == 2 && sx != SIDE_C && sy == SIDE_L) ||
+                                    (cls == 3 && sx != SIDE_C && sy == SIDE_H);
+                  if (!take) continue;
+                  const float by = st.by[sy][ra][jj];
+                  const float qf = fmaf(g00 * bx, bx, fmaf(g01x2 * bx, by, g11 * by * by));
+                  w += amp * exp2f(-qf * k2);
+                }
               }
             }
             acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, zv, acc[b], 0, 0, 0);
