@@ -469,6 +469,75 @@ __global__ __launch_bounds__(kFftThreads) void k_c2r_z(C2RArgs<R, OUT> a) {
   }
 }
 
+// Register-staged adjoint z pass (FP32, m = R1 * R2): stage 1 forms Z_k from
+// the pair (X_k, X_{m-k}) read straight from global memory, the second stage
+// runs from LDS into registers and stores the real pairs (x_2j, x_2j+1)
+// straight to global memory.  A block reads all of its rows before the
+// barrier, so `out` may alias the spectrum.
+template <typename R, int R1, int R2>
+__global__ __launch_bounds__(kFftThreads) void k_c2r_z_reg(C2RArgs<R, R> a) {
+  constexpr int m = R1 * R2;
+  constexpr int nq1 = m / R1;
+  constexpr int P = kFftThreads / nq1;
+  constexpr int pitch = m + 1;
+  constexpr int ns = m / R2;
+  __shared__ R re[P * pitch];
+  __shared__ R im[P * pitch];
+  const long long r0 = (long long)blockIdx.x * P;
+  const int pc = (int)min((long long)P, a.rows - r0);
+  const auto* src = a.main + r0 * m;
+  const void* tw = sizeof(R) == 8 ? (const void*)a.ax.tw64 : (const void*)a.ax.tw;
+  {
+    const int j = threadIdx.x % nq1, p = threadIdx.x / nq1;
+    Cx<R> v[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      const int k = j + r * nq1;
+      if (p < pc) {
+        Cx<R> x, xc;
+        if (k == 0) {
+          x = mk<R>(src[p * m].x, (R)0);
+          xc = mk<R>(a.nyq[r0 + p].x, (R)0);
+        } else {
+          const auto u = src[p * m + k];
+          const auto w = src[p * m + (m - k)];
+          x = mk<R>((R)0.5 * u.x, (R)0.5 * u.y);
+          xc = mk<R>((R)0.5 * w.x, (R)-0.5 * w.y);
+        }
+        const Cx<R> e = cadd(x, xc);
+        const Cx<R> o = cmulc(csub(x, xc), ld_tw<R>(a.tw2, k));  // (X - conj X') W^{-k}
+        v[r] = mk<R>(e.x - o.y, e.y + o.x);                       // e + i o
+      } else {
+        v[r] = mk<R>(0, 0);
+      }
+    }
+    dft<R1, true>(v);
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      re[p * pitch + j * R1 + r] = v[r].x;
+      im[p * pitch + j * R1 + r] = v[r].y;
+    }
+  }
+  __syncthreads();
+  R* dst = a.out + r0 * (2 * (long long)m);
+  for (int b = threadIdx.x; b < P * ns; b += kFftThreads) {
+    const int k = b % ns, p = b / ns;
+    Cx<R> v[R2];
+#pragma unroll
+    for (int r = 0; r < R2; ++r) v[r] = mk<R>(re[p * pitch + k + r * ns], im[p * pitch + k + r * ns]);
+    twiddle_in<R2, true>(v, k, 1, tw);
+    dft<R2, true>(v);
+    if (p < pc) {
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const long long o = 2 * ((long long)p * m + k + r * ns);
+        dst[o] = v[r].x;
+        dst[o + 1] = v[r].y;
+      }
+    }
+  }
+}
+
 // --- strided pencil pass ------------------------------------------------------
 // One launch transforms two jobs: the main half spectrum and the Nyquist
 // plane (workgroups [0, blocks0) take job 0, the rest job 1).
@@ -922,6 +991,18 @@ hipError_t inverse_impl(Plan& p, const double* w_main, const double* w_nyq, void
   a.P = tile_pencils<R>(m);
   a.ax = p.ax_zc;
   a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
+  if constexpr (sizeof(R) == 4) {
+    if (z_reg_rows(m) > 0) {
+      const int P = z_reg_rows(m);
+      const dim3 g((unsigned)((a.rows + P - 1) / P)), b(kFftThreads);
+      switch (m) {
+        case 64: hipLaunchKernelGGL((k_c2r_z_reg<R, 8, 8>), g, b, 0, s, a); break;
+        case 128: hipLaunchKernelGGL((k_c2r_z_reg<R, 16, 8>), g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_c2r_z_reg<R, 16, 16>), g, b, 0, s, a); break;
+      }
+      return hipGetLastError();
+    }
+  }
   const long long blocks = (a.rows + a.P - 1) / a.P;
   hipLaunchKernelGGL((k_c2r_z<R, R>), dim3((unsigned)blocks), dim3(kFftThreads),
                      tile_lds<R>(m, a.P), s, a);
