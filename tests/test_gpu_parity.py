@@ -182,7 +182,10 @@ def test_occupancy_histogram_bit_exact(rad, gpu):
 
 @pytest.mark.parametrize("fft64", [False, True])
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (64, 64, 64), (96, 160, 192),
-                                   (38, 14, 22), (30, 18, 50), (2, 4, 2), (256, 256, 256)])
+                                   (38, 14, 22), (30, 18, 50), (2, 4, 2), (256, 256, 256),
+                                   # register-staged z pass, Nz/2 = 64 / 128 / 256, with a
+                                   # partial last block of rows
+                                   (6, 10, 128), (5, 7, 256), (3, 5, 512)])
 def test_rfft3_matches_numpy(shape, fft64, gpu):
     import torch
     from mdsf import engine
