@@ -185,7 +185,7 @@ def test_occupancy_histogram_bit_exact(rad, gpu):
                                    (38, 14, 22), (30, 18, 50), (2, 4, 2), (256, 256, 256),
                                    # register-staged z pass, Nz/2 = 64 / 128 / 256, with a
                                    # partial last block of rows
-                                   (6, 10, 128), (5, 7, 256), (3, 5, 512)])
+                                   (6, 10, 128), (6, 6, 256), (2, 6, 512)])
 def test_rfft3_matches_numpy(shape, fft64, gpu):
     import torch
     from mdsf import engine
