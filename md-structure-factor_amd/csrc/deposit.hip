@@ -644,13 +644,16 @@ struct StageM {
   unsigned char wl[kThreads / 64][NV + 4];
 };
 
+// xy-coupled cells: per virtual atom and cell the offsets b of the three
+// images per axis and a 3-bit mask of the valid ones (a byte per cell)
+template <int NS>
 struct StageMXY {
-  __align__(16) float bx[3][NV][T];
-  __align__(16) float by[3][NV][T];
-  __align__(16) float vz[NV][T];
-  int xv[NV][T], yv[NV][T];
-  float amp[NV], k2[NV];
-  int vreal[NV], vcls[NV], vxm[NV];
+  __align__(16) float bx[3][NS][T];
+  __align__(16) float by[3][NS][T];
+  __align__(16) float vz[NS][T];
+  unsigned char xv[NS][T], yv[NS][T];
+  float amp[NS], k2[NS];
+  int vreal[NS], vcls[NS], vxm[NS];
 };
 
 // The gather is latency-bound (batch staging, candidate streaming): the
@@ -664,22 +667,25 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const double* __restrict__ s_r,
                                                            const float* __restrict__ slices,
                                                            float* __restrict__ rho) {
-  using Stage = typename std::conditional<MODE == MODE_DIAG, StageM, StageMXY>::type;
+  // virtual atom slots per batch: 64 for orthogonal cells, 48 for xy-coupled
+  // cells (whose staging area then fits 4 blocks per CU)
+  constexpr int NVK = MODE == MODE_DIAG ? NV : 48;
+  using Stage = typename std::conditional<MODE == MODE_DIAG, StageM, StageMXY<NVK>>::type;
   constexpr int EXTRA = MODE == MODE_DIAG ? 3 : 2;  // extra virtual slots of a corner atom
   __shared__ Stage st;
   __shared__ EnumShared<MT> sh;
-  constexpr int NXY = MODE == MODE_XY ? NV : 1;         // arrays only the xy path reads
+  constexpr int NXY = MODE == MODE_XY ? NVK : 1;         // arrays only the xy path reads
   constexpr int TXY = MODE == MODE_XY ? MT : 1;
   __shared__ struct {
     double r[NXY][3];    // wrapped position (FP64), xy-coupled cells
-    int vbase[NV], entry[NXY];
-    int4 q[NV];          // home cell, type | corner << 16
+    int vbase[NVK], entry[NXY];
+    int4 q[NVK];          // home cell, type | corner << 16
     int ok[NXY];         // 0 only after a reported error
     // tile slice offset per axis and corner component 0 of that slice (floats
     // into the slice buffer, which the plan keeps below 2^31 floats), -1: none
-    int so[MODE == MODE_DIAG ? NV : 1][3];
-    int co[MODE == MODE_DIAG ? NV : 1][3];
-    int clist[NV];       // batch indices of the corner atoms
+    int so[MODE == MODE_DIAG ? NVK : 1][3];
+    int co[MODE == MODE_DIAG ? NVK : 1][3];
+    int clist[NVK];       // batch indices of the corner atoms
   } bt;
   // per-type constants: g_dd / (2 sigma^2) per axis and Nel / sigma^3
   __shared__ double tgk[TXY][3];
@@ -735,20 +741,20 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
   for (int b = 0; b < 4; ++b) acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
   if constexpr (MODE == MODE_DIAG) {
     // (ordered before the first batch by the enumeration's barriers)
-    if (tid < T) st.vx[NV][tid] = st.vy[NV][tid] = st.vz[NV][tid] = 0.f;
+    if (tid < T) st.vx[NVK][tid] = st.vy[NVK][tid] = st.vz[NVK][tid] = 0.f;
   }
 
   auto process = [&](int count, int head) -> int {
     // separates the previous batch's MFMA reads of the staging area from
     // this batch's writes
     __syncthreads();
-    // how many real atoms fit into NV virtual slots (same result in every wave)
+    // how many real atoms fit into NVK virtual slots (same result in every wave)
     const int e = lane < count ? sh.list[(head + lane) & kListMask] : 0;
     const bool isc = lane < count && (e < 0);
     const unsigned long long cm = __ballot(isc);
     const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
     const int vupto = (lane + 1) + EXTRA * __popcll(cm & le);
-    const bool fits = lane < count && vupto <= NV;
+    const bool fits = lane < count && vupto <= NVK;
     const int nreal = __popcll(__ballot(fits));
     const int ncorner = __popcll(__ballot(fits && isc));
     const int vtot = nreal > 0 ? __shfl(vupto, nreal - 1, 64) : 0;
@@ -1024,7 +1030,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
       if (on) st.wl[wave][pos] = (unsigned char)lane;
       const int cnt = __popcll(bm);
       kend = (cnt + 3) & ~3;
-      if (lane >= cnt && lane < kend) st.wl[wave][lane] = (unsigned char)NV;
+      if (lane >= cnt && lane < kend) st.wl[wave][lane] = (unsigned char)NVK;
       __builtin_amdgcn_wave_barrier();
     }
     for (int k0 = 0; k0 < kend; k0 += 4) {
@@ -1095,7 +1101,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
     }
     return nreal;
   };
-  enumerate_tile(sys, td, boff, s_ir, tl, sh, NV, process);
+  enumerate_tile(sys, td, boff, s_ir, tl, sh, NVK, process);
 
   // D[y][z] of block b: lane l holds y = 4 (l / 16) + r, z = l % 16
   const int gz = tl.t0[2] + (lane & 15);
