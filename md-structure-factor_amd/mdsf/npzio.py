@@ -8,7 +8,10 @@ chunk but the last ends with a full flush, so the chunks concatenate into one
 valid raw deflate stream; the member's CRC-32 is computed on the calling
 thread while the pool compresses.
 The archive is a plain ZIP (ZIP64 records where sizes need them) that
-``np.load`` / ``zipfile`` read like any other.
+``np.load`` / ``zipfile`` read like any other.  A member may be given as a
+``concurrent.futures.Future`` of its array; it is resolved when the writer
+reaches it, so the caller can produce later members while earlier ones are
+deflated.
 """
 import concurrent.futures
 import io
@@ -57,6 +60,8 @@ def savez_compressed(path, threads=None, level=6, force_zip64=False, **arrays):
     entries = []
     with open(path, "wb") as f, concurrent.futures.ThreadPoolExecutor(threads) as pool:
         for key, val in arrays.items():
+            if isinstance(val, concurrent.futures.Future):
+                val = val.result()   # produced while earlier members were deflated
             arr = np.asanyarray(val)
             if arr.dtype.hasobject:
                 raise TypeError(f"{key}: object arrays are not written")

@@ -28,3 +28,18 @@ def test_parallel_npz_round_trip(tmp_path, force_zip64, monkeypatch):
     r = np.load(ref)
     for k in arrays:
         assert np.array_equal(r[k], d[k])
+
+
+def test_future_members_resolve_in_order(tmp_path):
+    import concurrent.futures
+    from mdsf import npzio
+    rng = np.random.default_rng(3)
+    a = rng.random((50, 40))
+    b = np.arange(1000, dtype=np.int32)
+    with concurrent.futures.ThreadPoolExecutor(1) as pool:
+        fa = pool.submit(lambda: a.copy())
+        fb = pool.submit(lambda: b.copy())
+        path = npzio.savez_compressed(str(tmp_path / "f"), threads=4, a=fa, L=np.float32(2.5), b=fb)
+    d = np.load(path)
+    assert d.files == ["a", "L", "b"]
+    assert np.array_equal(d["a"], a) and np.array_equal(d["b"], b) and d["L"] == np.float32(2.5)
