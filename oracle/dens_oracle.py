@@ -1,6 +1,6 @@
 """CPU oracle: FP64 numpy restatement of the reference S(q) path (``dens.py``).
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP path: only
+TEST INFRASTRUCTURE ONLY.  The HIP path is tested against this module: only
 ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it.  The product (``md-structure-factor_amd/``) never
 calls it and has no CPU fallback.

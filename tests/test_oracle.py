@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/dens_oracle.py) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/dens_oracle.py) before the HIP path is tested against it.
 
 * golden vectors written by the reference's own compute_sf (tests/golden);
 * brute force / closed forms for the pieces the goldens only cover as a
