@@ -18,6 +18,7 @@
 // values are broadcast with readlane.  Items write 3 FP64 partial sums; a
 // second kernel adds each atom's partials in a fixed order (deterministic).
 #include "mdsf_internal.h"
+#include "tiles.h"
 
 namespace mdsf {
 
@@ -208,7 +209,9 @@ __global__ void k_adjoint_reduce(SysDev sys, const TypeDev* __restrict__ td,
 // Tiled adjoint: grad[atom] = scale * (sum of the atom's tile slots).  16
 // lanes per atom read its slots coalesced (lane i takes slots i, i + 16, ...
 // in order) and a fixed xor-shuffle tree adds the 16 partial sums, so the
-// result is deterministic.
+// result is deterministic.  Only the slots of tiles the atom's window reaches
+// on all three axes are read: exactly the slots the tiled kernel wrote (its
+// candidate test is the same window_reaches), so the buffer needs no clearing.
 constexpr int kRedLanes = 16;
 
 __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td,
@@ -233,9 +236,31 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
   if (act) {
     const TypeDev& ty = td[t];
+    const int4 q = s_ir[s];
     const int per = ty.nsl[0] * ty.nsl[1] * ty.nsl[2];
     const double* src = tpart + 3 * (ty.adjt_base + (long long)slot * per);
+    // first tile of the window per axis (the slice numbering of k_slices)
+    int first[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+      int lo = (ir - ty.half[d]) % sys.n[d];
+      lo = lo < 0 ? lo + sys.n[d] : lo;
+      first[d] = lo / kTile;
+    }
     for (int it = li; it < per; it += kRedLanes) {
+      const int j2 = it % ty.nsl[2], j01 = it / ty.nsl[2];
+      const int j1 = j01 % ty.nsl[1], j0 = j01 / ty.nsl[1];
+      bool reached = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int ir = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+        const int j = d == 0 ? j0 : (d == 1 ? j1 : j2);
+        const int tile = (first[d] + j) % sys.ntiles[d];
+        const int t0 = tile * kTile, t1 = min(t0 + kTile, sys.n[d]);
+        reached = reached && window_reaches(ir, ty.half[d], t0, t1, sys.n[d]);
+      }
+      if (!reached) continue;
       g0 += src[3 * it + 0];
       g1 += src[3 * it + 1];
       g2 += src[3 * it + 2];
@@ -293,16 +318,15 @@ hipError_t launch_adjoint_weights(Plan& p, const double* dsf, hipStream_t s) {
 }
 
 hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale3, double* grad,
-                                  hipStream_t s, bool derivs_ready) {
+                                  hipStream_t s, bool derivs_ready, bool grad_zeroed) {
   const int n = p.sys.natoms;
   if (n == 0) return hipSuccess;
-  hipError_t e = launch_fill_zero(grad, 3LL * n, s);
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
+  if (!grad_zeroed && (e = launch_fill_zero(grad, 3LL * n, s)) != hipSuccess) return e;
   const int* nvalid = p.bucket_off + p.sys.nbuckets;
   if (p.adj_tslots > 0 && p.adj_tpart && p.dslices) {
     // FP32 orthogonal cells: MFMA transpose of the tiled deposition
     if (!derivs_ready && (e = launch_slice_derivs(p, s)) != hipSuccess) return e;
-    if ((e = launch_fill_zero(p.adj_tpart, 3 * p.adj_tslots, s)) != hipSuccess) return e;
     if ((e = launch_adjoint_tiles(p, static_cast<const float*>(grho), s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_adjoint_tile_reduce,
                        dim3((unsigned)(((long long)n * kRedLanes + 255) / 256)), dim3(256), 0, s,

@@ -678,8 +678,10 @@ int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nfra
   for (int64_t t = 0; t < nframes; ++t) {
     const int b = (int)(t & 1);
     char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
+    double* grad_t = d_grad + (size_t)t * p->sys.natoms * 3;
     p->use_slot(b);
     MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[b], 0));
+    MDSF_HIP(launch_fill_zero(grad_t, 3LL * p->sys.natoms, s));  // off the gather's stream
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          nullptr, nullptr, s));
     MDSF_HIP(launch_binning(*p, s, /*derivs=*/true));
@@ -690,9 +692,8 @@ int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nfra
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
     MDSF_HIP(fft_forward(*p, p->rho, false, f));
     MDSF_HIP(fft_inverse(*p, p->adj_w, w_nyq, p->rho, f));
-    MDSF_HIP(launch_deposit_adjoint(*p, p->rho, d_scale + 3 * t,
-                                    d_grad + (size_t)t * p->sys.natoms * 3, f,
-                                    /*derivs_ready=*/true));
+    MDSF_HIP(launch_deposit_adjoint(*p, p->rho, d_scale + 3 * t, grad_t, f,
+                                    /*derivs_ready=*/true, /*grad_zeroed=*/true));
     MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }
   MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) & 1], 0));

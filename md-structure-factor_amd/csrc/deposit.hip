@@ -24,6 +24,7 @@
 #include <type_traits>
 
 #include "mdsf_internal.h"
+#include "tiles.h"
 
 namespace mdsf {
 
@@ -67,13 +68,7 @@ __device__ __forceinline__ int bucket_ranges(int t0, int t1, int A, int N, int B
 }
 
 __device__ __forceinline__ bool overlap1(int ir, int A, int t0, int t1, int N) {
-  const int lo = ir - A, hi = ir + A;
-#pragma unroll
-  for (int j = -1; j <= 1; ++j) {
-    const int a = max(lo, t0 + j * N), b = min(hi, t1 + j * N);
-    if (a < b) return true;
-  }
-  return false;
+  return window_reaches(ir, A, t0, t1, N);
 }
 
 // Per-type bucket ranges for one tile.

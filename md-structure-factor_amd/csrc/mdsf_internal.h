@@ -234,9 +234,11 @@ hipError_t launch_fill_zero(double* p, long long n, hipStream_t s);
 hipError_t launch_adjoint_weights(Plan& p, const double* dsf, hipStream_t s);
 // grad[atom] = scale3 * dL/dr for the last prepared frame from dL/drho (grho,
 // plan precision); scale3 may be null (unit scale).  `derivs_ready`: the
-// binning of this frame already built the slice derivatives.
+// binning of this frame already built the slice derivatives; `grad_zeroed`:
+// the caller cleared grad (needed only for atoms the last frame failed to bin).
 hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale3, double* grad,
-                                  hipStream_t s, bool derivs_ready = false);
+                                  hipStream_t s, bool derivs_ready = false,
+                                  bool grad_zeroed = false);
 
 hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s);
 hipError_t launch_outputs(const int n[3], const double* sf, const double box[3], int box_f32,
