@@ -651,11 +651,12 @@ struct StageMXY {
   int vreal[NS], vcls[NS], vxm[NS];
 };
 
-// The gather is latency-bound (batch staging, candidate streaming): the
-// orthogonal-cell kernel asks for 6 waves per SIMD, which its register budget
-// allows without spills and its LDS footprint (< 26.6 KB) matches.
+// The gather is latency-bound (batch staging, candidate streaming).  The
+// orthogonal-cell kernel fits 7 blocks per CU (23.4 KB LDS, 72 VGPRs) but is
+// held to 6 waves per SIMD: alone it runs as fast as with 7, and in the frame
+// pipeline the LDS left free takes a block of the concurrent z pass.
 template <int MODE, int MT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == MODE_DIAG ? 7 : 1, 8)))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == MODE_DIAG ? 6 : 1, MODE == MODE_DIAG ? 6 : 8)))
 void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const int* __restrict__ boff,
                                                            const int4* __restrict__ s_ir,

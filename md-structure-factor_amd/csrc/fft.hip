@@ -337,18 +337,23 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z(R2CArgs<R> a) {
   }
 }
 
+// Rows per block of the register-staged z passes: half the threads own one
+// stage-1 butterfly (m / R1 per row).
+constexpr int z_rows_per_block(int m, int r1) { return kFftThreads / (m / r1) / 2; }
+
 // Register-staged z pass (FP32, m = R1 * R2, two stages).  Stage 1 reads its
 // R1 inputs per butterfly straight from global memory (row-contiguous across
 // lanes) into registers; stage 2 runs in place in LDS (each thread reads and
 // writes the same R2 slots); the split into the half spectrum then reads the
-// pairs (k, m - k) from LDS and stores whole rows contiguously.  P rows per
-// block give every thread one stage-1 butterfly, and the LDS footprint is one
-// plane pair (no ping-pong), so the pass shares CUs well with the deposition.
+// pairs (k, m - k) from LDS and stores whole rows contiguously.  A block holds
+// P rows (half the threads take a stage-1 butterfly) in one plane pair of
+// LDS (16.5 KB at m = 128): small enough to fit beside the deposition's
+// blocks when the frame pipeline runs both, and faster alone than 2P rows.
 template <typename R, int R1, int R2>
 __global__ __launch_bounds__(kFftThreads) void k_r2c_z_reg(R2CArgs<R> a) {
   constexpr int m = R1 * R2;
   constexpr int nq1 = m / R1;             // stage-1 butterflies per row
-  constexpr int P = kFftThreads / nq1;    // rows per block
+  constexpr int P = z_rows_per_block(m, R1);
   constexpr int pitch = m + 1;
   constexpr int ns = m / R2;              // stage-2 butterflies per row
   __shared__ R re[P * pitch];
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z_reg(R2CArgs<R> a) {
   const int pc = (int)min((long long)P, a.rows - r0);
   const auto* src = reinterpret_cast<const typename V2<R>::T*>(a.in) + r0 * m;
   const void* tw = sizeof(R) == 8 ? (const void*)a.ax.tw64 : (const void*)a.ax.tw;
-  {
+  if (threadIdx.x < P * nq1) {
     const int j = threadIdx.x % nq1, p = threadIdx.x / nq1;
     Cx<R> v[R1];
 #pragma unroll
@@ -478,7 +483,7 @@ template <typename R, int R1, int R2>
 __global__ __launch_bounds__(kFftThreads) void k_c2r_z_reg(C2RArgs<R, R> a) {
   constexpr int m = R1 * R2;
   constexpr int nq1 = m / R1;
-  constexpr int P = kFftThreads / nq1;
+  constexpr int P = z_rows_per_block(m, R1);
   constexpr int pitch = m + 1;
   constexpr int ns = m / R2;
   __shared__ R re[P * pitch];
@@ -487,7 +492,7 @@ __global__ __launch_bounds__(kFftThreads) void k_c2r_z_reg(C2RArgs<R, R> a) {
   const int pc = (int)min((long long)P, a.rows - r0);
   const auto* src = a.main + r0 * m;
   const void* tw = sizeof(R) == 8 ? (const void*)a.ax.tw64 : (const void*)a.ax.tw;
-  {
+  if (threadIdx.x < P * nq1) {
     const int j = threadIdx.x % nq1, p = threadIdx.x / nq1;
     Cx<R> v[R1];
 #pragma unroll
@@ -873,9 +878,9 @@ typename V2<R>::T* spec_nyq(Plan& p) {
 // when m has no two-stage split.
 inline int z_reg_rows(int m) {
   switch (m) {
-    case 64: return kFftThreads / (64 / 8);
-    case 128: return kFftThreads / (128 / 16);
-    case 256: return kFftThreads / (256 / 16);
+    case 64: return z_rows_per_block(64, 8);
+    case 128: return z_rows_per_block(128, 16);
+    case 256: return z_rows_per_block(256, 16);
     default: return 0;
   }
 }
