@@ -64,7 +64,8 @@ def cpu_baseline(coords0, dims, typ, rad, sres, budget_atoms=None):
     """Time the FP64 oracle on a type-stratified atom sample of frame 0 plus one
     fold + rfftn + |F|^2, and extrapolate to frames/s for the full frame."""
     from oracle import dens_oracle as O
-    budget_atoms = budget_atoms or {"H": 2000, "C": 2000, "O": 960, "Na": 80}
+    # about 10 s of single-core work at 256^3 (Na windows cost ~15 ms per atom)
+    budget_atoms = budget_atoms or {"H": 10000, "C": 8000, "O": 960, "Na": 300}
     r = np.array(coords0[None], copy=True)
     o = O.setup(r, np.array(dims[:1]), typ, rad, sres, better=False)
     n, nb = o["N"], o["Nb"]
