@@ -18,6 +18,7 @@
 // values are broadcast with readlane.  Items write 3 FP64 partial sums; a
 // second kernel adds each atom's partials in a fixed order (deterministic).
 #include "mdsf_internal.h"
+#include "spill.h"
 #include "tiles.h"
 
 namespace mdsf {
@@ -218,7 +219,9 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
                                       const int4* __restrict__ s_ir, const int* __restrict__ s_idx,
                                       const int* __restrict__ nvalid,
                                       const double* __restrict__ tpart,
-                                      const double* __restrict__ scale3, double* __restrict__ grad) {
+                                      const double* __restrict__ scale3,
+                                      const double* __restrict__ s_r,
+                                      const float* __restrict__ grho, double* __restrict__ grad) {
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const int s = (int)(g / kRedLanes), li = (int)(g % kRedLanes);
   // every lane of a 16-lane group takes the same branch, so the shuffles
@@ -264,6 +267,40 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
       g0 += src[3 * it + 0];
       g1 += src[3 * it + 1];
       g2 += src[3 * it + 2];
+    }
+    if (q.w >> 16) {
+      // corner atom: the quirk images whose destination z tile the window
+      // does not reach (the deposition's spill pass), dL/dr = sum grho dv/dr
+      const int N0 = sys.n[0], N1 = sys.n[1], N2 = sys.n[2], nb = sys.nb;
+      const double r[3] = {s_r[3 * (size_t)s + 0], s_r[3 * (size_t)s + 1],
+                           s_r[3 * (size_t)s + 2]};
+      const SpillRange xl = spill_side_l(q.x, ty.half[0]), xh = spill_side_h(q.x, ty.half[0], N0);
+      const int nxl = xl.len(), nx = nxl + xh.len();
+      for (int band = 0; band < 2; ++band) {
+        // band 0: y on H, z on L -> z + Nb; band 1: y on L, z on H -> z - Nb
+        const SpillRange yr = band == 0 ? spill_side_h(q.y, ty.half[1], N1)
+                                        : spill_side_l(q.y, ty.half[1]);
+        const SpillRange zr = band == 0 ? spill_side_l(q.z, ty.half[2])
+                                        : spill_side_h(q.z, ty.half[2], N2);
+        const int ny = yr.len(), nz = zr.len();
+        const long long total = (long long)nx * ny * nz;
+        for (long long it = li; it < total; it += kRedLanes) {
+          const int kz = (int)(it % nz), kxy = (int)(it / nz);
+          const int ky = kxy % ny, kx = kxy / ny;
+          const int ux = kx < nxl ? xl.lo + kx : xh.lo + (kx - nxl);
+          const int uy = yr.lo + ky, uz = zr.lo + kz;
+          const int zd = band == 0 ? uz + nb : uz - nb;
+          const int t0 = (zd / kTile) * kTile, t1 = min(t0 + kTile, N2);
+          if (!spill_tile_unreached(q.z, ty.half[2], t0, t1, N2)) continue;
+          const int xd = ux < 0 ? ux + N0 : ux - N0, yd = band == 0 ? uy - N1 : uy + N1;
+          const double w = (double)grho[((size_t)xd * N1 + yd) * N2 + zd];
+          double dv[3];
+          spill_value(sys, ty, r, ux, uy, uz, dv);
+          g0 += w * dv[0];
+          g1 += w * dv[1];
+          g2 += w * dv[2];
+        }
+      }
     }
   }
 #pragma unroll
@@ -330,7 +367,8 @@ hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale
     if ((e = launch_adjoint_tiles(p, static_cast<const float*>(grho), s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_adjoint_tile_reduce,
                        dim3((unsigned)(((long long)n * kRedLanes + 255) / 256)), dim3(256), 0, s,
-                       p.sys, p.types, p.s_ir, p.s_idx, nvalid, p.adj_tpart, scale3, grad);
+                       p.sys, p.types, p.s_ir, p.s_idx, nvalid, p.adj_tpart, scale3, p.s_r,
+                       static_cast<const float*>(grho), grad);
     return hipGetLastError();
   }
   if (p.adj_items > 0) {

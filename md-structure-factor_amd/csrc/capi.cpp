@@ -77,7 +77,7 @@ void free_plan(Plan* p) {
   for (FrameState& f : p->slot) {
     void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
                   f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices,
-                  f.dslices};
+                  f.dslices, f.corners};
     for (void* q : fs)
       if (q) (void)hipFree(q);
   }
@@ -155,6 +155,12 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     if (desc->nborder > n)
       return fail(MDSF_ERR_ARG, "Nborder exceeds the grid (the reference's fold needs Nborder <= N)");
     if (!(desc->dr[d] > 0.0)) return fail(MDSF_ERR_ARG, "dr must be positive");
+    // tiles of the corner-fold spill pass: those meeting [0, Nb) or [N - Nb, N)
+    int spill = 0;
+    for (int a = 0; a < n; a += kTile)
+      if (a < desc->nborder || std::min(a + kTile, n) > n - desc->nborder) ++spill;
+    if (spill > kMaxSpillTiles)
+      return fail(MDSF_ERR_UNSUPPORTED, "Nborder too large for the corner fold pass");
   }
   if (desc->nborder < 0) return fail(MDSF_ERR_ARG, "negative Nborder");
   if (desc->ntypes < 0 || desc->ntypes > kMaxTypes)
@@ -330,6 +336,8 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       f.tables = tb;
     }
     TRY(dalloc(&f.slices, (size_t)p->slice_floats));
+    TRY(dalloc(&f.corners, (size_t)2 + 2 * kCornerCap));
+    TRY(hipMemset(f.corners, 0, sizeof(int) * ((size_t)2 + 2 * kCornerCap)));
   }
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));

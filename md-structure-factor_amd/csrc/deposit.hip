@@ -24,6 +24,7 @@
 #include <type_traits>
 
 #include "mdsf_internal.h"
+#include "spill.h"
 #include "tiles.h"
 
 namespace mdsf {
@@ -1357,6 +1358,168 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
   enumerate_tile(sys, td, boff, s_ir, tl, sh, NV, process);
 }
 
+// ---------------------------------------------------------------------------
+// Corner-quirk spill (dens.py:107): one workgroup per destination tile of the
+// spill grid.  The candidates are the frame's corner atoms in sorted-index
+// order (the list k_gather and k_corner_order build); past kCornerCap of them,
+// per band, the atoms whose windows reach the source tile (the destination
+// tile shifted in z by -Nb for band 0, +Nb for band 1) in the deposition's
+// enumeration order.  A corner atom whose z window does not reach the
+// destination z tile adds its quirk images (x off-core, y on H / L, z on L /
+// H) to the tile.  Each lane owns one (x, y) column and adds to the density
+// after the tiled deposition has written it: one writer per cell, a fixed
+// order per cell, bitwise reproducible.
+constexpr int kSpillTrigger = 128;
+
+template <typename ACC>
+__global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys, SpillTiles sp,
+                                                            const TypeDev* __restrict__ td,
+                                                            const int* __restrict__ boff,
+                                                            const int4* __restrict__ s_ir,
+                                                            const double* __restrict__ s_r,
+                                                            const int* __restrict__ corners,
+                                                            ACC* __restrict__ rho) {
+  __shared__ EnumShared<kMaxTypes> sh;
+  __shared__ int4 sq[kThreads];
+  __shared__ double sr[kThreads][3];
+  const int ncorner = corners[0], nspill = corners[1 + 2 * kCornerCap];
+  const bool listed = ncorner <= kCornerCap;
+  if (ncorner == 0 || (listed && nspill == 0)) return;  // uniform over the block
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x;
+  const int iz = b % sp.n[2], iy = (b / sp.n[2]) % sp.n[1], ix = b / (sp.n[2] * sp.n[1]);
+  Tile dst;
+  dst.t0[0] = sp.t[0][ix] * T;
+  dst.t0[1] = sp.t[1][iy] * T;
+  dst.t0[2] = sp.t[2][iz] * T;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dst.t1[d] = min(dst.t0[d] + T, sys.n[d]);
+  const int N0 = sys.n[0], N1 = sys.n[1], N2 = sys.n[2], nb = sys.nb;
+  const int xd = dst.t0[0] + (tid & 15), yd = dst.t0[1] + (tid >> 4);
+  double acc[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) acc[k] = 0.0;
+
+  for (int band = 0; band < 2; ++band) {
+    const int zlo = band == 0 ? 0 : N2 - nb, zhi = band == 0 ? nb : N2;
+    if (dst.t1[2] <= zlo || dst.t0[2] >= zhi) continue;  // uniform over the block
+    const int shift = band == 0 ? -nb : nb;               // u_z = z + shift
+    // does atom (q) spill into this destination tile on this band?
+    auto reaches = [&](const int4& q, const TypeDev& ty) -> bool {
+      return ((spill_bands(q, ty, sys) >> band) & 1) &&
+             spill_tile_unreached(q.z, ty.half[2], dst.t0[2], dst.t1[2], N2) &&
+             window_reaches(q.x, ty.half[0], dst.t0[0], dst.t1[0], N0) &&
+             window_reaches(q.y, ty.half[1], dst.t0[1], dst.t1[1], N1);
+    };
+    // this lane's cells of the tile from one reaching atom
+    auto add = [&](const int4& q, const double* r) {
+      if (xd >= N0 || yd >= N1) return;
+      const TypeDev& ty = td[q.w & 0xffff];
+      const int Ax = ty.half[0], Ay = ty.half[1], Az = ty.half[2];
+      // y image on this band's side: H (u = y + N) for band 0, L (u = y - N) for band 1
+      const int uy = band == 0 ? yd + N1 : yd - N1;
+      if (uy < q.y - Ay || uy >= q.y + Ay) return;
+#pragma unroll
+      for (int sx = 0; sx < 2; ++sx) {
+        const int ux = sx == 0 ? xd - N0 : xd + N0;  // the off-core x images
+        if (ux < q.x - Ax || ux >= q.x + Ax) continue;
+#pragma unroll
+        for (int k = 0; k < T; ++k) {
+          const int z = dst.t0[2] + k;
+          if (z >= dst.t1[2] || z < zlo || z >= zhi) continue;
+          const int uz = z + shift;  // < 0 on band 0, >= N on band 1
+          if (uz < q.z - Az || uz >= q.z + Az) continue;
+          acc[k] += spill_value(sys, ty, r, ux, uy, uz, nullptr);
+        }
+      }
+    };
+    if (listed) {
+      // 256 listed atoms at a time: parallel loads, the ones reaching this
+      // tile compacted in list order into LDS, then every lane walks them
+      for (int base = 0; base < nspill; base += kThreads) {
+        const int a = base + tid;
+        int4 q = make_int4(0, 0, 0, 0);
+        int c = 0;
+        bool hit = false;
+        if (a < nspill) {
+          c = corners[1 + kCornerCap + a];
+          q = s_ir[c];
+          hit = reaches(q, td[q.w & 0xffff]);
+        }
+        int pos;
+        const int m = block_scan(sh, hit ? 1 : 0, &pos);
+        if (hit) {
+          sq[pos] = q;
+          sr[pos][0] = s_r[3 * (size_t)c + 0];
+          sr[pos][1] = s_r[3 * (size_t)c + 1];
+          sr[pos][2] = s_r[3 * (size_t)c + 2];
+        }
+        __syncthreads();
+        for (int j = 0; j < m; ++j) {
+          const double r[3] = {sr[j][0], sr[j][1], sr[j][2]};
+          add(sq[j], r);
+        }
+        __syncthreads();
+      }
+    } else {
+      Tile src = dst;
+      src.t0[2] += shift;
+      src.t1[2] += shift;
+      auto process = [&](int count, int head) -> int {
+        for (int i = 0; i < count; ++i) {
+          const int e = sh.list[(head + i) & kListMask];
+          if (e >= 0) continue;  // corner flag (bit 31) clear
+          const int c = e & 0x7fffffff;
+          const int4 q = s_ir[c];
+          if (!reaches(q, td[q.w & 0xffff])) continue;
+          const double r[3] = {s_r[3 * (size_t)c + 0], s_r[3 * (size_t)c + 1],
+                               s_r[3 * (size_t)c + 2]};
+          add(q, r);
+        }
+        return count;
+      };
+      enumerate_tile(sys, td, boff, s_ir, src, sh, kSpillTrigger, process);
+      __syncthreads();
+    }
+  }
+  if (xd < N0 && yd < N1) {
+    ACC* dcol = rho + ((size_t)xd * N1 + yd) * N2 + dst.t0[2];
+    const int nz = dst.t1[2] - dst.t0[2];
+#pragma unroll
+    for (int k = 0; k < T; ++k)
+      if (k < nz && acc[k] != 0.0) dcol[k] = (ACC)((double)dcol[k] + acc[k]);
+  }
+}
+
+// Tiles meeting [0, Nb) or [N - Nb, N) on one axis; false if more than fit.
+bool spill_axis_tiles(int N, int nb, int* t, int* n) {
+  const int nt = (N + T - 1) / T;
+  int c = 0;
+  for (int i = 0; i < nt; ++i) {
+    const int a = i * T, e = min(a + T, N);
+    if (a < nb || e > N - nb) {
+      if (c == kMaxSpillTiles) return false;
+      t[c++] = i;
+    }
+  }
+  *n = c;
+  return true;
+}
+
+template <typename ACC>
+hipError_t launch_spill(Plan& p, ACC* out, hipStream_t s) {
+  const SysDev& sys = p.sys;
+  if (sys.nb == 0 || sys.natoms == 0) return hipSuccess;
+  SpillTiles sp{};
+  for (int d = 0; d < 3; ++d)
+    if (!spill_axis_tiles(sys.n[d], sys.nb, sp.t[d], &sp.n[d])) return hipErrorInvalidConfiguration;
+  const long long nblk = (long long)sp.n[0] * sp.n[1] * sp.n[2];
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_deposit_spill<ACC>, dim3((unsigned)nblk), dim3(kThreads), 0, s, sys, sp,
+                     p.types, p.bucket_off, p.s_ir, p.s_r, p.corners, out);
+  return hipGetLastError();
+}
+
 template <typename ACC>
 hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
   const SysDev& sys = p.sys;
@@ -1392,8 +1555,13 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
 }  // namespace
 
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s) {
-  if (p.fft64) return launch_impl<double>(p, static_cast<double*>(out), s);
-  return launch_impl<float>(p, static_cast<float*>(out), s);
+  hipError_t e;
+  if (p.fft64) {
+    if ((e = launch_impl<double>(p, static_cast<double*>(out), s)) != hipSuccess) return e;
+    return launch_spill<double>(p, static_cast<double*>(out), s);
+  }
+  if ((e = launch_impl<float>(p, static_cast<float*>(out), s)) != hipSuccess) return e;
+  return launch_spill<float>(p, static_cast<float*>(out), s);
 }
 
 hipError_t launch_adjoint_tiles(Plan& p, const float* grho, hipStream_t s) {

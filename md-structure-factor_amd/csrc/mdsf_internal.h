@@ -16,6 +16,8 @@ constexpr int kTile = 16;       // deposition tile edge along x, y and z
 constexpr int kBatch = 32;      // atoms staged per deposition batch
 constexpr int kMaxStages = 16;  // FFT stages per axis
 constexpr int kTuneReg = 100;   // FFT pencil choice: register-staged kernel
+constexpr int kMaxSpillTiles = 96;  // per axis, tiles of the corner-fold spill pass
+constexpr int kCornerCap = 2048;    // corner atoms listed per frame for the spill pass
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
 enum Mode { MODE_DIAG = 0, MODE_XY = 1 };
@@ -99,6 +101,10 @@ struct FrameState {
   void* tables = nullptr;
   float* slices = nullptr;
   float* dslices = nullptr;  // derivatives of the slices in the atom position (adjoint)
+  // corner atoms (window off the core on all three axes): [0] count, then
+  // kCornerCap sorted indices in arrival order, then those that spill in index
+  // order, then their count
+  int* corners = nullptr;
 };
 
 struct Plan {
@@ -123,6 +129,7 @@ struct Plan {
   void* tables = nullptr;         // per-atom factor tables (FP64 plans)
   float* slices = nullptr;        // per-atom tile slices (FP32 plans, orthogonal-axis cells)
   float* dslices = nullptr;       // their derivatives (adjoint only)
+  int* corners = nullptr;         // corner-atom list of the current slot
   long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
   // grids: two density / spectrum buffers so that one frame's FFT can run
@@ -153,6 +160,7 @@ struct Plan {
     tables = f.tables;
     slices = f.slices;
     dslices = f.dslices;
+    corners = f.corners;
     rho = rho_buf[b & 1];
     spec_main = rho;
     spec_nyq = nyq_buf[b & 1];

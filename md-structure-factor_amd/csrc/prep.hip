@@ -9,6 +9,7 @@
 //   window         dens.py:287-297 padded indices [ir-A+Nb, ir+A+Nb)
 //   Gaussian       dens.py:299-308 (Nel/sig^3) exp(-|(r - x_v) ucell|^2 / (2 sig^2))
 #include "mdsf_internal.h"
+#include "spill.h"
 
 namespace mdsf {
 
@@ -142,8 +143,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int* __restrict__ 
 __global__ __launch_bounds__(kScanThreads) void k_scan_add(int* __restrict__ boff,
                                                            int* __restrict__ bcur,
                                                            const int* __restrict__ aux, int nbk,
-                                                           int nblocks) {
+                                                           int nblocks, int* __restrict__ corners) {
   __shared__ int base_sh;
+  if (blockIdx.x == 0 && threadIdx.x == 0) corners[0] = 0;  // refilled by k_gather
   if (threadIdx.x < 64) {
     int s = 0;
     for (int b = threadIdx.x; b < (int)blockIdx.x; b += 64) s += aux[b];
@@ -194,14 +196,53 @@ __global__ void k_rank_sort(const int* __restrict__ key, const int* __restrict__
 // Gathers atom data into sorted order (valid sorted positions only).
 __global__ void k_gather(const int* __restrict__ s_idx, const int4* __restrict__ ir4,
                          const double* __restrict__ rpos, int4* __restrict__ s_ir,
-                         double* __restrict__ s_r, const int* __restrict__ nvalid) {
+                         double* __restrict__ s_r, const int* __restrict__ nvalid,
+                         int* __restrict__ corners) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= *nvalid) return;
   const int i = s_idx[s];
-  s_ir[s] = ir4[i];
+  const int4 q = ir4[i];
+  s_ir[s] = q;
+  if (q.w >> 16) {
+    // corner atom: listed in arrival order; k_corner_order sorts the list
+    const int k = atomicAdd(corners, 1);
+    if (k < kCornerCap) corners[1 + k] = s;
+  }
   s_r[3 * (size_t)s + 0] = rpos[3 * (size_t)i + 0];
   s_r[3 * (size_t)s + 1] = rpos[3 * (size_t)i + 1];
   s_r[3 * (size_t)s + 2] = rpos[3 * (size_t)i + 2];
+}
+
+// The corner atoms that spill (spill_bands != 0), in sorted-index order, one
+// block: entry v goes to its rank among them, so the spill pass visits them
+// in a fixed order whatever order k_gather's atomics gave.  Their count goes
+// to corners[1 + 2 kCornerCap].
+constexpr int kOrderThreads = 256;
+__global__ __launch_bounds__(kOrderThreads) void k_corner_order(SysDev sys,
+                                                                const TypeDev* __restrict__ td,
+                                                                const int4* __restrict__ s_ir,
+                                                                int* __restrict__ corners) {
+  __shared__ int lst[kCornerCap];
+  __shared__ int nrel;
+  const int n = min(corners[0], kCornerCap);
+  if (threadIdx.x == 0) nrel = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kOrderThreads) {
+    const int c = corners[1 + i];
+    const int4 q = s_ir[c];
+    const bool spills = spill_bands(q, td[q.w & 0xffff], sys) != 0;
+    lst[i] = spills ? c : -1;
+    if (spills) atomicAdd(&nrel, 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kOrderThreads) {
+    const int v = lst[i];
+    if (v < 0) continue;
+    int r = 0;
+    for (int j = 0; j < n; ++j) r += (lst[j] >= 0 && lst[j] < v) ? 1 : 0;
+    corners[1 + kCornerCap + r] = v;
+  }
+  if (threadIdx.x == 0) corners[1 + 2 * kCornerCap] = nrel;
 }
 
 // Per-atom 1-D factor tables, one thread per entry:
@@ -378,14 +419,16 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
   hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
                      p.bucket_off, p.bucket_aux, nbk);
   hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off, p.bucket_cur,
-                     p.bucket_aux, nbk, nblk);
+                     p.bucket_aux, nbk, nblk, p.corners);
   if (n > 0) {
     hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
                        p.s_tmp, n);
     hipLaunchKernelGGL(k_rank_sort, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_off,
                        p.s_tmp, p.s_idx, p.bucket_off + nbk);
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.s_idx, p.ir4, p.rpos,
-                       p.s_ir, p.s_r, p.bucket_off + nbk);
+                       p.s_ir, p.s_r, p.bucket_off + nbk, p.corners);
+    hipLaunchKernelGGL(k_corner_order, dim3(1), dim3(kOrderThreads), 0, s, p.sys, p.types,
+                       p.s_ir, p.corners);
     if (p.slice_floats > 0)
       hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0,
                          s, p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices,

@@ -58,16 +58,33 @@ class Setup:
         self.N = grid_shape(self.L, sres, better)             # dens.py:181-189
         self.dr = np.divide(self.L, self.N)                   # dens.py:202
         typ = np.asarray(typ)
-        self.names, self.type_index = np.unique(typ, return_inverse=True)
+        self.names, self.name_index = np.unique(typ, return_inverse=True)
         self.names = [str(x) for x in self.names]
+        self.name_index = self.name_index.reshape(-1)
         self.bdict = borders(rad, self.dr, self.names)        # dens.py:223
         if len(self.bdict) == 0:
             raise ValueError("no atoms")
         self.nborder = int(max(max(v) for v in self.bdict.values()))  # dens.py:226-231
-        self.amp = np.array([rad[a][0] / np.power(rad[a][1], 3.) for a in self.names])
-        self.two_sig2 = np.array([2 * np.power(rad[a][1], 2.) for a in self.names])
-        self.half = np.array([self.bdict[a].astype(int) for a in self.names], dtype=np.int64)
-        self.counts = np.bincount(self.type_index.ravel(), minlength=len(self.names))
+        # The density is one sum over atoms (dens.py:283-308): an atom enters it only
+        # through its amplitude Nel/sigma^3, its 2 sigma^2 and its window half-widths,
+        # so names whose three agree exactly deposit as one device type.  Force-field
+        # systems name every atom of a residue apart (100+ names) with a handful of
+        # distinct parameter sets.
+        keys, self.type_of_name, self.type_names = {}, [], []
+        for a in self.names:
+            key = (float(rad[a][0] / np.power(rad[a][1], 3.)), float(2 * np.power(rad[a][1], 2.)),
+                   tuple(int(h) for h in self.bdict[a].astype(int)))
+            if key not in keys:
+                keys[key] = len(keys)
+                self.type_names.append([])
+            self.type_of_name.append(keys[key])
+            self.type_names[keys[key]].append(a)
+        self.type_of_name = np.asarray(self.type_of_name, dtype=np.int64)
+        self.type_index = self.type_of_name[self.name_index]
+        self.amp = np.array([k[0] for k in keys])
+        self.two_sig2 = np.array([k[1] for k in keys])
+        self.half = np.array([k[2] for k in keys], dtype=np.int64)
+        self.counts = np.bincount(self.type_index, minlength=len(keys))
         self.nframes, self.natoms = r.shape[0], r.shape[1]
         self.wrap = wrap_range(self.nframes, self.natoms)
         # numpy rescales / wraps in float32 only when both coords and dims are float32

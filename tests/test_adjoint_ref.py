@@ -33,6 +33,75 @@ def corner_system(seed, box=(7.0, 8.0, 9.0), frames=2, n_each=2, names=("C", "O"
     return coords, dims, typ
 
 
+def spill_system(seed, frames=1, box=(40.0, 44.0, 42.0), names=("H", "C", "O", "Na")):
+    """Atoms within about an Angstrom of all 8 cell corners in a ~40 A box, with
+    Na setting Nborder = 17 at Sres = 1: at N ~ 40, the reference's corner z
+    slot (u_z + Nb for y high / z low, u_z - Nb for y low / z high,
+    dens.py:107) sends contributions to z tiles of 16 cells that the atom's
+    own window does not reach."""
+    rng = np.random.default_rng(seed)
+    box = np.asarray(box)
+    pts = []
+    for cx in (0, 1):
+        for cy in (0, 1):
+            for cz in (0, 1):
+                for _ in range(3):
+                    pts.append(np.array([cx, cy, cz]) * box + rng.normal(0, 0.8, 3))
+    pts += list(rng.uniform(0, 1, (12, 3)) * box)
+    base = np.mod(np.array(pts), box)
+    coords = np.stack([np.mod(base + rng.normal(0, 0.05, base.shape), box) for _ in range(frames)])
+    dims = np.stack([box * (1.0 + 0.002 * t) for t in range(frames)])
+    typ = [names[i % len(names)] for i in range(base.shape[0])]
+    return coords, dims, typ
+
+
+def spill_images(rframe, typ, rad, sres, dims, tile=16):
+    """Quirk images (corner, y and z on opposite sides) whose destination z tile
+    the atom's z window [ir - A, ir + A) does not reach, on the oracle's set-up."""
+    r = np.array(rframe)[None]
+    o = O.setup(r, np.array(dims), typ, rad, sres, False)
+    n, nb, count = [int(v) for v in o["N"]], o["Nb"], 0
+    for a in range(r.shape[1]):
+        ir = (r[0, a] / o["dr"]).astype(int)
+        A = o["bdict"][typ[a]].astype(int)
+        win = [range(ir[d] - A[d], ir[d] + A[d]) for d in range(3)]
+        xs = [u for u in win[0] if u < 0 or u >= n[0]]
+        for ylo in (False, True):
+            ys = [u for u in win[1] if (u < 0 if ylo else u >= n[1])]
+            zs = [u for u in win[2] if (u >= n[2] if ylo else u < 0)]
+            for uz in zs:
+                zd = uz - nb if ylo else uz + nb
+                t0 = (zd // tile) * tile
+                t1 = min(t0 + tile, n[2])
+                reached = any(max(ir[2] - A[2], t0 + j * n[2]) < min(ir[2] + A[2], t1 + j * n[2])
+                              for j in (-1, 0, 1))
+                if not reached:
+                    count += len(xs) * len(ys)
+    return count
+
+
+@pytest.mark.parametrize("theta", [None, 120.0])
+def test_torch_reference_matches_oracle_spill_system(rad, theta):
+    coords, dims, typ = spill_system(11)
+    assert spill_images(coords[0], typ, rad, 1.0, dims) > 100
+    ucell = monoclinic(theta)
+    want = O.compute_sf(np.array(coords), np.array(dims), typ, None, rad, ucell, 1.0,
+                        better=False)["sf"]
+    got = T.structure_factor(torch.tensor(coords), dims, typ, rad, 1.0, ucell).numpy()
+    mask = want > 1e-12 * want.max()
+    err = np.abs(got[mask] - want[mask]) / want[mask]
+    assert err.max() < 1e-9, err.max()
+
+
+def test_spill_count_is_zero_without_corner_atoms(rad):
+    """The counter itself: atoms away from the corners give no spill images."""
+    rng = np.random.default_rng(4)
+    box = np.array([40.0, 44.0, 42.0])
+    r = 8.0 + rng.uniform(0, 1, (30, 3)) * (box - 16.0)
+    typ = [("H", "C", "O", "Na")[i % 4] for i in range(30)]
+    assert spill_images(r, typ, rad, 1.0, box[None]) == 0
+
+
 def monoclinic(theta_deg):
     if theta_deg is None:
         return np.eye(3)

@@ -16,7 +16,7 @@ import pytest
 import torch
 
 import sf_torch_ref as T
-from test_adjoint_ref import corner_system, monoclinic
+from test_adjoint_ref import corner_system, monoclinic, spill_images, spill_system
 
 pytestmark = pytest.mark.gpu
 
@@ -78,6 +78,28 @@ def test_deposit_adjoint_matches_autograd(theta, fp64, rad, gpu):
     plan.check_errors()
     geo = T.prepare(dims, typ, rad, 0.5)
     r = c[0].cpu().clone().requires_grad_(True)           # wrapped on the GPU
+    rho = T.frame_density(r, typ, rad, geo, ucell)
+    torch.sum(torch.tensor(gd) * rho).backward()
+    assert rel_err(got, r.grad.numpy()) <= TOL[fp64]
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("theta", [None, 120.0])
+def test_deposit_adjoint_corner_spill(theta, fp64, rad, gpu):
+    """Corner-slot images (dens.py:107) in z tiles the atom's window does not
+    reach (Nborder = 17 at N ~ 40): their gradient terms, against autograd."""
+    coords, dims, typ = spill_system(11)
+    assert spill_images(coords[0], typ, rad, 1.0, dims) > 100
+    ucell = monoclinic(theta)
+    s, plan, types, scale = _plan(coords, dims, typ, rad, 1.0, ucell, fp64)
+    c = torch.tensor(coords).to(plan.device)
+    plan.prepare_frame(c[0], types, scale[0], [float(v) for v in s.L], s.arith, s.wrap)
+    rng = np.random.default_rng(5)
+    gd = rng.normal(size=tuple(int(v) for v in s.N))
+    got = plan.deposit_adjoint(torch.tensor(gd)).cpu().numpy()
+    plan.check_errors()
+    geo = T.prepare(dims, typ, rad, 1.0)
+    r = c[0].cpu().clone().requires_grad_(True)
     rho = T.frame_density(r, typ, rad, geo, ucell)
     torch.sum(torch.tensor(gd) * rho).backward()
     assert rel_err(got, r.grad.numpy()) <= TOL[fp64]

@@ -24,9 +24,10 @@ def test_setup_matches_oracle(name, rad):
     assert s.nborder == o["Nb"]
     assert s.L.dtype == c["out_L"].dtype and np.array_equal(s.L, c["out_L"])
     for t, nm in enumerate(s.names):
-        assert list(s.half[t]) == list(o["bdict"][nm].astype(int))
+        assert list(s.half[s.type_of_name[t]]) == list(o["bdict"][nm].astype(int))
     assert s.counts.sum() == r.shape[1]
-    assert [s.names[i] for i in s.type_index] == c["typ"]
+    assert [s.names[i] for i in s.name_index] == c["typ"]
+    assert all(s.type_of_name[s.name_index[i]] == s.type_index[i] for i in range(r.shape[1]))
     # arithmetic dtype of the in-place rescale / wrap follows numpy promotion
     want = (r.dtype == np.float32 and np.asarray(c["in_dims"]).dtype == np.float32)
     assert (s.arith == 0) == want
@@ -37,6 +38,34 @@ def test_unknown_type_raises_keyerror(rad):
     r = np.zeros((1, 2, 3)) + 1.0
     with pytest.raises(KeyError):
         engine.Setup(r, np.ones((1, 3)) * 10, ["H", "OW"], rad, 0.5, True, np.eye(3))
+
+
+def test_names_with_equal_parameters_share_a_device_type(rad):
+    """Force-field names (C1, C2, H17 ...) that carry one element's radii
+    deposit as one type; the density is a sum over atoms (dens.py:283-308), so
+    the oracle run with each name replaced by its group's first name is
+    bit-identical to the run with the names as given."""
+    from mdsf import engine
+    rad = dict(rad)
+    for i in range(70):
+        rad[f"C{i}"] = rad["C"]
+        rad[f"H{i}"] = rad["H"]
+    rad["Hx"] = (rad["H"][0], rad["H"][1] * 1.0000001)
+    rng = np.random.default_rng(7)
+    typ = [rng.choice([f"C{i % 70}", f"H{i % 70}", "Hx", "O", "C"]) for i in range(400)]
+    r = rng.uniform(0, 12, (2, 400, 3))
+    dims = np.full((2, 3), 12.0)
+    s = engine.Setup(np.array(r), dims, typ, rad, 0.6, True, np.eye(3))
+    assert len(s.names) > 64 and len(s.amp) == 4
+    groups = {frozenset(g) for g in s.type_names}
+    assert frozenset(["Hx"]) in groups and frozenset(["O"]) in groups
+    assert s.counts.sum() == 400
+    for t in range(len(s.amp)):
+        assert s.counts[t] == sum(typ.count(n) for n in s.type_names[t])
+    merged = [s.type_names[s.type_index[i]][0] for i in range(400)]
+    a = O.compute_sf(np.array(r), dims, typ, None, rad, np.eye(3), 0.6)
+    b = O.compute_sf(np.array(r), dims, merged, None, rad, np.eye(3), 0.6)
+    assert np.array_equal(a["sf"], b["sf"])
 
 
 def test_border_larger_than_grid_rejected(rad):
