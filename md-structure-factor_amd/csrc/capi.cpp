@@ -10,6 +10,7 @@
 #include <utility>
 
 #include "mdsf_internal.h"
+#include "spill.h"
 
 namespace mdsf {
 
@@ -77,7 +78,7 @@ void free_plan(Plan* p) {
   for (FrameState& f : p->slot) {
     void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
                   f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices,
-                  f.dslices, f.corners};
+                  f.dslices, f.corners, f.spill};
     for (void* q : fs)
       if (q) (void)hipFree(q);
   }
@@ -155,12 +156,6 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     if (desc->nborder > n)
       return fail(MDSF_ERR_ARG, "Nborder exceeds the grid (the reference's fold needs Nborder <= N)");
     if (!(desc->dr[d] > 0.0)) return fail(MDSF_ERR_ARG, "dr must be positive");
-    // tiles of the corner-fold spill pass: those meeting [0, Nb) or [N - Nb, N)
-    int spill = 0;
-    for (int a = 0; a < n; a += kTile)
-      if (a < desc->nborder || std::min(a + kTile, n) > n - desc->nborder) ++spill;
-    if (spill > kMaxSpillTiles)
-      return fail(MDSF_ERR_UNSUPPORTED, "Nborder too large for the corner fold pass");
   }
   if (desc->nborder < 0) return fail(MDSF_ERR_ARG, "negative Nborder");
   if (desc->ntypes < 0 || desc->ntypes > kMaxTypes)
@@ -185,6 +180,11 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     s.ntiles[d] = (desc->n[d] + kTile - 1) / kTile;
   }
   s.nb = desc->nborder;
+  p->spill_tiles = 1;
+  for (int d = 0; d < 3; ++d) {
+    spill_axis(s.n[d], s.nb, &s.sp_k0[d], &s.sp_fh[d], &s.sp_cnt[d]);
+    p->spill_tiles *= s.sp_cnt[d];
+  }
   s.g00 = G[0][0];
   s.g01 = G[0][1];
   s.g11 = G[1][1];
@@ -338,6 +338,11 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     TRY(dalloc(&f.slices, (size_t)p->slice_floats));
     TRY(dalloc(&f.corners, (size_t)2 + 2 * kCornerCap));
     TRY(hipMemset(f.corners, 0, sizeof(int) * ((size_t)2 + 2 * kCornerCap)));
+    if (p->spill_tiles > 0 && p->spill_tiles <= kMaxSpillBufTiles) {
+      unsigned char* sb = nullptr;
+      TRY(dalloc(&sb, (size_t)p->spill_tiles * kTile * kTile * kTile * (p->fft64 ? 8 : 4)));
+      f.spill = sb;
+    }
   }
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));

@@ -567,6 +567,18 @@ __device__ __forceinline__ void tile_coords(const SysDev& sys, Tile& tl) {
   for (int d = 0; d < 3; ++d) tl.t1[d] = min(tl.t0[d] + T, sys.n[d]);
 }
 
+// The staged spill tile of tile tl (or null): added by the tiled deposition
+// as it stores the tile.
+template <typename ACC>
+__device__ __forceinline__ const ACC* spill_stage_tile(const SysDev& sys, const ACC* stage,
+                                                       const Tile& tl) {
+  if (!stage) return nullptr;
+  const int jx = spill_index(sys, 0, tl.t0[0] / T), jy = spill_index(sys, 1, tl.t0[1] / T),
+            jz = spill_index(sys, 2, tl.t0[2] / T);
+  if (jx < 0 || jy < 0 || jz < 0) return nullptr;
+  return stage + ((size_t)(jx * sys.sp_cnt[1] + jy) * sys.sp_cnt[2] + jz) * (T * T * T);
+}
+
 // ---------------------------------------------------------------------------
 // VALU path (FP64 plans): lane = one (x, y) column, 16 z accumulators.
 template <int MODE, typename ACC>
@@ -575,6 +587,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
                                                       const int4* __restrict__ s_ir,
                                                       const double* __restrict__ s_r,
                                                       const void* __restrict__ tables,
+                                                      const ACC* __restrict__ spill,
                                                       ACC* __restrict__ rho) {
   using Stage = typename std::conditional<MODE == MODE_DIAG, StageDiag<ACC>, StageXY<ACC>>::type;
   __shared__ Stage st;
@@ -607,12 +620,14 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
   enumerate_tile(sys, td, boff, s_ir, tl, sh, NB, process);
 
   const int gx = tl.t0[0] + ci, gy = tl.t0[1] + cj;
+  const ACC* sp = spill_stage_tile(sys, spill, tl);
   if (gx < sys.n[0] && gy < sys.n[1]) {
     ACC* dst = rho + ((size_t)gx * sys.n[1] + gy) * sys.n[2] + tl.t0[2];
     const int nz = tl.t1[2] - tl.t0[2];
+    const ACC* sc = sp ? sp + (size_t)(cj * T + ci) * T : nullptr;
 #pragma unroll
     for (int k = 0; k < T; ++k)
-      if (k < nz) dst[k] = acc[k];
+      if (k < nz) dst[k] = sc ? acc[k] + sc[k] : acc[k];
   }
 }
 
@@ -663,6 +678,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const int4* __restrict__ s_ir,
                                                            const double* __restrict__ s_r,
                                                            const float* __restrict__ slices,
+                                                           const float* __restrict__ spill,
                                                            float* __restrict__ rho) {
   // virtual atom slots per batch: 64 for orthogonal cells, 48 for xy-coupled
   // cells (whose staging area then fits 4 blocks per CU)
@@ -1102,14 +1118,18 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
 
   // D[y][z] of block b: lane l holds y = 4 (l / 16) + r, z = l % 16
   const int gz = tl.t0[2] + (lane & 15);
+  const float* sp = spill_stage_tile(sys, spill, tl);
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     const int gx = tl.t0[0] + 4 * wave + b;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int gy = tl.t0[1] + 4 * (lane >> 4) + r;
-      if (gx < tl.t1[0] && gy < tl.t1[1] && gz < tl.t1[2])
-        rho[((size_t)gx * sys.n[1] + gy) * sys.n[2] + gz] = acc[b][r];
+      if (gx < tl.t1[0] && gy < tl.t1[1] && gz < tl.t1[2]) {
+        float v = acc[b][r];
+        if (sp) v += sp[((gy - tl.t0[1]) * T + (gx - tl.t0[0])) * T + (gz - tl.t0[2])];
+        rho[((size_t)gx * sys.n[1] + gy) * sys.n[2] + gz] = v;
+      }
     }
   }
 }
@@ -1371,27 +1391,40 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 // order per cell, bitwise reproducible.
 constexpr int kSpillTrigger = 128;
 
+// With `stage` the block writes its whole tile (zeros included) to
+// stage[block * 16^3 + (ly * 16 + lx) * 16 + lz] (l = cell within the tile)
+// for the tiled deposition to add as it stores; otherwise it adds to rho after
+// the tiled deposition.
 template <typename ACC>
-__global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys, SpillTiles sp,
+__global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
                                                             const TypeDev* __restrict__ td,
                                                             const int* __restrict__ boff,
                                                             const int4* __restrict__ s_ir,
                                                             const double* __restrict__ s_r,
                                                             const int* __restrict__ corners,
-                                                            ACC* __restrict__ rho) {
+                                                            ACC* __restrict__ rho,
+                                                            ACC* __restrict__ stage) {
   __shared__ EnumShared<kMaxTypes> sh;
   __shared__ int4 sq[kThreads];
   __shared__ double sr[kThreads][3];
-  const int ncorner = corners[0], nspill = corners[1 + 2 * kCornerCap];
-  const bool listed = ncorner <= kCornerCap;
-  if (ncorner == 0 || (listed && nspill == 0)) return;  // uniform over the block
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
-  const int iz = b % sp.n[2], iy = (b / sp.n[2]) % sp.n[1], ix = b / (sp.n[2] * sp.n[1]);
+  const int ncorner = corners[0], nspill = corners[1 + 2 * kCornerCap];
+  const bool listed = ncorner <= kCornerCap;
+  if (ncorner == 0 || (listed && nspill == 0)) {  // uniform over the block
+    if (stage) {
+      ACC* o = stage + (size_t)b * (T * T * T) + (size_t)tid * T;
+#pragma unroll
+      for (int k = 0; k < T; ++k) o[k] = (ACC)0;
+    }
+    return;
+  }
+  const int cz = sys.sp_cnt[2], cy = sys.sp_cnt[1];
+  const int iz = b % cz, iy = (b / cz) % cy, ix = b / (cz * cy);
   Tile dst;
-  dst.t0[0] = sp.t[0][ix] * T;
-  dst.t0[1] = sp.t[1][iy] * T;
-  dst.t0[2] = sp.t[2][iz] * T;
+  dst.t0[0] = spill_tile(sys, 0, ix) * T;
+  dst.t0[1] = spill_tile(sys, 1, iy) * T;
+  dst.t0[2] = spill_tile(sys, 2, iz) * T;
 #pragma unroll
   for (int d = 0; d < 3; ++d) dst.t1[d] = min(dst.t0[d] + T, sys.n[d]);
   const int N0 = sys.n[0], N1 = sys.n[1], N2 = sys.n[2], nb = sys.nb;
@@ -1482,7 +1515,11 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys, SpillTil
       __syncthreads();
     }
   }
-  if (xd < N0 && yd < N1) {
+  if (stage) {
+    ACC* o = stage + (size_t)b * (T * T * T) + (size_t)tid * T;
+#pragma unroll
+    for (int k = 0; k < T; ++k) o[k] = (ACC)acc[k];
+  } else if (xd < N0 && yd < N1) {
     ACC* dcol = rho + ((size_t)xd * N1 + yd) * N2 + dst.t0[2];
     const int nz = dst.t1[2] - dst.t0[2];
 #pragma unroll
@@ -1491,37 +1528,20 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys, SpillTil
   }
 }
 
-// Tiles meeting [0, Nb) or [N - Nb, N) on one axis; false if more than fit.
-bool spill_axis_tiles(int N, int nb, int* t, int* n) {
-  const int nt = (N + T - 1) / T;
-  int c = 0;
-  for (int i = 0; i < nt; ++i) {
-    const int a = i * T, e = min(a + T, N);
-    if (a < nb || e > N - nb) {
-      if (c == kMaxSpillTiles) return false;
-      t[c++] = i;
-    }
-  }
-  *n = c;
-  return true;
-}
 
+// Corner-fold spill grid: staged into the slot's tiles (stage) or added to the
+// density after the tiled deposition (stage null, out the density).
 template <typename ACC>
-hipError_t launch_spill(Plan& p, ACC* out, hipStream_t s) {
+hipError_t launch_spill(Plan& p, ACC* out, ACC* stage, hipStream_t s) {
   const SysDev& sys = p.sys;
-  if (sys.nb == 0 || sys.natoms == 0) return hipSuccess;
-  SpillTiles sp{};
-  for (int d = 0; d < 3; ++d)
-    if (!spill_axis_tiles(sys.n[d], sys.nb, sp.t[d], &sp.n[d])) return hipErrorInvalidConfiguration;
-  const long long nblk = (long long)sp.n[0] * sp.n[1] * sp.n[2];
-  if (nblk == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_deposit_spill<ACC>, dim3((unsigned)nblk), dim3(kThreads), 0, s, sys, sp,
-                     p.types, p.bucket_off, p.s_ir, p.s_r, p.corners, out);
+  if (p.spill_tiles == 0 || sys.natoms == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_deposit_spill<ACC>, dim3((unsigned)p.spill_tiles), dim3(kThreads), 0, s,
+                     sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.corners, out, stage);
   return hipGetLastError();
 }
 
 template <typename ACC>
-hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
+hipError_t launch_impl(Plan& p, ACC* out, const ACC* stage, hipStream_t s) {
   const SysDev& sys = p.sys;
   const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
   if (ntiles == 0) return hipSuccess;
@@ -1531,23 +1551,23 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
     const bool few = sys.ntypes <= 16;
     if (sys.mode == MODE_DIAG && few)
       hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, 16>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
     else if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, kMaxTypes>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
     else if (few)
       hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, 16>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
     else
       hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, kMaxTypes>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
   } else {
     if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.tables, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.tables, stage, out);
     else
       hipLaunchKernelGGL((k_deposit<MODE_XY, ACC>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.tables, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.tables, stage, out);
   }
   return hipGetLastError();
 }
@@ -1555,13 +1575,25 @@ hipError_t launch_impl(Plan& p, ACC* out, hipStream_t s) {
 }  // namespace
 
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s) {
+  // staged spill tiles (launch_spill_stage, run with the frame's binning) are
+  // added as the tiles are stored; otherwise the spill pass follows in place
   hipError_t e;
   if (p.fft64) {
-    if ((e = launch_impl<double>(p, static_cast<double*>(out), s)) != hipSuccess) return e;
-    return launch_spill<double>(p, static_cast<double*>(out), s);
+    double* o = static_cast<double*>(out);
+    const double* st = static_cast<const double*>(p.spill);
+    if ((e = launch_impl<double>(p, o, st, s)) != hipSuccess || st) return e;
+    return launch_spill<double>(p, o, nullptr, s);
   }
-  if ((e = launch_impl<float>(p, static_cast<float*>(out), s)) != hipSuccess) return e;
-  return launch_spill<float>(p, static_cast<float*>(out), s);
+  float* o = static_cast<float*>(out);
+  const float* st = static_cast<const float*>(p.spill);
+  if ((e = launch_impl<float>(p, o, st, s)) != hipSuccess || st) return e;
+  return launch_spill<float>(p, o, nullptr, s);
+}
+
+hipError_t launch_spill_stage(Plan& p, hipStream_t s) {
+  if (!p.spill) return hipSuccess;
+  if (p.fft64) return launch_spill<double>(p, nullptr, static_cast<double*>(p.spill), s);
+  return launch_spill<float>(p, nullptr, static_cast<float*>(p.spill), s);
 }
 
 hipError_t launch_adjoint_tiles(Plan& p, const float* grho, hipStream_t s) {

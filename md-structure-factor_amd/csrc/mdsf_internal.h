@@ -16,7 +16,7 @@ constexpr int kTile = 16;       // deposition tile edge along x, y and z
 constexpr int kBatch = 32;      // atoms staged per deposition batch
 constexpr int kMaxStages = 16;  // FFT stages per axis
 constexpr int kTuneReg = 100;   // FFT pencil choice: register-staged kernel
-constexpr int kMaxSpillTiles = 96;  // per axis, tiles of the corner-fold spill pass
+constexpr int kMaxSpillBufTiles = 4096;  // spill tiles staged per frame slot (else added in place)
 constexpr int kCornerCap = 2048;    // corner atoms listed per frame for the spill pass
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
@@ -71,6 +71,9 @@ struct SysDev {
   int nbuckets;
   int ntiles[3];
   int mode;
+  // corner-fold spill tiles per axis: [0, sp_k0) and [sp_fh, ntiles), sp_cnt
+  // of them (all tiles when the two ranges meet); none when nb = 0
+  int sp_k0[3], sp_fh[3], sp_cnt[3];
 };
 
 // One FFT axis: length, radix sequence and twiddles W_n^k (k < n), FP32
@@ -105,6 +108,8 @@ struct FrameState {
   // kCornerCap sorted indices in arrival order, then those that spill in index
   // order, then their count
   int* corners = nullptr;
+  // corner-fold spill of the frame, one 16^3 tile per spill tile (plan precision)
+  void* spill = nullptr;
 };
 
 struct Plan {
@@ -130,6 +135,8 @@ struct Plan {
   float* slices = nullptr;        // per-atom tile slices (FP32 plans, orthogonal-axis cells)
   float* dslices = nullptr;       // their derivatives (adjoint only)
   int* corners = nullptr;         // corner-atom list of the current slot
+  void* spill = nullptr;          // staged spill tiles of the current slot (or null)
+  long long spill_tiles = 0;      // sp_cnt[0] sp_cnt[1] sp_cnt[2]
   long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
   // grids: two density / spectrum buffers so that one frame's FFT can run
@@ -161,6 +168,7 @@ struct Plan {
     slices = f.slices;
     dslices = f.dslices;
     corners = f.corners;
+    spill = f.spill;
     rho = rho_buf[b & 1];
     spec_main = rho;
     spec_nyq = nyq_buf[b & 1];
@@ -217,6 +225,9 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false);
 // only the slice derivatives of the last sorted frame
 hipError_t launch_slice_derivs(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);
+// corner-fold spill of the sorted frame into the slot's staging tiles (when the
+// plan stages them; launch_deposit adds them as it stores the tiles)
+hipError_t launch_spill_stage(Plan& p, hipStream_t s);
 // tiled adjoint: adj_tpart slots <- per (atom, tile) gradient sums from
 // dL/drho (FP32 DIAG plans; slots of tiles an atom does not reach are left
 // untouched)

@@ -429,6 +429,9 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
                        p.s_ir, p.s_r, p.bucket_off + nbk, p.corners);
     hipLaunchKernelGGL(k_corner_order, dim3(1), dim3(kOrderThreads), 0, s, p.sys, p.types,
                        p.s_ir, p.corners);
+    // the corner-fold spill, staged for the deposition of this frame
+    const hipError_t e = launch_spill_stage(p, s);
+    if (e != hipSuccess) return e;
     if (p.slice_floats > 0)
       hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0,
                          s, p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices,

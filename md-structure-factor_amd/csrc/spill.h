@@ -83,10 +83,31 @@ __device__ __forceinline__ double spill_value(const SysDev& sys, const TypeDev& 
 }
 
 // Spill tiles per axis: tiles meeting [0, Nb) or [N - Nb, N), where folded
-// images of off-core windows land (x, y) and the quirk bands lie (z).
-struct SpillTiles {
-  int n[3];
-  int t[3][kMaxSpillTiles];
-};
+// images of off-core windows land (x, y) and the quirk bands lie (z): the
+// low tiles [0, k0), k0 = ceil(Nb / 16), and the high tiles [fh, nt),
+// fh = floor((N - Nb) / 16); all tiles when the two ranges meet.
+__host__ __device__ inline void spill_axis(int N, int nb, int* k0, int* fh, int* cnt) {
+  const int nt = (N + kTile - 1) / kTile;
+  if (nb <= 0) {
+    *k0 = 0;
+    *fh = nt;
+    *cnt = 0;
+    return;
+  }
+  *k0 = (nb + kTile - 1) / kTile < nt ? (nb + kTile - 1) / kTile : nt;
+  *fh = (N - nb) / kTile;
+  *cnt = *fh <= *k0 ? nt : *k0 + nt - *fh;
+  if (*fh <= *k0) *fh = *k0;
+}
+// position of tile i among the spill tiles of axis d, or -1
+__device__ __forceinline__ int spill_index(const SysDev& sys, int d, int i) {
+  const int k0 = sys.sp_k0[d], fh = sys.sp_fh[d];
+  return i < k0 ? i : (i >= fh ? k0 + (i - fh) : -1);
+}
+// the j-th spill tile of axis d
+__device__ __forceinline__ int spill_tile(const SysDev& sys, int d, int j) {
+  const int k0 = sys.sp_k0[d];
+  return j < k0 ? j : sys.sp_fh[d] + (j - k0);
+}
 
 }  // namespace mdsf
