@@ -78,7 +78,7 @@ void free_plan(Plan* p) {
   for (FrameState& f : p->slot) {
     void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
                   f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices,
-                  f.dslices, f.corners, f.spill};
+                  f.dslices, f.corners, f.spill, f.spill_flag};
     for (void* q : fs)
       if (q) (void)hipFree(q);
   }
@@ -342,6 +342,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       unsigned char* sb = nullptr;
       TRY(dalloc(&sb, (size_t)p->spill_tiles * kTile * kTile * kTile * (p->fft64 ? 8 : 4)));
       f.spill = sb;
+      TRY(dalloc(&f.spill_flag, (size_t)p->spill_tiles));
     }
   }
   TRY(dalloc(&p->err, (size_t)1));

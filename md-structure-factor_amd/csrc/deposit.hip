@@ -571,12 +571,13 @@ __device__ __forceinline__ void tile_coords(const SysDev& sys, Tile& tl) {
 // as it stores the tile.
 template <typename ACC>
 __device__ __forceinline__ const ACC* spill_stage_tile(const SysDev& sys, const ACC* stage,
-                                                       const Tile& tl) {
+                                                       const int* flag, const Tile& tl) {
   if (!stage) return nullptr;
   const int jx = spill_index(sys, 0, tl.t0[0] / T), jy = spill_index(sys, 1, tl.t0[1] / T),
             jz = spill_index(sys, 2, tl.t0[2] / T);
   if (jx < 0 || jy < 0 || jz < 0) return nullptr;
-  return stage + ((size_t)(jx * sys.sp_cnt[1] + jy) * sys.sp_cnt[2] + jz) * (T * T * T);
+  const int j = (jx * sys.sp_cnt[1] + jy) * sys.sp_cnt[2] + jz;
+  return flag[j] ? stage + (size_t)j * (T * T * T) : nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -588,6 +589,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
                                                       const double* __restrict__ s_r,
                                                       const void* __restrict__ tables,
                                                       const ACC* __restrict__ spill,
+                                                      const int* __restrict__ spill_flag,
                                                       ACC* __restrict__ rho) {
   using Stage = typename std::conditional<MODE == MODE_DIAG, StageDiag<ACC>, StageXY<ACC>>::type;
   __shared__ Stage st;
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
   enumerate_tile(sys, td, boff, s_ir, tl, sh, NB, process);
 
   const int gx = tl.t0[0] + ci, gy = tl.t0[1] + cj;
-  const ACC* sp = spill_stage_tile(sys, spill, tl);
+  const ACC* sp = spill_stage_tile(sys, spill, spill_flag, tl);
   if (gx < sys.n[0] && gy < sys.n[1]) {
     ACC* dst = rho + ((size_t)gx * sys.n[1] + gy) * sys.n[2] + tl.t0[2];
     const int nz = tl.t1[2] - tl.t0[2];
@@ -679,6 +681,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
                                                            const double* __restrict__ s_r,
                                                            const float* __restrict__ slices,
                                                            const float* __restrict__ spill,
+                                                           const int* __restrict__ spill_flag,
                                                            float* __restrict__ rho) {
   // virtual atom slots per batch: 64 for orthogonal cells, 48 for xy-coupled
   // cells (whose staging area then fits 4 blocks per CU)
@@ -1118,7 +1121,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
 
   // D[y][z] of block b: lane l holds y = 4 (l / 16) + r, z = l % 16
   const int gz = tl.t0[2] + (lane & 15);
-  const float* sp = spill_stage_tile(sys, spill, tl);
+  const float* sp = spill_stage_tile(sys, spill, spill_flag, tl);
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     const int gx = tl.t0[0] + 4 * wave + b;
@@ -1390,6 +1393,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 // after the tiled deposition has written it: one writer per cell, a fixed
 // order per cell, bitwise reproducible.
 constexpr int kSpillTrigger = 128;
+constexpr int kSpillChunk = 64;  // listed atoms staged per step
 
 // With `stage` the block writes its whole tile (zeros included) to
 // stage[block * 16^3 + (ly * 16 + lx) * 16 + lz] (l = cell within the tile)
@@ -1402,23 +1406,19 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
                                                             const int4* __restrict__ s_ir,
                                                             const double* __restrict__ s_r,
                                                             const int* __restrict__ corners,
+                                                            const int* __restrict__ flag,
                                                             ACC* __restrict__ rho,
                                                             ACC* __restrict__ stage) {
   __shared__ EnumShared<kMaxTypes> sh;
-  __shared__ int4 sq[kThreads];
-  __shared__ double sr[kThreads][3];
+  __shared__ int4 sq[kSpillChunk];
+  __shared__ double sr[kSpillChunk][3];
+  __shared__ double sez[kSpillChunk][T];
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   const int ncorner = corners[0], nspill = corners[1 + 2 * kCornerCap];
   const bool listed = ncorner <= kCornerCap;
-  if (ncorner == 0 || (listed && nspill == 0)) {  // uniform over the block
-    if (stage) {
-      ACC* o = stage + (size_t)b * (T * T * T) + (size_t)tid * T;
-#pragma unroll
-      for (int k = 0; k < T; ++k) o[k] = (ACC)0;
-    }
-    return;
-  }
+  // uniform over the block; a staged tile nothing spills into is not read
+  if (ncorner == 0 || (listed && nspill == 0) || (stage && !flag[b])) return;
   const int cz = sys.sp_cnt[2], cy = sys.sp_cnt[1];
   const int iz = b % cz, iy = (b / cz) % cy, ix = b / (cz * cy);
   Tile dst;
@@ -1444,37 +1444,48 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
              window_reaches(q.x, ty.half[0], dst.t0[0], dst.t1[0], N0) &&
              window_reaches(q.y, ty.half[1], dst.t0[1], dst.t1[1], N1);
     };
-    // this lane's cells of the tile from one reaching atom
-    auto add = [&](const int4& q, const double* r) {
+    // z factor exp(-g22 bz^2 / 2 s^2) of tile z cell k (0 off the band or the window)
+    auto zfactor = [&](const int4& q, const TypeDev& ty, double rz, int k) -> double {
+      const int z = dst.t0[2] + k;
+      const int uz = z + shift;  // < 0 on band 0, >= N on band 1
+      if (z >= dst.t1[2] || z < zlo || z >= zhi || uz < q.z - ty.half[2] || uz >= q.z + ty.half[2])
+        return 0.0;
+      const double bz = rz - (double)uz * sys.dr[2];
+      return exp(-(sys.g22 * bz * bz) / ty.two_sig2);
+    };
+    // this lane's cells of the tile from one reaching atom; ez: its z factors
+    // (null: evaluated here)
+    auto add = [&](const int4& q, const double* r, const double* ez) {
       if (xd >= N0 || yd >= N1) return;
       const TypeDev& ty = td[q.w & 0xffff];
-      const int Ax = ty.half[0], Ay = ty.half[1], Az = ty.half[2];
+      const int Ax = ty.half[0], Ay = ty.half[1];
       // y image on this band's side: H (u = y + N) for band 0, L (u = y - N) for band 1
       const int uy = band == 0 ? yd + N1 : yd - N1;
       if (uy < q.y - Ay || uy >= q.y + Ay) return;
+      const double by = r[1] - (double)uy * sys.dr[1];
 #pragma unroll
       for (int sx = 0; sx < 2; ++sx) {
         const int ux = sx == 0 ? xd - N0 : xd + N0;  // the off-core x images
         if (ux < q.x - Ax || ux >= q.x + Ax) continue;
+        const double bx = r[0] - (double)ux * sys.dr[0];
+        // amp exp(-(b^T G b) / 2 s^2) split into its xy and z factors (dens.py:297-308)
+        const double exy =
+            ty.amp * exp(-(sys.g00 * bx * bx + 2.0 * sys.g01 * bx * by + sys.g11 * by * by) /
+                         ty.two_sig2);
 #pragma unroll
-        for (int k = 0; k < T; ++k) {
-          const int z = dst.t0[2] + k;
-          if (z >= dst.t1[2] || z < zlo || z >= zhi) continue;
-          const int uz = z + shift;  // < 0 on band 0, >= N on band 1
-          if (uz < q.z - Az || uz >= q.z + Az) continue;
-          acc[k] += spill_value(sys, ty, r, ux, uy, uz, nullptr);
-        }
+        for (int k = 0; k < T; ++k) acc[k] += exy * (ez ? ez[k] : zfactor(q, ty, r[2], k));
       }
     };
     if (listed) {
-      // 256 listed atoms at a time: parallel loads, the ones reaching this
-      // tile compacted in list order into LDS, then every lane walks them
-      for (int base = 0; base < nspill; base += kThreads) {
+      // kSpillChunk listed atoms at a time: parallel loads, the ones reaching
+      // this tile compacted in list order into LDS with their z factors, then
+      // every lane walks them
+      for (int base = 0; base < nspill; base += kSpillChunk) {
         const int a = base + tid;
         int4 q = make_int4(0, 0, 0, 0);
         int c = 0;
         bool hit = false;
-        if (a < nspill) {
+        if (tid < kSpillChunk && a < nspill) {
           c = corners[1 + kCornerCap + a];
           q = s_ir[c];
           hit = reaches(q, td[q.w & 0xffff]);
@@ -1488,9 +1499,14 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
           sr[pos][2] = s_r[3 * (size_t)c + 2];
         }
         __syncthreads();
+        for (int i = tid; i < m * T; i += kThreads) {
+          const int j = i / T, k = i - (i / T) * T;
+          sez[j][k] = zfactor(sq[j], td[sq[j].w & 0xffff], sr[j][2], k);
+        }
+        __syncthreads();
         for (int j = 0; j < m; ++j) {
           const double r[3] = {sr[j][0], sr[j][1], sr[j][2]};
-          add(sq[j], r);
+          add(sq[j], r, sez[j]);
         }
         __syncthreads();
       }
@@ -1507,7 +1523,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
           if (!reaches(q, td[q.w & 0xffff])) continue;
           const double r[3] = {s_r[3 * (size_t)c + 0], s_r[3 * (size_t)c + 1],
                                s_r[3 * (size_t)c + 2]};
-          add(q, r);
+          add(q, r, nullptr);
         }
         return count;
       };
@@ -1536,7 +1552,8 @@ hipError_t launch_spill(Plan& p, ACC* out, ACC* stage, hipStream_t s) {
   const SysDev& sys = p.sys;
   if (p.spill_tiles == 0 || sys.natoms == 0) return hipSuccess;
   hipLaunchKernelGGL(k_deposit_spill<ACC>, dim3((unsigned)p.spill_tiles), dim3(kThreads), 0, s,
-                     sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.corners, out, stage);
+                     sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.corners, p.spill_flag, out,
+                     stage);
   return hipGetLastError();
 }
 
@@ -1551,23 +1568,23 @@ hipError_t launch_impl(Plan& p, ACC* out, const ACC* stage, hipStream_t s) {
     const bool few = sys.ntypes <= 16;
     if (sys.mode == MODE_DIAG && few)
       hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, 16>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
     else if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, kMaxTypes>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
     else if (few)
       hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, 16>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
     else
       hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, kMaxTypes>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
   } else {
     if (sys.mode == MODE_DIAG)
       hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.tables, stage, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.tables, stage, p.spill_flag, out);
     else
       hipLaunchKernelGGL((k_deposit<MODE_XY, ACC>), grid, block, 0, s, sys, p.types,
-                         p.bucket_off, p.s_ir, p.s_r, p.tables, stage, out);
+                         p.bucket_off, p.s_ir, p.s_r, p.tables, stage, p.spill_flag, out);
   }
   return hipGetLastError();
 }

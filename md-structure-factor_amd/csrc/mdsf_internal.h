@@ -108,8 +108,10 @@ struct FrameState {
   // kCornerCap sorted indices in arrival order, then those that spill in index
   // order, then their count
   int* corners = nullptr;
-  // corner-fold spill of the frame, one 16^3 tile per spill tile (plan precision)
+  // corner-fold spill of the frame, one 16^3 tile per spill tile (plan precision),
+  // and per spill tile whether any atom spills into it
   void* spill = nullptr;
+  int* spill_flag = nullptr;
 };
 
 struct Plan {
@@ -136,6 +138,7 @@ struct Plan {
   float* dslices = nullptr;       // their derivatives (adjoint only)
   int* corners = nullptr;         // corner-atom list of the current slot
   void* spill = nullptr;          // staged spill tiles of the current slot (or null)
+  int* spill_flag = nullptr;      // their occupancy flags
   long long spill_tiles = 0;      // sp_cnt[0] sp_cnt[1] sp_cnt[2]
   long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
@@ -169,6 +172,7 @@ struct Plan {
     dslices = f.dslices;
     corners = f.corners;
     spill = f.spill;
+    spill_flag = f.spill_flag;
     rho = rho_buf[b & 1];
     spec_main = rho;
     spec_nyq = nyq_buf[b & 1];

@@ -221,11 +221,17 @@ constexpr int kOrderThreads = 256;
 __global__ __launch_bounds__(kOrderThreads) void k_corner_order(SysDev sys,
                                                                 const TypeDev* __restrict__ td,
                                                                 const int4* __restrict__ s_ir,
-                                                                int* __restrict__ corners) {
+                                                                int* __restrict__ corners,
+                                                                int* __restrict__ flag,
+                                                                int nflag) {
   __shared__ int lst[kCornerCap];
   __shared__ int nrel;
   const int n = min(corners[0], kCornerCap);
   if (threadIdx.x == 0) nrel = 0;
+  // spill tile flags (staged spill): cleared, then set per spilling atom below;
+  // past kCornerCap corner atoms every tile is flagged
+  if (flag)
+    for (int i = threadIdx.x; i < nflag; i += kOrderThreads) flag[i] = corners[0] > kCornerCap;
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += kOrderThreads) {
     const int c = corners[1 + i];
@@ -241,6 +247,11 @@ __global__ __launch_bounds__(kOrderThreads) void k_corner_order(SysDev sys,
     int r = 0;
     for (int j = 0; j < n; ++j) r += (lst[j] >= 0 && lst[j] < v) ? 1 : 0;
     corners[1 + kCornerCap + r] = v;
+    if (flag) {
+      const int4 q = s_ir[v];
+      const TypeDev& ty = td[q.w & 0xffff];
+      spill_mark(sys, q, ty, spill_bands(q, ty, sys), flag);
+    }
   }
   if (threadIdx.x == 0) corners[1 + 2 * kCornerCap] = nrel;
 }
@@ -428,7 +439,8 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
     hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.s_idx, p.ir4, p.rpos,
                        p.s_ir, p.s_r, p.bucket_off + nbk, p.corners);
     hipLaunchKernelGGL(k_corner_order, dim3(1), dim3(kOrderThreads), 0, s, p.sys, p.types,
-                       p.s_ir, p.corners);
+                       p.s_ir, p.corners, p.spill ? p.spill_flag : nullptr,
+                       (int)(p.spill ? p.spill_tiles : 0));
     // the corner-fold spill, staged for the deposition of this frame
     const hipError_t e = launch_spill_stage(p, s);
     if (e != hipSuccess) return e;

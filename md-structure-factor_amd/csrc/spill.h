@@ -110,4 +110,36 @@ __device__ __forceinline__ int spill_tile(const SysDev& sys, int d, int j) {
   return j < k0 ? j : sys.sp_fh[d] + (j - k0);
 }
 
+// Marks the spill tiles an atom spills into on the given bands (its
+// destination cells: off-core x images, y images on the band's side, z
+// destinations in tiles its z window does not reach).
+__device__ inline void spill_mark(const SysDev& sys, const int4& q, const TypeDev& ty, int bands,
+                                  int* __restrict__ flag) {
+  const int N0 = sys.n[0], N1 = sys.n[1], N2 = sys.n[2], nb = sys.nb;
+  const int Ax = ty.half[0], Ay = ty.half[1], Az = ty.half[2];
+  const SpillRange xl = spill_side_l(q.x, Ax), xh = spill_side_h(q.x, Ax, N0);
+  for (int band = 0; band < 2; ++band) {
+    if (!((bands >> band) & 1)) continue;
+    const SpillRange yr = band == 0 ? spill_side_h(q.y, Ay, N1) : spill_side_l(q.y, Ay);
+    const SpillRange zr = band == 0 ? spill_side_l(q.z, Az) : spill_side_h(q.z, Az, N2);
+    // destination cell ranges: x and y wrapped into [0, N), z shifted by -/+ Nb
+    const int ylo = band == 0 ? yr.lo - N1 : yr.lo + N1, yhi = band == 0 ? yr.hi - N1 : yr.hi + N1;
+    const int zlo = band == 0 ? zr.lo + nb : zr.lo - nb, zhi = band == 0 ? zr.hi + nb : zr.hi - nb;
+    for (int part = 0; part < 2; ++part) {
+      const SpillRange xr = part == 0 ? xl : xh;
+      if (xr.len() == 0) continue;
+      const int xlo = part == 0 ? xr.lo + N0 : xr.lo - N0, xhi = part == 0 ? xr.hi + N0 : xr.hi - N0;
+      for (int tx = xlo / kTile; tx * kTile < xhi; ++tx)
+        for (int ty_ = ylo / kTile; ty_ * kTile < yhi; ++ty_)
+          for (int tz = zlo / kTile; tz * kTile < zhi; ++tz) {
+            if (!spill_tile_unreached(q.z, Az, tz * kTile, min(tz * kTile + kTile, N2), N2)) continue;
+            const int jx = spill_index(sys, 0, tx), jy = spill_index(sys, 1, ty_),
+                      jz = spill_index(sys, 2, tz);
+            if (jx >= 0 && jy >= 0 && jz >= 0)
+              flag[(jx * sys.sp_cnt[1] + jy) * sys.sp_cnt[2] + jz] = 1;
+          }
+    }
+  }
+}
+
 }  // namespace mdsf
