@@ -1438,14 +1438,15 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
     if (dst.t1[2] <= zlo || dst.t0[2] >= zhi) continue;  // uniform over the block
     const int shift = band == 0 ? -nb : nb;               // u_z = z + shift
     // does atom (q) spill into this destination tile on this band?
-    auto reaches = [&](const int4& q, const TypeDev& ty) -> bool {
+    auto reaches = [&](const int4& q, const TypeDev& ty) __attribute__((always_inline)) -> bool {
       return ((spill_bands(q, ty, sys) >> band) & 1) &&
              spill_tile_unreached(q.z, ty.half[2], dst.t0[2], dst.t1[2], N2) &&
              window_reaches(q.x, ty.half[0], dst.t0[0], dst.t1[0], N0) &&
              window_reaches(q.y, ty.half[1], dst.t0[1], dst.t1[1], N1);
     };
     // z factor exp(-g22 bz^2 / 2 s^2) of tile z cell k (0 off the band or the window)
-    auto zfactor = [&](const int4& q, const TypeDev& ty, double rz, int k) -> double {
+    auto zfactor = [&](const int4& q, const TypeDev& ty, double rz, int k)
+                       __attribute__((always_inline)) -> double {
       const int z = dst.t0[2] + k;
       const int uz = z + shift;  // < 0 on band 0, >= N on band 1
       if (z >= dst.t1[2] || z < zlo || z >= zhi || uz < q.z - ty.half[2] || uz >= q.z + ty.half[2])
@@ -1455,7 +1456,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
     };
     // this lane's cells of the tile from one reaching atom; ez: its z factors
     // (null: evaluated here)
-    auto add = [&](const int4& q, const double* r, const double* ez) {
+    auto add = [&](const int4& q, const double* r, const double* ez) __attribute__((always_inline)) {
       if (xd >= N0 || yd >= N1) return;
       const TypeDev& ty = td[q.w & 0xffff];
       const int Ax = ty.half[0], Ay = ty.half[1];
@@ -1476,54 +1477,56 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
         for (int k = 0; k < T; ++k) acc[k] += exy * (ez ? ez[k] : zfactor(q, ty, r[2], k));
       }
     };
+    // One step over up to kSpillChunk candidates (lanes below kSpillChunk hold
+    // one each, c < 0 for none): the ones reaching this tile are compacted in
+    // candidate order into LDS with their z factors, then every lane walks them.
+    // Called by the whole block.
+    auto step = [&](int c) __attribute__((always_inline)) {
+      int4 q = make_int4(0, 0, 0, 0);
+      bool hit = false;
+      if (c >= 0) {
+        q = s_ir[c];
+        hit = reaches(q, td[q.w & 0xffff]);
+      }
+      int pos;
+      const int m = block_scan(sh, hit ? 1 : 0, &pos);
+      if (m == 0) return;  // uniform
+      if (hit) {
+        sq[pos] = q;
+        sr[pos][0] = s_r[3 * (size_t)c + 0];
+        sr[pos][1] = s_r[3 * (size_t)c + 1];
+        sr[pos][2] = s_r[3 * (size_t)c + 2];
+      }
+      __syncthreads();
+      for (int i = tid; i < m * T; i += kThreads) {
+        const int j = i / T, k = i - (i / T) * T;
+        sez[j][k] = zfactor(sq[j], td[sq[j].w & 0xffff], sr[j][2], k);
+      }
+      __syncthreads();
+      for (int j = 0; j < m; ++j) {
+        const double r[3] = {sr[j][0], sr[j][1], sr[j][2]};
+        add(sq[j], r, sez[j]);
+      }
+      __syncthreads();
+    };
     if (listed) {
-      // kSpillChunk listed atoms at a time: parallel loads, the ones reaching
-      // this tile compacted in list order into LDS with their z factors, then
-      // every lane walks them
       for (int base = 0; base < nspill; base += kSpillChunk) {
         const int a = base + tid;
-        int4 q = make_int4(0, 0, 0, 0);
-        int c = 0;
-        bool hit = false;
-        if (tid < kSpillChunk && a < nspill) {
-          c = corners[1 + kCornerCap + a];
-          q = s_ir[c];
-          hit = reaches(q, td[q.w & 0xffff]);
-        }
-        int pos;
-        const int m = block_scan(sh, hit ? 1 : 0, &pos);
-        if (hit) {
-          sq[pos] = q;
-          sr[pos][0] = s_r[3 * (size_t)c + 0];
-          sr[pos][1] = s_r[3 * (size_t)c + 1];
-          sr[pos][2] = s_r[3 * (size_t)c + 2];
-        }
-        __syncthreads();
-        for (int i = tid; i < m * T; i += kThreads) {
-          const int j = i / T, k = i - (i / T) * T;
-          sez[j][k] = zfactor(sq[j], td[sq[j].w & 0xffff], sr[j][2], k);
-        }
-        __syncthreads();
-        for (int j = 0; j < m; ++j) {
-          const double r[3] = {sr[j][0], sr[j][1], sr[j][2]};
-          add(sq[j], r, sez[j]);
-        }
-        __syncthreads();
+        step(tid < kSpillChunk && a < nspill ? corners[1 + kCornerCap + a] : -1);
       }
     } else {
+      // the atoms whose windows reach the source tile, in enumeration order
       Tile src = dst;
       src.t0[2] += shift;
       src.t1[2] += shift;
-      auto process = [&](int count, int head) -> int {
-        for (int i = 0; i < count; ++i) {
-          const int e = sh.list[(head + i) & kListMask];
-          if (e >= 0) continue;  // corner flag (bit 31) clear
-          const int c = e & 0x7fffffff;
-          const int4 q = s_ir[c];
-          if (!reaches(q, td[q.w & 0xffff])) continue;
-          const double r[3] = {s_r[3 * (size_t)c + 0], s_r[3 * (size_t)c + 1],
-                               s_r[3 * (size_t)c + 2]};
-          add(q, r, nullptr);
+      auto process = [&](int count, int head) __attribute__((always_inline)) -> int {
+        for (int base = 0; base < count; base += kSpillChunk) {
+          int c = -1;
+          if (tid < kSpillChunk && base + tid < count) {
+            const int e = sh.list[(head + base + tid) & kListMask];
+            if (e < 0) c = e & 0x7fffffff;  // corner flag in bit 31
+          }
+          step(c);
         }
         return count;
       };

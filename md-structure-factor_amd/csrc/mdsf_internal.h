@@ -17,7 +17,7 @@ constexpr int kBatch = 32;      // atoms staged per deposition batch
 constexpr int kMaxStages = 16;  // FFT stages per axis
 constexpr int kTuneReg = 100;   // FFT pencil choice: register-staged kernel
 constexpr int kMaxSpillBufTiles = 4096;  // spill tiles staged per frame slot (else added in place)
-constexpr int kCornerCap = 2048;    // corner atoms listed per frame for the spill pass
+constexpr int kCornerCap = 512;     // corner atoms listed per frame for the spill pass
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
 enum Mode { MODE_DIAG = 0, MODE_XY = 1 };

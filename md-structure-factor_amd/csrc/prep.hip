@@ -229,9 +229,12 @@ __global__ __launch_bounds__(kOrderThreads) void k_corner_order(SysDev sys,
   const int n = min(corners[0], kCornerCap);
   if (threadIdx.x == 0) nrel = 0;
   // spill tile flags (staged spill): cleared, then set per spilling atom below;
-  // past kCornerCap corner atoms every tile is flagged
+  // past kCornerCap corner atoms every tile is flagged and the spill pass
+  // enumerates its candidates per tile instead of walking the list
+  const bool over = corners[0] > kCornerCap;
   if (flag)
-    for (int i = threadIdx.x; i < nflag; i += kOrderThreads) flag[i] = corners[0] > kCornerCap;
+    for (int i = threadIdx.x; i < nflag; i += kOrderThreads) flag[i] = over ? 1 : 0;
+  if (over) return;  // uniform
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += kOrderThreads) {
     const int c = corners[1 + i];

@@ -33,7 +33,7 @@ def corner_system(seed, box=(7.0, 8.0, 9.0), frames=2, n_each=2, names=("C", "O"
     return coords, dims, typ
 
 
-def spill_system(seed, frames=1, box=(40.0, 44.0, 42.0), names=("H", "C", "O", "Na")):
+def spill_system(seed, frames=1, box=(40.0, 44.0, 42.0), names=("H", "C", "O", "Na"), per_corner=3):
     """Atoms within about an Angstrom of all 8 cell corners in a ~40 A box, with
     Na setting Nborder = 17 at Sres = 1: at N ~ 40, the reference's corner z
     slot (u_z + Nb for y high / z low, u_z - Nb for y low / z high,
@@ -45,7 +45,7 @@ def spill_system(seed, frames=1, box=(40.0, 44.0, 42.0), names=("H", "C", "O", "
     for cx in (0, 1):
         for cy in (0, 1):
             for cz in (0, 1):
-                for _ in range(3):
+                for _ in range(per_corner):
                     pts.append(np.array([cx, cy, cz]) * box + rng.normal(0, 0.8, 3))
     pts += list(rng.uniform(0, 1, (12, 3)) * box)
     base = np.mod(np.array(pts), box)

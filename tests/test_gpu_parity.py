@@ -198,6 +198,24 @@ def test_corner_fold_spill_beyond_window_tiles(theta, fp64, rad, gpu, tmp_path):
     assert_sq_close(got, want, rel=1e-5 if fp64 else 1e-4)
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("theta", [None, 120.0])
+def test_corner_fold_spill_many_corner_atoms(theta, fp64, rad, gpu):
+    """640 atoms at the 8 corners: more corner atoms than the spill pass lists
+    per frame (512), so its candidates come from the per-tile enumeration."""
+    from test_adjoint_ref import spill_images, spill_system
+    coords, dims, typ = spill_system(13, per_corner=80)
+    assert spill_images(coords[0], typ, rad, 1.0, dims) > 5000  # 635 corner atoms
+    ucell = np.eye(3) if theta is None else np.array(
+        [[1, 0, 0], [np.cos(math.radians(theta)), np.sin(math.radians(theta)), 0], [0, 0, 1]])
+    args = (coords, dims, typ, rad, 1.0, False, ucell)
+    s, plan, c, types, scale = _gpu(*args, fp64=fp64)
+    _, o, ref = _oracle_frames(*args)
+    d = plan.deposit_fold(c[0], types, scale[0], [float(x) for x in s.L], s.arith, s.wrap)
+    plan.check_errors()
+    assert_density_close(d.cpu().numpy().astype(np.float64), ref[0][0], fp64)
+
+
 def test_occupancy_histogram_bit_exact(rad, gpu):
     import torch
     from mdsf import synth
