@@ -48,7 +48,10 @@ typedef struct mdsf_plan mdsf_plan;
 typedef void* mdsf_stream; /* a hipStream_t */
 
 /* One atom type: replaces the per-type values `compute_sf` reads from the
- * radii dictionary and `get_borders` (dens.py:38-43, 287, 305-308). */
+ * radii dictionary and `get_borders` (dens.py:38-43, 287, 305-308).  A plan
+ * takes at most 64.  An atom enters the density only through these values,
+ * so names that share them (force-field names C1, C2, H17 ... of one element)
+ * can and should be passed as one type. */
 typedef struct mdsf_type_desc {
   double amp;        /* Nel / sigma**3                               dens.py:308 */
   double two_sig2;   /* 2 * sigma**2                                 dens.py:308 */
