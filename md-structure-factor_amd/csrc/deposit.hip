@@ -1395,6 +1395,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 constexpr int kSpillTrigger = 128;
 constexpr int kSpillChunk = 64;  // listed atoms staged per step
 
+// exp(-q) of an FP64 exponent in the plan precision (FP32 exp for FP32 plans,
+// as the tile slices take it)
+template <typename ACC>
+__device__ __forceinline__ ACC spill_exp(double q);
+template <>
+__device__ __forceinline__ float spill_exp<float>(double q) { return expf(-(float)q); }
+template <>
+__device__ __forceinline__ double spill_exp<double>(double q) { return exp(-q); }
+
 // With `stage` the block writes its whole tile (zeros included) to
 // stage[block * 16^3 + (ly * 16 + lx) * 16 + lz] (l = cell within the tile)
 // for the tiled deposition to add as it stores; otherwise it adds to rho after
@@ -1412,7 +1421,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
   __shared__ EnumShared<kMaxTypes> sh;
   __shared__ int4 sq[kSpillChunk];
   __shared__ double sr[kSpillChunk][3];
-  __shared__ double sez[kSpillChunk][T];
+  __shared__ ACC sez[kSpillChunk][T];
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   const int ncorner = corners[0], nspill = corners[1 + 2 * kCornerCap];
@@ -1429,9 +1438,9 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
   for (int d = 0; d < 3; ++d) dst.t1[d] = min(dst.t0[d] + T, sys.n[d]);
   const int N0 = sys.n[0], N1 = sys.n[1], N2 = sys.n[2], nb = sys.nb;
   const int xd = dst.t0[0] + (tid & 15), yd = dst.t0[1] + (tid >> 4);
-  double acc[T];
+  ACC acc[T];
 #pragma unroll
-  for (int k = 0; k < T; ++k) acc[k] = 0.0;
+  for (int k = 0; k < T; ++k) acc[k] = (ACC)0;
 
   for (int band = 0; band < 2; ++band) {
     const int zlo = band == 0 ? 0 : N2 - nb, zhi = band == 0 ? nb : N2;
@@ -1446,17 +1455,17 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
     };
     // z factor exp(-g22 bz^2 / 2 s^2) of tile z cell k (0 off the band or the window)
     auto zfactor = [&](const int4& q, const TypeDev& ty, double rz, int k)
-                       __attribute__((always_inline)) -> double {
+                       __attribute__((always_inline)) -> ACC {
       const int z = dst.t0[2] + k;
       const int uz = z + shift;  // < 0 on band 0, >= N on band 1
       if (z >= dst.t1[2] || z < zlo || z >= zhi || uz < q.z - ty.half[2] || uz >= q.z + ty.half[2])
-        return 0.0;
+        return (ACC)0;
       const double bz = rz - (double)uz * sys.dr[2];
-      return exp(-(sys.g22 * bz * bz) / ty.two_sig2);
+      return spill_exp<ACC>((sys.g22 * bz * bz) / ty.two_sig2);
     };
     // this lane's cells of the tile from one reaching atom; ez: its z factors
     // (null: evaluated here)
-    auto add = [&](const int4& q, const double* r, const double* ez) __attribute__((always_inline)) {
+    auto add = [&](const int4& q, const double* r, const ACC* ez) __attribute__((always_inline)) {
       if (xd >= N0 || yd >= N1) return;
       const TypeDev& ty = td[q.w & 0xffff];
       const int Ax = ty.half[0], Ay = ty.half[1];
@@ -1470,9 +1479,9 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
         if (ux < q.x - Ax || ux >= q.x + Ax) continue;
         const double bx = r[0] - (double)ux * sys.dr[0];
         // amp exp(-(b^T G b) / 2 s^2) split into its xy and z factors (dens.py:297-308)
-        const double exy =
-            ty.amp * exp(-(sys.g00 * bx * bx + 2.0 * sys.g01 * bx * by + sys.g11 * by * by) /
-                         ty.two_sig2);
+        const ACC exy = (ACC)ty.amp *
+                        spill_exp<ACC>((sys.g00 * bx * bx + 2.0 * sys.g01 * bx * by + sys.g11 * by * by) /
+                                       ty.two_sig2);
 #pragma unroll
         for (int k = 0; k < T; ++k) acc[k] += exy * (ez ? ez[k] : zfactor(q, ty, r[2], k));
       }
@@ -1543,7 +1552,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
     const int nz = dst.t1[2] - dst.t0[2];
 #pragma unroll
     for (int k = 0; k < T; ++k)
-      if (k < nz && acc[k] != 0.0) dcol[k] = (ACC)((double)dcol[k] + acc[k]);
+      if (k < nz && acc[k] != (ACC)0) dcol[k] += acc[k];
   }
 }
 
