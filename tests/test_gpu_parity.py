@@ -171,10 +171,11 @@ def test_corner_fold_spill_beyond_window_tiles(theta, fp64, rad, gpu, tmp_path):
     """Corner-slot contributions (dens.py:107) that land in z tiles the atom's
     own window does not reach: N = (40, 44, 42), Nborder = 17 (Na), atoms at
     all 8 corners (316 such images, counted on the oracle's set-up).  Folded
-    density against the oracle, then S(q) through the drop-in."""
+    density against the oracle, then S(q) of 3 frames through the drop-in's
+    pipelined frame loop (the staged spill alternates between frame slots)."""
     from test_adjoint_ref import spill_images, spill_system
     import dens
-    coords, dims, typ = spill_system(11)
+    coords, dims, typ = spill_system(11, frames=3)
     assert spill_images(coords[0], typ, rad, 1.0, dims) > 100
     ucell = np.eye(3) if theta is None else np.array(
         [[1, 0, 0], [np.cos(math.radians(theta)), np.sin(math.radians(theta)), 0], [0, 0, 1]])
@@ -200,11 +201,13 @@ def test_corner_fold_spill_beyond_window_tiles(theta, fp64, rad, gpu, tmp_path):
 
 @pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("theta", [None, 120.0])
-def test_corner_fold_spill_many_corner_atoms(theta, fp64, rad, gpu):
+def test_corner_fold_spill_many_corner_atoms(theta, fp64, rad, gpu, tmp_path):
     """640 atoms at the 8 corners: more corner atoms than the spill pass lists
-    per frame (512), so its candidates come from the per-tile enumeration."""
+    per frame (512), so its candidates come from the per-tile enumeration.
+    Folded density of frame 0, then S(q) of 2 frames through the frame loop."""
     from test_adjoint_ref import spill_images, spill_system
-    coords, dims, typ = spill_system(13, per_corner=80)
+    import dens
+    coords, dims, typ = spill_system(13, frames=2, per_corner=80)
     assert spill_images(coords[0], typ, rad, 1.0, dims) > 5000  # 635 corner atoms
     ucell = np.eye(3) if theta is None else np.array(
         [[1, 0, 0], [np.cos(math.radians(theta)), np.sin(math.radians(theta)), 0], [0, 0, 1]])
@@ -214,6 +217,17 @@ def test_corner_fold_spill_many_corner_atoms(theta, fp64, rad, gpu):
     d = plan.deposit_fold(c[0], types, scale[0], [float(x) for x in s.L], s.arith, s.wrap)
     plan.check_errors()
     assert_density_close(d.cpu().numpy().astype(np.float64), ref[0][0], fp64)
+    want = O.compute_sf(np.array(coords), np.array(dims), typ, None, rad, ucell, 1.0,
+                        better=False)["sf"]
+    saved = (dens.PRINT_DETAILS, dens.FFT_PRECISION, dens.USE_BETTER_RESOLUTION)
+    dens.PRINT_DETAILS, dens.USE_BETTER_RESOLUTION = False, False
+    dens.FFT_PRECISION = "fp64" if fp64 else "fp32"
+    try:
+        dens.compute_sf(np.array(coords), dims, typ, str(tmp_path / "s"), rad, ucell, 1.0)
+    finally:
+        dens.PRINT_DETAILS, dens.FFT_PRECISION, dens.USE_BETTER_RESOLUTION = saved
+    got = np.load(str(tmp_path / "s.npz"))["sf"]
+    assert_sq_close(got, want, rel=1e-5 if fp64 else 1e-4)
 
 
 def test_occupancy_histogram_bit_exact(rad, gpu):
