@@ -128,6 +128,30 @@ def test_adjoint_frames_match_autograd(theta, fp64, rad, gpu):
     assert rel_err(got, xr.grad.numpy()) <= TOL[fp64]
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("theta", [None, 120.0])
+def test_adjoint_frames_corner_spill(theta, fp64, rad, gpu):
+    """The pipelined forward and backward frame loops (frame slots alternate)
+    on 3 frames with corner-slot spill images, against autograd of the torch
+    reference."""
+    from mdsf.autograd import SFModel
+    coords, dims, typ = spill_system(11, frames=3)
+    ucell = monoclinic(theta)
+    model = SFModel(coords, dims, typ, rad, 1.0, ucell=ucell, fp64=fp64)
+    rng = np.random.default_rng(6)
+    x = torch.tensor(coords, device=model.plan.device, requires_grad=True)
+    S = model(x)
+    w = torch.tensor(rng.normal(size=tuple(S.shape)), device=S.device) / S.detach().max()
+    torch.sum(w * S).backward()
+    got = x.grad.cpu().numpy()
+    xr = torch.tensor(coords, requires_grad=True)
+    Sr = T.structure_factor(xr, dims, typ, rad, 1.0, ucell)
+    np.testing.assert_allclose(S.detach().cpu().numpy(), Sr.detach().numpy(),
+                               rtol=1e-4 if not fp64 else 1e-10, atol=1e-6 * float(Sr.max()))
+    torch.sum(w.cpu() * Sr).backward()
+    assert rel_err(got, xr.grad.numpy()) <= TOL[fp64]
+
+
 def test_adjoint_fd_adjoint_workload(rad, gpu):
     """20k C/O/H atoms, 60 A box, 128^3, 16 frames; L = sum (S - S_target)^2.
     FP32 and FP64 adjoints against central FD (h = 1e-3 A) of the FP64 forward
