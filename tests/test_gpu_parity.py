@@ -417,13 +417,13 @@ def test_gentraj_features_end_to_end(rad, gpu, tmp_path):
     assert_sq_close(got["sf"], want["sf"])
 
 
-@pytest.mark.parametrize("precision", ["auto", "fp32"])
-def test_reference_hii_frame_end_to_end(precision, rad, gpu, tmp_path):
+@pytest.mark.parametrize("precision,water", [("auto", "drop"), ("fp32", "drop"), ("auto", "map")])
+def test_reference_hii_frame_end_to_end(precision, water, rad, gpu, tmp_path):
     """The reference's own test fixture test/test_system.gro (one frame of an
     inverted-hexagonal phase: 55,680 atoms in a 120-degree cell), read through
     the drop-in load_traj as main_gromacs.py:197 does.  The 480 water atoms
     (HW1 / HW2 / OW, which radii.txt lacks, so the reference raises KeyError on
-    them) are left out.  The rest is moved to the monoclinic frame as
+    them) are left out ("drop") or renamed O / H ("map", all 55,680 atoms).  The rest is moved to the monoclinic frame as
     main_gromacs.py:204-207 does, then compute_sf vs the oracle at
     Sres = 1 Angstrom (88 x 88 x 84, Nborder 17).  The frame has 138 atom
     names (4 parameter sets) and atoms exactly on the cell faces (home cell N).
@@ -444,10 +444,16 @@ def test_reference_hii_frame_end_to_end(precision, rad, gpu, tmp_path):
         shutil.copyfileobj(fi, fo)
     load_traj.process_gro_mdtraj(gro, gro, str(tmp_path / "traj"))
     traj = np.load(str(tmp_path / "traj.npz"))
-    keep = np.array([str(n) in rad for n in traj["typ"]])
-    assert keep.sum() == 55200
-    coords = np.array(traj["coords"][:, keep])
-    typ = [str(n) for n in traj["typ"][keep]]
+    if water == "map":
+        names = {"OW": "O", "HW1": "H", "HW2": "H"}
+        typ = [names.get(str(n), str(n)) for n in traj["typ"]]
+        coords = np.array(traj["coords"])
+        assert len(typ) == 55680 and all(n in rad for n in typ)
+    else:
+        keep = np.array([str(n) in rad for n in traj["typ"]])
+        assert keep.sum() == 55200
+        coords = np.array(traj["coords"][:, keep])
+        typ = [str(n) for n in traj["typ"][keep]]
     dims = traj["dims"]
     theta = math.radians(120.0)
     coords[..., 1] = coords[..., 1] / np.sin(theta)
@@ -476,4 +482,5 @@ def test_reference_hii_frame_end_to_end(precision, rad, gpu, tmp_path):
         mask = want["sf"] > SQ_FLOOR * want["sf"].max()
         floor32 = float(np.max(np.abs(s32[mask] - want["sf"][mask]) / want["sf"][mask]))
         err = assert_sq_close(got["sf"], want["sf"], rel=2.0 * floor32)
-    print(f"test_system.gro ({precision}): S(q) max rel err {err.max():.2e}, median {np.median(err):.2e}")
+    print(f"test_system.gro ({precision}, water {water}): S(q) max rel err {err.max():.2e}, "
+          f"median {np.median(err):.2e}")
