@@ -343,6 +343,9 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       TRY(dalloc(&sb, (size_t)p->spill_tiles * kTile * kTile * kTile * (p->fft64 ? 8 : 4)));
       f.spill = sb;
       TRY(dalloc(&f.spill_flag, (size_t)p->spill_tiles));
+      // no tile is read before a frame's binning flags it (a frame without atoms
+      // never runs the binning)
+      TRY(hipMemset(f.spill_flag, 0, sizeof(int) * (size_t)p->spill_tiles));
     }
   }
   TRY(dalloc(&p->err, (size_t)1));

@@ -230,6 +230,23 @@ def test_corner_fold_spill_many_corner_atoms(theta, fp64, rad, gpu, tmp_path):
     assert_sq_close(got, want, rel=1e-5 if fp64 else 1e-4)
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_empty_system_deposits_zeros(fp64, gpu):
+    """No atoms, Nborder 17 on a 32^3 grid (every tile a spill tile): the
+    density is exactly zero, twice in a row on the same plan."""
+    import torch
+    from mdsf import engine
+    plan = engine.Plan([32, 32, 32], 17, [1.0, 1.0, 1.0], np.eye(3), [1.0], [0.5], [[4, 4, 4]],
+                       [0], fft64=fp64)
+    coords = torch.zeros((0, 3), dtype=torch.float32, device=plan.device)
+    types = torch.zeros((0,), dtype=torch.int32, device=plan.device)
+    scale = torch.ones(3, dtype=torch.float64, device=plan.device)
+    for _ in range(2):
+        d = plan.deposit_fold(coords, types, scale, [32.0, 32.0, 32.0], 0, (0, 0))
+        plan.check_errors()
+        assert int(torch.count_nonzero(d)) == 0
+
+
 def test_occupancy_histogram_bit_exact(rad, gpu):
     import torch
     from mdsf import synth
