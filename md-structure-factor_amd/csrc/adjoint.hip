@@ -268,7 +268,8 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
       g1 += src[3 * it + 1];
       g2 += src[3 * it + 2];
     }
-    if (q.w >> 16) {
+    const int bands = (q.w >> 16) ? spill_bands(q, ty, sys) : 0;
+    if (bands) {
       // corner atom: the quirk images whose destination z tile the window
       // does not reach (the deposition's spill pass), dL/dr = sum grho dv/dr
       const int N0 = sys.n[0], N1 = sys.n[1], N2 = sys.n[2], nb = sys.nb;
@@ -277,6 +278,7 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
       const SpillRange xl = spill_side_l(q.x, ty.half[0]), xh = spill_side_h(q.x, ty.half[0], N0);
       const int nxl = xl.len(), nx = nxl + xh.len();
       for (int band = 0; band < 2; ++band) {
+        if (!((bands >> band) & 1)) continue;
         // band 0: y on H, z on L -> z + Nb; band 1: y on L, z on H -> z - Nb
         const SpillRange yr = band == 0 ? spill_side_h(q.y, ty.half[1], N1)
                                         : spill_side_l(q.y, ty.half[1]);
