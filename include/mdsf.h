@@ -173,6 +173,20 @@ int mdsf_accumulate_density(mdsf_plan* plan, const void* d_density,
 int mdsf_rfft3(mdsf_plan* plan, const void* d_density, double* d_spectrum,
                mdsf_stream stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the transform of mdsf_rfft3 without a plan, on the grid
+ * n[3] (even sizes) in `precision` (MDSF_F32 / MDSF_F64, the element type of
+ * d_density).  d_spectrum receives complex128 (Nx, Ny, Nz/2+1), bit for bit
+ * what mdsf_rfft3 gives on a plan of this grid and precision created without
+ * MDSF_AUTOTUNE.  Allocates nothing: its twiddles, half spectrum and Nyquist
+ * plane live in one fixed 80 MiB device array inside the library (one copy
+ * per GPU, held until the process ends), which fits every FP32 grid up to
+ * 256^3 and every FP64 grid up to 208^3; larger grids are rejected with
+ * MDSF_ERR_ARG.  Every call on a GPU uses that GPU's one array, so calls that
+ * overlap in time on one GPU (other streams, other host threads) race. */
+int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void* d_density,
+                       double* d_spectrum, mdsf_stream stream);
 /* The accumulated S(q) in the reference layout: d_sf double (Nx, Ny, Nz/2+1)
  * (the `sf` key of the .npz, dens.py:318,346). */
 int mdsf_sf_read(mdsf_plan* plan, double* d_sf, mdsf_stream stream);

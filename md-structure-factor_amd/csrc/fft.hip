@@ -1208,4 +1208,103 @@ hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s) {
   return hipGetLastError();
 }
 
+// --- plan-free transform for callers that cannot allocate -----------------------
+// The working set of mdsf_rfft3_noalloc (twiddles, half spectrum, Nyquist
+// plane) lives in this fixed device array, one copy per GPU, held until the
+// process ends.  Host code never passes the array's name to a kernel: it is a
+// host-side stand-in.  The device address of each GPU's copy comes from
+// hipGetSymbolAddress on the first call on that device and is kept per device.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)80 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (resolved once per device).
+hipError_t scratch_address(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
 }  // namespace mdsf
+
+extern "C" int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void* d_density,
+                                  double* d_spectrum, mdsf_stream stream) {
+  using namespace mdsf;
+  if (!n || !d_density || !d_spectrum) return fail(MDSF_ERR_ARG, "null argument");
+  if (precision != MDSF_F32 && precision != MDSF_F64)
+    return fail(MDSF_ERR_ARG, "precision must be MDSF_F32 or MDSF_F64");
+  for (int d = 0; d < 3; ++d)
+    if (n[d] < 2 || n[d] % 2 != 0 || n[d] > 4096)
+      return fail(MDSF_ERR_ARG, "grid sizes must be even and in [2, 4096]");
+  const size_t esz = precision == MDSF_F64 ? 8 : 4;
+  const size_t nx = n[0], ny = n[1], nz = n[2], m = nz / 2;
+  // layout: main half spectrum (nx ny m complex), Nyquist plane (nx ny
+  // complex), then the FP32 and FP64 twiddles of the x, y, z/2 axes and of
+  // the R2C split (length nz)
+  const size_t lens[4] = {nx, ny, m, nz};
+  size_t off = 0;
+  const size_t off_main = off;
+  off = align256(off + 2 * esz * nx * ny * m);
+  const size_t off_nyq = off;
+  off = align256(off + 2 * esz * nx * ny);
+  size_t off_tw[4], off_tw64[4];
+  for (int k = 0; k < 4; ++k) {
+    off_tw[k] = off;
+    off = align256(off + sizeof(float2) * lens[k]);
+    off_tw64[k] = off;
+    off = align256(off + sizeof(double2) * lens[k]);
+  }
+  if (off > kScratchBytes)
+    return fail(MDSF_ERR_ARG, "grid too large for mdsf_rfft3_noalloc: it needs " +
+                                  std::to_string(off) + " bytes of its " +
+                                  std::to_string(kScratchBytes) +
+                                  "-byte device scratch array (use a plan and mdsf_rfft3)");
+  unsigned char* base = nullptr;
+  MDSF_HIP(scratch_address(&base));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Plan p;  // host-side description only: every device pointer is into `base`
+  for (int d = 0; d < 3; ++d) p.sys.n[d] = n[d];
+  p.fft64 = precision == MDSF_F64 ? 1 : 0;
+  p.spec_main = p.rho = base + off_main;
+  p.spec_nyq = base + off_nyq;
+  FftAxis* axes[3] = {&p.ax_x, &p.ax_y, &p.ax_zc};
+  for (int k = 0; k < 4; ++k) {
+    float2* tw = reinterpret_cast<float2*>(base + off_tw[k]);
+    double2* tw64 = reinterpret_cast<double2*>(base + off_tw64[k]);
+    if (k < 3) {
+      FftAxis& ax = *axes[k];
+      ax.n = (int)lens[k];
+      if (!fft_factor(ax.n, ax.radix, &ax.nstages))
+        return fail(MDSF_ERR_UNSUPPORTED,
+                    "FFT length " + std::to_string(ax.n) + " needs too many stages");
+      ax.tw = tw;
+      ax.tw64 = tw64;
+    } else {
+      p.tw_r2c = tw;
+      p.tw_r2c64 = tw64;
+    }
+    MDSF_HIP(fft_make_twiddles(tw, tw64, (int)lens[k], s));
+  }
+  MDSF_HIP(fft_forward(p, d_density, false, s));
+  MDSF_HIP(fft_spectrum_read(p, d_spectrum, s));
+  return MDSF_OK;
+}

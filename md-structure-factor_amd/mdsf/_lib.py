@@ -67,6 +67,8 @@ SIGNATURES = {
     "mdsf_fft_pass": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "mdsf_accumulate_density": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mdsf_rfft3": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mdsf_rfft3_noalloc": (c_int32, [ctypes.POINTER(c_int32), c_int32, c_void_p, c_void_p,
+                                     c_void_p]),
     "mdsf_sf_read": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "mdsf_outputs": (c_int32, [c_void_p, c_void_p, ctypes.POINTER(c_double), c_int32, c_void_p,
                                c_void_p, c_void_p, c_void_p]),
