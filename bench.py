@@ -7,11 +7,12 @@ axis), FP32 deposition and FFT, FP64 frame sum of |F|^2.
 
 One step = one frame through the whole path (rescale/wrap, binning, factor
 tables, deposition + fold, z/y/x FFT passes, |F|^2 accumulation), with the
-trajectory already resident in HBM.  Each stage is bracketed by HIP events on
-the stream the kernels run on, which gives the per-kernel averages used for
-the roofline figures.  The CPU baseline times the FP64 numpy oracle (the
-reference's per-atom op sequence) on a bounded, type-stratified sample of
-one frame and extrapolates to frames/s.
+trajectory already resident in HBM.  In the timed region HIP events bracket
+the deposition (the dominant kernel) on the stream it runs on; a second pass
+runs every stage alone on one stream for the per-pass FFT figures.  The CPU
+baseline runs one whole frame through the FP64 numpy oracle (the reference's
+per-atom op sequence) on one host core; the precision leg compares the FP32
+and FP64 plans with that oracle frame.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu]
 """
@@ -60,57 +61,47 @@ def voxel_updates(setup):
     return int(np.sum(per_type * setup.counts))
 
 
-def cpu_baseline(coords0, dims, typ, rad, sres, budget_atoms=None):
-    """Time the FP64 oracle on a type-stratified atom sample of frame 0 plus one
-    fold + rfftn + |F|^2, and extrapolate to frames/s for the full frame."""
-    from oracle import dens_oracle as O
-    # about 10 s of single-core work at 256^3 (Na windows cost ~15 ms per atom)
-    budget_atoms = budget_atoms or {"H": 10000, "C": 8000, "O": 960, "Na": 300}
-    r = np.array(coords0[None], copy=True)
-    o = O.setup(r, np.array(dims[:1]), typ, rad, sres, better=False)
-    n, nb = o["N"], o["Nb"]
-    padded = tuple(int(n[i]) + 2 * nb for i in range(3))
-    d0 = np.zeros(padded)
-    typ = np.asarray(typ)
-    t_dep = 0.0
-    sample = {}
-    for name in sorted(set(typ.tolist())):
-        idx = np.nonzero(typ == name)[0]
-        k = min(len(idx), budget_atoms.get(name, 200))
-        sel = idx[:k]
-        t0 = time.perf_counter()
-        O.deposit_frame(r[0][sel], typ[sel].tolist(), rad, o["bdict"], o["dr"], nb, np.eye(3),
-                        padded, out=d0)
-        dt = time.perf_counter() - t0
-        t_dep += dt / k * len(idx)
-        sample[name] = [int(k), int(len(idx)), round(dt / k * 1e3, 4)]
-    t0 = time.perf_counter()
-    d1 = O.fold(d0, *O.fold_tables(n, nb))
-    t_fold = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.frame_power(d1)
-    t_fft = time.perf_counter() - t0
-    frame_s = t_dep + t_fold + t_fft
-    cpu = "unknown"
+def _cpu_name():
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {
+    return "unknown"
+
+
+def cpu_baseline(coords0, dims, typ, rad, sres):
+    """One whole frame of the workload through the FP64 numpy oracle (the
+    reference's per-atom op sequence, fold, rfftn and |F|^2, dens.py:283-318)
+    on one host core: frames/s = 1 / its wall time.  Returns the baseline
+    record and the oracle's S(q) of that frame (the precision leg's reference)."""
+    from oracle import dens_oracle as O
+    r = np.array(coords0[None], copy=True)
+    o = O.setup(r, np.array(dims[:1]), typ, rad, sres, better=False)
+    n, nb = o["N"], o["Nb"]
+    padded = tuple(int(n[i]) + 2 * nb for i in range(3))
+    t0 = time.perf_counter()
+    d0, _ = O.deposit_frame(r[0], list(typ), rad, o["bdict"], o["dr"], nb, np.eye(3), padded)
+    t1 = time.perf_counter()
+    d1 = O.fold(d0, *O.fold_tables(n, nb))
+    del d0
+    t2 = time.perf_counter()
+    sf = O.frame_power(d1)
+    t3 = time.perf_counter()
+    frame_s = t3 - t0
+    rec = {
         "value": 1.0 / frame_s,
         "unit": "frames/s",
         "cores": 1,
         "kind": "port",
-        "sample": ("FP64 numpy oracle (reference per-atom op sequence) on frame 0: per-type atom "
-                   "samples [timed, total, ms/atom] " + json.dumps(sample) +
-                   f"; fold {t_fold:.2f} s, rfftn+|F|^2 {t_fft:.2f} s; deposition extrapolated "
-                   f"to all atoms: {t_dep:.1f} s/frame"),
-        "host": f"{cpu}; nproc={os.cpu_count()}; {platform.machine()}",
+        "sample": (f"frame 0 of the workload, whole ({len(typ)} atoms, {list(int(x) for x in n)} "
+                   f"grid), through the FP64 numpy oracle: deposition {t1 - t0:.1f} s, fold "
+                   f"{t2 - t1:.2f} s, rfftn + |F|^2 {t3 - t2:.2f} s"),
+        "host": f"{_cpu_name()}; nproc={os.cpu_count()}; {platform.machine()}",
     }
+    return rec, sf
 
 
 def copy_calibration(nbytes=256 << 20):
@@ -199,26 +190,35 @@ def sq_rel_err(got, want, floor=1e-6):
             "voxels": int(mask.sum()), "of": int(want.numel())}
 
 
-def precision_leg(coords, s, nframes=2):
-    """FP32 plan (the measured path) against the FP64 plan on the first frames of
-    the workload; the FP64 plan is pinned to the FP64 numpy oracle by the tests."""
+def precision_leg(coords, s, oracle_sf=None, nframes=2):
+    """S(q) of the FP32 plan (the measured path) and of the FP64 plan against
+    the FP64 numpy oracle on frame 0 (when ``oracle_sf`` is given), and the
+    FP32 plan against the FP64 plan over the first ``nframes`` frames."""
     import torch
     from mdsf import engine
     dev = torch.device("cuda", 0)
-    out = {}
-    for fp64 in (False, True):
+    types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
+
+    def run(fp64, frames):
         plan = engine.Plan.from_setup(s, fft64=fp64)
-        c = torch.from_numpy(np.ascontiguousarray(coords[:nframes])).to(dev)
-        types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
-        scale = torch.from_numpy(np.ascontiguousarray(s.scale[:nframes], dtype=np.float64)).to(dev)
+        c = torch.from_numpy(np.ascontiguousarray(coords[:frames])).to(dev)
+        scale = torch.from_numpy(np.ascontiguousarray(s.scale[:frames], dtype=np.float64)).to(dev)
         plan.accumulate_frames(c, types, scale, [float(x) for x in s.L], s.arith, s.wrap)
         plan.check_errors()
-        out[fp64] = plan.sf()
+        out = plan.sf()
         plan.close()
-    res = sq_rel_err(out[False], out[True])
-    res.update({"reference": "fp64 plan (FP64 deposition, FFT and sum)", "frames": nframes,
-                "floor": "1e-6 * max S"})
-    return res
+        return out
+
+    out = {"floor": "voxels above 1e-6 * max S"}
+    if oracle_sf is not None:
+        want = torch.from_numpy(oracle_sf).to(dev)
+        for fp64 in (False, True):
+            out[("fp64" if fp64 else "fp32") + "_vs_oracle"] = sq_rel_err(run(fp64, 1), want)
+        out["oracle"] = "FP64 numpy oracle, frame 0 (the cpu_baseline run)"
+    res = sq_rel_err(run(False, nframes), run(True, nframes))
+    res["frames"] = nframes
+    out["fp32_vs_fp64_plan"] = res
+    return out
 
 
 def batch_leg(rad, nframes):
@@ -313,19 +313,57 @@ def cylindrical_leg(plan, box, ucell):
             "interpolations_per_s": samples / dt, "ucell": "orthogonal (the workload's cell)"}
 
 
+def mono_leg(coords, dims, typ, rad, sres, steps, warmup):
+    """The same frames in the reference driver's default cell (main_gromacs.py
+    -ct 120, :61,79-80): coordinates moved to the monoclinic frame as
+    main_gromacs.py:204-207 does (float32), ucell = [[1,0,0],[cos,sin,0],[0,0,1]]
+    (the xy-coupled deposition); frames/s of the pipelined frame loop."""
+    import torch
+    from mdsf import engine
+    th = np.radians(120.0)
+    c = np.array(coords[:max(steps, warmup)], copy=True)
+    c[..., 1] = c[..., 1] / np.sin(th)
+    c[..., 0] = c[..., 0] - c[..., 1] * np.cos(th)
+    ucell = np.array([[1, 0, 0], [np.cos(th), np.sin(th), 0], [0, 0, 1]])
+    s = engine.Setup(c, dims[:c.shape[0]], typ, rad, sres, False, ucell)
+    plan = engine.Plan.from_setup(s, autotune=True)
+    dev = plan.device
+    d_c = torch.from_numpy(c).to(dev)
+    d_t = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
+    d_s = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
+    box = [float(x) for x in s.L]
+    plan.accumulate_frames(d_c[:warmup], d_t, d_s[:warmup], box, s.arith, s.wrap)
+    plan.reset()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    plan.accumulate_frames(d_c[:steps], d_t, d_s[:steps], box, s.arith, s.wrap)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plan.check_errors()
+    ms = e0.elapsed_time(e1)
+    plan.close()
+    return {"workload": "hii_phase_256^3_50k_atoms in the 120-degree cell (main_gromacs.py "
+                        "-ct 120 transform)", "frames": steps, "frames_per_s": steps / (ms * 1e-3),
+            "ms_per_frame": ms / steps, "deposition": "xy-coupled (MODE_XY)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline leg (and the precision leg's oracle comparison)")
     ap.add_argument("--fft", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+bwd leg")
     ap.add_argument("--no-precision", action="store_true",
-                    help="skip the fp32-vs-fp64 S(q) error report")
+                    help="skip the S(q) error report")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-isolated", action="store_true",
                     help="skip the one-stage-at-a-time timing pass")
+    ap.add_argument("--no-mono", action="store_true", help="skip the 120-degree cell leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the default FFT launch choices instead of timing them at plan creation")
     ap.add_argument("--batch-frames", type=int, default=50)
@@ -366,11 +404,11 @@ def main():
     d_scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
     box = [float(x) for x in s.L]
     stream = torch.cuda.current_stream(dev)
-    # the timed region is the library's own frame loop (mdsf_accumulate_frames):
+    # The timed region is the library's own frame loop (mdsf_accumulate_frames):
     # frame t + 1 prepared on this stream while frame t is deposited and frame
-    # t - 1 transformed on the plan's two other streams; the plan records HIP
-    # events around every stage on the stream it runs on
-    plan.set_stage_timing(True)
+    # t - 1 transformed on the plan's two other streams.  HIP events bracket
+    # only the deposition (on the stream it runs on), the dominant kernel.
+    plan.set_stage_timing(2)
     log(f"warmup {args.warmup} frames")
     plan.accumulate_frames(d_coords[:args.warmup], d_types, d_scale[:args.warmup], box, s.arith,
                            s.wrap)
@@ -385,8 +423,9 @@ def main():
     torch.cuda.synchronize()
     plan.check_errors()
     total_ms = start.elapsed_time(end)
-    per_stage = plan.stage_times()
-    log("per-stage avg ms (pipelined): " + json.dumps({k: round(v, 4) for k, v in per_stage.items()}))
+    dep_ms = plan.stage_times()["deposit"]
+    log(f"{args.steps / (total_ms * 1e-3):.1f} frames/s; deposition {dep_ms * 1e3:.1f} us "
+        f"per frame in the timed region")
     iso = None
     if not args.no_isolated:
         iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, min(args.steps, 30), stream)
@@ -394,29 +433,14 @@ def main():
 
     n = [int(x) for x in s.N]
     fb = fft_bytes(n)
-    fft_roof = {}
-    for key, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc")):
-        gbs = fb[key] / (per_stage[st] * 1e-3) / 1e9
-        fft_roof[key] = {"achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
-                         "bytes": fb[key], "avg_ms": round(per_stage[st], 4)}
     updates = voxel_updates(s)
-    dep_tflops = 2.0 * updates / (per_stage["deposit"] * 1e-3) / 1e12
-    # dominant kernel: the single launch with the largest average time in the
-    # timed (pipelined) region ("prepare" is several small kernels)
-    kernels = {k: v for k, v in per_stage.items() if k != "prepare"}
-    dominant = max(kernels, key=kernels.get)
-    if dominant == "deposit":
-        roofline = {"bound": "mfma", "kernel": "k_deposit_mfma (deposition + fold, FP32 MFMA)",
-                    "achieved": round(dep_tflops, 3), "peak": FP32_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(dep_tflops / FP32_PEAK_TFLOPS, 4),
-                    "traffic": DEPOSIT_TRAFFIC,
-                    "basis": f"{updates} voxel updates/frame x 2 flop (one FMA each) / avg "
-                             f"deposit time {per_stage['deposit'] * 1e3:.1f} us"}
-    else:
-        key = {"fft_z": "z", "fft_y": "y", "fft_x_acc": "x"}[dominant]
-        roofline = {"bound": "hbm", "kernel": f"FFT {key} pass",
-                    "achieved": fft_roof[key]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": fft_roof[key]["frac"], "traffic": None}
+    dep_tflops = 2.0 * updates / (dep_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": "k_deposit_mfma (deposition + fold, FP32 MFMA)",
+                "achieved": round(dep_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(dep_tflops / FP32_PEAK_TFLOPS, 4),
+                "traffic": DEPOSIT_TRAFFIC,
+                "basis": f"{updates} voxel updates/frame x 2 flop (one FMA each) / avg deposition "
+                         f"{dep_ms * 1e3:.1f} us (HIP events on its stream, timed region)"}
 
     out = {
         "metric": METRIC,
@@ -435,26 +459,35 @@ def main():
                    "box_A": [float(x) for x in s.L], "fft_precision": args.fft,
                    "sf_accumulation": "f64"},
         "roofline": roofline,
-        "fft_passes": fft_roof,
         "deposit": {"voxel_updates_per_frame": updates,
-                    "updates_per_s": round(updates / (per_stage["deposit"] * 1e-3), 1),
-                    "tflops_fma": round(dep_tflops, 3)},
-        "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
+                    "updates_per_s": round(updates / (dep_ms * 1e-3), 1),
+                    "tflops_fma": round(dep_tflops, 3), "avg_ms_timed_region": round(dep_ms, 4)},
     }
     if iso is not None:
         out["stage_ms_isolated"] = {k: round(v, 4) for k, v in iso.items()}
-        out["roofline_isolated"] = {
-            "deposit_tflops": round(2.0 * updates / (iso["deposit"] * 1e-3) / 1e12, 3),
-            "deposit_frac": round(2.0 * updates / (iso["deposit"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
-            "fft_gbs": {k: round(fb[k] / (iso[st] * 1e-3) / 1e9, 1)
-                        for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"))},
-            "note": "the same kernels run one at a time on one stream"}
+        out["fft_passes"] = {
+            k: {"achieved": round(fb[k] / (iso[st] * 1e-3) / 1e9, 1),
+                "frac": round(fb[k] / (iso[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "bytes": fb[k], "avg_ms": round(iso[st], 4)}
+            for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"))}
+        out["fft_passes_note"] = ("each pass alone on one stream (HIP events around its "
+                                  "kernel); algorithmic bytes / time vs 8 TB/s")
+        out["deposit"]["isolated_frac"] = round(
+            2.0 * updates / (iso["deposit"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)
+    oracle_sf = None
+    if not args.no_cpu:
+        log("cpu baseline: frame 0 through the FP64 numpy oracle on one core")
+        out["cpu_baseline"], oracle_sf = cpu_baseline(coords[0], dims, typ, rad, sres)
+    if not args.no_precision:
+        log("precision: fp32 / fp64 plans vs the oracle (frame 0), fp32 vs fp64 (2 frames)")
+        out["precision"] = precision_leg(coords, s, oracle_sf)
+    del oracle_sf
+    if not args.no_mono:
+        log("120-degree cell leg")
+        out["mono120"] = mono_leg(coords, dims, typ, rad, sres, args.steps, args.warmup)
     if not args.no_cylindrical:
         log("cylindrical average of the 256^3 kgridplt")
         out["cylindrical"] = cylindrical_leg(plan, box, np.eye(3))
-    if not args.no_precision:
-        log("precision: fp32 plan vs fp64 plan on 2 frames")
-        out["precision"] = precision_leg(coords, s)
     if not args.no_batch:
         log(f"batched leg: 32 systems x {args.batch_frames} frames")
         out["batched"] = batch_leg(rad, args.batch_frames)
@@ -464,10 +497,6 @@ def main():
     if not args.no_adjoint:
         log("fwd+bwd leg (adjoint workload)")
         out["fwd_bwd"] = adjoint_leg(rad, args.adj_iters)
-    if not args.no_cpu:
-        log("cpu baseline (oracle on a stratified sample of frame 0)")
-        c0, d0, t0 = synth.hii(seed=2002, nframes=1)
-        out["cpu_baseline"] = cpu_baseline(c0[0], d0, t0, rad, sres)
     print(json.dumps(out), flush=True)
 
 

@@ -32,7 +32,8 @@ extern "C" {
 #define MDSF_OK 0
 #define MDSF_ERR_ARG 1        /* invalid argument (the reference raises ValueError)   */
 #define MDSF_ERR_HIP 2        /* HIP runtime error                                    */
-#define MDSF_ERR_RANGE 3      /* an atom window leaves the padded grid (ValueError)   */
+#define MDSF_ERR_RANGE 3      /* reserved: windows leaving the padded grid are clipped
+                                 as the reference's numpy slices clip them           */
 #define MDSF_ERR_TYPE 4       /* an atom type index is out of range (KeyError)        */
 #define MDSF_ERR_UNSUPPORTED 5
 
@@ -71,9 +72,12 @@ typedef struct mdsf_grid_desc {
   int32_t flags;         /* MDSF_FFT_FP64 or 0                                          */
 } mdsf_grid_desc;
 
-/* Plan lifecycle.  Validates the descriptor (Nborder <= N, even N, cell
- * metric without xz / yz coupling), allocates workspaces on the current
- * device and generates the FFT twiddles in FP64 on the GPU. */
+/* Plan lifecycle.  Validates the descriptor (Nborder <= N, even N),
+ * allocates workspaces on the current device and generates the FFT twiddles
+ * in FP64 on the GPU.  Any 3x3 ucell is accepted (dens.py:301): cells whose
+ * metric couples z with x or y deposit through an FP64 per-voxel gather (any
+ * plan precision); the adjoint entry points reject such plans with
+ * MDSF_ERR_UNSUPPORTED. */
 int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out);
 int mdsf_plan_destroy(mdsf_plan* plan);
 const char* mdsf_last_error(void);
@@ -85,9 +89,12 @@ int mdsf_sf_reset(mdsf_plan* plan, mdsf_stream stream);
 
 /* Synchronises `stream` and reports (then clears) the error flags the
  * kernels raised since the last check: MDSF_ERR_TYPE for a type index out of
- * range (the reference's KeyError, dens.py:287) and MDSF_ERR_RANGE for an
- * atom whose window leaves the padded grid (the reference's broadcast
- * ValueError at dens.py:308). */
+ * range (the reference's KeyError, dens.py:287).  An atom whose window leaves
+ * the padded grid is no error: the reference slices its grid and d0 with the
+ * same indices (dens.py:292-308), which numpy clips alike, and so does the
+ * deposition (a window across the low edge or past the high edge adds
+ * nothing, one cut by the high edge adds its kept part, one wholly at
+ * negative indices is counted from the far end of the padded grid). */
 int mdsf_check_errors(mdsf_plan* plan, mdsf_stream stream);
 
 /* The frame loop of dens.py:277-321 for `nframes` frames.
@@ -102,7 +109,8 @@ int mdsf_check_errors(mdsf_plan* plan, mdsf_stream stream);
  *   wrap_lo/hi atom index range the wrap touches (all atoms below 1e6 atoms;
  *              see dens.py:213-221 for the buffered branch).
  *   d_occupancy optional uint32[Nx*Ny*Nz]: += 1 at (ir mod N) per atom and frame.
- *   d_home_cells optional int32[natoms*3]: ir of the LAST frame (dens.py:285).
+ *   d_home_cells optional int32[natoms*3]: ir of the LAST frame (dens.py:285);
+ *              INT32_MIN for a non-finite coordinate or |r / dr| >= 1e9.
  * Adds sum_frames |rfftn(folded density)|^2 into the plan accumulator. */
 int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype,
                            int64_t nframes, const int32_t* d_types,
@@ -117,6 +125,23 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype,
 int mdsf_rescale_wrap(mdsf_plan* plan, void* d_coords, int dtype, int64_t nframes,
                       const double* d_scale, const double box[3], int arith, int64_t wrap_lo,
                       int64_t wrap_hi, mdsf_stream stream);
+
+/* The in-place rescale of dens.py:46-62 (`rescale(coords, dims)`) alone, no
+ * plan: d_coords (nframes, natoms, 3) of dtype MDSF_F32 / MDSF_F64 is
+ * multiplied per frame and axis by d_scale[3 t + i] = avg(dims)[i] / dims[t, i]
+ * in the arithmetic `arith` (as for mdsf_accumulate_frames). */
+int mdsf_rescale(void* d_coords, int dtype, int64_t nframes, int64_t natoms,
+                 const double* d_scale, int arith, mdsf_stream stream);
+
+/* The border fold of a padded FP64 grid onto the periodic cell, with the
+ * tables compute_sf builds (des = [0, Nb, N - Nb, N], ori = [0, Nb, Nb + N,
+ * 2 Nb + N] per axis, dens.py:258-263): remap_grid_tcl (dens.py:86-108),
+ * corner z slot included, and remap_grid (dens.py:65-83) with one table for
+ * every axis.  d_padded (n + 2 nborder per axis) -> d_out (n); the additions
+ * run in the reference's order, so the result is bitwise the reference's.
+ * Requires nborder[d] <= n[d]. */
+int mdsf_fold_padded(const double* d_padded, const int32_t n[3], const int32_t nborder[3],
+                     double* d_out, mdsf_stream stream);
 
 /* One frame of deposition + periodic fold (dens.py:283-311), no FFT:
  * writes the folded density d_density[Nx*Ny*Nz] (plan precision).  Coordinates are
@@ -152,12 +177,14 @@ int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_str
  * reference (its frame loop is sequential, dens.py:277-321). */
 int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which);
 
-/* Stage timing of mdsf_accumulate_frames (off by default): when enabled, the
- * frame loop records HIP events around each frame's preparation, deposition
- * and three FFT passes on the stream each runs on; mdsf_stage_times then
- * waits for the last call's events and returns the average milliseconds per
- * frame of {prepare, deposit, z pass, y pass, x pass + |F|^2 sum} in
- * avg_ms[5].  No counterpart in the reference. */
+/* Stage timing of mdsf_accumulate_frames (off by default): with enable = 1
+ * the frame loop records HIP events around each frame's preparation,
+ * deposition and three FFT passes on the stream each runs on; with enable = 2
+ * only around the deposition (the least perturbation of the overlapped loop).
+ * mdsf_stage_times then waits for the last call's events and returns the
+ * average milliseconds per frame of {prepare, deposit, z pass, y pass, x pass
+ * + |F|^2 sum} in avg_ms[5] (0 for the stages not timed).  No counterpart in
+ * the reference. */
 int mdsf_set_stage_timing(mdsf_plan* plan, int32_t enable);
 int mdsf_stage_times(mdsf_plan* plan, double* avg_ms);
 
@@ -187,6 +214,7 @@ int mdsf_rfft3(mdsf_plan* plan, const void* d_density, double* d_spectrum,
  * overlap in time on one GPU (other streams, other host threads) race. */
 int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void* d_density,
                        double* d_spectrum, mdsf_stream stream);
+
 /* The accumulated S(q) in the reference layout: d_sf double (Nx, Ny, Nz/2+1)
  * (the `sf` key of the .npz, dens.py:318,346). */
 int mdsf_sf_read(mdsf_plan* plan, double* d_sf, mdsf_stream stream);
@@ -208,12 +236,14 @@ int mdsf_outputs(mdsf_plan* plan, const double* d_sf, const double box[3],
  * double (Nx, Ny, Nz/2+1).  Nothing per frame is stored by the forward pass:
  * each frame's density and spectrum are recomputed. */
 
-/* dL/d(coords) for `nframes` frames laid out as for mdsf_accumulate_frames.
+/* dL/d(coords) for `nframes` frames laid out as for mdsf_accumulate_frames
+ * (plans whose cell couples z with x or y: MDSF_ERR_UNSUPPORTED, as for
+ * mdsf_deposit_adjoint).
  * The coordinates are rescaled and wrapped in place exactly as the forward
  * pass does (pass a copy of the original coordinates); d_grad (nframes,
  * natoms, 3) double receives dL/d(original coordinates) = scale * dL/dr.
- * Atoms that fail the window check get a zero gradient and raise the plan's
- * error flag (mdsf_check_errors). */
+ * Atoms whose windows the reference's slices empty get a zero gradient;
+ * windows cut by the padded grid's high edge differentiate their kept part. */
 int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype,
                         int64_t nframes, const int32_t* d_types,
                         const double* d_scale, const double box[3], int arith,

@@ -17,6 +17,7 @@
 // its (y, z) pairs with an inner loop over the chunk's x entries, whose
 // values are broadcast with readlane.  Items write 3 FP64 partial sums; a
 // second kernel adds each atom's partials in a fixed order (deterministic).
+#include "clip.h"
 #include "mdsf_internal.h"
 #include "spill.h"
 #include "tiles.h"
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_deposit_adjoint(
     const int xc = rem / ty.adj_npc, pc = rem - xc * ty.adj_npc;
     const int s = ty.type_start + slot;
     const int4 q = s < *nvalid ? s_ir[s] : make_int4(0, 0, 0, -1);
-    if ((q.w & 0xffff) == t) {
+    if ((q.w & 0xffff) == t && !(q.w & kAtomParked)) {  // parked: k_adjoint_reduce
       const int n0 = sys.n[0], n1 = sys.n[1], n2 = sys.n[2], nb = sys.nb;
       const int Ax = ty.half[0], Ay = ty.half[1], Az = ty.half[2];
       const int Wz = 2 * Az;
@@ -179,25 +180,35 @@ __global__ __launch_bounds__(kAdjThreads) void k_deposit_adjoint(
   }
 }
 
-// grad[atom] = scale * (sum of the atom's item partials, in item order).
+// grad[atom] = scale * (sum of the atom's item partials, in item order);
+// parked atoms sum their clipped window here (one thread each).
+template <typename R>
 __global__ void k_adjoint_reduce(SysDev sys, const TypeDev* __restrict__ td,
                                  const int4* __restrict__ s_ir, const int* __restrict__ s_idx,
                                  const int* __restrict__ nvalid, const double* __restrict__ part,
-                                 const double* __restrict__ scale3, double* __restrict__ grad) {
+                                 const double* __restrict__ scale3, const double* __restrict__ s_r,
+                                 const R* __restrict__ grho, double* __restrict__ grad) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= *nvalid) return;
-  const int t = s_ir[s].w & 0xffff;
+  const int4 q = s_ir[s];
+  const int t = q.w & 0xffff;
   if (t >= sys.ntypes) return;
   const TypeDev& ty = td[t];
   const int slot = s - ty.type_start;
   if (slot < 0 || slot >= ty.count) return;
-  const int per = ty.adj_nxc * ty.adj_npc;
-  const long long b = ty.adj_item_base + (long long)slot * per;
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-  for (int it = 0; it < per; ++it) {
-    g0 += part[3 * (b + it) + 0];
-    g1 += part[3 * (b + it) + 1];
-    g2 += part[3 * (b + it) + 2];
+  if (q.w & kAtomParked) {
+    // window clipped by the padded grid: the gradient over its kept part
+    parked_grad<R>(sys, ty, q, s_r[3 * (size_t)s + 0], s_r[3 * (size_t)s + 1],
+                   s_r[3 * (size_t)s + 2], grho, 0, 1, g0, g1, g2);
+  } else {
+    const int per = ty.adj_nxc * ty.adj_npc;
+    const long long b = ty.adj_item_base + (long long)slot * per;
+    for (int it = 0; it < per; ++it) {
+      g0 += part[3 * (b + it) + 0];
+      g1 += part[3 * (b + it) + 1];
+      g2 += part[3 * (b + it) + 2];
+    }
   }
   const double c0 = scale3 ? scale3[0] : 1.0, c1 = scale3 ? scale3[1] : 1.0,
                c2 = scale3 ? scale3[2] : 1.0;
@@ -228,6 +239,7 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
   // below run with the whole group
   bool act = s < *nvalid;
   int t = 0, slot = 0;
+  bool parked = false;
   if (act) {
     t = s_ir[s].w & 0xffff;
     act = t < sys.ntypes;
@@ -235,9 +247,14 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
   if (act) {
     slot = s - td[t].type_start;
     act = slot >= 0 && slot < td[t].count;
+    parked = (s_ir[s].w & kAtomParked) != 0;
   }
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-  if (act) {
+  if (act && parked) {
+    // window clipped by the padded grid: the 16 lanes share its kept part
+    parked_grad<float>(sys, td[t], s_ir[s], s_r[3 * (size_t)s + 0], s_r[3 * (size_t)s + 1],
+                       s_r[3 * (size_t)s + 2], grho, li, kRedLanes, g0, g1, g2);
+  } else if (act) {
     const TypeDev& ty = td[t];
     const int4 q = s_ir[s];
     const int per = ty.nsl[0] * ty.nsl[1] * ty.nsl[2];
@@ -268,7 +285,7 @@ __global__ void k_adjoint_tile_reduce(SysDev sys, const TypeDev* __restrict__ td
       g1 += src[3 * it + 1];
       g2 += src[3 * it + 2];
     }
-    const int bands = (q.w >> 16) ? spill_bands(q, ty, sys) : 0;
+    const int bands = ((q.w >> 16) & 1) ? spill_bands(q, ty, sys) : 0;
     if (bands) {
       // corner atom: the quirk images whose destination z tile the window
       // does not reach (the deposition's spill pass), dL/dr = sum grho dv/dr
@@ -397,8 +414,14 @@ hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_adjoint_reduce, dim3((n + 255) / 256), dim3(256), 0, s, p.sys, p.types,
-                     p.s_ir, p.s_idx, nvalid, p.adj_part, scale3, grad);
+  if (p.fft64)
+    hipLaunchKernelGGL(k_adjoint_reduce<double>, dim3((n + 255) / 256), dim3(256), 0, s, p.sys,
+                       p.types, p.s_ir, p.s_idx, nvalid, p.adj_part, scale3, p.s_r,
+                       static_cast<const double*>(grho), grad);
+  else
+    hipLaunchKernelGGL(k_adjoint_reduce<float>, dim3((n + 255) / 256), dim3(256), 0, s, p.sys,
+                       p.types, p.s_ir, p.s_idx, nvalid, p.adj_part, scale3, p.s_r,
+                       static_cast<const float*>(grho), grad);
   return hipGetLastError();
 }
 

@@ -117,6 +117,9 @@ int ensure_adjoint(Plan& p) {
   return MDSF_OK;
 }
 
+constexpr const char* kNoGenAdjoint =
+    "the adjoint supports orthogonal and xy-coupled cells (no ucell coupling z with x or y)";
+
 int run_fft_accumulate(Plan& p, const void* density, hipStream_t s) {
   MDSF_HIP(fft_forward(p, density, true, s));
   return MDSF_OK;
@@ -167,10 +170,6 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   double G[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) G[i][j] = U[3 * i + 0] * U[3 * j + 0] + U[3 * i + 1] * U[3 * j + 1] + U[3 * i + 2] * U[3 * j + 2];
-  if (G[0][2] != 0.0 || G[1][2] != 0.0)
-    return fail(MDSF_ERR_UNSUPPORTED,
-                "cell matrices coupling z with x or y are not supported (callers use monoclinic "
-                "xy cells)");
 
   Plan* p = new Plan();
   SysDev& s = p->sys;
@@ -189,7 +188,11 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   s.g01 = G[0][1];
   s.g11 = G[1][1];
   s.g22 = G[2][2];
-  s.mode = (G[0][1] == 0.0) ? MODE_DIAG : MODE_XY;
+  s.g02 = G[0][2];
+  s.g12 = G[1][2];
+  // z coupled to x or y: the general gather (dens.py:301 with any 3x3 ucell)
+  s.mode = (G[0][2] != 0.0 || G[1][2] != 0.0) ? MODE_GEN
+                                               : ((G[0][1] == 0.0) ? MODE_DIAG : MODE_XY);
   s.ntypes = desc->ntypes;
   s.natoms = desc->natoms;
   p->fft64 = (desc->flags & MDSF_FFT_FP64) ? 1 : 0;
@@ -237,10 +240,11 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     ty.stride = stride;
     ty.table_base = table_floats;
     ty.bucket_base = bucket_base;
+    ty.park = bucket_base + ty.nbk[0] * ty.nbk[1] * ty.nbk[2];
     table_floats += (long long)in.count * stride;
     start += in.count;
     counted += in.count;
-    bucket_base += ty.nbk[0] * ty.nbk[1] * ty.nbk[2];
+    bucket_base = ty.park + 1;
   }
   if (counted != desc->natoms) {
     free_plan(p);
@@ -480,8 +484,11 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     }
     p->timed_frames = nframes;
   }
+  // timing mode 1: every stage; mode 2: only the deposition (two events per
+  // frame on its stream, the least perturbation of the overlapped loop)
   auto mark = [&](int64_t t, int k, hipStream_t st) -> hipError_t {
-    return p->timing ? hipEventRecord(p->tev[10 * t + k], st) : hipSuccess;
+    if (!p->timing || (p->timing == 2 && k != 2 && k != 3)) return hipSuccess;
+    return hipEventRecord(p->tev[10 * t + k], st);
   };
   MDSF_HIP(hipEventRecord(p->ev_fork, s));
   MDSF_HIP(hipStreamWaitEvent(d, p->ev_fork, 0));
@@ -509,6 +516,8 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(mark(t, 3, d));
     MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
+    // parked atoms' clipped windows: on the FFT stream, which has room
+    MDSF_HIP(launch_parked(*p, p->rho, f));
     for (int pass = 0; pass < 3; ++pass) {
       MDSF_HIP(mark(t, 4 + 2 * pass, f));
       MDSF_HIP(fft_pass(*p, pass, p->rho, f));
@@ -554,6 +563,7 @@ int mdsf_deposit_fold(mdsf_plan* plan, void* d_coords, int dtype, const int32_t*
                        nullptr, d_home_cells, s));
   MDSF_HIP(launch_binning(*p, s));
   MDSF_HIP(launch_deposit(*p, d_density, s));
+  MDSF_HIP(launch_parked(*p, d_density, s));
   return MDSF_OK;
 }
 
@@ -574,7 +584,8 @@ int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t
 int mdsf_set_stage_timing(mdsf_plan* plan, int32_t enable) {
   Plan* p = reinterpret_cast<Plan*>(plan);
   if (!p) return fail(MDSF_ERR_ARG, "null plan");
-  p->timing = enable != 0;
+  if (enable < 0 || enable > 2) return fail(MDSF_ERR_ARG, "stage timing mode must be 0, 1 or 2");
+  p->timing = enable;
   p->timed_frames = 0;
   return MDSF_OK;
 }
@@ -585,10 +596,12 @@ int mdsf_stage_times(mdsf_plan* plan, double* avg_ms) {
   const int64_t n = p->timed_frames;
   for (int k = 0; k < 5; ++k) avg_ms[k] = 0.0;
   if (n == 0) return fail(MDSF_ERR_ARG, "no timed mdsf_accumulate_frames call");
-  MDSF_HIP(hipEventSynchronize(p->tev[10 * (n - 1) + 9]));
+  const bool all = p->timing != 2;
+  if (all) MDSF_HIP(hipEventSynchronize(p->tev[10 * (n - 1) + 9]));
   MDSF_HIP(hipEventSynchronize(p->tev[10 * (n - 1) + 3]));
   for (int64_t t = 0; t < n; ++t) {
     for (int k = 0; k < 5; ++k) {
+      if (!all && k != 1) continue;
       float ms = 0.f;
       MDSF_HIP(hipEventElapsedTime(&ms, p->tev[10 * t + 2 * k], p->tev[10 * t + 2 * k + 1]));
       avg_ms[k] += ms;
@@ -609,8 +622,9 @@ int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which) {
 int mdsf_deposit(mdsf_plan* plan, void* d_density, mdsf_stream stream) {
   Plan* p = reinterpret_cast<Plan*>(plan);
   if (!p) return fail(MDSF_ERR_ARG, "null plan");
-  MDSF_HIP(launch_deposit(*p, d_density ? d_density : p->rho,
-                          reinterpret_cast<hipStream_t>(stream)));
+  void* out = d_density ? d_density : p->rho;
+  MDSF_HIP(launch_deposit(*p, out, reinterpret_cast<hipStream_t>(stream)));
+  MDSF_HIP(launch_parked(*p, out, reinterpret_cast<hipStream_t>(stream)));
   return MDSF_OK;
 }
 
@@ -657,6 +671,7 @@ int mdsf_deposit_adjoint(mdsf_plan* plan, const void* d_grad_density, double* d_
   Plan* p = reinterpret_cast<Plan*>(plan);
   if (!p || !d_grad_density) return fail(MDSF_ERR_ARG, "null plan or density gradient");
   if (p->sys.natoms > 0 && !d_grad) return fail(MDSF_ERR_ARG, "null coordinate gradient");
+  if (p->sys.mode == MODE_GEN) return fail(MDSF_ERR_UNSUPPORTED, kNoGenAdjoint);
   int rc = ensure_adjoint(*p);
   if (rc != MDSF_OK) return rc;
   MDSF_HIP(launch_deposit_adjoint(*p, d_grad_density, nullptr, d_grad,
@@ -674,6 +689,7 @@ int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nfra
   if (nframes < 0) return fail(MDSF_ERR_ARG, "negative frame count");
   if (!d_dsf) return fail(MDSF_ERR_ARG, "null dL/dS");
   if (p->sys.natoms > 0 && nframes > 0 && !d_grad) return fail(MDSF_ERR_ARG, "null gradient");
+  if (p->sys.mode == MODE_GEN) return fail(MDSF_ERR_UNSUPPORTED, kNoGenAdjoint);
   if ((rc = ensure_adjoint(*p)) != MDSF_OK) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const long long nxy = (long long)p->sys.n[0] * p->sys.n[1];
@@ -707,6 +723,7 @@ int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nfra
     MDSF_HIP(launch_deposit(*p, p->rho, d));
     MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
+    MDSF_HIP(launch_parked(*p, p->rho, f));
     MDSF_HIP(fft_forward(*p, p->rho, false, f));
     MDSF_HIP(fft_inverse(*p, p->adj_w, w_nyq, p->rho, f));
     MDSF_HIP(launch_deposit_adjoint(*p, p->rho, d_scale + 3 * t, grad_t, f,
@@ -730,6 +747,37 @@ int mdsf_outputs(mdsf_plan* plan, const double* d_sf, const double box[3], int b
   if (!p || !d_sf || !box) return fail(MDSF_ERR_ARG, "null argument");
   MDSF_HIP(launch_outputs(p->sys.n, d_sf, box, box_f32, d_sfplt, d_kgrid, d_kgridplt,
                           reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_rescale(void* d_coords, int dtype, int64_t nframes, int64_t natoms,
+                 const double* d_scale, int arith, mdsf_stream stream) {
+  if (nframes < 0 || natoms < 0) return fail(MDSF_ERR_ARG, "negative frame or atom count");
+  if (natoms > 2147483647LL) return fail(MDSF_ERR_UNSUPPORTED, "more than 2^31 - 1 atoms");
+  if (nframes == 0 || natoms == 0) return MDSF_OK;
+  if (!d_coords || !d_scale) return fail(MDSF_ERR_ARG, "null coordinates or scale");
+  if (dtype != MDSF_F32 && dtype != MDSF_F64) return fail(MDSF_ERR_ARG, "dtype must be F32 or F64");
+  if (arith != MDSF_F32 && arith != MDSF_F64) return fail(MDSF_ERR_ARG, "arith must be F32 or F64");
+  if (dtype == MDSF_F64 && arith == MDSF_F32)
+    return fail(MDSF_ERR_ARG, "float64 coordinates are rescaled in float64");
+  MDSF_HIP(launch_rescale(d_coords, dtype, arith, nframes, (int)natoms, d_scale,
+                          reinterpret_cast<hipStream_t>(stream)));
+  return MDSF_OK;
+}
+
+int mdsf_fold_padded(const double* d_padded, const int32_t n[3], const int32_t nborder[3],
+                     double* d_out, mdsf_stream stream) {
+  if (!n || !nborder) return fail(MDSF_ERR_ARG, "null grid sizes");
+  int nn[3], nb[3];
+  for (int d = 0; d < 3; ++d) {
+    nn[d] = n[d];
+    nb[d] = nborder[d];
+    if (nn[d] < 1 || nb[d] < 0) return fail(MDSF_ERR_ARG, "grid sizes must be positive");
+    if (nb[d] > nn[d])
+      return fail(MDSF_ERR_ARG, "the border exceeds the grid (the fold needs Nborder <= N)");
+  }
+  if (!d_padded || !d_out) return fail(MDSF_ERR_ARG, "null grid");
+  MDSF_HIP(launch_fold_padded(d_padded, nn, nb, d_out, reinterpret_cast<hipStream_t>(stream)));
   return MDSF_OK;
 }
 

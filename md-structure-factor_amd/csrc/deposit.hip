@@ -23,6 +23,7 @@
 // reproducible.  ACC is the arithmetic and output type (float or double).
 #include <type_traits>
 
+#include "clip.h"
 #include "mdsf_internal.h"
 #include "spill.h"
 #include "tiles.h"
@@ -150,7 +151,7 @@ __device__ void stage_diag(StageDiag<ACC>& st, int a, int part, int c, const Sys
                            const void* __restrict__ tables, const Tile& tl) {
   const int4 q = s_ir[c];
   const int t = q.w & 0xffff;
-  const int corner = q.w >> 16;
+  const int corner = (q.w >> 16) & 1;
   const TypeDev& ty = td[t];
   const int slot = c - ty.type_start;
   if (slot < 0 || slot >= ty.count) {  // only after a reported range / type error
@@ -238,7 +239,7 @@ __device__ void stage_xy(StageXY<ACC>& st, int a, int part, int c, const SysDev&
                          const Tile& tl) {
   const int4 q = s_ir[c];
   const int t = q.w & 0xffff;
-  const int corner = q.w >> 16;
+  const int corner = (q.w >> 16) & 1;
   const TypeDev& ty = td[t];
   const int slot = c - ty.type_start;
   if (slot < 0 || slot >= ty.count) {
@@ -531,7 +532,7 @@ __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td
         hit = overlap1(q.x, sh.half[t][0], tl.t0[0], tl.t1[0], sys.n[0]) &&
               overlap1(q.y, sh.half[t][1], tl.t0[1], tl.t1[1], sys.n[1]) &&
               overlap1(q.z, sh.half[t][2], tl.t0[2], tl.t1[2], sys.n[2]);
-        entry = c | ((q.w >> 16) ? (int)0x80000000u : 0);
+        entry = c | (((q.w >> 16) & 1) ? (int)0x80000000u : 0);
       }
       int pos;
       const int nhit = block_scan(sh, hit ? 1 : 0, &pos);
@@ -626,10 +627,129 @@ __global__ __launch_bounds__(kThreads) void k_deposit(SysDev sys, const TypeDev*
   if (gx < sys.n[0] && gy < sys.n[1]) {
     ACC* dst = rho + ((size_t)gx * sys.n[1] + gy) * sys.n[2] + tl.t0[2];
     const int nz = tl.t1[2] - tl.t0[2];
+
     const ACC* sc = sp ? sp + (size_t)(cj * T + ci) * T : nullptr;
 #pragma unroll
     for (int k = 0; k < T; ++k)
       if (k < nz) dst[k] = sc ? acc[k] + sc[k] : acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// General cell (the metric G = U U^T couples z with x or y, dens.py:301 with
+// any ucell): nothing separates, so every image is evaluated per voxel in
+// FP64, amp exp(-(b^T G b) / 2 s^2), b = r - u dr, and rounded once to the
+// plan precision.  Lane = one (x, y) column with 16 z accumulators; the
+// enumeration and visit order are the other kernels'.  Per (x, y) image pair
+// the z sources of tile cell c are u = c - N, c, c + N, except where the
+// corner slot applies (x and y off the core): for y on L the H source is
+// u = c + Nb (u >= N), for y on H the L source is u = c - Nb (u < 0).
+struct StageGen {
+  double r[NB][3];
+  double amp[NB], k2[NB];  // Nel / sigma^3, 1 / (2 sigma^2)
+  int lo[NB][3], len[NB][3];
+};
+
+template <typename ACC>
+__global__ __launch_bounds__(kThreads) void k_deposit_gen(SysDev sys,
+                                                          const TypeDev* __restrict__ td,
+                                                          const int* __restrict__ boff,
+                                                          const int4* __restrict__ s_ir,
+                                                          const double* __restrict__ s_r,
+                                                          const ACC* __restrict__ spill,
+                                                          const int* __restrict__ spill_flag,
+                                                          ACC* __restrict__ rho) {
+  __shared__ StageGen st;
+  __shared__ EnumShared<kMaxTypes> sh;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  Tile tl;
+  tile_coords(sys, tl);
+  const int ci = (wave & 1) * 8 + (lane & 7);
+  const int cj = (wave >> 1) * 8 + (lane >> 3);
+  const int gx = tl.t0[0] + ci, gy = tl.t0[1] + cj;
+  const int nz = tl.t1[2] - tl.t0[2];
+  const int N0 = sys.n[0], N1 = sys.n[1], N2 = sys.n[2], nb = sys.nb;
+  const double g00 = sys.g00, g11 = sys.g11, g22 = sys.g22;
+  const double g01x2 = 2.0 * sys.g01, g02x2 = 2.0 * sys.g02, g12x2 = 2.0 * sys.g12;
+  double acc[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) acc[k] = 0.0;
+
+  auto process = [&](int count, int head) -> int {
+    const int nbatch = min(count, NB);
+    __syncthreads();
+    if (tid < nbatch) {
+      const int c = sh.list[(head + tid) & kListMask] & 0x7fffffff;
+      const int4 q = s_ir[c];
+      const TypeDev& ty = td[q.w & 0xffff];
+      const int slot = c - ty.type_start;
+      const bool ok = slot >= 0 && slot < ty.count;  // false only after a reported error
+      const int ir[3] = {q.x, q.y, q.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        st.r[tid][d] = s_r[3 * (size_t)c + d];
+        st.lo[tid][d] = ir[d] - ty.half[d];
+        st.len[tid][d] = ok ? 2 * ty.half[d] : 0;
+      }
+      st.amp[tid] = ty.amp;
+      st.k2[tid] = 1.0 / ty.two_sig2;
+    }
+    __syncthreads();
+    if (gx < N0 && gy < N1) {
+      for (int a = 0; a < nbatch; ++a) {
+        const int lox = st.lo[a][0], loy = st.lo[a][1], loz = st.lo[a][2];
+        const int lnx = st.len[a][0], lny = st.len[a][1], lnz = st.len[a][2];
+        const double rx = st.r[a][0], ry = st.r[a][1], rz = st.r[a][2];
+        const double amp = st.amp[a], k2 = st.k2[a];
+#pragma unroll 1
+        for (int sx = 0; sx < 3; ++sx) {
+          const int ux = gx + (sx - 1) * N0;
+          if (ux < lox || ux >= lox + lnx) continue;
+          const double bx = rx - (double)ux * sys.dr[0];
+#pragma unroll 1
+          for (int sy = 0; sy < 3; ++sy) {
+            const int uy = gy + (sy - 1) * N1;
+            if (uy < loy || uy >= loy + lny) continue;
+            const double by = ry - (double)uy * sys.dr[1];
+            const double qxy = g00 * bx * bx + g01x2 * bx * by + g11 * by * by;
+            const double cz = g02x2 * bx + g12x2 * by;
+            const bool slot_rule = sx != SIDE_C && sy != SIDE_C;
+            // z offsets of the L and H sources (the C source is u = c)
+            const int offL = (slot_rule && sy == SIDE_H) ? -nb : -N2;
+            const int offH = (slot_rule && sy == SIDE_L) ? nb : N2;
+#pragma unroll
+            for (int k = 0; k < T; ++k) {
+              if (k >= nz) continue;
+              const int c = tl.t0[2] + k;
+              const int us[3] = {c + offL, c, c + offH};
+#pragma unroll
+              for (int j = 0; j < 3; ++j) {
+                const int uz = us[j];
+                // the moved slots take only off-core sources (u < 0 / u >= N)
+                const bool side_ok = j == 1 || (j == 0 ? uz < 0 : uz >= N2);
+                if (!side_ok || uz < loz || uz >= loz + lnz) continue;
+                const double bz = rz - (double)uz * sys.dr[2];
+                const double qv = qxy + g22 * bz * bz + cz * bz;
+                acc[k] += amp * exp(-qv * k2);
+              }
+            }
+          }
+        }
+      }
+    }
+    return nbatch;
+  };
+  enumerate_tile(sys, td, boff, s_ir, tl, sh, NB, process);
+
+  const ACC* sp = spill_stage_tile(sys, spill, spill_flag, tl);
+  if (gx < N0 && gy < N1) {
+    ACC* dst = rho + ((size_t)gx * N1 + gy) * N2 + tl.t0[2];
+
+    const ACC* sc = sp ? sp + (size_t)(cj * T + ci) * T : nullptr;
+#pragma unroll
+    for (int k = 0; k < T; ++k)
+      if (k < nz) dst[k] = sc ? (ACC)acc[k] + sc[k] : (ACC)acc[k];
   }
 }
 
@@ -823,7 +943,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
       // lane, all waves); the last group's padding is zeroed alongside
       for (int it = tid; it < nreal * 12; it += kThreads) {
         const int a = it / 12, r12 = it - 12 * (it / 12), d = r12 >> 2, q4 = r12 & 3;
-        if ((bt.q[a].w >> 16) != 0) continue;  // corner atom: evaluated below
+        if (((bt.q[a].w >> 16) & 1) != 0) continue;  // corner atom: evaluated below
         const int off = bt.so[a][d];
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (off >= 0) v = *reinterpret_cast<const float4*>(slices + off + 4 * q4);
@@ -877,7 +997,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
         const int vb = bt.vbase[a];
         const int4 q = bt.q[a];
         const int t = q.w & 0xffff;
-        const bool corner = (q.w >> 16) != 0;
+        const bool corner = ((q.w >> 16) & 1) != 0;
         const bool valid = act && bt.ok[a];
         const int nv = corner ? EXTRA + 1 : 1;
         bool hC = false, hN = false;
@@ -1003,7 +1123,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
           const int vb = bt.vbase[a];
           const int4 q = bt.q[a];
           const TypeDev& ty = td[q.w & 0xffff];
-          const bool corner = (q.w >> 16) != 0;
+          const bool corner = ((q.w >> 16) & 1) != 0;
           st.amp[vb] = ty.amp_f;
           st.k2[vb] = ty.c2;
           const int nv = corner ? EXTRA + 1 : 1;
@@ -1122,6 +1242,7 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
   // D[y][z] of block b: lane l holds y = 4 (l / 16) + r, z = l % 16
   const int gz = tl.t0[2] + (lane & 15);
   const float* sp = spill_stage_tile(sys, spill, spill_flag, tl);
+
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
     const int gx = tl.t0[0] + 4 * wave + b;
@@ -1276,7 +1397,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     // tile slices (float4 per thread)
     for (int it = tid; it < nreal * 12; it += kThreads) {
       const int a = it / 12, r12 = it - 12 * (it / 12), d = r12 >> 2, qq = r12 & 3;
-      if ((bt.q[a].w >> 16) != 0) continue;  // corner atom: below
+      if (((bt.q[a].w >> 16) & 1) != 0) continue;  // corner atom: below
       const int off = bt.so[a][d];
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f), w = v;
       if (off >= 0) {
@@ -1482,8 +1603,16 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
         const ACC exy = (ACC)ty.amp *
                         spill_exp<ACC>((sys.g00 * bx * bx + 2.0 * sys.g01 * bx * by + sys.g11 * by * by) /
                                        ty.two_sig2);
+        // general cells: the z couplings add exp(-2 (g02 bx + g12 by) bz / 2 s^2)
+        const double cz = sys.mode == MODE_GEN ? 2.0 * (sys.g02 * bx + sys.g12 * by) : 0.0;
 #pragma unroll
-        for (int k = 0; k < T; ++k) acc[k] += exy * (ez ? ez[k] : zfactor(q, ty, r[2], k));
+        for (int k = 0; k < T; ++k) {
+          ACC f = ez ? ez[k] : zfactor(q, ty, r[2], k);
+          if (cz != 0.0)
+            f *= spill_exp<ACC>(cz * (r[2] - (double)(dst.t0[2] + k + shift) * sys.dr[2]) /
+                                ty.two_sig2);
+          acc[k] += exy * f;
+        }
       }
     };
     // One step over up to kSpillChunk candidates (lanes below kSpillChunk hold
@@ -1557,6 +1686,37 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
 }
 
 
+// ---------------------------------------------------------------------------
+// Parked atoms (windows that leave the padded grid, clipped as the
+// reference's numpy slices clip them; clip.h): a block per parked atom adds
+// its clipped window through the fold's closed form, after the tiled
+// deposition, with atomics (only parked atoms take this path, so the order
+// of their few additions is the one thing left unfixed).  Blocks of types
+// without parked atoms return at once.
+template <typename ACC>
+__global__ __launch_bounds__(kThreads) void k_parked(SysDev sys, const TypeDev* __restrict__ td,
+                                                     const int* __restrict__ boff,
+                                                     const int4* __restrict__ s_ir,
+                                                     const double* __restrict__ s_r,
+                                                     ACC* __restrict__ rho) {
+  const TypeDev& ty = td[blockIdx.y];
+  const int first = boff[ty.park], last = boff[ty.park + 1];
+  for (int s = first + (int)blockIdx.x; s < last; s += (int)gridDim.x) {
+    const ClipWin w = clip_window(sys, ty, s_ir[s]);
+    if (w.empty) continue;  // uniform over the block
+    const double rx = s_r[3 * (size_t)s + 0], ry = s_r[3 * (size_t)s + 1],
+                 rz = s_r[3 * (size_t)s + 2];
+    const long long total = w.n0 * w.n1 * w.n2;
+    for (long long i = threadIdx.x; i < total; i += kThreads) {
+      const long long px = w.lo0 + i / (w.n1 * w.n2), py = w.lo1 + (i / w.n2) % w.n1,
+                      pz = w.lo2 + i % w.n2;
+      double g0, g1, g2;
+      const double v = clip_value(sys, ty, rx, ry, rz, px, py, pz, g0, g1, g2);
+      if (v != 0.0) atomicAdd(&rho[clip_dest(sys, px, py, pz)], (ACC)v);
+    }
+  }
+}
+
 // Corner-fold spill grid: staged into the slot's tiles (stage) or added to the
 // density after the tiled deposition (stage null, out the density).
 template <typename ACC>
@@ -1575,7 +1735,10 @@ hipError_t launch_impl(Plan& p, ACC* out, const ACC* stage, hipStream_t s) {
   const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
   if (ntiles == 0) return hipSuccess;
   const dim3 grid((unsigned)ntiles), block(kThreads);
-  if constexpr (std::is_same<ACC, float>::value) {
+  if (sys.mode == MODE_GEN) {
+    hipLaunchKernelGGL((k_deposit_gen<ACC>), grid, block, 0, s, sys, p.types, p.bucket_off,
+                       p.s_ir, p.s_r, stage, p.spill_flag, out);
+  } else if constexpr (std::is_same<ACC, float>::value) {
     // blocks size their per-type LDS for 16 types when the system has no more
     const bool few = sys.ntypes <= 16;
     if (sys.mode == MODE_DIAG && few)
@@ -1617,6 +1780,19 @@ hipError_t launch_deposit(Plan& p, void* out, hipStream_t s) {
   const float* st = static_cast<const float*>(p.spill);
   if ((e = launch_impl<float>(p, o, st, s)) != hipSuccess || st) return e;
   return launch_spill<float>(p, o, nullptr, s);
+}
+
+hipError_t launch_parked(Plan& p, void* out, hipStream_t s) {
+  const SysDev& sys = p.sys;
+  if (sys.natoms == 0 || sys.ntypes == 0) return hipSuccess;
+  const dim3 grid(8, (unsigned)sys.ntypes), block(kThreads);
+  if (p.fft64)
+    hipLaunchKernelGGL(k_parked<double>, grid, block, 0, s, sys, p.types, p.bucket_off, p.s_ir,
+                       p.s_r, static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL(k_parked<float>, grid, block, 0, s, sys, p.types, p.bucket_off, p.s_ir,
+                       p.s_r, static_cast<float*>(out));
+  return hipGetLastError();
 }
 
 hipError_t launch_spill_stage(Plan& p, hipStream_t s) {

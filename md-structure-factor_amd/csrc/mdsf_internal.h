@@ -20,7 +20,29 @@ constexpr int kMaxSpillBufTiles = 4096;  // spill tiles staged per frame slot (e
 constexpr int kCornerCap = 512;     // corner atoms listed per frame for the spill pass
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
-enum Mode { MODE_DIAG = 0, MODE_XY = 1 };
+// Cell metric G = U U^T of the plan: diagonal, xy-coupled (g01 != 0), or
+// general (z coupled to x or y: g02 or g12 != 0).
+enum Mode { MODE_DIAG = 0, MODE_XY = 1, MODE_GEN = 2 };
+
+// Flags in the w component of an atom's int4 (home cell x, y, z; w = type |
+// flags): the atom is a corner atom (its window leaves the core on all three
+// axes), or it is parked -- its window leaves the padded grid on some axis,
+// so the tiled passes skip it and the clipped-window passes handle it.
+constexpr int kAtomCorner = 1 << 16;
+constexpr int kAtomParked = 1 << 17;
+
+// numpy's slice [i0:i1] of an axis of length len: negative bounds count from
+// the end, then both clamp to [0, len]; empty when *s1 <= *s0.  The reference
+// slices its padded grids this way (dens.py:292-308).
+__host__ __device__ inline void numpy_slice(long long i0, long long i1, long long len,
+                                            long long* s0, long long* s1) {
+  if (i0 < 0) i0 = i0 + len < 0 ? 0 : i0 + len;
+  else if (i0 > len) i0 = len;
+  if (i1 < 0) i1 = i1 + len < 0 ? 0 : i1 + len;
+  else if (i1 > len) i1 = len;
+  *s0 = i0;
+  *s1 = i1;
+}
 
 // Error bits raised by kernels into the plan's device flag word.
 enum ErrBit { ERRBIT_RANGE = 1, ERRBIT_TYPE = 2 };
@@ -39,6 +61,7 @@ struct TypeDev {
   int bsz[3];        // bucket edge per axis
   int nbk[3];        // buckets per axis
   int bucket_base;   // first bucket id of this type
+  int park;          // bucket id of its parked atoms (after its grid buckets)
   // tile slices (FP32 DIAG deposition): per atom and axis nsl[d] slices of
   // kTile floats, the image-summed 1-D factor over one tile along that axis
   int nsl[3];
@@ -65,7 +88,8 @@ struct SysDev {
   int n[3];
   int nb;
   double dr[3];
-  double g00, g01, g11, g22;  // cell metric G = U U^T (no xz / yz coupling)
+  double g00, g01, g11, g22;  // cell metric G = U U^T
+  double g02, g12;            // its z couplings (nonzero only in MODE_GEN)
   int ntypes;
   int natoms;
   int nbuckets;
@@ -193,7 +217,7 @@ struct Plan {
   hipStream_t dep_stream = nullptr, fft_stream = nullptr;
   // optional stage timing of mdsf_accumulate_frames: 10 timing events per
   // frame (prepare, deposit, z, y, x: start / end, each on its stream)
-  bool timing = false;
+  int timing = 0;  // 0 off, 1 every stage, 2 the deposition only
   std::vector<hipEvent_t> tev;
   int64_t timed_frames = 0;
   hipEvent_t ev_fork = nullptr;
@@ -229,6 +253,9 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false);
 // only the slice derivatives of the last sorted frame
 hipError_t launch_slice_derivs(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);
+// the clipped windows of the frame's parked atoms (windows leaving the padded
+// grid), added into the deposited density `out`; after launch_deposit
+hipError_t launch_parked(Plan& p, void* out, hipStream_t s);
 // corner-fold spill of the sorted frame into the slot's staging tiles (when the
 // plan stages them; launch_deposit adds them as it stores the tiles)
 hipError_t launch_spill_stage(Plan& p, hipStream_t s);
@@ -264,6 +291,12 @@ hipError_t launch_deposit_adjoint(Plan& p, const void* grho, const double* scale
                                   bool grad_zeroed = false);
 
 hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s);
+// fold of a padded FP64 grid (n + 2 nb per axis) onto n (remap_grid_tcl)
+hipError_t launch_fold_padded(const double* d0, const int n[3], const int nb[3], double* d1,
+                              hipStream_t s);
+// in-place rescale of `nframes` frames of `natoms` coordinates (no wrap)
+hipError_t launch_rescale(void* coords, int dtype, int arith, long long nframes, int natoms,
+                          const double* scale, hipStream_t s);
 hipError_t launch_outputs(const int n[3], const double* sf, const double box[3], int box_f32,
                           double* sfplt, double* kgrid, double* kgridplt, hipStream_t s);
 hipError_t launch_ewald(const double* kp, int nx, int ny, int nz, double wl, double* xy,

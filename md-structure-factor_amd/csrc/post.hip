@@ -3,6 +3,7 @@
 //   sfplt = get_dplot(sf)                      dens.py:142-163,323
 //   kgrid, kgridplt                            dens.py:325-344
 //   Ewald-sphere sections of kgridplt          plot2d.py:218-263
+//   fold of a padded grid (remap_grid[_tcl])   dens.py:65-108
 #include <cmath>
 
 #include "mdsf_internal.h"
@@ -228,7 +229,78 @@ unsigned grid_for(long long total) {
   return (unsigned)b;
 }
 
+// Fold of a padded FP64 grid onto the periodic cell with the reference's
+// tables des = [0, Nb, N - Nb, N], ori = [0, Nb, Nb + N, 2 Nb + N] per axis
+// (dens.py:86-108; remap_grid, dens.py:65-83, is the same with one table for
+// every axis).  One thread per destination cell adds, in the reference's
+// order of slab additions, the sources that land on it: the core copy, the 6
+// sides (z, y, x normal for the L side, then for the H side inside the
+// same io1 loop), the 12 edges and the 8 corners, the corners' z destination
+// taken from the y side's range (dens.py:107).  Same operations in the same
+// order, so the sums are bitwise the reference's.
+struct FoldAxis {
+  int n, nb;
+};
+
+// Source index on one axis for destination c and source side io (0 = L,
+// 2 = H), or -1 when c lies outside that side's destination range
+// des[2 - io] .. des[3 - io].
+__device__ __forceinline__ int fold_src(int c, int io, const FoldAxis& a) {
+  if (io == 0) return c >= a.n - a.nb ? c - (a.n - a.nb) : -1;   // ori [0, Nb) -> des [N-Nb, N)
+  return c < a.nb ? a.nb + a.n + c : -1;                          // ori [Nb+N, 2Nb+N) -> des [0, Nb)
+}
+
+__global__ void k_fold_padded(const double* __restrict__ d0, double* __restrict__ d1,
+                              FoldAxis ax, FoldAxis ay, FoldAxis az) {
+#pragma clang fp contract(off)
+  const long long total = (long long)ax.n * ay.n * az.n;
+  const long long py = ay.n + 2 * ay.nb, pz = az.n + 2 * az.nb;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int cz = (int)(i % az.n);
+    const int cy = (int)((i / az.n) % ay.n);
+    const int cx = (int)(i / ((long long)az.n * ay.n));
+    auto at = [&](int px, int qy, int qz) { return d0[((long long)px * py + qy) * pz + qz]; };
+    const int kx = cx + ax.nb, ky = cy + ay.nb, kz = cz + az.nb;   // core sources
+    double v = at(kx, ky, kz);
+    for (int io1 = 0; io1 < 4; io1 += 2) {
+      const int sx1 = fold_src(cx, io1, ax), sy1 = fold_src(cy, io1, ay),
+                sz1 = fold_src(cz, io1, az);
+      if (sz1 >= 0) v += at(kx, ky, sz1);   // sides normal to z
+      if (sy1 >= 0) v += at(kx, sy1, kz);   // sides normal to y
+      if (sx1 >= 0) v += at(sx1, ky, kz);   // sides normal to x
+      for (int io2 = 0; io2 < 4; io2 += 2) {
+        const int sy2 = fold_src(cy, io2, ay), sz2 = fold_src(cz, io2, az);
+        if (sy1 >= 0 && sz2 >= 0) v += at(kx, sy1, sz2);   // edges parallel to x
+        if (sx1 >= 0 && sz2 >= 0) v += at(sx1, ky, sz2);   // edges parallel to y
+        if (sx1 >= 0 && sy2 >= 0) v += at(sx1, sy2, kz);   // edges parallel to z
+        // corners: z destination range of the y side io2, z source side io3
+        const bool zdst = io2 == 0 ? cz >= az.n - az.nb : cz < az.nb;
+        if (sx1 >= 0 && sy2 >= 0 && zdst) {
+          const int rel = io2 == 0 ? cz - (az.n - az.nb) : cz;   // offset in that range
+          for (int io3 = 0; io3 < 4; io3 += 2) {
+            const int sz3 = io3 == 0 ? rel : az.nb + az.n + rel;
+            v += at(sx1, sy2, sz3);
+          }
+        }
+      }
+    }
+    d1[i] = v;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_fold_padded(const double* d0, const int n[3], const int nb[3], double* d1,
+                              hipStream_t s) {
+  const long long total = (long long)n[0] * n[1] * n[2];
+  if (total == 0) return hipSuccess;
+  long long blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fold_padded, dim3((unsigned)blocks), dim3(256), 0, s, d0, d1,
+                     FoldAxis{n[0], nb[0]}, FoldAxis{n[1], nb[1]}, FoldAxis{n[2], nb[2]});
+  return hipGetLastError();
+}
 
 hipError_t launch_sf_read(const Plan& p, double* sf, hipStream_t s) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;

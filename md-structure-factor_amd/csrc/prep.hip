@@ -8,6 +8,8 @@
 //   home cell      dens.py:285     ir = int(r / dr)   (FP64 IEEE division, truncation)
 //   window         dens.py:287-297 padded indices [ir-A+Nb, ir+A+Nb)
 //   Gaussian       dens.py:299-308 (Nel/sig^3) exp(-|(r - x_v) ucell|^2 / (2 sig^2))
+#include <climits>
+
 #include "mdsf_internal.h"
 #include "spill.h"
 
@@ -66,38 +68,45 @@ __global__ void k_prep(T* __restrict__ coords, SysDev sys, const TypeDev* __rest
   }
   const TypeDev& ty = td[t];
   int ir[3], h[3];
-  bool ok = true;
+  bool finite = true, parked = false;
   bool nonc[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double q = (double)x[d] / sys.dr[d];
-    if (!(q > -1.0e9 && q < 1.0e9)) {
-      ok = false;
-      ir[d] = 0;
-    } else {
+    if (q > -1.0e9 && q < 1.0e9) {
       ir[d] = (int)q;  // truncation toward zero, as ndarray.astype(int)
+    } else {
+      // non-finite or beyond any window: the reference's slices come out
+      // empty; this home cell makes the recomputed slice empty too
+      finite = false;
+      ir[d] = -(1 << 30);
     }
+    // The reference adds the window into d0[x0:x1] on each axis, padded
+    // indices x0 = ir - A + Nb, x1 = ir + A + Nb of a grid of len = N + 2 Nb,
+    // at grid[x0:x1] (dens.py:292-308).  Both are the same numpy slice, so a
+    // window that leaves the padded grid is clipped (negative bounds count
+    // from the end, then clamp), never an error.  Such atoms are parked:
+    // their own bucket, which the tiled passes never enumerate, and the
+    // clipped-window passes add them (or nothing, when a slice is empty).
     const int A = ty.half[d];
-    if (ir[d] - A < -sys.nb || ir[d] + A > sys.n[d] + sys.nb) ok = false;
+    const long long len = (long long)sys.n[d] + 2LL * sys.nb;
+    const long long x0 = (long long)ir[d] - A + sys.nb, x1 = (long long)ir[d] + A + sys.nb;
+    if (A == 0 || x0 < 0 || x1 > len) parked = true;
     nonc[d] = A > 0 && (ir[d] - A < 0 || ir[d] + A > sys.n[d]);
     h[d] = pos_mod(ir[d], sys.n[d]);
-    if (home) home[3 * (size_t)i + d] = ir[d];
+    if (home) home[3 * (size_t)i + d] = finite ? ir[d] : INT_MIN;
   }
-  if (!ok) {
-    atomicOr(err, ERRBIT_RANGE);
-    key[i] = -1;
-    return;
-  }
-  const int corner = (nonc[0] && nonc[1] && nonc[2]) ? 1 : 0;
-  const int b = ty.bucket_base +
-                ((h[0] / ty.bsz[0]) * ty.nbk[1] + h[1] / ty.bsz[1]) * ty.nbk[2] + h[2] / ty.bsz[2];
+  const int corner = (!parked && nonc[0] && nonc[1] && nonc[2]) ? kAtomCorner : 0;
+  const int b = parked ? ty.park
+                       : ty.bucket_base + ((h[0] / ty.bsz[0]) * ty.nbk[1] + h[1] / ty.bsz[1]) *
+                                              ty.nbk[2] + h[2] / ty.bsz[2];
   key[i] = b;
-  ir4[i] = make_int4(ir[0], ir[1], ir[2], t | (corner << 16));
+  ir4[i] = make_int4(ir[0], ir[1], ir[2], t | corner | (parked ? kAtomParked : 0));
   rpos[3 * (size_t)i + 0] = (double)x[0];
   rpos[3 * (size_t)i + 1] = (double)x[1];
   rpos[3 * (size_t)i + 2] = (double)x[2];
   atomicAdd(&bcount[b], 1);
-  if (occ) atomicAdd(&occ[((size_t)h[0] * sys.n[1] + h[1]) * sys.n[2] + h[2]], 1u);
+  if (occ && finite) atomicAdd(&occ[((size_t)h[0] * sys.n[1] + h[1]) * sys.n[2] + h[2]], 1u);
 }
 
 // Exclusive scan of the bucket counts in two coalesced passes: per-block
@@ -203,7 +212,7 @@ __global__ void k_gather(const int* __restrict__ s_idx, const int4* __restrict__
   const int i = s_idx[s];
   const int4 q = ir4[i];
   s_ir[s] = q;
-  if (q.w >> 16) {
+  if ((q.w >> 16) & 1) {
     // corner atom: listed in arrival order; k_corner_order sorts the list
     const int k = atomicAdd(corners, 1);
     if (k < kCornerCap) corners[1 + k] = s;
@@ -278,6 +287,7 @@ __global__ void k_tables(SysDev sys, const TypeDev* __restrict__ td, const int4*
   const int e = (int)(local - (long long)slot * ty.stride);
   const int s = ty.type_start + slot;
   const int4 q = s_ir[s];
+  if (q.w & kAtomParked) return;  // parked atoms have no tables
   const int len0 = 2 * ty.half[0], len1 = 2 * ty.half[1];
   int d, k;
   if (e < len0) {
@@ -322,7 +332,7 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
   if (s >= *nvalid) return;
   const int4 q = s_ir[s];
   const int t = q.w & 0xffff;
-  if (t >= sys.ntypes) return;
+  if (t >= sys.ntypes || (q.w & kAtomParked)) return;  // parked atoms have no slices
   const TypeDev& ty = td[t];
   const int slot = s - ty.type_start;
   if (slot < 0 || slot >= ty.count) return;  // only after a reported error
@@ -339,7 +349,7 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
   const int lo = ir - A, hi = ir + A;
   const int off_d = d == 0 ? 0 : (d == 1 ? kTile * ty.nsl[0] : kTile * (ty.nsl[0] + ty.nsl[1]));
   const long long aoff = ty.slice_base + (long long)slot * ty.slice_stride + off_d + i;
-  const bool corner = (q.w >> 16) != 0;
+  const bool corner = ((q.w >> 16) & 1) != 0;
   const int coff_d = d == 0 ? 0 : (d == 1 ? 2 * kTile * ty.nsl[0]
                                           : 2 * kTile * ty.nsl[0] + 3 * kTile * ty.nsl[1]);
   const long long acoff = ty.corner_base + (long long)slot * ty.corner_stride + coff_d + i;
@@ -412,6 +422,32 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
     hipLaunchKernelGGL((k_prep<double, double>), grid, block, 0, s, (double*)coords, p.sys,
                        p.types, types, scale3, box[0], box[1], box[2], (long long)wrap_lo,
                        (long long)wrap_hi, p.ir4, p.key, p.rpos, p.bucket_count, occ, home, p.err, co, wb);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rescale(void* coords, int dtype, int arith, long long nframes, int natoms,
+                          const double* scale, hipStream_t s) {
+  if (natoms == 0 || nframes == 0) return hipSuccess;
+  SysDev sys{};
+  sys.natoms = natoms;
+  const dim3 grid((natoms + 255) / 256), block(256);
+  const size_t fstride = (size_t)natoms * 3;
+  for (long long t = 0; t < nframes; ++t) {
+    // wrap range [0, 0): the coordinates are only rescaled (dens.py:54-58)
+    if (dtype == MDSF_F32 && arith == MDSF_F32)
+      hipLaunchKernelGGL((k_prep<float, float>), grid, block, 0, s, (float*)coords + t * fstride,
+                         sys, nullptr, nullptr, scale + 3 * t, 0.0, 0.0, 0.0, 0LL, 0LL, nullptr,
+                         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1);
+    else if (dtype == MDSF_F32)
+      hipLaunchKernelGGL((k_prep<float, double>), grid, block, 0, s, (float*)coords + t * fstride,
+                         sys, nullptr, nullptr, scale + 3 * t, 0.0, 0.0, 0.0, 0LL, 0LL, nullptr,
+                         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1);
+    else
+      hipLaunchKernelGGL((k_prep<double, double>), grid, block, 0, s,
+                         (double*)coords + t * fstride, sys, nullptr, nullptr, scale + 3 * t, 0.0,
+                         0.0, 0.0, 0LL, 0LL, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, 1, 1);
   }
   return hipGetLastError();
 }

@@ -59,6 +59,58 @@ def get_borders(ad, dr, keys):
     return engine.borders(ad, dr, keys)
 
 
+def rescale(coords, dims):
+    """Rescale ``coords`` (frames, atoms, 3) in place so that every frame has the
+    mean box; returns ``(coords, avgdims)`` (dens.py:46-62).  The products run
+    on the GPU in numpy's arithmetic (float32 only when coords and dims are)."""
+    import torch
+    if coords.dtype not in (np.float32, np.float64):
+        raise TypeError("coordinates must be float32 or float64")
+    avgdims = np.average(dims, axis=0)
+    a = avgdims / dims
+    arith = engine._lib.F32 if (coords.dtype == np.float32 and a.dtype == np.float32) \
+        else engine._lib.F64
+    dev = engine._device(None)
+    c = torch.from_numpy(np.ascontiguousarray(coords)).to(dev)
+    sc = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    engine.rescale_device(c, sc, arith)
+    coords[...] = c.cpu().numpy()
+    return coords, avgdims
+
+
+def _fold_tables(des, ori, ndim):
+    """(N, Nborder) per axis from the reference's fold tables, which compute_sf
+    builds as des = [0, Nb, N - Nb, N], ori = [0, Nb, Nb + N, 2 Nb + N]
+    (dens.py:258-263); other tables are rejected."""
+    n, nb = [], []
+    for d in range(ndim):
+        de = [int(v) for v in np.asarray(des[d]).reshape(-1)]
+        oi = [int(v) for v in np.asarray(ori[d]).reshape(-1)]
+        ok = len(de) == 4 and len(oi) == 4
+        b, nn = (oi[1], oi[2] - oi[1]) if ok else (0, 0)
+        if not ok or de != [0, b, nn - b, nn] or oi != [0, b, b + nn, 2 * b + nn]:
+            raise ValueError("only the fold tables compute_sf builds are supported: "
+                             "des = [0, Nb, N - Nb, N], ori = [0, Nb, Nb + N, 2 Nb + N]")
+        n.append(nn)
+        nb.append(b)
+    return n, nb
+
+
+def remap_grid_tcl(d0, des, ori):
+    """Fold the padded grid ``d0`` onto the periodic cell with per-axis tables
+    (dens.py:86-108, corner z slot included), on the GPU; returns FP64."""
+    import torch
+    n, nb = _fold_tables(des, ori, 3)
+    dev = engine._device(None)
+    d = torch.from_numpy(np.ascontiguousarray(d0, dtype=np.float64)).to(dev)
+    return engine.fold_padded(d, n, nb).cpu().numpy()
+
+
+def remap_grid(d0, des, ori):
+    """The cubic form of remap_grid_tcl: one table for every axis (dens.py:65-83)."""
+    return remap_grid_tcl(d0, [des] * 3, [ori] * 3)
+
+
 def get_dplot(dmag):
     """Half spectrum -> centred plottable grid (dens.py:142-163), on the GPU."""
     import torch
@@ -127,7 +179,7 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
         plan.rescale_wrap(coords, scale, box, s.arith, s.wrap)
         for _ in range(s.nframes):
             d1 = np.random.rand(int(s.N[0]), int(s.N[1]), int(s.N[2]))   # dens.py:280
-            plan.accumulate_density(torch.from_numpy(d1.astype(np.float32)))
+            plan.accumulate_density(torch.from_numpy(d1))   # cast to the plan precision
     else:
         types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
         plan.accumulate_frames(coords, types, scale, box, s.arith, s.wrap)

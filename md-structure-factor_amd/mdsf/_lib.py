@@ -54,6 +54,10 @@ SIGNATURES = {
     "mdsf_rescale_wrap": (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_void_p,
                                     ctypes.POINTER(c_double), c_int32, c_int64, c_int64,
                                     c_void_p]),
+    "mdsf_rescale": (c_int32, [c_void_p, c_int32, c_int64, c_int64, c_void_p, c_int32,
+                               c_void_p]),
+    "mdsf_fold_padded": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
+                                   c_void_p, c_void_p]),
     "mdsf_deposit_fold": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p,
                                     ctypes.POINTER(c_double), c_int32, c_int64, c_int64,
                                     c_void_p, c_void_p, c_void_p]),

@@ -239,7 +239,9 @@ class Plan:
             self._ptr(home), self._stream()))
 
     def set_stage_timing(self, enable=True):
-        _lib.check(self.lib.mdsf_set_stage_timing(self._handle, 1 if enable else 0))
+        """True / 1: events around every stage; 2: around the deposition only; False: off."""
+        mode = int(enable) if not isinstance(enable, bool) else (1 if enable else 0)
+        _lib.check(self.lib.mdsf_set_stage_timing(self._handle, mode))
 
     def stage_times(self):
         """Average ms per frame of the last timed accumulate_frames call:
@@ -386,6 +388,34 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def rescale_device(coords, scale, arith):
+    """In-place rescale (dens.py:54-58) of device coordinates (F, natoms, 3) by
+    device per-frame factors ``scale`` (F, 3) FP64, in numpy's arithmetic ``arith``."""
+    lib = _lib.load()
+    dtype = _lib.F32 if coords.dtype == torch.float32 else _lib.F64
+    s = _lib.c_void_p(torch.cuda.current_stream(coords.device).cuda_stream)
+    _lib.check(lib.mdsf_rescale(_lib.c_void_p(coords.data_ptr()), dtype, int(coords.shape[0]),
+                                int(coords.shape[1]), _lib.c_void_p(scale.data_ptr()), int(arith),
+                                s))
+
+
+def fold_padded(d0, n, nborder):
+    """Fold of a padded FP64 device grid (n + 2 nborder per axis) onto the
+    periodic cell n (remap_grid_tcl, dens.py:86-108, corner z slot included)."""
+    lib = _lib.load()
+    d = d0.to(dtype=torch.float64).contiguous()
+    n = [int(x) for x in n]
+    nb = [int(x) for x in nborder]
+    if list(d.shape) != [n[i] + 2 * nb[i] for i in range(3)]:
+        raise ValueError(f"padded grid shape {tuple(d.shape)} does not match N={n}, "
+                         f"Nborder={nb}")
+    out = torch.empty(n, dtype=torch.float64, device=d.device)
+    s = _lib.c_void_p(torch.cuda.current_stream(d.device).cuda_stream)
+    _lib.check(lib.mdsf_fold_padded(_lib.c_void_p(d.data_ptr()), (_lib.c_int32 * 3)(*n),
+                                    (_lib.c_int32 * 3)(*nb), _lib.c_void_p(out.data_ptr()), s))
+    return out
 
 
 def ewald_slices(kgridplt, wavelength):

@@ -115,6 +115,12 @@ def wrap_atom_range(nframes, natoms):
 
 
 # ----------------------------------------------------------------- deposition
+def home_cells(rframe, dr):
+    """Integer home cell of every atom, ``ir = (r / dr).astype(int)`` (dens.py:285):
+    FP64 division (``dr`` is FP64), truncation toward zero."""
+    return (rframe / dr).astype(int)
+
+
 def deposit_frame(rframe, typ, rad, bdict, dr, nb, ucell, shape_padded, out=None):
     """Sum of truncated Gaussians on the padded FP64 grid for one frame (dens.py:283-309).
 
@@ -123,18 +129,21 @@ def deposit_frame(rframe, typ, rad, bdict, dr, nb, ucell, shape_padded, out=None
     Returns the padded grid and the per-atom integer home cells ``ir``.
     """
     d0 = np.zeros(shape_padded) if out is None else out
-    irs = np.zeros((rframe.shape[0], 3), dtype=np.int64)
+    irs = home_cells(rframe, dr)
+    # padded indices per axis; the reference slices grid and d0 with the same
+    # [i0:i1] (dens.py:292-308), so numpy's slice rules (negative indices
+    # count from the end, bounds clamp) pick the same cells of both
+    pidx = [np.arange(int(shape_padded[a])) for a in range(3)]
     for im in range(rframe.shape[0]):
-        ir = (rframe[im, :] / dr).astype(int)
-        irs[im] = ir
+        ir = irs[im]
         ax, ay, az = bdict[typ[im]].astype(int)
         x0, x1 = ir[0] - ax + nb, ir[0] + ax + nb
         y0, y1 = ir[1] - ay + nb, ir[1] + ay + nb
         z0, z1 = ir[2] - az + nb, ir[2] + az + nb
-        gx = (np.arange(x0, x1) - nb) * dr[0]
-        gy = (np.arange(y0, y1) - nb) * dr[1]
-        gz = (np.arange(z0, z1) - nb) * dr[2]
-        win = np.empty((x1 - x0, y1 - y0, z1 - z0, 3))
+        gx = (pidx[0][x0:x1] - nb) * dr[0]
+        gy = (pidx[1][y0:y1] - nb) * dr[1]
+        gz = (pidx[2][z0:z1] - nb) * dr[2]
+        win = np.empty((len(gx), len(gy), len(gz), 3))
         win[..., 0] = gx[:, None, None]
         win[..., 1] = gy[None, :, None]
         win[..., 2] = gz[None, None, :]

@@ -15,13 +15,17 @@ Factor ``plot2d.py``:
   normalisation (``:653-655``); everything after that is plotting.
 * ``normalize_alkanes`` follows ``plot2d.py:874-918`` loop for loop.
 
-How it is pinned (``tests/test_plot2d_oracle.py``): a constant S (every bin
-equals the constant, ``final`` is all ones); an S linear in the grid
-coordinates, which the trilinear interpolant reproduces exactly, so that the
-theta average has the closed form a + z (b_inv c)_z on a monoclinic cell (a
-transposed ``b_inv`` or a wrong bin grid fails it); the interpolator against a
-hand-computed trilinear value; ``normalize_alkanes`` on a field whose ring
-mean is known.
+How it is pinned (``tests/test_plot2d_oracle.py``): ``reciprocal_inverse``
+equals the transposed cell matrix on monoclinic cells; a constant S (every
+bin equals the constant, ``final`` is all ones); an S linear in the grid
+coordinates on an orthogonal cell, which the trilinear interpolant reproduces
+exactly, so that the theta average is a + d z (pins the r / z bins and the
+theta sampling); S = X Y + Z on 120- and 60-degree monoclinic cells, whose
+theta average has the closed form r^2 cos(theta) / 2 + z only when the samples
+are taken at ``pts @ b_inv`` (the transposed product gives
+r^2 cos(theta) sin(theta) / 2 + z); the out-of-grid fill; the interpolator
+against a hand-computed trilinear value; ``normalize_alkanes`` on a field
+whose ring mean is known.
 """
 import numpy as np
 from scipy.interpolate import RegularGridInterpolator

@@ -13,7 +13,9 @@ providing ``old_div = a / b`` is inserted into ``sys.modules`` first.
 
 Run from the repository root (needs ``/root/reference``, this container only)::
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
+
+With case names, only those fixtures are (re)written.
 """
 import math
 import os
@@ -77,10 +79,46 @@ def _run(dens, name, coords, dims, typ, ucell, sres, *, better=True, noise=0,
     print(f"{name}: N={res['N']} sf{res['sf'].shape} kgridplt{res['kgridplt'].shape}")
 
 
+def _clipped_windows(dens, eye):
+    """G: windows that leave the padded grid.  The reference slices the same
+    padded index range out of `grid` and `d0` (dens.py:292-308), so numpy
+    clips both alike instead of raising: a window crossing the low edge
+    becomes empty, one crossing the high edge is cut there, one entirely at
+    negative indices counts from the far end.  Coordinates outside (-L, 2L)
+    keep such windows after the single wrap of dens.py:206-212."""
+    rng = np.random.default_rng(20261203)
+    L = np.array([12.0, 12.0, 12.0])                 # Sres 0.75: N = 16, Nb = 7 (C)
+    r0, typ = _water_like(rng, 10, L, ["C", "O", "H"])
+    out = np.array([
+        [-14.25, 6.0, 6.0],      # -> -2.25: x window crosses the low edge (empty)
+        [-24.0, 5.0, 7.0],       # -> -12: x window wholly at negative indices
+        [27.2, 6.5, 5.5],        # -> 15.2: x window cut at the high edge
+        [34.5, 6.0, 6.0],        # -> 22.5: x window starts past the high edge (empty)
+        [5.0, 26.9, 4.0],        # -> 14.9: y window cut at the high edge
+        [27.1, 27.3, 27.0],      # -> 15.1 .. 15.3: cut on all three axes (corner slot)
+        [11.9, 27.4, -0.05],     # cut on y, z wraps to 11.95
+    ])
+    otyp = np.array(["C", "C", "C", "O", "H", "C", "O"])
+    r = np.vstack([r0, out])
+    t = np.concatenate([typ, otyp])
+    frames = np.stack([r, r + rng.normal(0, 0.05, r.shape)])
+    dims = np.tile(L, (2, 1))
+    _run(dens, "clipped_windows", frames, dims, t, eye, 0.75,
+         note="windows off the padded grid: clipped as numpy slices, N=16, Nb=7")
+
+
 def main():
     dens = _import_reference()
-    rng = np.random.default_rng(20261202)
     eye = np.eye(3)
+    only = set(sys.argv[1:])
+    if only:
+        if "clipped_windows" in only:
+            _clipped_windows(dens, eye)
+        if only - {"clipped_windows"}:
+            raise SystemExit("cases A-F are written together (one random stream); run "
+                             "without arguments to rewrite them")
+        return
+    rng = np.random.default_rng(20261202)
 
     # A: orthorhombic float64, atoms in all 8 corners and one at exactly 0.0
     #    (wraps to L, home cell ir = N), Nborder > N/2.
@@ -159,6 +197,8 @@ def main():
     _run(dens, "mixed_f32_24", np.asarray(fr, np.float32),
          np.asarray(dd, np.float32), typ, eye, 0.5,
          note="24^3 float32, per-frame dims, N bumped 23->24")
+
+    _clipped_windows(dens, eye)
 
 
 if __name__ == "__main__":

@@ -41,6 +41,27 @@ def fold_index(u, n, nb):
     return (std[0] * n[1] + std[1]) * n[2] + z
 
 
+def _clipped_atom(rho, r_a, ir, A, name, rad, geo, U):
+    """One atom whose window leaves the padded grid: the reference slices the
+    same padded range of its grid and of d0 (dens.py:292-308), so each axis
+    keeps the cells numpy's slice rules keep, at their own grid coordinates."""
+    n, nb, dr = geo["N"], geo["Nb"], geo["dr"]
+    us = []
+    for d in range(3):
+        ln = n[d] + 2 * nb
+        p = np.arange(ln)[int(ir[d]) - A[d] + nb:int(ir[d]) + A[d] + nb]
+        if len(p) == 0:
+            return rho
+        us.append(torch.tensor(p - nb, dtype=torch.int64))
+    ux, uy, uz = torch.meshgrid(us[0], us[1], us[2], indexing="ij")
+    g = torch.stack([ux.to(torch.float64) * dr[0], uy.to(torch.float64) * dr[1],
+                     uz.to(torch.float64) * dr[2]], dim=-1)
+    d2 = (((r_a - g) @ U) ** 2).sum(-1)
+    nel, sig = rad[name]
+    val = (nel / sig ** 3) * torch.exp(-d2 / (2 * sig ** 2))
+    return rho.index_add(0, fold_index((ux, uy, uz), n, nb).reshape(-1), val.reshape(-1))
+
+
 def frame_density(r, typ, rad, geo, ucell):
     """Folded density (Nx, Ny, Nz) of one wrapped frame ``r`` (natoms, 3), differentiable in r."""
     n, nb, dr = geo["N"], geo["Nb"], geo["dr"]
@@ -50,9 +71,18 @@ def frame_density(r, typ, rad, geo, ucell):
     typ = np.asarray(typ)
     for name in sorted(set(typ.tolist())):
         idx = np.nonzero(typ == name)[0]
+        A = geo["bdict"][name].astype(int)
+        ir_all = (r[idx].detach() / dr_t).to(torch.int64)   # truncation, as astype(int)
+        lo = ir_all - torch.tensor(A) + nb
+        hi = ir_all + torch.tensor(A) + nb
+        inside = ((lo >= 0) & (hi <= torch.tensor(n) + 2 * nb)).all(dim=1).numpy()
+        for k in np.nonzero(~inside)[0]:
+            rho = _clipped_atom(rho, r[idx[k]], ir_all[k].numpy(), A, name, rad, geo, U)
+        idx = idx[inside]
+        if len(idx) == 0:
+            continue
         rt = r[idx]
         ir = (rt.detach() / dr_t).to(torch.int64)          # truncation, as astype(int)
-        A = geo["bdict"][name].astype(int)
         offs = [torch.arange(-A[d], A[d], dtype=torch.int64) for d in range(3)]
         ux = ir[:, 0, None, None, None] + offs[0][None, :, None, None]
         uy = ir[:, 1, None, None, None] + offs[1][None, None, :, None]

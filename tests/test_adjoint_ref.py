@@ -121,6 +121,18 @@ def test_torch_reference_matches_oracle(rad, theta):
     assert err.max() < 1e-9, err.max()
 
 
+def test_torch_reference_matches_clipped_windows_golden(rad):
+    """Windows that leave the padded grid (clipped as the reference's numpy
+    slices clip them): the torch reference against the reference's own
+    output for the clipped_windows fixture."""
+    from conftest import load_golden
+    c = load_golden("clipped_windows")
+    S = T.structure_factor(torch.tensor(c["in_coords"]), c["in_dims"], c["typ"], rad, c["sres"],
+                           c["in_ucell"], better=bool(c["in_better"]))
+    want = c["out_sf"]
+    np.testing.assert_allclose(S.numpy(), want, rtol=1e-9, atol=1e-12 * want.max())
+
+
 def test_torch_reference_gradient_matches_fd(rad):
     """Autograd of the torch reference against central differences of the oracle."""
     coords, dims, typ = corner_system(6, frames=1)

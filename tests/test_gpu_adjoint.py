@@ -152,6 +152,32 @@ def test_adjoint_frames_corner_spill(theta, fp64, rad, gpu):
     assert rel_err(got, xr.grad.numpy()) <= TOL[fp64]
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_adjoint_clipped_windows(fp64, rad, gpu):
+    """The reference's clipped_windows fixture: windows cut at the padded
+    grid's high edge (their gradient comes from the clipped-window pass) and
+    windows the reference's slices empty (zero gradient), against autograd of
+    the torch reference (pinned to the fixture on the CPU)."""
+    from conftest import load_golden
+    from mdsf.autograd import SFModel
+    c = load_golden("clipped_windows")
+    coords, dims, typ = c["in_coords"], c["in_dims"], c["typ"]
+    model = SFModel(coords, dims, typ, rad, c["sres"], better=bool(c["in_better"]), fp64=fp64)
+    rng = np.random.default_rng(12)
+    x = torch.tensor(coords, device=model.plan.device, requires_grad=True)
+    S = model(x)
+    w = torch.tensor(rng.normal(size=tuple(S.shape)), device=S.device) / S.detach().max()
+    torch.sum(w * S).backward()
+    got = x.grad.cpu().numpy()
+    xr = torch.tensor(coords, requires_grad=True)
+    Sr = T.structure_factor(xr, dims, typ, rad, c["sres"], np.eye(3), better=bool(c["in_better"]))
+    torch.sum(w.cpu() * Sr).backward()
+    want = xr.grad.numpy()
+    assert rel_err(got, want) <= TOL[fp64]
+    cut = [12, 14, 15, 16]        # fixture atoms whose windows the high edge cuts
+    assert np.all(np.abs(want[:, cut]).max(axis=-1) > 0)
+
+
 def test_adjoint_fd_adjoint_workload(rad, gpu):
     """20k C/O/H atoms, 60 A box, 128^3, 16 frames; L = sum (S - S_target)^2.
     FP32 and FP64 adjoints against central FD (h = 1e-3 A) of the FP64 forward

@@ -114,7 +114,7 @@ def test_dropin_fp32_fft_on_goldens(name, rad, gpu, tmp_path):
 
 @pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("name", ["orth_corners_f64", "mono120_f32", "na_wide",
-                                  "aniso_sres_array", "mixed_f32_24"])
+                                  "aniso_sres_array", "mixed_f32_24", "clipped_windows"])
 def test_deposit_fold_and_home_cells(name, fp64, rad, gpu):
     import torch
     c = load_golden(name)
@@ -335,13 +335,22 @@ def test_water_64_end_to_end(rad, gpu, tmp_path):
     assert_sq_close(got["sf"], want["sf"])
 
 
-def test_range_error_raises(rad, gpu, tmp_path):
+def test_atom_far_outside_deposits_nothing(rad, gpu, tmp_path):
+    """An atom 3.5 boxes out stays 2.5 boxes out after the single wrap
+    (dens.py:206-212); its window starts past the padded grid, so the
+    reference's slices (dens.py:292-308) come out empty: no error, no
+    density.  S(q) equals the oracle's, and that of the first atom alone."""
     import dens
-    r = np.array([[[1.0, 1.0, 1.0], [35.0, 2.0, 2.0]]])   # second atom 3.5 boxes out
+    r = np.array([[[1.0, 1.0, 1.0], [35.0, 2.0, 2.0]]])
+    dims = np.array([[10.0, 10.0, 10.0]])
+    want = O.compute_sf(np.array(r), dims, ["O", "H"], None, rad, np.eye(3), 0.5)["sf"]
+    alone = O.compute_sf(np.array(r[:, :1]), dims, ["O"], None, rad, np.eye(3), 0.5)["sf"]
     dens.PRINT_DETAILS = False
-    with pytest.raises(ValueError):
-        dens.compute_sf(r, np.array([[10.0, 10.0, 10.0]]), ["O", "H"], str(tmp_path / "e"), rad,
-                        np.eye(3), 0.5)
+    dens.compute_sf(r, dims, ["O", "H"], str(tmp_path / "e"), rad, np.eye(3), 0.5)
+    got = np.load(str(tmp_path / "e.npz"))["sf"]
+    assert r[0, 1, 0] == 25.0
+    assert_sq_close(got, want)
+    assert np.array_equal(want, alone)
 
 
 def test_type_index_error_raises(rad, gpu):

@@ -49,6 +49,28 @@ def test_fp32_vs_fp64_hii_256(rad, gpu):
     assert emax <= 1e-5
 
 
+def test_hii_frame_vs_oracle(rad, gpu):
+    """One frame of the 256^3 HII workload (Nborder 55: the corner-slot bands
+    of dens.py:107 span 4 z tiles, 85 corner atoms) through the FP32 and FP64
+    plans against the FP64 numpy oracle (dens.py:283-318, about 20 s on one core)."""
+    from oracle import dens_oracle as O
+    from mdsf import synth
+    coords, dims, typ = synth.hii(seed=2002, nframes=1)
+    t0 = time.perf_counter()
+    want = O.compute_sf(np.array(coords), np.array(dims), typ, None, rad, np.eye(3),
+                        synth.hii_sres(), better=False)["sf"]
+    t_oracle = time.perf_counter() - t0
+    want = torch.from_numpy(want).to(gpu)
+    for fp64 in (False, True):
+        s, plan, got, _ = _sq(coords, dims, typ, rad, synth.hii_sres(), fp64)
+        assert s.nborder == 55
+        emax, emed, nv = _err(got, want)
+        print(f"\nHII 256^3 frame 0, {'fp64' if fp64 else 'fp32'} plan vs oracle: max rel "
+              f"{emax:.3e} median {emed:.3e} over {nv} voxels (oracle {t_oracle:.1f} s)")
+        assert emax <= (1e-9 if fp64 else 1e-5)
+        plan.close()
+
+
 def test_large_512_precision_and_ewald(rad, gpu):
     """1M H/C/N/O atoms, 200 A box, 512^3, one frame; then the Ewald slices at 1.542 A."""
     from mdsf import engine, synth

@@ -63,6 +63,32 @@ def test_linear_field_closed_form():
     np.testing.assert_allclose(un["oa"], want / np.average(want), rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("theta", [120.0, 60.0])
+def test_bilinear_field_monoclinic_closed_form(theta):
+    """S = X Y + Z on a monoclinic cell.  The trilinear interpolant reproduces
+    X Y exactly; a sample at (r cos t, r sin t, z) is taken at p = (r cos t,
+    r sin t, z) @ b_inv (plot2d.py:619-627), and with b_inv = A^T,
+    X = r cos t, Y = r (cos t cos(theta) + sin t sin(theta)), so the average
+    over n >= 3 equally spaced t is r^2 cos(theta) / 2 + z.  The transposed
+    product b_inv @ p gives r^2 cos(theta) sin(theta) / 2 instead, and a wrong
+    r or z bin grid shifts the values."""
+    u = monoclinic(theta)
+    D = grid(lambda x, y, z: x * y + z)
+    rb = 50
+    out = P2.rz_average(D, u, format=True, rbins=rb)   # format: no mean normalisation
+    r = out["rarr"][:, None]
+    zar = out["zar"][None, :]
+    X, Y = D[:, 0, 0, 0], D[0, :, 0, 1]
+    rmax = 0.95 * min((X[-1] - X[0]) * u[0][0], (Y[-1] - Y[0]) * u[1][1]) / 2.0
+    np.testing.assert_allclose(out["rarr"], np.linspace(0.0, rmax, rb), rtol=0, atol=0)
+    nth = np.maximum((P2.THETA_BINS_PER_INV_ANG * 2 * np.pi * out["rarr"]).astype(int), 1)
+    ok = (nth >= 3) | (out["rarr"] == 0.0)
+    want = r ** 2 * np.cos(np.radians(theta)) / 2.0 + zar
+    np.testing.assert_allclose(out["oa"][ok], (want + 0 * zar)[ok], rtol=0, atol=1e-12)
+    wrong = r ** 2 * np.cos(np.radians(theta)) * np.sin(np.radians(theta)) / 2.0 + zar
+    assert np.max(np.abs(out["oa"][ok] - wrong[ok])) > 1e-2
+
+
 def test_out_of_grid_bins_take_the_minimum():
     """A cell whose r range leaves the grid on the y axis: those bins are NaN
     before the fill and take the smallest finite bin (plot2d.py:629-630)."""
