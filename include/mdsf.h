@@ -206,8 +206,8 @@ int mdsf_rfft3(mdsf_plan* plan, const void* d_density, double* d_spectrum,
  * n[3] (even sizes) in `precision` (MDSF_F32 / MDSF_F64, the element type of
  * d_density).  d_spectrum receives complex128 (Nx, Ny, Nz/2+1), bit for bit
  * what mdsf_rfft3 gives on a plan of this grid and precision created without
- * MDSF_AUTOTUNE.  Allocates nothing: its twiddles, half spectrum and Nyquist
- * plane live in one fixed 80 MiB device array inside the library (one copy
+ * MDSF_AUTOTUNE.  Allocates nothing: its twiddles, pass outputs and Nyquist
+ * plane live in one fixed 144 MiB device array inside the library (one copy
  * per GPU, held until the process ends), which fits every FP32 grid up to
  * 256^3 and every FP64 grid up to 208^3; larger grids are rejected with
  * MDSF_ERR_ARG.  Every call on a GPU uses that GPU's one array, so calls that

@@ -343,17 +343,21 @@ __global__ void k_fill_zero(double* __restrict__ p, long long n) {
   if (i < n) p[i] = 0.0;
 }
 
-// w_main[x, y, kz] = 2 g[x, y, kz] (kz < m), w_nyq[x, y] = 2 g[x, y, m]
+// w_main[kz][y][x] = 2 g[x, y, kz] (kz < m), w_nyq[x][y] = 2 g[x, y, m]: the
+// layouts the FFT keeps its spectrum and S(q) in
 __global__ void k_adjoint_weights(const double* __restrict__ g, double* __restrict__ wmain,
-                                  double* __restrict__ wnyq, long long nxy, int m) {
+                                  double* __restrict__ wnyq, int nx, int ny, int m) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long nxy = (long long)nx * ny;
   if (i >= nxy * (m + 1)) return;
   const long long xy = i / (m + 1);
   const int kz = (int)(i - xy * (m + 1));
-  if (kz < m)
-    wmain[xy * m + kz] = 2.0 * g[i];
-  else
+  if (kz < m) {
+    const long long x = xy / ny, y = xy - x * ny;
+    wmain[((long long)kz * ny + y) * nx + x] = 2.0 * g[i];
+  } else {
     wnyq[xy] = 2.0 * g[i];
+  }
 }
 
 }  // namespace
@@ -369,7 +373,7 @@ hipError_t launch_adjoint_weights(Plan& p, const double* dsf, hipStream_t s) {
   const int m = p.sys.n[2] / 2;
   const long long n = nxy * (m + 1);
   hipLaunchKernelGGL(k_adjoint_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dsf,
-                     p.adj_w, p.adj_w + nxy * m, nxy, m);
+                     p.adj_w, p.adj_w + nxy * m, p.sys.n[0], p.sys.n[1], m);
   return hipGetLastError();
 }
 

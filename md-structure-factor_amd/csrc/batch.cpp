@@ -78,6 +78,7 @@ hipError_t enqueue_system(Batch& b, int i, hipStream_t s) {
                     nullptr, nullptr, s, false, /*write_back=*/false);
     if (e == hipSuccess) e = launch_binning(p, s);
     if (e == hipSuccess) e = launch_deposit(p, p.rho, s);
+    if (e == hipSuccess) e = launch_parked(p, p.rho, s);
     if (e == hipSuccess) e = fft_forward(p, p.rho, true, s);
   }
   return e;

@@ -83,7 +83,8 @@ void free_plan(Plan* p) {
       if (q) (void)hipFree(q);
   }
   void* ptrs[] = {p->types,     p->err,       p->rho_buf[0], p->nyq_buf[0], p->sf_main,
-                  p->rho_buf[1], p->nyq_buf[1],
+                  p->rho_buf[1], p->nyq_buf[1], p->spec_buf[0], p->spec_buf[1],
+                  p->res_buf[0], p->res_buf[1],
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
                   p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
                   p->adj_w,     p->adj_tpart};
@@ -355,23 +356,22 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));
   for (int b = 0; b < 2; ++b) {
-    if (p->fft64) {
-      double* rho = nullptr;
-      double2* nyq = nullptr;
-      TRY(dalloc(&rho, nx * ny * nz));
-      TRY(dalloc(&nyq, nx * ny));
-      p->rho_buf[b] = rho;
-      p->nyq_buf[b] = nyq;
-    } else {
-      float* rho = nullptr;
-      float2* nyq = nullptr;
-      TRY(dalloc(&rho, nx * ny * nz));
-      TRY(dalloc(&nyq, nx * ny));
-      p->rho_buf[b] = rho;
-      p->nyq_buf[b] = nyq;
+    unsigned char* rho = nullptr;
+    unsigned char* spec = nullptr;
+    double2* nyq = nullptr;
+    TRY(dalloc(&rho, nx * ny * nz * (p->fft64 ? 8 : 4)));
+    TRY(dalloc(&spec, fft_spec_bytes(s.n, p->fft64)));
+    TRY(dalloc(&nyq, nx * ny));
+    p->rho_buf[b] = rho;
+    p->spec_buf[b] = spec;
+    p->nyq_buf[b] = nyq;
+    if (fft_res_bytes(s.n, p->fft64) > 0) {
+      unsigned char* res = nullptr;
+      TRY(dalloc(&res, fft_res_bytes(s.n, p->fft64)));
+      p->res_buf[b] = res;
     }
   }
-  p->use_slot(0);  // (the z pass writes the half spectrum over the density)
+  p->use_slot(0);
   TRY(dalloc(&p->sf_main, nx * ny * m));
   TRY(dalloc(&p->sf_nyq, nx * ny));
   TRY(hipMemset(p->sf_main, 0, sizeof(double) * nx * ny * m));

@@ -1,20 +1,36 @@
-// 3-D real-to-complex FFT for gfx950: Stockham autosort stages in LDS.
+// 3-D real-to-complex FFT for gfx950: Stockham autosort stages, FP64 arithmetic.
 //
 // Replaces np.fft.rfftn (dens.py:313) and the |F|^2 frame sum
-// (dens.py:315-318).  Unnormalised forward transform, exponent -2*pi*i*k*n/N.
-// Element type R = float (FP32 storage and arithmetic) or double (FP64); the
-// twiddles of both are generated in FP64 on the device.
+// (dens.py:315-318): unnormalised forward transform, exponent -2 pi i k n / N.
 //
-// Layout between passes (internal): the z pass packs each real row of Nz
-// floats as Nz/2 complex values and writes the half spectrum,
-// kz in [0, Nz/2) -> main[x][y][kz], and the kz = Nz/2 plane -> nyq[x][y].
-// The y and x passes are strided pencil transforms over those arrays; the
-// x pass can fold |F|^2 into FP64 accumulators of the same layout.
+// Precision.  Storage is the plan precision R (float or double); every
+// butterfly, twiddle product and the real-to-complex split is evaluated in
+// FP64, with FP64 twiddles generated on the device.  An FP32 plan rounds to
+// FP32 only where values are stored between passes, and the y pass's output,
+// whose rounding error the x pass would spread along whole rows of the
+// spectrum (the error that dominates faint bins next to bright ones), keeps a
+// 4-bit residual per value: the rounding error in units of 1/16 of the FP32
+// ulp, so that the x pass reads it back 16 times more precisely.
 //
-// Each workgroup owns a tile of P pencils.  LDS holds re/im planes of
-// P x pitch values (pitch odd) twice (ping-pong between stages), and threads
-// are mapped pencil-fastest, so every Stockham read and write is
-// conflict-free for any radix and span.
+// Layout.  Three passes, each reading whole contiguous rows and writing P
+// pencils side by side (runs of P values):
+//   z  rho[x][y][0..Nz) (density, R)  ->  spec[x][kz][y] (kz < Nz/2), nyq[x][y] (kz = Nz/2)
+//   y  spec[x][kz][0..Ny)             ->  rho[kz][ky][x] (the density buffer, free by then)
+//                                         + res[kz][ky][x] (FP32 plans)
+//   x  rho[kz][ky][0..Nx)             ->  F in place, or |F|^2 added into sf[kz][ky][kx] (FP64)
+// The Nyquist plane nyq (FP64 complex) is transformed in FP64 by a few extra
+// blocks of the y and x launches (sf_nyq[x][y]).  The inverse passes of the
+// adjoint run the same kernels backwards: x in place (weights applied on
+// load), y from rho[kz][ky][x] back to spec[x][kz][y], the complex-to-real z
+// pass from spec to density rows.
+//
+// Register-staged passes (n = R1 R2 or R1 R2 R3, n a power of two up to
+// 1024): a block holds 4096 values (16 per thread); the first stage reads its
+// inputs straight from global memory, each later stage takes its inputs from
+// the previous one through LDS (real and imaginary parts in turn, one FP64
+// plane), and the last stage stores straight to global memory or folds |X|^2
+// into the accumulator.  Any other length takes the LDS-staged pass (every
+// stage in LDS, radices 2/3/4/5/8/16 and generic odd primes).
 #include <map>
 #include <mutex>
 
@@ -25,107 +41,98 @@ namespace mdsf {
 namespace {
 
 constexpr int kFftThreads = 256;
+constexpr int kRegElems = 4096;  // values per block of the register-staged passes
+constexpr int kZRegElems = 2048; // of the register-staged z pass (room beside the deposition)
 
 template <typename R> struct V2;
 template <> struct V2<float> { using T = float2; };
 template <> struct V2<double> { using T = double2; };
 
-template <typename R>
 struct Cx {
-  R x, y;
+  double x, y;
 };
 
-template <typename R>
-__device__ __forceinline__ Cx<R> mk(R a, R b) {
-  Cx<R> c;
+__device__ __forceinline__ Cx mk(double a, double b) {
+  Cx c;
   c.x = a;
   c.y = b;
   return c;
 }
-template <typename R>
-__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b) {
-  return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-template <typename R>
-__device__ __forceinline__ Cx<R> cmulc(Cx<R> a, Cx<R> b) {  // a * conj(b)
-  return mk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
-}
-template <typename R>
-__device__ __forceinline__ Cx<R> cadd(Cx<R> a, Cx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
-template <typename R>
-__device__ __forceinline__ Cx<R> csub(Cx<R> a, Cx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
-template <typename R>
-__device__ __forceinline__ Cx<R> cscale(R s, Cx<R> a) { return mk<R>(s * a.x, s * a.y); }
+__device__ __forceinline__ Cx cmul(Cx a, Cx b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ Cx cmulc(Cx a, Cx b) { return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+__device__ __forceinline__ Cx cadd(Cx a, Cx b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ Cx csub(Cx a, Cx b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ Cx cscale(double s, Cx a) { return mk(s * a.x, s * a.y); }
+__device__ __forceinline__ Cx csel(bool c, Cx a, Cx b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
 // multiply by -i (forward) or +i (inverse)
-template <bool INV, typename R>
-__device__ __forceinline__ Cx<R> mul_mi(Cx<R> a) {
-  return INV ? mk<R>(-a.y, a.x) : mk<R>(a.y, -a.x);
+template <bool INV>
+__device__ __forceinline__ Cx mul_mi(Cx a) {
+  return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
 }
-template <typename R>
-__device__ __forceinline__ Cx<R> ld_tw(const void* tw, int k) {
-  const auto v = static_cast<const typename V2<R>::T*>(tw)[k];
-  return mk<R>(v.x, v.y);
+__device__ __forceinline__ Cx ld_tw(const double2* tw, int k) {
+  const double2 v = tw[k];
+  return mk(v.x, v.y);
 }
 
-// --- small DFT kernels (in registers) ---------------------------------------
-template <bool INV, typename R>
-__device__ __forceinline__ void dft2(Cx<R>* v) {
-  const Cx<R> a = v[0], b = v[1];
+// --- small DFTs in registers -------------------------------------------------
+template <bool INV>
+__device__ __forceinline__ void dft2(Cx* v) {
+  const Cx a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
 
-template <bool INV, typename R>
-__device__ __forceinline__ void dft3(Cx<R>* v) {
-  const R s = (R)0.86602540378443864676;  // sin(2 pi / 3)
-  const Cx<R> t1 = cadd(v[1], v[2]);
-  const Cx<R> t2 = mk<R>(v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y);
-  const Cx<R> t3 = mul_mi<INV>(cscale(s, csub(v[1], v[2])));
+template <bool INV>
+__device__ __forceinline__ void dft3(Cx* v) {
+  const double s = 0.86602540378443864676;  // sin(2 pi / 3)
+  const Cx t1 = cadd(v[1], v[2]);
+  const Cx t2 = mk(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+  const Cx t3 = mul_mi<INV>(cscale(s, csub(v[1], v[2])));
   v[0] = cadd(v[0], t1);
   v[1] = cadd(t2, t3);
   v[2] = csub(t2, t3);
 }
 
-template <bool INV, typename R>
-__device__ __forceinline__ void dft4(Cx<R>* v) {
-  const Cx<R> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-  const Cx<R> t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+template <bool INV>
+__device__ __forceinline__ void dft4(Cx* v) {
+  const Cx t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  const Cx t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
   v[0] = cadd(t0, t2);
   v[2] = csub(t0, t2);
   v[1] = cadd(t1, t3);
   v[3] = csub(t1, t3);
 }
 
-template <bool INV, typename R>
-__device__ __forceinline__ void dft5(Cx<R>* v) {
-  const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
-  const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
-  const Cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-  const Cx<R> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  const Cx<R> x0 = v[0];
-  const Cx<R> a1 = mk<R>(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
-  const Cx<R> a2 = mk<R>(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
-  const Cx<R> b1 = mul_mi<INV>(mk<R>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
-  const Cx<R> b2 = mul_mi<INV>(mk<R>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
-  v[0] = mk<R>(x0.x + t1.x + t2.x, x0.y + t1.y + t2.y);
+template <bool INV>
+__device__ __forceinline__ void dft5(Cx* v) {
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+  const Cx t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const Cx t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const Cx x0 = v[0];
+  const Cx a1 = mk(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+  const Cx a2 = mk(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+  const Cx b1 = mul_mi<INV>(mk(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+  const Cx b2 = mul_mi<INV>(mk(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+  v[0] = mk(x0.x + t1.x + t2.x, x0.y + t1.y + t2.y);
   v[1] = cadd(a1, b1);
   v[4] = csub(a1, b1);
   v[2] = cadd(a2, b2);
   v[3] = csub(a2, b2);
 }
 
-template <bool INV, typename R>
-__device__ __forceinline__ void dft8(Cx<R>* v) {
-  const R h = (R)0.70710678118654752440;
-  Cx<R> e[4] = {v[0], v[2], v[4], v[6]};
-  Cx<R> o[4] = {v[1], v[3], v[5], v[7]};
+template <bool INV>
+__device__ __forceinline__ void dft8(Cx* v) {
+  const double h = 0.70710678118654752440;
+  Cx e[4] = {v[0], v[2], v[4], v[6]};
+  Cx o[4] = {v[1], v[3], v[5], v[7]};
   dft4<INV>(e);
   dft4<INV>(o);
-  const Cx<R> w1 = mk<R>(h, INV ? h : -h);
-  const Cx<R> w3 = mk<R>(-h, INV ? h : -h);
-  const Cx<R> o1 = cmul(o[1], w1);
-  const Cx<R> o2 = mul_mi<INV>(o[2]);
-  const Cx<R> o3 = cmul(o[3], w3);
+  const Cx w1 = mk(h, INV ? h : -h);
+  const Cx w3 = mk(-h, INV ? h : -h);
+  const Cx o1 = cmul(o[1], w1);
+  const Cx o2 = mul_mi<INV>(o[2]);
+  const Cx o3 = cmul(o[3], w3);
   v[0] = cadd(e[0], o[0]);
   v[4] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);
@@ -136,9 +143,9 @@ __device__ __forceinline__ void dft8(Cx<R>* v) {
   v[7] = csub(e[3], o3);
 }
 
-template <bool INV, typename R>
-__device__ __forceinline__ void dft16(Cx<R>* v) {
-  Cx<R> e[8], o[8];
+template <bool INV>
+__device__ __forceinline__ void dft16(Cx* v) {
+  Cx e[8], o[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     e[i] = v[2 * i];
@@ -146,22 +153,21 @@ __device__ __forceinline__ void dft16(Cx<R>* v) {
   }
   dft8<INV>(e);
   dft8<INV>(o);
-  const R c1 = (R)0.92387953251128675613, s1 = (R)0.38268343236508977173;
-  const R h = (R)0.70710678118654752440;
-  const R sg = INV ? (R)1 : (R)-1;
-  const Cx<R> w[8] = {mk<R>(1, 0),        mk<R>(c1, sg * s1),  mk<R>(h, sg * h),
-                      mk<R>(s1, sg * c1), mk<R>(0, sg),        mk<R>(-s1, sg * c1),
-                      mk<R>(-h, sg * h),  mk<R>(-c1, sg * s1)};
+  const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;
+  const double h = 0.70710678118654752440;
+  const double sg = INV ? 1.0 : -1.0;
+  const Cx w[8] = {mk(1, 0),        mk(c1, sg * s1),  mk(h, sg * h),  mk(s1, sg * c1),
+                   mk(0, sg),       mk(-s1, sg * c1), mk(-h, sg * h), mk(-c1, sg * s1)};
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const Cx<R> t = k == 0 ? o[0] : cmul(o[k], w[k]);
+    const Cx t = k == 0 ? o[0] : cmul(o[k], w[k]);
     v[k] = cadd(e[k], t);
     v[k + 8] = csub(e[k], t);
   }
 }
 
-template <int RAD, bool INV, typename R>
-__device__ __forceinline__ void dft(Cx<R>* v) {
+template <int RAD, bool INV>
+__device__ __forceinline__ void dft(Cx* v) {
   if constexpr (RAD == 2) dft2<INV>(v);
   else if constexpr (RAD == 3) dft3<INV>(v);
   else if constexpr (RAD == 4) dft4<INV>(v);
@@ -170,12 +176,54 @@ __device__ __forceinline__ void dft(Cx<R>* v) {
   else dft16<INV>(v);
 }
 
-// One Stockham stage, out of place (in -> out planes), one butterfly per
-// thread iteration: X[(j/ns)*ns*RAD + k + r*ns] = DFT(x[j + r*n/RAD] * W^(k r n/(ns RAD))).
-template <int RAD, bool INV, typename R>
-__device__ void stage_fixed(const R* __restrict__ ire, const R* __restrict__ iim,
-                            R* __restrict__ ore, R* __restrict__ oim, int n, int P, int pitch,
-                            int ns, const void* __restrict__ tw) {
+// inputs r >= 1 of a butterfly at k times W^(k r step)
+template <int RAD, bool INV>
+__device__ __forceinline__ void twiddle_in(Cx* v, int k, int step, const double2* tw) {
+  if (k == 0) return;
+#pragma unroll
+  for (int r = 1; r < RAD; ++r) {
+    const Cx w = ld_tw(tw, k * r * step);
+    v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+  }
+}
+
+// --- storage conversions ---------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ Cx load_cx(const typename V2<R>::T* p, long long i) {
+  const auto v = p[i];
+  return mk((double)v.x, (double)v.y);
+}
+template <typename R>
+__device__ __forceinline__ void store_cx(typename V2<R>::T* p, long long i, Cx v) {
+  typename V2<R>::T o;
+  o.x = (R)v.x;
+  o.y = (R)v.y;
+  p[i] = o;
+}
+
+// 4-bit residual of an FP32 rounding: round((v - (float)v) * 16 / ulp) in
+// [-8, 7], ulp = 2^(max(e, 1) - 150) the FP32 spacing at (float)v (biased
+// exponent e).  Both scalings are exact powers of two (ldexp), so encoding
+// and decoding take a few instructions and no division.
+__device__ __forceinline__ int res_exp(float h) {
+  const int e = (int)((__float_as_uint(h) >> 23) & 0xffu);
+  return e > 1 ? e : 1;
+}
+__device__ __forceinline__ unsigned res_nibble(double v, float h) {
+  const double q = rint(ldexp(v - (double)h, 154 - res_exp(h)));
+  const double c = fmin(fmax(q, -8.0), 7.0);
+  return (unsigned)(int)c & 15u;
+}
+__device__ __forceinline__ double res_value(unsigned nib, float h) {
+  const int qi = (int)((nib ^ 8u) & 15u) - 8;
+  return ldexp((double)qi, res_exp(h) - 154);
+}
+
+// --- LDS-staged stages (any radix), pencil-fastest, FP64 planes --------------------
+template <int RAD, bool INV>
+__device__ void stage_fixed(const double* __restrict__ ire, const double* __restrict__ iim,
+                            double* __restrict__ ore, double* __restrict__ oim, int n, int P,
+                            int pitch, int ns, const double2* __restrict__ tw) {
   const int nq = n / RAD;
   const int nbf = P * nq;
   const int twstep = n / (ns * RAD);
@@ -183,17 +231,13 @@ __device__ void stage_fixed(const R* __restrict__ ire, const R* __restrict__ iim
     const int p = b % P, j = b / P;
     const int k = j % ns;
     const int row = p * pitch;
-    Cx<R> v[RAD];
+    Cx v[RAD];
 #pragma unroll
     for (int r = 0; r < RAD; ++r) {
       const int a = row + j + r * nq;
-      Cx<R> x = mk<R>(ire[a], iim[a]);
-      if (r > 0 && k > 0) {
-        const Cx<R> w = ld_tw<R>(tw, k * r * twstep);
-        x = INV ? cmulc(x, w) : cmul(x, w);
-      }
-      v[r] = x;
+      v[r] = mk(ire[a], iim[a]);
     }
+    twiddle_in<RAD, INV>(v, k, twstep, tw);
     dft<RAD, INV>(v);
     const int d0 = row + (j / ns) * ns * RAD + k;
 #pragma unroll
@@ -207,10 +251,10 @@ __device__ void stage_fixed(const R* __restrict__ ire, const R* __restrict__ iim
 
 // Any other radix (odd primes): one output element per thread iteration,
 // direct DFT with W_RAD^m = W_n^(m n / RAD) from the twiddle table.
-template <bool INV, typename R>
-__device__ void stage_generic(const R* __restrict__ ire, const R* __restrict__ iim,
-                              R* __restrict__ ore, R* __restrict__ oim, int n, int P, int pitch,
-                              int ns, int RAD, const void* __restrict__ tw) {
+template <bool INV>
+__device__ void stage_generic(const double* __restrict__ ire, const double* __restrict__ iim,
+                              double* __restrict__ ore, double* __restrict__ oim, int n, int P,
+                              int pitch, int ns, int RAD, const double2* __restrict__ tw) {
   const int nq = n / RAD;
   const int twstep = n / (ns * RAD);
   const int step = n / RAD;
@@ -219,16 +263,16 @@ __device__ void stage_generic(const R* __restrict__ ire, const R* __restrict__ i
     const int j = rest % nq, m = rest / nq;
     const int k = j % ns;
     const int row = p * pitch;
-    Cx<R> acc = mk<R>(0, 0);
+    Cx acc = mk(0, 0);
     int wm = 0;  // (r * m) mod RAD
     for (int r = 0; r < RAD; ++r) {
       const int a = row + j + r * nq;
-      Cx<R> x = mk<R>(ire[a], iim[a]);
+      Cx x = mk(ire[a], iim[a]);
       if (r > 0 && k > 0) {
-        const Cx<R> w = ld_tw<R>(tw, k * r * twstep);
+        const Cx w = ld_tw(tw, k * r * twstep);
         x = INV ? cmulc(x, w) : cmul(x, w);
       }
-      const Cx<R> w = ld_tw<R>(tw, wm * step);
+      const Cx w = ld_tw(tw, wm * step);
       acc = cadd(acc, INV ? cmulc(x, w) : cmul(x, w));
       wm += m;
       if (wm >= RAD) wm -= RAD;
@@ -240,509 +284,389 @@ __device__ void stage_generic(const R* __restrict__ ire, const R* __restrict__ i
   __syncthreads();
 }
 
-// Runs every stage of `ax`, ping-ponging between plane pairs 0 and 1 of
-// `lds` ([re0 | im0 | re1 | im1], P*pitch values each).  Returns the plane
-// pair holding the result.
-template <bool INV, typename R>
-__device__ int run_stages(R* lds, int n, int P, int pitch, const FftAxis& ax) {
+// Every stage of `ax` on P pencils in LDS, ping-ponging between plane pairs
+// 0 and 1 of `lds` ([re0 | im0 | re1 | im1], P * pitch values each).
+// Returns the plane pair holding the result.  A radix-16 stage runs as two
+// radix-4 stages (the same factorisation of n): a 16-point butterfly held
+// with its twiddles would set the register budget of every kernel that
+// contains this loop.
+template <bool INV>
+__device__ int run_stages(double* lds, int n, int P, int pitch, const FftAxis& ax) {
   const int plane = P * pitch;
-  const void* tw = sizeof(R) == 8 ? (const void*)ax.tw64 : (const void*)ax.tw;
   int ns = 1, cur = 0;
   for (int s = 0; s < ax.nstages; ++s) {
-    const int RAD = ax.radix[s];
-    const R* ire = lds + (2 * cur) * plane;
-    const R* iim = ire + plane;
-    R* ore = lds + (2 * (cur ^ 1)) * plane;
-    R* oim = ore + plane;
-    switch (RAD) {
-      case 2: stage_fixed<2, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
-      case 3: stage_fixed<3, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
-      case 4: stage_fixed<4, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
-      case 5: stage_fixed<5, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
-      case 8: stage_fixed<8, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
-      case 16: stage_fixed<16, INV>(ire, iim, ore, oim, n, P, pitch, ns, tw); break;
-      default: stage_generic<INV>(ire, iim, ore, oim, n, P, pitch, ns, RAD, tw); break;
+    const int RAD0 = ax.radix[s];
+    const int sub = RAD0 == 16 ? 2 : 1;
+    const int RAD = RAD0 == 16 ? 4 : RAD0;
+    for (int h = 0; h < sub; ++h) {
+      const double* ire = lds + (2 * cur) * plane;
+      const double* iim = ire + plane;
+      double* ore = lds + (2 * (cur ^ 1)) * plane;
+      double* oim = ore + plane;
+      switch (RAD) {
+        case 2: stage_fixed<2, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw64); break;
+        case 3: stage_fixed<3, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw64); break;
+        case 4: stage_fixed<4, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw64); break;
+        case 5: stage_fixed<5, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw64); break;
+        case 8: stage_fixed<8, INV>(ire, iim, ore, oim, n, P, pitch, ns, ax.tw64); break;
+        default: stage_generic<INV>(ire, iim, ore, oim, n, P, pitch, ns, RAD, ax.tw64); break;
+      }
+      ns *= RAD;
+      cur ^= 1;
     }
-    ns *= RAD;
-    cur ^= 1;
   }
   return cur;
 }
 
 __host__ __device__ inline int lds_pitch(int n) { return (n % 2 == 0) ? n + 1 : n; }
 
-// --- twiddles: W_n^k = exp(-2 pi i k / n), generated in FP64 ------------------
+// --- twiddles: W_n^k = exp(-2 pi i k / n), generated in FP64 ----------------------
 __global__ void k_twiddles(float2* __restrict__ tw, double2* __restrict__ tw64, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   double s, c;
   sincospi(-2.0 * (double)k / (double)n, &s, &c);
-  tw[k] = make_float2((float)c, (float)s);
+  if (tw) tw[k] = make_float2((float)c, (float)s);
   tw64[k] = make_double2(c, s);
 }
 
-// --- z pass: real rows -> half spectrum + Nyquist plane ------------------------
-template <typename R>
-struct R2CArgs {
-  const R* in;                      // rows of nz reals (plan precision)
-  typename V2<R>::T* main;          // rows of m = nz/2 complex (may alias `in` for R = float)
-  typename V2<R>::T* nyq;           // one complex per row
-  long long rows;
-  int m;                            // nz / 2
-  int P;                            // rows per tile
-  FftAxis ax;                       // length m
-  const void* tw2;                  // W_nz^k, k < nz
-};
-
-template <typename R>
-__global__ __launch_bounds__(kFftThreads) void k_r2c_z(R2CArgs<R> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  R* lds = reinterpret_cast<R*>(lds_raw);
-  const int m = a.m, P = a.P, pitch = lds_pitch(m);
-  R* re = lds;
-  R* im = lds + P * pitch;
-  const long long r0 = (long long)blockIdx.x * P;
-  const int pc = (int)min((long long)P, a.rows - r0);
-  const auto* src = reinterpret_cast<const typename V2<R>::T*>(a.in) + r0 * m;
-  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
-    const int p = idx / m, e = idx - p * m;
-    const auto v = src[idx];
-    re[p * pitch + e] = (R)v.x;
-    im[p * pitch + e] = (R)v.y;
-  }
-  __syncthreads();
-  {
-    const int cur = run_stages<false, R>(lds, m, P, pitch, a.ax);
-    re = lds + 2 * cur * P * pitch;
-    im = re + P * pitch;
-  }
-  // split: X_k = A_k + W_nz^k B_k, A = (Z_k + conj Z_{m-k})/2, B = (Z_k - conj Z_{m-k})/(2i)
-  auto* dst = a.main + r0 * m;
-  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
-    const int p = idx / m, k = idx - p * m;
-    const int kk = (k == 0) ? 0 : m - k;
-    const Cx<R> z = mk<R>(re[p * pitch + k], im[p * pitch + k]);
-    const Cx<R> zc = mk<R>(re[p * pitch + kk], -im[p * pitch + kk]);
-    const Cx<R> A = cscale((R)0.5, cadd(z, zc));
-    const Cx<R> d = cscale((R)0.5, csub(z, zc));
-    const Cx<R> B = mk<R>(d.y, -d.x);  // d / i
-    const Cx<R> X = cadd(A, cmul(B, ld_tw<R>(a.tw2, k)));
-    dst[idx].x = X.x;
-    dst[idx].y = X.y;
-  }
-  for (int p = threadIdx.x; p < pc; p += kFftThreads) {
-    // X_m = A_0 - B_0 = Re z0 - Im z0 (real)
-    a.nyq[r0 + p].x = re[p * pitch] - im[p * pitch];
-    a.nyq[r0 + p].y = (R)0;
-  }
-}
-
-// Rows per block of the register-staged z passes: half the threads own one
-// stage-1 butterfly (m / R1 per row).
-constexpr int z_rows_per_block(int m, int r1) { return kFftThreads / (m / r1) / 2; }
-
-// Register-staged z pass (FP32, m = R1 * R2, two stages).  Stage 1 reads its
-// R1 inputs per butterfly straight from global memory (row-contiguous across
-// lanes) into registers; stage 2 runs in place in LDS (each thread reads and
-// writes the same R2 slots); the split into the half spectrum then reads the
-// pairs (k, m - k) from LDS and stores whole rows contiguously.  A block holds
-// P rows (half the threads take a stage-1 butterfly) in one plane pair of
-// LDS (16.5 KB at m = 128): small enough to fit beside the deposition's
-// blocks when the frame pipeline runs both, and faster alone than 2P rows.
-template <typename R, int R1, int R2>
-__global__ __launch_bounds__(kFftThreads) void k_r2c_z_reg(R2CArgs<R> a) {
-  constexpr int m = R1 * R2;
-  constexpr int nq1 = m / R1;             // stage-1 butterflies per row
-  constexpr int P = z_rows_per_block(m, R1);
-  constexpr int pitch = m + 1;
-  constexpr int ns = m / R2;              // stage-2 butterflies per row
-  __shared__ R re[P * pitch];
-  __shared__ R im[P * pitch];
-  const long long r0 = (long long)blockIdx.x * P;
-  const int pc = (int)min((long long)P, a.rows - r0);
-  const auto* src = reinterpret_cast<const typename V2<R>::T*>(a.in) + r0 * m;
-  const void* tw = sizeof(R) == 8 ? (const void*)a.ax.tw64 : (const void*)a.ax.tw;
-  if (threadIdx.x < P * nq1) {
-    const int j = threadIdx.x % nq1, p = threadIdx.x / nq1;
-    Cx<R> v[R1];
-#pragma unroll
-    for (int r = 0; r < R1; ++r) {
-      if (p < pc) {
-        const auto x = src[p * m + j + r * nq1];
-        v[r] = mk<R>(x.x, x.y);
-      } else {
-        v[r] = mk<R>(0, 0);
-      }
-    }
-    dft<R1, false>(v);
-#pragma unroll
-    for (int r = 0; r < R1; ++r) {
-      re[p * pitch + j * R1 + r] = v[r].x;
-      im[p * pitch + j * R1 + r] = v[r].y;
-    }
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < P * ns; b += kFftThreads) {
-    const int k = b % ns, p = b / ns;
-    Cx<R> v[R2];
-#pragma unroll
-    for (int r = 0; r < R2; ++r) v[r] = mk<R>(re[p * pitch + k + r * ns], im[p * pitch + k + r * ns]);
-    twiddle_in<R2, false>(v, k, 1, tw);
-    dft<R2, false>(v);
-#pragma unroll
-    for (int r = 0; r < R2; ++r) {
-      re[p * pitch + k + r * ns] = v[r].x;
-      im[p * pitch + k + r * ns] = v[r].y;
-    }
-  }
-  __syncthreads();
-  // split: X_k = A_k + W_nz^k B_k, A = (Z_k + conj Z_{m-k})/2, B = (Z_k - conj Z_{m-k})/(2i)
-  auto* dst = a.main + r0 * m;
-  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
-    const int p = idx / m, k = idx - p * m;
-    const int kk = (k == 0) ? 0 : m - k;
-    const Cx<R> z = mk<R>(re[p * pitch + k], im[p * pitch + k]);
-    const Cx<R> zc = mk<R>(re[p * pitch + kk], -im[p * pitch + kk]);
-    const Cx<R> A = cscale((R)0.5, cadd(z, zc));
-    const Cx<R> d = cscale((R)0.5, csub(z, zc));
-    const Cx<R> B = mk<R>(d.y, -d.x);  // d / i
-    const Cx<R> X = cadd(A, cmul(B, ld_tw<R>(a.tw2, k)));
-    dst[idx].x = X.x;
-    dst[idx].y = X.y;
-  }
-  for (int p = threadIdx.x; p < pc; p += kFftThreads) {
-    // X_m = A_0 - B_0 = Re z0 - Im z0 (real)
-    a.nyq[r0 + p].x = re[p * pitch] - im[p * pitch];
-    a.nyq[r0 + p].y = (R)0;
-  }
-}
-
-// --- adjoint z pass: half spectrum -> real rows ----------------------------------
-// Real adjoint of the R2C along z:  x_n = Re sum_{k=0..m} X_k e^{+2 pi i k n / nz}.
-// With X'_0 = Re X_0, X'_m = Re X_m and X'_k = X_k / 2 inside, this is the
-// Hermitian inverse of X': Z_k = (X'_k + conj X'_{m-k}) + i W^{-k} (X'_k - conj X'_{m-k})
-// and x_{2j} + i x_{2j+1} = sum_k Z_k e^{+2 pi i jk/m}.
-template <typename R, typename OUT>
-struct C2RArgs {
-  const typename V2<R>::T* main;
-  const typename V2<R>::T* nyq;
-  OUT* out;                         // rows of nz reals
-  long long rows;
-  int m;
-  int P;
-  FftAxis ax;
-  const void* tw2;
-};
-
-template <typename R, typename OUT>
-__global__ __launch_bounds__(kFftThreads) void k_c2r_z(C2RArgs<R, OUT> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  R* lds = reinterpret_cast<R*>(lds_raw);
-  const int m = a.m, P = a.P, pitch = lds_pitch(m);
-  R* re = lds;
-  R* im = lds + P * pitch;
-  const long long r0 = (long long)blockIdx.x * P;
-  const int pc = (int)min((long long)P, a.rows - r0);
-  const auto* src = a.main + r0 * m;
-  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
-    const int p = idx / m, k = idx - p * m;
-    Cx<R> x, xc;
-    if (k == 0) {
-      x = mk<R>(src[idx].x, (R)0);
-      xc = mk<R>(a.nyq[r0 + p].x, (R)0);
-    } else {
-      const auto v = src[p * m + (m - k)];
-      x = mk<R>((R)0.5 * src[idx].x, (R)0.5 * src[idx].y);
-      xc = mk<R>((R)0.5 * v.x, (R)-0.5 * v.y);
-    }
-    const Cx<R> e = cadd(x, xc);
-    const Cx<R> o = cmulc(csub(x, xc), ld_tw<R>(a.tw2, k));  // (X - conj X') W^{-k}
-    re[p * pitch + k] = e.x - o.y;                            // e + i o
-    im[p * pitch + k] = e.y + o.x;
-  }
-  __syncthreads();
-  {
-    const int cur = run_stages<true, R>(lds, m, P, pitch, a.ax);
-    re = lds + 2 * cur * P * pitch;
-    im = re + P * pitch;
-  }
-  OUT* dst = a.out + r0 * (2 * (long long)m);
-  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
-    const int p = idx / m, j = idx - p * m;
-    dst[2 * (long long)idx] = (OUT)re[p * pitch + j];
-    dst[2 * (long long)idx + 1] = (OUT)im[p * pitch + j];
-  }
-}
-
-// Register-staged adjoint z pass (FP32, m = R1 * R2): stage 1 forms Z_k from
-// the pair (X_k, X_{m-k}) read straight from global memory, the second stage
-// runs from LDS into registers and stores the real pairs (x_2j, x_2j+1)
-// straight to global memory.  A block reads all of its rows before the
-// barrier, so `out` may alias the spectrum.
-template <typename R, int R1, int R2>
-__global__ __launch_bounds__(kFftThreads) void k_c2r_z_reg(C2RArgs<R, R> a) {
-  constexpr int m = R1 * R2;
-  constexpr int nq1 = m / R1;
-  constexpr int P = z_rows_per_block(m, R1);
-  constexpr int pitch = m + 1;
-  constexpr int ns = m / R2;
-  __shared__ R re[P * pitch];
-  __shared__ R im[P * pitch];
-  const long long r0 = (long long)blockIdx.x * P;
-  const int pc = (int)min((long long)P, a.rows - r0);
-  const auto* src = a.main + r0 * m;
-  const void* tw = sizeof(R) == 8 ? (const void*)a.ax.tw64 : (const void*)a.ax.tw;
-  if (threadIdx.x < P * nq1) {
-    const int j = threadIdx.x % nq1, p = threadIdx.x / nq1;
-    Cx<R> v[R1];
-#pragma unroll
-    for (int r = 0; r < R1; ++r) {
-      const int k = j + r * nq1;
-      if (p < pc) {
-        Cx<R> x, xc;
-        if (k == 0) {
-          x = mk<R>(src[p * m].x, (R)0);
-          xc = mk<R>(a.nyq[r0 + p].x, (R)0);
-        } else {
-          const auto u = src[p * m + k];
-          const auto w = src[p * m + (m - k)];
-          x = mk<R>((R)0.5 * u.x, (R)0.5 * u.y);
-          xc = mk<R>((R)0.5 * w.x, (R)-0.5 * w.y);
-        }
-        const Cx<R> e = cadd(x, xc);
-        const Cx<R> o = cmulc(csub(x, xc), ld_tw<R>(a.tw2, k));  // (X - conj X') W^{-k}
-        v[r] = mk<R>(e.x - o.y, e.y + o.x);                       // e + i o
-      } else {
-        v[r] = mk<R>(0, 0);
-      }
-    }
-    dft<R1, true>(v);
-#pragma unroll
-    for (int r = 0; r < R1; ++r) {
-      re[p * pitch + j * R1 + r] = v[r].x;
-      im[p * pitch + j * R1 + r] = v[r].y;
-    }
-  }
-  __syncthreads();
-  R* dst = a.out + r0 * (2 * (long long)m);
-  for (int b = threadIdx.x; b < P * ns; b += kFftThreads) {
-    const int k = b % ns, p = b / ns;
-    Cx<R> v[R2];
-#pragma unroll
-    for (int r = 0; r < R2; ++r) v[r] = mk<R>(re[p * pitch + k + r * ns], im[p * pitch + k + r * ns]);
-    twiddle_in<R2, true>(v, k, 1, tw);
-    dft<R2, true>(v);
-    if (p < pc) {
-#pragma unroll
-      for (int r = 0; r < R2; ++r) {
-        const long long o = 2 * ((long long)p * m + k + r * ns);
-        dst[o] = v[r].x;
-        dst[o + 1] = v[r].y;
-      }
-    }
-  }
-}
-
-// --- strided pencil pass ------------------------------------------------------
-// One launch transforms two jobs: the main half spectrum and the Nyquist
-// plane (workgroups [0, blocks0) take job 0, the rest job 1).
-template <typename R>
-struct PencilJob {
-  typename V2<R>::T* data;
-  double* acc;              // EPI_ACC: acc += |X|^2 (FP64) at the same index
-  const double* scale;      // if set: X *= scale[index] before the transform
-  long long estride;        // between elements of a pencil
-  long long pstride;        // between adjacent pencils of a tile
-  long long outer_stride;   // between outer groups
-  int pencils;              // pencils per outer group
-  int tiles_inner;          // ceil(pencils / P)
-  int P;
-  int pencil_major;         // elements contiguous (estride == 1)
-};
-
-template <typename R>
-struct PencilArgs {
-  PencilJob<R> job[2];
-  long long blocks0;
-  int n;
-  FftAxis ax;
+// --- pencil passes -----------------------------------------------------------------
+// Pencil p of a block at slow index s and first fast index f0 reads its element
+// e at in + s*in_s + (f0+p)*in_f + e*in_e and writes its element k to
+// out + s*out_s + (f0+p)*out_f + k*out_e.  Blocks run over (s, f-tile).
+struct Geom {
+  long long in_s, in_f, in_e, out_s, out_f, out_e;
+  int nf;            // fast indices per slow index
+  int tiles_f;       // ceil(nf / P)
+  long long blocks;  // slow count * tiles_f
 };
 
 enum Epilogue { EPI_STORE = 0, EPI_ACC = 1 };
+enum Residual { RES_NONE = 0, RES_OUT = 1, RES_IN = 2 };
 
 template <typename R>
-int tile_pencils(int n);
+struct PassArgs {
+  const typename V2<R>::T* in;
+  typename V2<R>::T* out;
+  double* acc;                  // EPI_ACC: |X|^2 added at the output index
+  const double* w;              // X *= w[input index] before the transform (inverse x pass)
+  const unsigned char* res_in;  // RES_IN: residual nibbles at the input index
+  unsigned char* res_out;       // RES_OUT: residual nibbles at the output index
+  Geom g;
+  int P;
+  int n;
+  FftAxis ax;
+  // Nyquist plane (FP64, in place): blocks [g.blocks, g.blocks + nq_pencils)
+  // take one pencil each: element e at nyq + pencil*nq_p + e*nq_e
+  double2* nyq;
+  double* nyq_acc;
+  const double* nyq_w;
+  long long nq_p, nq_e;
+  int nq_pencils;
+  int tpb;                      // tiles per block of the register-staged pass
+};
 
-// Generic tile: load P pencils into LDS, all stages in LDS, store / accumulate.
-template <bool INV, typename R, int EPI>
-__device__ void generic_tile(const PencilJob<R>& J, long long tile, int n, const FftAxis& ax,
-                             R* lds) {
-  const int P = J.P, pitch = lds_pitch(n);
-  R* re = lds;
-  R* im = lds + P * pitch;
-  const long long outer = tile / J.tiles_inner;
-  const int inner = (int)(tile - outer * J.tiles_inner);
-  const int p0 = inner * P;
-  const int pc = min(P, J.pencils - p0);
-  const long long base = outer * J.outer_stride + (long long)p0 * J.pstride;
-  const int total = pc * n;
-  for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
-    int p, e;
-    if (J.pencil_major) {
-      p = idx / n;
-      e = idx - p * n;
-    } else {
-      e = idx / pc;
-      p = idx - e * pc;
-    }
-    const long long g = base + p * J.pstride + e * J.estride;
-    const auto v = J.data[g];
-    R sc = (R)1;
-    if (J.scale) sc = (R)J.scale[g];
-    re[p * pitch + e] = v.x * sc;
-    im[p * pitch + e] = v.y * sc;
+// One Nyquist-plane pencil per block, every stage in LDS (FP64).
+template <bool INV, int EPI, typename A>
+__device__ void nyq_pencil(const A& a, int pencil, double* lds) {
+  const int n = a.n, pitch = lds_pitch(n);
+  double* re = lds;
+  double* im = lds + pitch;
+  const long long base = (long long)pencil * a.nq_p;
+  __syncthreads();  // the LDS may still be read for the previous pencil or tile
+  for (int e = threadIdx.x; e < n; e += kFftThreads) {
+    const long long g = base + e * a.nq_e;
+    const double2 v = a.nyq[g];
+    const double sc = a.nyq_w ? a.nyq_w[g] : 1.0;
+    re[e] = v.x * sc;
+    im[e] = v.y * sc;
   }
   __syncthreads();
-  {
-    const int cur = run_stages<INV, R>(lds, n, P, pitch, ax);
-    re = lds + 2 * cur * P * pitch;
-    im = re + P * pitch;
-  }
-  for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
-    int p, e;
-    if (J.pencil_major) {
-      p = idx / n;
-      e = idx - p * n;
-    } else {
-      e = idx / pc;
-      p = idx - e * pc;
-    }
-    const R xr = re[p * pitch + e], xi = im[p * pitch + e];
-    const long long g = base + p * J.pstride + e * J.estride;
-    if (EPI == EPI_STORE) {
-      J.data[g].x = xr;
-      J.data[g].y = xi;
-    } else {
-      J.acc[g] += (double)xr * (double)xr + (double)xi * (double)xi;
-    }
+  const int cur = run_stages<INV>(lds, n, 1, pitch, a.ax);
+  re = lds + 2 * cur * pitch;
+  im = re + pitch;
+  for (int e = threadIdx.x; e < n; e += kFftThreads) {
+    const long long g = base + e * a.nq_e;
+    if (EPI == EPI_STORE) a.nyq[g] = make_double2(re[e], im[e]);
+    else a.nyq_acc[g] += re[e] * re[e] + im[e] * im[e];
   }
 }
 
-template <bool INV, typename R, int EPI>
-__global__ __launch_bounds__(kFftThreads) void k_pencils(PencilArgs<R> a) {
+// Global element access of a pass: load with the optional weight and
+// residual, store / accumulate with the optional residual.
+template <typename R, int RES>
+__device__ __forceinline__ Cx pass_load(const PassArgs<R>& a, long long gi) {
+  Cx x = load_cx<R>(a.in, gi);
+  if constexpr (RES == RES_IN) {
+    const unsigned b = a.res_in[gi];
+    x.x += res_value(b & 15u, (float)x.x);
+    x.y += res_value(b >> 4, (float)x.y);
+  }
+  if (a.w) x = cscale(a.w[gi], x);
+  return x;
+}
+template <typename R, int EPI, int RES>
+__device__ __forceinline__ void pass_store(const PassArgs<R>& a, long long go, Cx v) {
+  if constexpr (EPI == EPI_ACC) {
+    a.acc[go] += v.x * v.x + v.y * v.y;
+  } else {
+    store_cx<R>(a.out, go, v);
+    if constexpr (RES == RES_OUT)
+      a.res_out[go] = (unsigned char)(res_nibble(v.x, (float)v.x) | (res_nibble(v.y, (float)v.y) << 4));
+  }
+}
+
+// The register-staged pass for n = R1 R2 R3 (R3 = 1: two stages).  A tile is
+// P = 4096 / n pencils; each thread holds 16 values per stage (16 / R
+// butterflies of radix R).  IN_E / OUT_E: the input / output elements of a
+// pencil are adjacent in memory (lanes element-fastest, and in_e / out_e = 1),
+// else the pencils are (lanes pencil-fastest).
+template <int n>
+constexpr int reg_pencils() { return kRegElems / n; }
+
+template <int n>
+__host__ __device__ constexpr int reg_pitch() { return n + n / 16 + 1; }
+__device__ __forceinline__ int reg_pos(int i) { return i + (i >> 4); }
+
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it (the next tile's inputs) stay in flight across it.  The empty
+// asm statements keep the compiler from moving memory accesses (twiddle
+// loads in particular) across it, which would hold them in registers
+// through the exchange.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  asm volatile("" ::: "memory");
+}
+
+// x, hidden from the optimiser: per-lane LDS positions derived from it are
+// recomputed in each tile iteration instead of being hoisted out of the tile
+// loop and held in registers for the whole kernel.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// Element `off` (a 32-bit offset) of an array at a wave-uniform base.
+template <typename T>
+__device__ __forceinline__ const T& at(const T* base, unsigned off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T& at(T* base, unsigned off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off * (unsigned)sizeof(T));
+}
+
+// Moves each thread's values v[i] (slot i at LDS position pos_out(i) of the
+// pencil plane, negative: unused) to its next-stage inputs, v[i] read back
+// from pos_in(i), real and imaginary parts in turn through one FP64 plane.
+// Positions are recomputed per slot rather than kept in registers.
+template <int CNT, typename PO, typename PI>
+__device__ __forceinline__ void lds_exchange(double* plane, Cx* v, PO pos_out, PI pos_in) {
+  lds_barrier();
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) {
+    const int q = pos_out(i);
+    if (q >= 0) plane[q] = v[i].x;
+  }
+  lds_barrier();
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) {
+    const int q = pos_in(i);
+    if (q >= 0) v[i].x = plane[q];
+  }
+  lds_barrier();
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) {
+    const int q = pos_out(i);
+    if (q >= 0) plane[q] = v[i].y;
+  }
+  lds_barrier();
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) {
+    const int q = pos_in(i);
+    if (q >= 0) v[i].y = plane[q];
+  }
+}
+
+// Tiles per block of the register-staged passes: about two blocks per
+// resident slot on the device (blocks resident at once: occupancy of this
+// kernel times the CU count, cached per kernel and device), at most 8.  A
+// block transforms its tiles in turn and loads the next one's inputs while it
+// transforms the current one; the block scheduler still balances the load
+// when the pass shares the GPU with the deposition.
+inline int stream_tpb(const void* kernel, size_t lds, long long ntiles) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, long long> cap;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  long long c;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cap.find({kernel, dev});
+    if (it != cap.end()) {
+      c = it->second;
+    } else {
+      int nb = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kFftThreads, lds) != hipSuccess ||
+          nb <= 0)
+        nb = 1;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          cus <= 0)
+        cus = 1;
+      c = (long long)nb * cus;
+      cap[{kernel, dev}] = c;
+    }
+  }
+  const long long t = ntiles / (2 * c);
+  return (int)(t < 1 ? 1 : (t > 8 ? 8 : t));
+}
+
+template <typename R, int R1, int R2, int R3, bool INV, bool IN_E, bool OUT_E, int EPI, int RES,
+          bool WGT>
+__global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  R* lds = reinterpret_cast<R*>(lds_raw);
-  const bool second = (long long)blockIdx.x >= a.blocks0;
-  const long long tile = second ? (long long)blockIdx.x - a.blocks0 : (long long)blockIdx.x;
-  generic_tile<INV, R, EPI>(second ? a.job[1] : a.job[0], tile, a.n, a.ax, lds);
-}
-
-// Register-staged tile for n = R1*R2*R3 (R3 = 1: two stages) with pencils
-// adjacent in memory: the first stage reads its inputs straight from global
-// memory (16 pencils = 128-byte rows), the last stage writes its outputs
-// straight to global memory (or folds |X|^2 into the FP64 accumulator), so
-// a two-stage transform makes a single round trip through LDS.
-template <typename R>
-constexpr int reg_tile_pencils(int stages) {
-  return (sizeof(R) == 4 ? 16 : 8) / (stages == 3 ? 2 : 1);
-}
-
-template <int RAD, bool INV, typename R>
-__device__ __forceinline__ void twiddle_in(Cx<R>* v, int k, int step, const void* tw) {
-#pragma unroll
-  for (int r = 1; r < RAD; ++r) {
-    if (k > 0) {
-      const Cx<R> w = ld_tw<R>(tw, k * r * step);
-      v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
-    }
-  }
-}
-
-template <typename R, int R1, int R2, int R3, bool INV, int EPI>
-__device__ void reg_tile(const PencilJob<R>& J, long long tile, const FftAxis& ax, R* lds) {
+  double* plane = reinterpret_cast<double*>(lds_raw);
   constexpr int n = R1 * R2 * R3;
-  constexpr int P = reg_tile_pencils<R>(R3 == 1 ? 2 : 3);
-  constexpr int pitch = n + 1;
-  constexpr int plane = P * pitch;
-  const void* tw = sizeof(R) == 8 ? (const void*)ax.tw64 : (const void*)ax.tw;
-  const long long outer = tile / J.tiles_inner;
-  const int inner = (int)(tile - outer * J.tiles_inner);
-  const int p0 = inner * P;
-  const int pc = min(P, J.pencils - p0);
-  const long long base = outer * J.outer_stride + (long long)p0 * J.pstride;
-  R* re0 = lds;
-  R* im0 = lds + plane;
-  // stage 1 (span 1): global -> DFT_R1 -> LDS plane 0 at j*R1 + r
-  {
-    constexpr int nq = n / R1;
-    for (int b = threadIdx.x; b < P * nq; b += kFftThreads) {
-      const int p = b % P, j = b / P;
-      Cx<R> v[R1];
+  constexpr int P = reg_pencils<n>();
+  constexpr int pitch = reg_pitch<n>();
+  constexpr int RL = R3 > 1 ? R3 : R2;  // radix of the last stage
+  constexpr int nq1 = n / R1, nsL = n / RL;
+  constexpr bool PF = sizeof(R) == 4;   // prefetch (FP32 storage)
+  using T2 = typename V2<R>::T;
+  const Geom& g = a.g;
+  int tid = threadIdx.x;
+  const unsigned in_e = IN_E ? 1u : (unsigned)g.in_e, out_e = OUT_E ? 1u : (unsigned)g.out_e;
+  const unsigned in_f = (unsigned)g.in_f, out_f = (unsigned)g.out_f;
+  const double2* tw = a.ax.tw64;
+  const long long ntiles = g.blocks;
+  const unsigned tiles_f = (unsigned)g.tiles_f;
+  // stage-1 slot i = t R1 + r: pencil p, element j + r nq1
+  auto in_pos = [&](int t, int pc, unsigned* o) __attribute__((always_inline)) {
+    const int b = tid + t * kFftThreads;
+    int p = IN_E ? b / nq1 : b % P;
+    const int j = IN_E ? b % nq1 : b / P;
+    p = p < pc ? p : pc - 1;  // pencils past a partial tile read the last one
+    *o = (unsigned)p * in_f + (unsigned)j * in_e;
+  };
+  T2 raw[16];
+  unsigned char rr[RES == RES_IN ? 16 : 1];
+  auto load = [&](long long tile) __attribute__((always_inline)) {
+    const unsigned s = (unsigned)(tile / tiles_f);
+    const int f0 = (int)((unsigned)tile - s * tiles_f) * P;
+    const int pc = min(P, g.nf - f0);
+    const long long ib = (long long)s * g.in_s + (long long)f0 * g.in_f;
+    const T2* src = a.in + ib;
+#pragma unroll
+    for (int t = 0; t < 16 / R1; ++t) {
+      unsigned o;
+      in_pos(t, pc, &o);
 #pragma unroll
       for (int r = 0; r < R1; ++r) {
-        const long long g = base + p * J.pstride + (long long)(j + r * nq) * J.estride;
-        if (p < pc) {
-          const auto x = J.data[g];
-          const R sc = J.scale ? (R)J.scale[g] : (R)1;
-          v[r] = mk<R>(x.x * sc, x.y * sc);
-        } else {
-          v[r] = mk<R>(0, 0);
-        }
-      }
-      dft<R1, INV>(v);
-#pragma unroll
-      for (int r = 0; r < R1; ++r) {
-        re0[p * pitch + j * R1 + r] = v[r].x;
-        im0[p * pitch + j * R1 + r] = v[r].y;
+        const unsigned oe = o + (unsigned)(r * nq1) * in_e;
+        raw[t * R1 + r] = at(src, oe);
+        if constexpr (RES == RES_IN) rr[t * R1 + r] = at(a.res_in + ib, oe);
       }
     }
-    __syncthreads();
+  };
+  const long long nblk = (ntiles + a.tpb - 1) / a.tpb;
+  if ((long long)blockIdx.x >= nblk) {
+    // the Nyquist-plane pencils: one per block after the tile blocks
+    if (a.nyq) nyq_pencil<INV, EPI>(a, (int)((long long)blockIdx.x - nblk), plane);
+    return;
   }
-  const R* sre = re0;
-  const R* sim = im0;
-  if constexpr (R3 > 1) {
-    // middle stage (span R1): LDS plane 0 -> DFT_R2 -> LDS plane 1
-    R* re1 = lds + 2 * plane;
-    R* im1 = lds + 3 * plane;
-    constexpr int ns = R1, nq = n / R2;
-    for (int b = threadIdx.x; b < P * nq; b += kFftThreads) {
-      const int p = b % P, j = b / P, k = j % ns;
-      Cx<R> v[R2];
+  long long tile = (long long)blockIdx.x * a.tpb;
+  const long long tend = min(ntiles, tile + a.tpb);
+  if (PF) load(tile);
+  for (; tile < tend; ++tile) {
+    tid = opaque(tid);
+    if (!PF) load(tile);
+    const unsigned s = (unsigned)(tile / tiles_f);
+    const int f0 = (int)((unsigned)tile - s * tiles_f) * P;
+    const int pc = min(P, g.nf - f0);
+    const long long ib = (long long)s * g.in_s + (long long)f0 * g.in_f;
+    const long long ob = (long long)s * g.out_s + (long long)f0 * g.out_f;
+    Cx v[16];
 #pragma unroll
-      for (int r = 0; r < R2; ++r) v[r] = mk<R>(re0[p * pitch + j + r * nq], im0[p * pitch + j + r * nq]);
-      twiddle_in<R2, INV>(v, k, n / (ns * R2), tw);
-      dft<R2, INV>(v);
-      const int d0 = (j / ns) * ns * R2 + k;
-#pragma unroll
-      for (int r = 0; r < R2; ++r) {
-        re1[p * pitch + d0 + r * ns] = v[r].x;
-        im1[p * pitch + d0 + r * ns] = v[r].y;
+    for (int i = 0; i < 16; ++i) {
+      v[i] = mk((double)raw[i].x, (double)raw[i].y);
+      if constexpr (RES == RES_IN) {
+        v[i].x += res_value(rr[i] & 15u, raw[i].x);
+        v[i].y += res_value(rr[i] >> 4, raw[i].y);
       }
     }
-    __syncthreads();
-    sre = re1;
-    sim = im1;
-  }
-  // last stage (span n / RL): LDS -> DFT_RL -> global, element k + r * ns
-  {
-    constexpr int RL = R3 > 1 ? R3 : R2;
-    constexpr int ns = n / RL, nq = n / RL;
-    for (int b = threadIdx.x; b < P * nq; b += kFftThreads) {
-      const int p = b % P, k = b / P;  // k < ns
-      Cx<R> v[RL];
+    if constexpr (WGT) {
 #pragma unroll
-      for (int r = 0; r < RL; ++r) v[r] = mk<R>(sre[p * pitch + k + r * nq], sim[p * pitch + k + r * nq]);
-      twiddle_in<RL, INV>(v, k, 1, tw);
-      dft<RL, INV>(v);
+      for (int t = 0; t < 16 / R1; ++t) {
+        unsigned o;
+        in_pos(t, pc, &o);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) v[t * R1 + r] = cscale(at(a.w + ib, o + (unsigned)(r * nq1) * in_e), v[t * R1 + r]);
+      }
+    }
+    if (PF && tile + 1 < tend) load(tile + 1);
+    // stage 1 (span 1): DFT_R1; output position j R1 + r
+#pragma unroll
+    for (int t = 0; t < 16 / R1; ++t) dft<R1, INV>(v + t * R1);
+    auto out1 = [&](int i) __attribute__((always_inline)) {
+      const int b = tid + (i / R1) * kFftThreads, r = i % R1;
+      const int p = IN_E ? b / nq1 : b % P, j = IN_E ? b % nq1 : b / P;
+      return p * pitch + reg_pos(j * R1 + r);
+    };
+    // last stage inputs: element k + r nsL of pencil p
+    auto inL = [&](int i) __attribute__((always_inline)) {
+      const int b = tid + (i / RL) * kFftThreads, r = i % RL;
+      const int p = OUT_E ? b / nsL : b % P, k = OUT_E ? b % nsL : b / P;
+      return p * pitch + reg_pos(k + r * nsL);
+    };
+    if constexpr (R3 > 1) {
+      // middle stage (span R1, radix R2): LDS -> DFT_R2 -> LDS
+      constexpr int nq2 = n / R2, ns2 = R1;
+      auto in2 = [&](int i) __attribute__((always_inline)) {
+        const int b = tid + (i / R2) * kFftThreads, r = i % R2;
+        return (b % P) * pitch + reg_pos(b / P + r * nq2);
+      };
+      auto out2 = [&](int i) __attribute__((always_inline)) {
+        const int b = tid + (i / R2) * kFftThreads, r = i % R2;
+        const int j = b / P, k = j % ns2;
+        return (b % P) * pitch + reg_pos((j / ns2) * ns2 * R2 + k + r * ns2);
+      };
+      lds_exchange<16>(plane, v, out1, in2);
+#pragma unroll
+      for (int t = 0; t < 16 / R2; ++t) {
+        const int b = tid + t * kFftThreads;
+        twiddle_in<R2, INV>(v + t * R2, (b / P) % ns2, n / (ns2 * R2), tw);
+        dft<R2, INV>(v + t * R2);
+      }
+      lds_exchange<16>(plane, v, out2, inL);
+    } else {
+      lds_exchange<16>(plane, v, out1, inL);
+    }
+#pragma unroll
+    for (int t = 0; t < 16 / RL; ++t) {
+      const int b = tid + t * kFftThreads;
+      const int p = OUT_E ? b / nsL : b % P, k = OUT_E ? b % nsL : b / P;
+      Cx* x = v + t * RL;
+      twiddle_in<RL, INV>(x, k, 1, tw);
+      dft<RL, INV>(x);
       if (p < pc) {
+        const unsigned o = (unsigned)p * out_f + (unsigned)k * out_e;
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
-          const long long g = base + p * J.pstride + (long long)(k + r * ns) * J.estride;
-          if (EPI == EPI_STORE) {
-            J.data[g].x = v[r].x;
-            J.data[g].y = v[r].y;
+          const unsigned oe = o + (unsigned)(r * nsL) * out_e;
+          if constexpr (EPI == EPI_ACC) {
+            double& d = at(a.acc + ob, oe);
+            d += x[r].x * x[r].x + x[r].y * x[r].y;
           } else {
-            J.acc[g] += (double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y;
+            T2 st;
+            st.x = (R)x[r].x;
+            st.y = (R)x[r].y;
+            at(a.out + ob, oe) = st;
+            if constexpr (RES == RES_OUT)
+              at(a.res_out + ob, oe) =
+                  (unsigned char)(res_nibble(x[r].x, st.x) | (res_nibble(x[r].y, st.y) << 4));
           }
         }
       }
@@ -750,30 +674,360 @@ __device__ void reg_tile(const PencilJob<R>& J, long long tile, const FftAxis& a
   }
 }
 
-template <typename R, int R1, int R2, int R3, bool INV, int EPI>
-__global__ __launch_bounds__(kFftThreads) void k_pencils_reg(PencilArgs<R> a) {
+// The LDS-staged pass (any n): P pencils loaded into LDS, every stage there.
+template <typename R, bool INV, int EPI, int RES>
+__global__ __launch_bounds__(kFftThreads) void k_pass_lds(PassArgs<R> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  R* lds = reinterpret_cast<R*>(lds_raw);
-  if ((long long)blockIdx.x < a.blocks0) {
-    reg_tile<R, R1, R2, R3, INV, EPI>(a.job[0], (long long)blockIdx.x, a.ax, lds);
+  double* lds = reinterpret_cast<double*>(lds_raw);
+  const long long blk = blockIdx.x;
+  if (blk >= a.g.blocks) {
+    nyq_pencil<INV, EPI>(a, (int)(blk - a.g.blocks), lds);
+    return;
+  }
+  const Geom& g = a.g;
+  const int n = a.n, P = a.P, pitch = lds_pitch(n);
+  const long long s = blk / g.tiles_f;
+  const int f0 = (int)(blk - s * g.tiles_f) * P;
+  const int pc = min(P, g.nf - f0);
+  const long long ib = s * g.in_s + (long long)f0 * g.in_f;
+  const long long ob = s * g.out_s + (long long)f0 * g.out_f;
+  double* re = lds;
+  double* im = lds + P * pitch;
+  const bool in_e = g.in_e == 1, out_e = g.out_e == 1;
+  for (int idx = threadIdx.x; idx < pc * n; idx += kFftThreads) {
+    const int p = in_e ? idx / n : idx % pc, e = in_e ? idx % n : idx / pc;
+    const Cx x = pass_load<R, RES>(a, ib + (long long)p * g.in_f + (long long)e * g.in_e);
+    re[p * pitch + e] = x.x;
+    im[p * pitch + e] = x.y;
+  }
+  __syncthreads();
+  const int cur = run_stages<INV>(lds, n, P, pitch, a.ax);
+  re = lds + 2 * cur * P * pitch;
+  im = re + P * pitch;
+  for (int idx = threadIdx.x; idx < pc * n; idx += kFftThreads) {
+    const int p = out_e ? idx / n : idx % pc, k = out_e ? idx % n : idx / pc;
+    pass_store<R, EPI, RES>(a, ob + (long long)p * g.out_f + (long long)k * g.out_e,
+                            mk(re[p * pitch + k], im[p * pitch + k]));
+  }
+}
+
+// --- z pass: real rows -> half spectrum --------------------------------------------
+// Rows (x, y) of nz reals are read as m = nz/2 complex values Z and
+// transformed; the half spectrum follows from the real-pair split
+//   X_k = A_k + W_nz^k B_k,  A = (Z_k + conj Z_{m-k}) / 2,  B = (Z_k - conj Z_{m-k}) / 2i,
+//   X_m = Re Z_0 - Im Z_0.
+// Block = P rows y0..y0+P-1 of one x; output spec[(x m + k) Ny + y] in runs
+// of P, nyq[x Ny + y] (FP64).
+template <typename R>
+struct ZArgs {
+  const R* in;                       // density rows (x, y) of nz reals
+  typename V2<R>::T* out;            // spec[x][kz][y]
+  double2* nyq;                      // nyq[x][y]
+  int nx, ny, m;
+  int P, tiles_y;
+  FftAxis ax;                        // length m
+  const double2* tw2;                // W_nz^k, k < nz
+  int tpb;                           // tiles per block of the register-staged pass
+};
+
+__device__ __forceinline__ Cx split_bin(Cx z, Cx zc_conj_src, const double2* tw2, int k) {
+  // zc_conj_src = Z_{m-k} (Z_0 for k = 0)
+  const Cx zc = mk(zc_conj_src.x, -zc_conj_src.y);
+  const Cx A = cscale(0.5, cadd(z, zc));
+  const Cx d = cscale(0.5, csub(z, zc));
+  const Cx B = mk(d.y, -d.x);  // d / i
+  return cadd(A, cmul(B, ld_tw(tw2, k)));
+}
+
+template <typename R>
+__device__ __forceinline__ void z_store(const ZArgs<R>& a, int x, int y, int k, Cx v) {
+  store_cx<R>(a.out, ((long long)x * a.m + k) * a.ny + y, v);
+}
+
+// Register-staged z pass (c2r form below keeps 2048 values per block).
+template <int m>
+constexpr int zreg_rows() { return kZRegElems / m; }
+
+// Persistent register-staged z pass, m = R1 R2 R3: a tile is P = 4096 / m rows
+// y0..y0+P-1 of one x, 16 values per thread and stage.  The last stage pairs
+// its butterflies k and ns - k (0 with ns/2) in one thread, which then holds
+// Z_k and Z_{m-k} for each of its bins and applies the real-pair split in
+// registers.  Block b takes tiles b, b + gridDim.x, ... and loads the next
+// tile's rows while it transforms the current one (FP32 storage).
+template <typename R, int R1, int R2, int R3>
+__global__ __launch_bounds__(kFftThreads) void k_r2c_z_stream(ZArgs<R> a) {
+  constexpr int m = R1 * R2 * R3;
+  constexpr int P = kRegElems / m;
+  constexpr int pitch = reg_pitch<m>();
+  constexpr int RL = R3 > 1 ? R3 : R2;
+  constexpr int nq1 = m / R1, ns = m / RL, NB = 16 / RL;
+  static_assert(NB % 2 == 0, "the last stage pairs butterflies within a thread");
+  constexpr bool PF = sizeof(R) == 4;
+  using T2 = typename V2<R>::T;
+  __shared__ double plane[P * pitch];
+  int tid = threadIdx.x;
+  const double2* tw = a.ax.tw64;
+  const long long ntiles = (long long)a.nx * a.tiles_y;
+  T2 raw[16];
+  auto load = [&](long long tile) __attribute__((always_inline)) {
+    const int x = (int)(tile / a.tiles_y);
+    const int y0 = (int)(tile - (long long)x * a.tiles_y) * P;
+    const int pc = min(P, a.ny - y0);
+    const T2* src = reinterpret_cast<const T2*>(a.in) + ((long long)x * a.ny + y0) * m;
+#pragma unroll
+    for (int t = 0; t < 16 / R1; ++t) {
+      const int b = tid + t * kFftThreads;
+      int p = b / nq1;
+      const int j = b % nq1;
+      p = p < pc ? p : pc - 1;
+#pragma unroll
+      for (int r = 0; r < R1; ++r) raw[t * R1 + r] = at(src, (unsigned)(p * m + j + r * nq1));
+    }
+  };
+  long long tile = (long long)blockIdx.x * a.tpb;
+  const long long tend = min(ntiles, tile + a.tpb);
+  if (PF && tile < tend) load(tile);
+  for (; tile < tend; ++tile) {
+    tid = opaque(tid);
+    if (!PF) load(tile);
+    const int x = (int)(tile / a.tiles_y);
+    const int y0 = (int)(tile - (long long)x * a.tiles_y) * P;
+    const int pc = min(P, a.ny - y0);
+    Cx v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = mk((double)raw[i].x, (double)raw[i].y);
+    if (PF && tile + 1 < tend) load(tile + 1);
+    // stage 1: rows along the lanes; output position j R1 + r
+#pragma unroll
+    for (int t = 0; t < 16 / R1; ++t) dft<R1, false>(v + t * R1);
+    auto out1 = [&](int i) __attribute__((always_inline)) {
+      const int b = tid + (i / R1) * kFftThreads, r = i % R1;
+      return (b / nq1) * pitch + reg_pos((b % nq1) * R1 + r);
+    };
+    // last stage: slot i = (2u + h) RL + r, pair c = tid + u 256 of row c % P,
+    // butterfly kA = c / P (h = 0) or its partner kB (h = 1), bin k + r ns
+    auto kpair = [&](int u, int h) __attribute__((always_inline)) {
+      const int kp = (tid + u * kFftThreads) / P;
+      return h == 0 ? kp : (kp == 0 ? ns / 2 : ns - kp);
+    };
+    auto inL = [&](int i) __attribute__((always_inline)) {
+      const int r = i % RL, h = (i / RL) & 1, u = i / (2 * RL);
+      const int p = (tid + u * kFftThreads) % P;
+      return p * pitch + reg_pos(kpair(u, h) + r * ns);
+    };
+    if constexpr (R3 > 1) {
+      constexpr int nq2 = m / R2, ns2 = R1;
+      auto in2 = [&](int i) __attribute__((always_inline)) {
+        const int b = tid + (i / R2) * kFftThreads, r = i % R2;
+        return (b % P) * pitch + reg_pos(b / P + r * nq2);
+      };
+      auto out2 = [&](int i) __attribute__((always_inline)) {
+        const int b = tid + (i / R2) * kFftThreads, r = i % R2;
+        const int j = b / P, k = j % ns2;
+        return (b % P) * pitch + reg_pos((j / ns2) * ns2 * R2 + k + r * ns2);
+      };
+      lds_exchange<16>(plane, v, out1, in2);
+#pragma unroll
+      for (int t = 0; t < 16 / R2; ++t) {
+        const int b = tid + t * kFftThreads;
+        twiddle_in<R2, false>(v + t * R2, (b / P) % ns2, m / (ns2 * R2), tw);
+        dft<R2, false>(v + t * R2);
+      }
+      lds_exchange<16>(plane, v, out2, inL);
+    } else {
+      lds_exchange<16>(plane, v, out1, inL);
+    }
+#pragma unroll
+    for (int u = 0; u < NB / 2; ++u) {
+      const int c = tid + u * kFftThreads;
+      const int p = c % P, kA = kpair(u, 0), kB = kpair(u, 1);
+      Cx* za = v + (2 * u) * RL;
+      Cx* zb = v + (2 * u + 1) * RL;
+      twiddle_in<RL, false>(za, kA, 1, tw);
+      dft<RL, false>(za);
+      twiddle_in<RL, false>(zb, kB, 1, tw);
+      dft<RL, false>(zb);
+      if (p >= pc) continue;
+      T2* dst = a.out + (long long)x * m * a.ny + y0 + p;
+      // bin k of this row at dst[k Ny]; Z_{m-k} of bin kA + r ns is za[RL - r]
+      // (kA = 0, r > 0), za[0] (kA = r = 0) or zb[RL - 1 - r]; of bin kB + r ns
+      // it is zb[RL - 1 - r] (kA = 0) or za[RL - 1 - r]
+      const bool self = kA == 0;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int k1 = kA + r * ns, k2 = kB + r * ns;
+        const Cx m1 = csel(self, za[r == 0 ? 0 : RL - r], zb[RL - 1 - r]);
+        const Cx m2 = csel(self, zb[RL - 1 - r], za[RL - 1 - r]);
+        const Cx x1 = split_bin(za[r], m1, a.tw2, k1);
+        const Cx x2 = split_bin(zb[r], m2, a.tw2, k2);
+        T2 s1, s2;
+        s1.x = (R)x1.x;
+        s1.y = (R)x1.y;
+        s2.x = (R)x2.x;
+        s2.y = (R)x2.y;
+        at(dst, (unsigned)(k1 * a.ny)) = s1;
+        at(dst, (unsigned)(k2 * a.ny)) = s2;
+      }
+      if (self) a.nyq[(long long)x * a.ny + y0 + p] = make_double2(za[0].x - za[0].y, 0.0);
+    }
+  }
+}
+
+// LDS-staged z pass (any m): P rows per block, every stage in LDS.
+template <typename R>
+__global__ __launch_bounds__(kFftThreads) void k_r2c_z(ZArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* lds = reinterpret_cast<double*>(lds_raw);
+  const int m = a.m, P = a.P, pitch = lds_pitch(m);
+  const int x = blockIdx.x / a.tiles_y;
+  const int y0 = (blockIdx.x - x * a.tiles_y) * P;
+  const int pc = min(P, a.ny - y0);
+  const auto* src = reinterpret_cast<const typename V2<R>::T*>(a.in) + ((long long)x * a.ny + y0) * m;
+  double* re = lds;
+  double* im = lds + P * pitch;
+  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
+    const int p = idx / m, e = idx - p * m;
+    const Cx v = load_cx<R>(src, idx);
+    re[p * pitch + e] = v.x;
+    im[p * pitch + e] = v.y;
+  }
+  __syncthreads();
+  const int cur = run_stages<false>(lds, m, P, pitch, a.ax);
+  re = lds + 2 * cur * P * pitch;
+  im = re + P * pitch;
+  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
+    const int p = idx % pc, k = idx / pc;
+    const int kk = k == 0 ? 0 : m - k;
+    const Cx z = mk(re[p * pitch + k], im[p * pitch + k]);
+    const Cx zm = mk(re[p * pitch + kk], im[p * pitch + kk]);
+    z_store<R>(a, x, y0 + p, k, split_bin(z, zm, a.tw2, k));
+  }
+  for (int p = threadIdx.x; p < pc; p += kFftThreads)
+    a.nyq[(long long)x * a.ny + y0 + p] = make_double2(re[p * pitch] - im[p * pitch], 0.0);
+}
+
+// --- inverse z pass: half spectrum -> real rows ---------------------------------------
+// Real adjoint of the R2C along z: x_n = Re sum_{k=0..m} X_k e^{+2 pi i k n / nz}.
+// With X'_0 = Re X_0, X'_m = Re X_m and X'_k = X_k / 2 inside, this is the
+// Hermitian inverse of X': Z_k = (X'_k + conj X'_{m-k}) + i W^{-k} (X'_k - conj X'_{m-k})
+// and x_{2j} + i x_{2j+1} = sum_k Z_k e^{+2 pi i jk/m}.  Reads spec[x][kz][y]
+// in runs of P (pencils y), writes density rows.
+template <typename R>
+struct CArgs {
+  const typename V2<R>::T* spec;  // spec[x][kz][y]
+  const double2* nyq;              // nyq[x][y]
+  R* out;                          // rows (x, y) of nz reals
+  int nx, ny, m;
+  int P, tiles_y;
+  FftAxis ax;
+  const double2* tw2;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx c2r_input(const CArgs<R>& a, int x, int y, int k) {
+  const long long base = (long long)x * a.m * a.ny + y;
+  Cx xk, xc;
+  if (k == 0) {
+    xk = mk(load_cx<R>(a.spec, base).x, 0.0);
+    xc = mk(a.nyq[(long long)x * a.ny + y].x, 0.0);
   } else {
-    generic_tile<INV, R, EPI>(a.job[1], (long long)blockIdx.x - a.blocks0, a.n, a.ax, lds);
+    const Cx u = load_cx<R>(a.spec, base + (long long)k * a.ny);
+    const Cx w = load_cx<R>(a.spec, base + (long long)(a.m - k) * a.ny);
+    xk = cscale(0.5, u);
+    xc = mk(0.5 * w.x, -0.5 * w.y);
+  }
+  const Cx e = cadd(xk, xc);
+  const Cx o = cmulc(csub(xk, xc), ld_tw(a.tw2, k));  // (X - conj X') W^{-k}
+  return mk(e.x - o.y, e.y + o.x);                    // e + i o
+}
+
+template <typename R, int R1, int R2>
+__global__ __launch_bounds__(kFftThreads) void k_c2r_z_reg(CArgs<R> a) {
+  constexpr int m = R1 * R2;
+  constexpr int P = zreg_rows<m>();
+  constexpr int pitch = reg_pitch<m>();
+  constexpr int nq1 = m / R1, ns = m / R2;
+  constexpr int T1 = (P * nq1 + kFftThreads - 1) / kFftThreads;
+  constexpr int T2 = (P * ns + kFftThreads - 1) / kFftThreads;
+  constexpr int VPT = T1 * R1 > T2 * R2 ? T1 * R1 : T2 * R2;
+  __shared__ double plane[P * pitch];
+  const int x = blockIdx.x / a.tiles_y;
+  const int y0 = (blockIdx.x - x * a.tiles_y) * P;
+  const int pc = min(P, a.ny - y0);
+  const double2* tw = a.ax.tw64;
+  const int tid = threadIdx.x;
+  Cx v[VPT];
+  // stage 1: inputs Z_k built from spec runs (pencil-fastest lanes)
+  auto out1 = [&](int i) __attribute__((always_inline)) {
+    const int b = tid + (i / R1) * kFftThreads, r = i % R1;
+    if (i >= T1 * R1 || b >= P * nq1) return -1;
+    return (b % P) * pitch + reg_pos((b / P) * R1 + r);
+  };
+  auto in2 = [&](int i) __attribute__((always_inline)) {
+    const int b = tid + (i / R2) * kFftThreads, r = i % R2;
+    if (i >= T2 * R2 || b >= P * ns) return -1;
+    return (b / ns) * pitch + reg_pos(b % ns + r * ns);
+  };
+#pragma unroll
+  for (int t = 0; t < T1; ++t) {
+    const int b = tid + t * kFftThreads;
+    if (b >= P * nq1) break;
+    const int p = b % P, j = b / P;
+    Cx* xv = v + t * R1;
+#pragma unroll
+    for (int r = 0; r < R1; ++r) xv[r] = p < pc ? c2r_input<R>(a, x, y0 + p, j + r * nq1) : mk(0, 0);
+    dft<R1, true>(xv);
+  }
+  lds_exchange<VPT>(plane, v, out1, in2);
+  R* dst = a.out + ((long long)x * a.ny + y0) * (2LL * m);
+#pragma unroll
+  for (int t = 0; t < T2; ++t) {
+    const int b = tid + t * kFftThreads;
+    if (b >= P * ns) break;
+    const int p = b / ns, k = b % ns;
+    Cx* xv = v + t * R2;
+    twiddle_in<R2, true>(xv, k, 1, tw);
+    dft<R2, true>(xv);
+    if (p < pc) {
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const long long o = 2 * ((long long)p * m + k + r * ns);
+        dst[o] = (R)xv[r].x;
+        dst[o + 1] = (R)xv[r].y;
+      }
+    }
   }
 }
 
 template <typename R>
-int tile_pencils(int n) {
-  int P = (int)(8192 / sizeof(R)) / (n > 0 ? n : 1);
-  if (P > 16) P = 16;
-  if (P < 1) P = 1;
-  return P;
+__global__ __launch_bounds__(kFftThreads) void k_c2r_z(CArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* lds = reinterpret_cast<double*>(lds_raw);
+  const int m = a.m, P = a.P, pitch = lds_pitch(m);
+  const int x = blockIdx.x / a.tiles_y;
+  const int y0 = (blockIdx.x - x * a.tiles_y) * P;
+  const int pc = min(P, a.ny - y0);
+  double* re = lds;
+  double* im = lds + P * pitch;
+  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
+    const int p = idx % pc, k = idx / pc;
+    const Cx z = c2r_input<R>(a, x, y0 + p, k);
+    re[p * pitch + k] = z.x;
+    im[p * pitch + k] = z.y;
+  }
+  __syncthreads();
+  const int cur = run_stages<true>(lds, m, P, pitch, a.ax);
+  re = lds + 2 * cur * P * pitch;
+  im = re + P * pitch;
+  R* dst = a.out + ((long long)x * a.ny + y0) * (2LL * m);
+  for (int idx = threadIdx.x; idx < pc * m; idx += kFftThreads) {
+    const int p = idx / m, j = idx - p * m;
+    dst[2 * (long long)idx] = (R)re[p * pitch + j];
+    dst[2 * (long long)idx + 1] = (R)im[p * pitch + j];
+  }
 }
 
-template <typename R>
-size_t tile_lds(int n, int P) {
-  return (size_t)P * lds_pitch(n) * 4 * sizeof(R);
-}
-
+// --- launch helpers --------------------------------------------------------------------
 // LDS bytes one workgroup may allocate on the current device (queried once per device).
 inline size_t max_lds_per_block() {
   static int cached[64];
@@ -789,7 +1043,16 @@ inline size_t max_lds_per_block() {
   return (size_t)cached[dev];
 }
 
-// Radix split of the register-staged kernel for n (R3 = 1: two stages), or 0.
+// pencils per block of the LDS-staged passes (4 FP64 planes of P * pitch)
+inline int lds_pencils(int n) {
+  int P = 2048 / (n > 0 ? n : 1);
+  if (P > 16) P = 16;
+  if (P < 1) P = 1;
+  return P;
+}
+inline size_t lds_bytes(int n, int P) { return (size_t)P * lds_pitch(n) * 4 * sizeof(double); }
+
+// Radix split of the register-staged passes for n, or 0.
 inline int reg_split(int n, int* r1, int* r2, int* r3) {
   switch (n) {
     case 16: *r1 = 4; *r2 = 4; *r3 = 1; return 1;
@@ -803,229 +1066,331 @@ inline int reg_split(int n, int* r1, int* r2, int* r3) {
   }
 }
 
-template <typename R, bool INV, int EPI>
-void launch_reg(int n, dim3 g, dim3 b, size_t lds, hipStream_t s, const PencilArgs<R>& a) {
+template <typename R, int R1, int R2, int R3, bool INV, bool IN_E, bool OUT_E, int EPI, int RES,
+          bool WGT>
+void launch_stream(size_t lds, hipStream_t s, PassArgs<R> a) {
+  const void* k =
+      reinterpret_cast<const void*>(&k_pass_stream<R, R1, R2, R3, INV, IN_E, OUT_E, EPI, RES, WGT>);
+  a.tpb = stream_tpb(k, lds, a.g.blocks);
+  const long long grid = (a.g.blocks + a.tpb - 1) / a.tpb + (a.nyq ? a.nq_pencils : 0);
+  if (grid == 0) return;
+  hipLaunchKernelGGL((k_pass_stream<R, R1, R2, R3, INV, IN_E, OUT_E, EPI, RES, WGT>),
+                     dim3((unsigned)grid), dim3(kFftThreads), lds, s, a);
+}
+
+template <typename R, bool INV, bool IN_E, bool OUT_E, int EPI, int RES, bool WGT>
+void launch_reg_n(int n, size_t lds, hipStream_t s, const PassArgs<R>& a) {
   switch (n) {
-    case 16: hipLaunchKernelGGL((k_pencils_reg<R, 4, 4, 1, INV, EPI>), g, b, lds, s, a); break;
-    case 32: hipLaunchKernelGGL((k_pencils_reg<R, 8, 4, 1, INV, EPI>), g, b, lds, s, a); break;
-    case 64: hipLaunchKernelGGL((k_pencils_reg<R, 8, 8, 1, INV, EPI>), g, b, lds, s, a); break;
-    case 128: hipLaunchKernelGGL((k_pencils_reg<R, 16, 8, 1, INV, EPI>), g, b, lds, s, a); break;
-    case 256: hipLaunchKernelGGL((k_pencils_reg<R, 16, 16, 1, INV, EPI>), g, b, lds, s, a); break;
-    case 512: hipLaunchKernelGGL((k_pencils_reg<R, 8, 8, 8, INV, EPI>), g, b, lds, s, a); break;
-    default: hipLaunchKernelGGL((k_pencils_reg<R, 16, 8, 8, INV, EPI>), g, b, lds, s, a); break;
+    case 16: launch_stream<R, 4, 4, 1, INV, IN_E, OUT_E, EPI, RES, WGT>(lds, s, a); break;
+    case 32: launch_stream<R, 8, 4, 1, INV, IN_E, OUT_E, EPI, RES, WGT>(lds, s, a); break;
+    case 64: launch_stream<R, 8, 8, 1, INV, IN_E, OUT_E, EPI, RES, WGT>(lds, s, a); break;
+    case 128: launch_stream<R, 16, 8, 1, INV, IN_E, OUT_E, EPI, RES, WGT>(lds, s, a); break;
+    case 256: launch_stream<R, 16, 16, 1, INV, IN_E, OUT_E, EPI, RES, WGT>(lds, s, a); break;
+    case 512: launch_stream<R, 8, 8, 8, INV, IN_E, OUT_E, EPI, RES, WGT>(lds, s, a); break;
+    default: launch_stream<R, 16, 8, 8, INV, IN_E, OUT_E, EPI, RES, WGT>(lds, s, a); break;
   }
 }
 
-// job[0] runs over outer0 outer groups, job[1] over outer1.  job[0] uses the
-// register-staged kernel when n has a supported split and its pencils are
-// adjacent in memory; job[1] (the Nyquist plane) always the generic tile.
-// `choice`: 0 = default (register-staged when available), kTuneReg = the
-// register-staged kernel, 1..16 = the generic LDS tile with that many pencils.
+// The pass kinds the transform uses (forward z is separate):
+enum PassKind {
+  PK_Y = 0,       // forward y: spec[x][kz][:] -> rho[kz][:][x]
+  PK_X_ACC,       // forward x, |X|^2 into sf
+  PK_X_STORE,     // forward x, X in place
+  PK_X_INV,       // inverse x (weights), in place
+  PK_Y_INV,       // inverse y: rho[kz][:][x] -> spec[x][kz][:]
+};
+
+// choice: 0 = default (register-staged when n has a split), kTuneReg = that,
+// 1..16 = the LDS-staged pass with that many pencils.
 template <typename R>
-hipError_t run_pencils(PencilArgs<R> a, long long outer0, long long outer1, bool inv, int epi,
-                       hipStream_t s, int choice = 0) {
+hipError_t run_pass(PassArgs<R> a, PassKind kind, hipStream_t s, int choice) {
   int r1, r2, r3;
-  bool reg = (choice == 0 || choice == kTuneReg) && reg_split(a.n, &r1, &r2, &r3) &&
-             !a.job[0].pencil_major;
-  if (reg) {
-    const size_t need =
-        (size_t)reg_tile_pencils<R>(r3 == 1 ? 2 : 3) * (a.n + 1) * (r3 == 1 ? 2 : 4) * sizeof(R);
-    reg = need <= max_lds_per_block();
-  }
-  long long blocks[2];
-  const long long outer[2] = {outer0, outer1};
+  const int n = a.n;
+  const size_t nyq_lds = (size_t)4 * lds_pitch(n) * sizeof(double);
+  bool reg = (choice == 0 || choice == kTuneReg) && reg_split(n, &r1, &r2, &r3);
   size_t lds = 0;
-  for (int j = 0; j < 2; ++j) {
-    PencilJob<R>& J = a.job[j];
-    const bool use_reg = reg && j == 0;
-    const bool forced = j == 0 && choice >= 1 && choice <= 16 &&
-                        tile_lds<R>(a.n, choice) <= max_lds_per_block();
-    J.P = use_reg ? reg_tile_pencils<R>(r3 == 1 ? 2 : 3) : (forced ? choice : tile_pencils<R>(a.n));
-    J.tiles_inner = (J.pencils + J.P - 1) / J.P;
-    blocks[j] = J.pencils > 0 ? outer[j] * J.tiles_inner : 0;
-    if (blocks[j] > 0) {
-      const size_t need = use_reg ? (size_t)J.P * (a.n + 1) * (r3 == 1 ? 2 : 4) * sizeof(R)
-                                  : tile_lds<R>(a.n, J.P);
-      lds = need > lds ? need : lds;
-    }
-  }
-  a.blocks0 = blocks[0];
-  const long long total = blocks[0] + blocks[1];
-  if (total == 0) return hipSuccess;
-  const dim3 g((unsigned)total), b(kFftThreads);
   if (reg) {
-    if (inv && epi == EPI_STORE) launch_reg<R, true, EPI_STORE>(a.n, g, b, lds, s, a);
-    else if (!inv && epi == EPI_STORE) launch_reg<R, false, EPI_STORE>(a.n, g, b, lds, s, a);
-    else launch_reg<R, false, EPI_ACC>(a.n, g, b, lds, s, a);
+    a.P = kRegElems / n;
+    lds = (size_t)a.P * (n + n / 16 + 1) * sizeof(double);
   } else {
-    if (inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<true, R, EPI_STORE>), g, b, lds, s, a);
-    else if (!inv && epi == EPI_STORE) hipLaunchKernelGGL((k_pencils<false, R, EPI_STORE>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_pencils<false, R, EPI_ACC>), g, b, lds, s, a);
+    a.P = (choice >= 1 && choice <= 16) ? choice : lds_pencils(n);
+    while (a.P > 1 && lds_bytes(n, a.P) > max_lds_per_block()) a.P /= 2;
+    lds = lds_bytes(n, a.P);
+  }
+  if (nyq_lds > lds) lds = nyq_lds;
+  if (lds > max_lds_per_block()) return hipErrorInvalidValue;
+  a.g.tiles_f = (a.g.nf + a.P - 1) / a.P;
+  a.g.blocks *= a.g.tiles_f;   // caller set the slow count
+  const long long total = a.g.blocks + (a.nyq ? a.nq_pencils : 0);
+  if (total == 0) return hipSuccess;
+  const dim3 g((unsigned)total);
+  const bool f32 = sizeof(R) == 4;
+  switch (kind) {
+    case PK_Y:
+      if (reg) {
+        if (f32) launch_reg_n<R, false, true, false, EPI_STORE, RES_OUT, false>(n, lds, s, a);
+        else launch_reg_n<R, false, true, false, EPI_STORE, RES_NONE, false>(n, lds, s, a);
+      } else {
+        if (f32) hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_OUT>), g, dim3(kFftThreads), lds, s, a);
+        else hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_NONE>), g, dim3(kFftThreads), lds, s, a);
+      }
+      break;
+    case PK_X_ACC:
+      if (reg) {
+        if (f32) launch_reg_n<R, false, true, true, EPI_ACC, RES_IN, false>(n, lds, s, a);
+        else launch_reg_n<R, false, true, true, EPI_ACC, RES_NONE, false>(n, lds, s, a);
+      } else {
+        if (f32) hipLaunchKernelGGL((k_pass_lds<R, false, EPI_ACC, RES_IN>), g, dim3(kFftThreads), lds, s, a);
+        else hipLaunchKernelGGL((k_pass_lds<R, false, EPI_ACC, RES_NONE>), g, dim3(kFftThreads), lds, s, a);
+      }
+      break;
+    case PK_X_STORE:
+      if (reg) {
+        if (f32) launch_reg_n<R, false, true, true, EPI_STORE, RES_IN, false>(n, lds, s, a);
+        else launch_reg_n<R, false, true, true, EPI_STORE, RES_NONE, false>(n, lds, s, a);
+      } else {
+        if (f32) hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_IN>), g, dim3(kFftThreads), lds, s, a);
+        else hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_NONE>), g, dim3(kFftThreads), lds, s, a);
+      }
+      break;
+    case PK_X_INV:
+      if (reg) launch_reg_n<R, true, true, true, EPI_STORE, RES_NONE, true>(n, lds, s, a);
+      else hipLaunchKernelGGL((k_pass_lds<R, true, EPI_STORE, RES_NONE>), g, dim3(kFftThreads), lds, s, a);
+      break;
+    default:  // PK_Y_INV
+      if (reg) launch_reg_n<R, true, false, true, EPI_STORE, RES_NONE, false>(n, lds, s, a);
+      else hipLaunchKernelGGL((k_pass_lds<R, true, EPI_STORE, RES_NONE>), g, dim3(kFftThreads), lds, s, a);
+      break;
   }
   return hipGetLastError();
 }
 
 template <typename R>
-typename V2<R>::T* spec_main(Plan& p) {
-  return static_cast<typename V2<R>::T*>(p.spec_main);
-}
-template <typename R>
-typename V2<R>::T* spec_nyq(Plan& p) {
-  return static_cast<typename V2<R>::T*>(p.spec_nyq);
+typename V2<R>::T* cx(void* p) {
+  return static_cast<typename V2<R>::T*>(p);
 }
 
-// Rows per block of the register-staged z pass for half-length m (FP32), 0
-// when m has no two-stage split.
+// Register-staged z pass rows per block for half-length m, 0 without a split.
 inline int z_reg_rows(int m) {
   switch (m) {
-    case 64: return z_rows_per_block(64, 8);
-    case 128: return z_rows_per_block(128, 16);
-    case 256: return z_rows_per_block(256, 16);
+    case 32: return zreg_rows<32>();
+    case 64: return zreg_rows<64>();
+    case 128: return zreg_rows<128>();
+    case 256: return zreg_rows<256>();
     default: return 0;
   }
+}
+
+// Half lengths m the persistent z pass handles (the others take k_r2c_z).
+inline bool z_stream_ok(int m) {
+  return m == 16 || m == 32 || m == 64 || m == 128 || m == 256 || m == 512 || m == 1024;
+}
+
+template <typename R, int R1, int R2, int R3>
+void launch_z_stream(long long ntiles, hipStream_t s, ZArgs<R> a) {
+  a.tpb = stream_tpb(reinterpret_cast<const void*>(&k_r2c_z_stream<R, R1, R2, R3>), 0, ntiles);
+  const long long grid = (ntiles + a.tpb - 1) / a.tpb;
+  if (grid == 0) return;
+  hipLaunchKernelGGL((k_r2c_z_stream<R, R1, R2, R3>), dim3((unsigned)grid), dim3(kFftThreads), 0,
+                     s, a);
 }
 
 template <typename R>
 hipError_t pass_z(Plan& p, const void* density, hipStream_t s) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
-  R2CArgs<R> a;
+  ZArgs<R> a;
   a.in = static_cast<const R*>(density);
-  a.main = spec_main<R>(p);
-  a.nyq = spec_nyq<R>(p);
-  a.rows = (long long)nx * ny;
+  a.out = cx<R>(p.spec);
+  a.nyq = static_cast<double2*>(p.spec_nyq);
+  a.nx = nx;
+  a.ny = ny;
   a.m = m;
-  a.P = (p.tune_z >= 1 && p.tune_z <= 16 && tile_lds<R>(m, p.tune_z) <= max_lds_per_block())
-            ? p.tune_z
-            : tile_pencils<R>(m);
   a.ax = p.ax_zc;
-  a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
-  if constexpr (sizeof(R) == 4) {
-    if (z_reg_rows(m) > 0 && (p.tune_z == 0 || p.tune_z == kTuneReg)) {
-      // register-staged kernel: the default where m has a two-stage split
-      const int P = z_reg_rows(m);
-      const dim3 g((unsigned)((a.rows + P - 1) / P)), b(kFftThreads);
-      switch (m) {
-        case 64: hipLaunchKernelGGL((k_r2c_z_reg<R, 8, 8>), g, b, 0, s, a); break;
-        case 128: hipLaunchKernelGGL((k_r2c_z_reg<R, 16, 8>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_r2c_z_reg<R, 16, 16>), g, b, 0, s, a); break;
-      }
-      return hipGetLastError();
+  a.tw2 = p.tw_r2c64;
+  if (z_stream_ok(m) && (p.tune_z == 0 || p.tune_z == kTuneReg)) {
+    a.P = kRegElems / m;
+    a.tiles_y = (ny + a.P - 1) / a.P;
+    const long long ntiles = (long long)nx * a.tiles_y;
+    switch (m) {
+      case 16: launch_z_stream<R, 4, 4, 1>(ntiles, s, a); break;
+      case 32: launch_z_stream<R, 8, 4, 1>(ntiles, s, a); break;
+      case 64: launch_z_stream<R, 8, 8, 1>(ntiles, s, a); break;
+      case 128: launch_z_stream<R, 16, 8, 1>(ntiles, s, a); break;
+      case 256: launch_z_stream<R, 16, 4, 4>(ntiles, s, a); break;
+      case 512: launch_z_stream<R, 16, 8, 4>(ntiles, s, a); break;
+      default: launch_z_stream<R, 16, 8, 8>(ntiles, s, a); break;
     }
+    return hipGetLastError();
   }
-  const long long blocks = (a.rows + a.P - 1) / a.P;
-  hipLaunchKernelGGL((k_r2c_z<R>), dim3((unsigned)blocks), dim3(kFftThreads),
-                     tile_lds<R>(m, a.P), s, a);
+  a.P = (p.tune_z >= 1 && p.tune_z <= 16) ? p.tune_z : lds_pencils(m);
+  while (a.P > 1 && lds_bytes(m, a.P) > max_lds_per_block()) a.P /= 2;
+  a.tiles_y = (ny + a.P - 1) / a.P;
+  hipLaunchKernelGGL((k_r2c_z<R>), dim3((unsigned)((long long)nx * a.tiles_y)), dim3(kFftThreads),
+                     lds_bytes(m, a.P), s, a);
   return hipGetLastError();
 }
 
-// y pass: main (pencils = kz, elements = y) + nyq (pencils = x, elements contiguous)
 template <typename R>
-hipError_t pass_y(Plan& p, bool inv, hipStream_t s) {
-  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
-  PencilArgs<R> a{};
-  a.n = ny;
-  a.ax = p.ax_y;
-  PencilJob<R>& J = a.job[0];
-  J.data = spec_main<R>(p);
-  J.estride = m;
-  J.pstride = 1;
-  J.outer_stride = (long long)ny * m;
-  J.pencils = m;
-  PencilJob<R>& K = a.job[1];
-  K.data = spec_nyq<R>(p);
-  K.estride = 1;
-  K.pstride = ny;
-  K.pencils = nx;
-  K.pencil_major = 1;
-  return run_pencils(a, nx, 1, inv, EPI_STORE, s, p.tune_y);
+PassArgs<R> base_args(Plan& p, int n, const FftAxis& ax) {
+  PassArgs<R> a{};
+  a.n = n;
+  a.ax = ax;
+  return a;
 }
 
-// x pass: main (pencils = (y, kz), elements = x) + nyq (pencils = y).  Forward
-// with `accumulate` adds |F|^2 into sf_main / sf_nyq; the inverse applies the
-// per-bin FP64 weights w first.
+// y pass: spec[x][kz][0..Ny) -> rho[kz][ky][x]; Nyquist plane nyq[x][:] in place
 template <typename R>
-hipError_t pass_x(Plan& p, bool inv, bool accumulate, const double* w_main, const double* w_nyq,
-                  hipStream_t s) {
+hipError_t pass_y(Plan& p, hipStream_t s) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
-  PencilArgs<R> a{};
-  a.n = nx;
-  a.ax = p.ax_x;
-  PencilJob<R>& J = a.job[0];
-  J.data = spec_main<R>(p);
-  J.acc = p.sf_main;
-  J.scale = w_main;
-  J.estride = (long long)ny * m;
-  J.pstride = 1;
-  J.pencils = ny * m;
-  PencilJob<R>& K = a.job[1];
-  K.data = spec_nyq<R>(p);
-  K.acc = p.sf_nyq;
-  K.scale = w_nyq;
-  K.estride = ny;
-  K.pstride = 1;
-  K.pencils = ny;
-  return run_pencils(a, 1, 1, inv, accumulate ? EPI_ACC : EPI_STORE, s, p.tune_x);
+  PassArgs<R> a = base_args<R>(p, ny, p.ax_y);
+  a.in = cx<R>(p.spec);
+  a.out = cx<R>(p.rho);
+  a.res_out = static_cast<unsigned char*>(p.res);
+  a.g.in_s = ny;                         // slow = kz
+  a.g.in_f = (long long)m * ny;          // fast = x
+  a.g.in_e = 1;
+  a.g.out_s = (long long)ny * nx;
+  a.g.out_f = 1;
+  a.g.out_e = nx;
+  a.g.nf = nx;
+  a.g.blocks = m;
+  a.nyq = static_cast<double2*>(p.spec_nyq);
+  a.nq_p = ny;
+  a.nq_e = 1;
+  a.nq_pencils = nx;
+  return run_pass<R>(a, PK_Y, s, p.tune_y);
+}
+
+// x pass: rho[kz][ky][0..Nx) in place (F) or |F|^2 into sf_main; the Nyquist
+// plane nyq[:][y] in place or into sf_nyq
+template <typename R>
+hipError_t pass_x(Plan& p, bool accumulate, hipStream_t s) {
+  const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
+  PassArgs<R> a = base_args<R>(p, nx, p.ax_x);
+  a.in = cx<R>(p.rho);
+  a.out = cx<R>(p.rho);
+  a.acc = p.sf_main;
+  a.res_in = static_cast<const unsigned char*>(p.res);
+  a.g.in_s = 0;
+  a.g.in_f = nx;                         // fast = row kz * Ny + ky
+  a.g.in_e = 1;
+  a.g.out_s = 0;
+  a.g.out_f = nx;
+  a.g.out_e = 1;
+  a.g.nf = m * ny;
+  a.g.blocks = 1;
+  a.nyq = static_cast<double2*>(p.spec_nyq);
+  a.nyq_acc = p.sf_nyq;
+  a.nq_p = 1;
+  a.nq_e = ny;
+  a.nq_pencils = ny;
+  return run_pass<R>(a, accumulate ? PK_X_ACC : PK_X_STORE, s, p.tune_x);
 }
 
 template <typename R>
 hipError_t forward_impl(Plan& p, const void* density, bool accumulate, hipStream_t s) {
   hipError_t e = pass_z<R>(p, density, s);
   if (e != hipSuccess) return e;
-  e = pass_y<R>(p, false, s);
+  e = pass_y<R>(p, s);
   if (e != hipSuccess) return e;
-  return pass_x<R>(p, false, accumulate, nullptr, nullptr, s);
+  return pass_x<R>(p, accumulate, s);
 }
 
-// Real adjoint of the forward transform applied to the stored spectrum, each
-// bin first multiplied by the FP64 weight w (internal layout):
+// Real adjoint of the forward transform applied to the stored spectrum
+// (rho[kz][ky][kx] + nyq after a storing forward pass), each bin first
+// multiplied by the FP64 weight w (w_main[kz][ky][kx], w_nyq[x][y]):
 //   out(r) = Re sum_{kx, ky, kz in [0, Nz/2]} w(q) X(q) e^{+i q.r}.
 template <typename R>
 hipError_t inverse_impl(Plan& p, const double* w_main, const double* w_nyq, void* out,
                         hipStream_t s) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
-  hipError_t e = pass_x<R>(p, true, false, w_main, w_nyq, s);
-  if (e != hipSuccess) return e;
-  e = pass_y<R>(p, true, s);
-  if (e != hipSuccess) return e;
-  C2RArgs<R, R> a;
-  a.main = spec_main<R>(p);
-  a.nyq = spec_nyq<R>(p);
-  a.out = static_cast<R*>(out);
-  a.rows = (long long)nx * ny;
-  a.m = m;
-  a.P = tile_pencils<R>(m);
-  a.ax = p.ax_zc;
-  a.tw2 = sizeof(R) == 8 ? (const void*)p.tw_r2c64 : (const void*)p.tw_r2c;
-  if constexpr (sizeof(R) == 4) {
-    if (z_reg_rows(m) > 0) {
-      const int P = z_reg_rows(m);
-      const dim3 g((unsigned)((a.rows + P - 1) / P)), b(kFftThreads);
-      switch (m) {
-        case 64: hipLaunchKernelGGL((k_c2r_z_reg<R, 8, 8>), g, b, 0, s, a); break;
-        case 128: hipLaunchKernelGGL((k_c2r_z_reg<R, 16, 8>), g, b, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_c2r_z_reg<R, 16, 16>), g, b, 0, s, a); break;
-      }
-      return hipGetLastError();
-    }
+  {  // x: in place, weights on load
+    PassArgs<R> a = base_args<R>(p, nx, p.ax_x);
+    a.in = cx<R>(p.rho);
+    a.out = cx<R>(p.rho);
+    a.w = w_main;
+    a.g.in_s = 0;
+    a.g.in_f = nx;
+    a.g.in_e = 1;
+    a.g.out_s = 0;
+    a.g.out_f = nx;
+    a.g.out_e = 1;
+    a.g.nf = m * ny;
+    a.g.blocks = 1;
+    a.nyq = static_cast<double2*>(p.spec_nyq);
+    a.nyq_w = w_nyq;
+    a.nq_p = 1;
+    a.nq_e = ny;
+    a.nq_pencils = ny;
+    const hipError_t e = run_pass<R>(a, PK_X_INV, s, 0);
+    if (e != hipSuccess) return e;
   }
-  const long long blocks = (a.rows + a.P - 1) / a.P;
-  hipLaunchKernelGGL((k_c2r_z<R, R>), dim3((unsigned)blocks), dim3(kFftThreads),
-                     tile_lds<R>(m, a.P), s, a);
+  {  // y: rho[kz][:][x] -> spec[x][kz][:]
+    PassArgs<R> a = base_args<R>(p, ny, p.ax_y);
+    a.in = cx<R>(p.rho);
+    a.out = cx<R>(p.spec);
+    a.g.in_s = (long long)ny * nx;       // slow = kz
+    a.g.in_f = 1;                        // fast = x
+    a.g.in_e = nx;
+    a.g.out_s = ny;
+    a.g.out_f = (long long)m * ny;
+    a.g.out_e = 1;
+    a.g.nf = nx;
+    a.g.blocks = m;
+    a.nyq = static_cast<double2*>(p.spec_nyq);
+    a.nq_p = ny;
+    a.nq_e = 1;
+    a.nq_pencils = nx;
+    const hipError_t e = run_pass<R>(a, PK_Y_INV, s, 0);
+    if (e != hipSuccess) return e;
+  }
+  CArgs<R> c;
+  c.spec = cx<R>(p.spec);
+  c.nyq = static_cast<const double2*>(p.spec_nyq);
+  c.out = static_cast<R*>(out);
+  c.nx = nx;
+  c.ny = ny;
+  c.m = m;
+  c.ax = p.ax_zc;
+  c.tw2 = p.tw_r2c64;
+  if (z_reg_rows(m) > 0) {
+    c.P = z_reg_rows(m);
+    c.tiles_y = (ny + c.P - 1) / c.P;
+    const dim3 g((unsigned)((long long)nx * c.tiles_y)), b(kFftThreads);
+    switch (m) {
+      case 32: hipLaunchKernelGGL((k_c2r_z_reg<R, 8, 4>), g, b, 0, s, c); break;
+      case 64: hipLaunchKernelGGL((k_c2r_z_reg<R, 8, 8>), g, b, 0, s, c); break;
+      case 128: hipLaunchKernelGGL((k_c2r_z_reg<R, 16, 8>), g, b, 0, s, c); break;
+      default: hipLaunchKernelGGL((k_c2r_z_reg<R, 16, 16>), g, b, 0, s, c); break;
+    }
+    return hipGetLastError();
+  }
+  c.P = lds_pencils(m);
+  while (c.P > 1 && lds_bytes(m, c.P) > max_lds_per_block()) c.P /= 2;
+  c.tiles_y = (ny + c.P - 1) / c.P;
+  hipLaunchKernelGGL((k_c2r_z<R>), dim3((unsigned)((long long)nx * c.tiles_y)), dim3(kFftThreads),
+                     lds_bytes(m, c.P), s, c);
   return hipGetLastError();
 }
 
+// stored spectrum (rho[kz][ky][kx] complex, nyq[x][y] FP64) -> complex128 (Nx, Ny, Nz/2+1)
 template <typename R>
 __global__ void k_spec_read(const typename V2<R>::T* __restrict__ main,
-                            const typename V2<R>::T* __restrict__ nyq, double2* __restrict__ out,
-                            int nx, int ny, int m) {
+                            const double2* __restrict__ nyq, double2* __restrict__ out, int nx,
+                            int ny, int m) {
   const long long h = m + 1;
   const long long total = (long long)nx * ny * h;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
     const long long row = i / h;
     const int kz = (int)(i - row * h);
-    const auto v = kz < m ? main[row * m + kz] : nyq[row];
-    out[i] = make_double2((double)v.x, (double)v.y);
+    const int y = (int)(row % ny), x = (int)(row / ny);
+    if (kz < m) {
+      const auto v = main[((long long)kz * ny + y) * nx + x];
+      out[i] = make_double2((double)v.x, (double)v.y);
+    } else {
+      out[i] = nyq[row];
+    }
   }
 }
 
@@ -1111,34 +1476,36 @@ hipError_t autotune_impl(Plan& p, hipStream_t s) {
     (void)hipEventDestroy(e0);
     return err;
   }
-  const int cand_p[5] = {kTuneReg, 2, 4, 8, 16};
-  // z pass: the register-staged kernel (FP32, m with a two-stage split) or
-  // the generic tile with that many rows
+  const int cand[5] = {kTuneReg, 2, 4, 8, 16};
+  // z pass: the register-staged kernel (m with a two-stage split) or the
+  // LDS-staged pass with that many rows
   float best = 1e30f;
   int pick = 0;
-  for (int P : cand_p) {
-    if (P == kTuneReg && !(sizeof(R) == 4 && z_reg_rows(m) > 0)) continue;
-    if (P != kTuneReg && tile_lds<R>(m, P) > max_lds_per_block()) continue;
-    p.tune_z = P;
+  for (int c : cand) {
+    if (c == kTuneReg && !z_stream_ok(m)) continue;
+    if (c != kTuneReg && lds_bytes(m, c) > max_lds_per_block()) continue;
+    p.tune_z = c;
     const float t = time_best([&] { return pass_z<R>(p, p.rho, s); }, s, e0, e1, &err);
     if (err != hipSuccess) break;
     if (t < best) {
       best = t;
-      pick = P;
+      pick = c;
     }
   }
   p.tune_z = pick;
-  // y and x passes: the register-staged kernel or the generic tile
-  const int cand_c[5] = {kTuneReg, 2, 4, 8, 16};
+  // y and x passes: the register-staged pass or the LDS-staged one
   for (int axis = 0; axis < 2 && err == hipSuccess; ++axis) {
     int& slot = axis == 0 ? p.tune_y : p.tune_x;
+    const int n = p.sys.n[axis == 0 ? 1 : 0];
+    int r1, r2, r3;
     best = 1e30f;
     pick = 0;
-    for (int c : cand_c) {
+    for (int c : cand) {
+      if (c == kTuneReg && !reg_split(n, &r1, &r2, &r3)) continue;
+      if (c != kTuneReg && lds_bytes(n, c) > max_lds_per_block()) continue;
       slot = c;
       const float t = time_best(
-          [&] { return axis == 0 ? pass_y<R>(p, false, s) : pass_x<R>(p, false, true, nullptr, nullptr, s); },
-          s, e0, e1, &err);
+          [&] { return axis == 0 ? pass_y<R>(p, s) : pass_x<R>(p, true, s); }, s, e0, e1, &err);
       if (err != hipSuccess) break;
       if (t < best) {
         best = t;
@@ -1154,7 +1521,7 @@ hipError_t autotune_impl(Plan& p, hipStream_t s) {
 
 }  // namespace
 
-// Picks the z tile height and the y / x pencil kernels for this grid by timing
+// Picks the z tile height and the y / x pass kernels for this grid by timing
 // the candidates on the plan's own buffers (results cached per grid shape and
 // precision).  Clobbers the density, spectrum and S(q) accumulators.
 hipError_t fft_autotune(Plan& p, hipStream_t s) {
@@ -1185,10 +1552,8 @@ hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, void*
 hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s) {
   switch (pass) {
     case 0: return p.fft64 ? pass_z<double>(p, density, s) : pass_z<float>(p, density, s);
-    case 1: return p.fft64 ? pass_y<double>(p, false, s) : pass_y<float>(p, false, s);
-    default:
-      return p.fft64 ? pass_x<double>(p, false, true, nullptr, nullptr, s)
-                     : pass_x<float>(p, false, true, nullptr, nullptr, s);
+    case 1: return p.fft64 ? pass_y<double>(p, s) : pass_y<float>(p, s);
+    default: return p.fft64 ? pass_x<double>(p, true, s) : pass_x<float>(p, true, s);
   }
 }
 
@@ -1199,24 +1564,32 @@ hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s) {
   if (blocks > 65536) blocks = 65536;
   if (p.fft64)
     hipLaunchKernelGGL(k_spec_read<double>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       spec_main<double>(p), spec_nyq<double>(p), reinterpret_cast<double2*>(out),
-                       nx, ny, m);
+                       cx<double>(p.rho), static_cast<const double2*>(p.spec_nyq),
+                       reinterpret_cast<double2*>(out), nx, ny, m);
   else
     hipLaunchKernelGGL(k_spec_read<float>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       spec_main<float>(p), spec_nyq<float>(p), reinterpret_cast<double2*>(out),
-                       nx, ny, m);
+                       cx<float>(p.rho), static_cast<const double2*>(p.spec_nyq),
+                       reinterpret_cast<double2*>(out), nx, ny, m);
   return hipGetLastError();
 }
 
+size_t fft_spec_bytes(const int n[3], int fft64) {
+  return (size_t)n[0] * n[1] * (n[2] / 2) * (fft64 ? 16 : 8);
+}
+
+size_t fft_res_bytes(const int n[3], int fft64) {
+  return fft64 ? 0 : (size_t)n[0] * n[1] * (n[2] / 2);
+}
+
 // --- plan-free transform for callers that cannot allocate -----------------------
-// The working set of mdsf_rfft3_noalloc (twiddles, half spectrum, Nyquist
-// plane) lives in this fixed device array, one copy per GPU, held until the
+// The working set of mdsf_rfft3_noalloc (twiddles, the z and y pass outputs,
+// the y pass residuals, Nyquist plane) lives in this fixed device array, one copy per GPU, held until the
 // process ends.  Host code never passes the array's name to a kernel: it is a
 // host-side stand-in.  The device address of each GPU's copy comes from
 // hipGetSymbolAddress on the first call on that device and is kept per device.
 namespace {
 
-constexpr size_t kScratchBytes = (size_t)80 << 20;
+constexpr size_t kScratchBytes = (size_t)144 << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -1255,21 +1628,25 @@ extern "C" int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void*
   for (int d = 0; d < 3; ++d)
     if (n[d] < 2 || n[d] % 2 != 0 || n[d] > 4096)
       return fail(MDSF_ERR_ARG, "grid sizes must be even and in [2, 4096]");
-  const size_t esz = precision == MDSF_F64 ? 8 : 4;
+  const int fft64 = precision == MDSF_F64 ? 1 : 0;
+  const int nn[3] = {n[0], n[1], n[2]};
   const size_t nx = n[0], ny = n[1], nz = n[2], m = nz / 2;
-  // layout: main half spectrum (nx ny m complex), Nyquist plane (nx ny
-  // complex), then the FP32 and FP64 twiddles of the x, y, z/2 axes and of
-  // the R2C split (length nz)
+  // layout: the spectrum buffer (the y pass output, transformed in place by
+  // the x pass), the z pass output, the y pass residuals (FP32), the Nyquist
+  // plane (FP64 complex), then the FP64 twiddles of the x, y, z/2 axes and
+  // of the R2C split (length nz)
   const size_t lens[4] = {nx, ny, m, nz};
   size_t off = 0;
-  const size_t off_main = off;
-  off = align256(off + 2 * esz * nx * ny * m);
+  const size_t off_rho = off;
+  off = align256(off + fft_spec_bytes(nn, fft64));
+  const size_t off_spec = off;
+  off = align256(off + fft_spec_bytes(nn, fft64));
+  const size_t off_res = off;
+  off = align256(off + fft_res_bytes(nn, fft64));
   const size_t off_nyq = off;
-  off = align256(off + 2 * esz * nx * ny);
-  size_t off_tw[4], off_tw64[4];
+  off = align256(off + sizeof(double2) * nx * ny);
+  size_t off_tw64[4];
   for (int k = 0; k < 4; ++k) {
-    off_tw[k] = off;
-    off = align256(off + sizeof(float2) * lens[k]);
     off_tw64[k] = off;
     off = align256(off + sizeof(double2) * lens[k]);
   }
@@ -1283,12 +1660,13 @@ extern "C" int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void*
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Plan p;  // host-side description only: every device pointer is into `base`
   for (int d = 0; d < 3; ++d) p.sys.n[d] = n[d];
-  p.fft64 = precision == MDSF_F64 ? 1 : 0;
-  p.spec_main = p.rho = base + off_main;
+  p.fft64 = fft64;
+  p.rho = base + off_rho;
+  p.spec = base + off_spec;
+  p.res = fft_res_bytes(nn, fft64) > 0 ? base + off_res : nullptr;
   p.spec_nyq = base + off_nyq;
   FftAxis* axes[3] = {&p.ax_x, &p.ax_y, &p.ax_zc};
   for (int k = 0; k < 4; ++k) {
-    float2* tw = reinterpret_cast<float2*>(base + off_tw[k]);
     double2* tw64 = reinterpret_cast<double2*>(base + off_tw64[k]);
     if (k < 3) {
       FftAxis& ax = *axes[k];
@@ -1296,13 +1674,13 @@ extern "C" int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void*
       if (!fft_factor(ax.n, ax.radix, &ax.nstages))
         return fail(MDSF_ERR_UNSUPPORTED,
                     "FFT length " + std::to_string(ax.n) + " needs too many stages");
-      ax.tw = tw;
+      ax.tw = nullptr;
       ax.tw64 = tw64;
     } else {
-      p.tw_r2c = tw;
+      p.tw_r2c = nullptr;
       p.tw_r2c64 = tw64;
     }
-    MDSF_HIP(fft_make_twiddles(tw, tw64, (int)lens[k], s));
+    MDSF_HIP(fft_make_twiddles(nullptr, tw64, (int)lens[k], s));
   }
   MDSF_HIP(fft_forward(p, d_density, false, s));
   MDSF_HIP(fft_spectrum_read(p, d_spectrum, s));

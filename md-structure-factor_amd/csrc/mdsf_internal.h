@@ -166,13 +166,19 @@ struct Plan {
   long long spill_tiles = 0;      // sp_cnt[0] sp_cnt[1] sp_cnt[2]
   long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
-  // grids: two density / spectrum buffers so that one frame's FFT can run
-  // while the next frame is deposited; rho / spec_* point at the current one
+  // grids: two sets of density / spectrum buffers so that one frame's FFT can
+  // run while the next frame is deposited; rho / spec / res / spec_nyq point at
+  // the current one
   void* rho_buf[2] = {nullptr, nullptr};
+  void* spec_buf[2] = {nullptr, nullptr};
+  void* res_buf[2] = {nullptr, nullptr};
   void* nyq_buf[2] = {nullptr, nullptr};
-  void* rho = nullptr;           // Nx*Ny*Nz folded density (FP32, or FP64 when fft64)
-  void* spec_main = nullptr;     // Nx*Ny*(Nz/2) complex, in place over rho
-  void* spec_nyq = nullptr;      // Nx*Ny complex: the kz = Nz/2 plane
+  // rho: Nx*Ny*Nz folded density (plan precision), which the y pass then
+  // overwrites with the spectrum rho[kz][ky][x] (Nx*Ny*Nz/2 complex)
+  void* rho = nullptr;
+  void* spec = nullptr;          // z pass output spec[x][kz][y], Nx*Ny*Nz/2 complex
+  void* res = nullptr;           // y pass rounding residuals, a byte per complex (FP32 plans)
+  void* spec_nyq = nullptr;      // Nx*Ny FP64 complex: the kz = Nz/2 plane, nyq[x][y]
   FrameState slot[2];
   // Frame slot b: its atom state and density / spectrum buffer become the
   // ones the launches below use (kernel arguments are taken at launch).
@@ -198,11 +204,12 @@ struct Plan {
     spill = f.spill;
     spill_flag = f.spill_flag;
     rho = rho_buf[b & 1];
-    spec_main = rho;
+    spec = spec_buf[b & 1];
+    res = res_buf[b & 1];
     spec_nyq = nyq_buf[b & 1];
   }
-  double* sf_main = nullptr;     // Nx*Ny*(Nz/2) FP64 accumulator
-  double* sf_nyq = nullptr;      // Nx*Ny FP64 accumulator
+  double* sf_main = nullptr;     // FP64 accumulator sf[kz][ky][kx], kz < Nz/2
+  double* sf_nyq = nullptr;      // FP64 accumulator of the kz = Nz/2 plane, [kx][ky]
   // FFT
   int fft64 = 0;                 // deposition, density and FFT in FP64 (else FP32)
   FftAxis ax_x, ax_y, ax_zc;     // zc: complex length Nz/2
@@ -228,7 +235,7 @@ struct Plan {
   double* adj_part = nullptr;    // 3 FP64 partial sums per work item
   long long adj_tslots = 0;      // tiled adjoint result slots
   double* adj_tpart = nullptr;   // 3 FP64 per slot
-  double* adj_w = nullptr;       // 2 dL/dS split into main (Nx*Ny*Nz/2) + Nyquist (Nx*Ny)
+  double* adj_w = nullptr;       // 2 dL/dS in the sf layouts: main [kz][ky][kx] + Nyquist [kx][ky]
 };
 
 // error handling -------------------------------------------------------------
@@ -277,6 +284,9 @@ hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s);
 // stored spectrum -> complex128 (Nx, Ny, Nz/2+1)
 hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s);
 bool fft_factor(int n, int* radix, int* nstages);
+// bytes of the z pass output (spec) and of the y pass residuals (res) of a grid
+size_t fft_spec_bytes(const int n[3], int fft64);
+size_t fft_res_bytes(const int n[3], int fft64);
 hipError_t fft_autotune(Plan& p, hipStream_t s);
 
 hipError_t launch_fill_zero(double* p, long long n, hipStream_t s);

@@ -12,6 +12,8 @@ namespace mdsf {
 
 namespace {
 
+// sf (Nx, Ny, Nz/2+1) from the accumulators sf_main[kz][ky][kx] (kz < Nz/2)
+// and sf_nyq[kx][ky]
 __global__ void k_sf_read(const double* __restrict__ sfm, const double* __restrict__ sfn,
                           double* __restrict__ out, int nx, int ny, int m) {
   const long long h = m + 1;
@@ -20,7 +22,8 @@ __global__ void k_sf_read(const double* __restrict__ sfm, const double* __restri
        i += (long long)gridDim.x * blockDim.x) {
     const long long row = i / h;
     const int kz = (int)(i - row * h);
-    out[i] = kz < m ? sfm[row * m + kz] : sfn[row];
+    const long long x = row / ny, y = row - x * ny;
+    out[i] = kz < m ? sfm[((long long)kz * ny + y) * nx + x] : sfn[row];
   }
 }
 
