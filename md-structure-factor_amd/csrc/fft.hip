@@ -6,15 +6,16 @@
 // Precision.  Storage is the plan precision R (float or double); every
 // butterfly, twiddle product and the real-to-complex split is evaluated in
 // FP64, with FP64 twiddles generated on the device.  An FP32 plan rounds to
-// FP32 only where values are stored between passes, and the y pass's output,
-// whose rounding error the x pass would spread along whole rows of the
-// spectrum (the error that dominates faint bins next to bright ones), keeps a
-// 4-bit residual per value: the rounding error in units of 1/16 of the FP32
-// ulp, so that the x pass reads it back 16 times more precisely.
+// FP32 only where values are stored between passes, and the z and y pass
+// outputs, whose rounding errors the next passes would spread along whole
+// rows of the spectrum (the error that dominates faint bins next to bright
+// ones), keep a 4-bit residual per value: the rounding error in units of 1/16
+// of the FP32 ulp, so that the next pass reads them 16 times more precisely.
 //
 // Layout.  Three passes, each reading whole contiguous rows and writing P
 // pencils side by side (runs of P values):
 //   z  rho[x][y][0..Nz) (density, R)  ->  spec[x][kz][y] (kz < Nz/2), nyq[x][y] (kz = Nz/2)
+//                                         + zres[x][kz][y] (FP32 plans)
 //   y  spec[x][kz][0..Ny)             ->  rho[kz][ky][x] (the density buffer, free by then)
 //                                         + res[kz][ky][x] (FP32 plans)
 //   x  rho[kz][ky][0..Nx)             ->  F in place, or |F|^2 added into sf[kz][ky][kx] (FP64)
@@ -179,7 +180,7 @@ __device__ __forceinline__ void dft(Cx* v) {
 // inputs r >= 1 of a butterfly at k times W^(k r step)
 template <int RAD, bool INV>
 __device__ __forceinline__ void twiddle_in(Cx* v, int k, int step, const double2* tw) {
-  if (k == 0) return;
+  // k = 0 multiplies by the table's exact 1 (no divergent branch)
 #pragma unroll
   for (int r = 1; r < RAD; ++r) {
     const Cx w = ld_tw(tw, k * r * step);
@@ -342,7 +343,8 @@ struct Geom {
 };
 
 enum Epilogue { EPI_STORE = 0, EPI_ACC = 1 };
-enum Residual { RES_NONE = 0, RES_OUT = 1, RES_IN = 2 };
+// residual nibbles read with the input (RES_IN) and / or written with the output (RES_OUT)
+enum Residual { RES_NONE = 0, RES_OUT = 1, RES_IN = 2, RES_INOUT = 3 };
 
 template <typename R>
 struct PassArgs {
@@ -397,7 +399,7 @@ __device__ void nyq_pencil(const A& a, int pencil, double* lds) {
 template <typename R, int RES>
 __device__ __forceinline__ Cx pass_load(const PassArgs<R>& a, long long gi) {
   Cx x = load_cx<R>(a.in, gi);
-  if constexpr (RES == RES_IN) {
+  if constexpr ((RES & RES_IN) != 0) {
     const unsigned b = a.res_in[gi];
     x.x += res_value(b & 15u, (float)x.x);
     x.y += res_value(b >> 4, (float)x.y);
@@ -411,7 +413,7 @@ __device__ __forceinline__ void pass_store(const PassArgs<R>& a, long long go, C
     a.acc[go] += v.x * v.x + v.y * v.y;
   } else {
     store_cx<R>(a.out, go, v);
-    if constexpr (RES == RES_OUT)
+    if constexpr ((RES & RES_OUT) != 0)
       a.res_out[go] = (unsigned char)(res_nibble(v.x, (float)v.x) | (res_nibble(v.y, (float)v.y) << 4));
   }
 }
@@ -540,7 +542,10 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
   int tid = threadIdx.x;
   const unsigned in_e = IN_E ? 1u : (unsigned)g.in_e, out_e = OUT_E ? 1u : (unsigned)g.out_e;
   const unsigned in_f = (unsigned)g.in_f, out_f = (unsigned)g.out_f;
-  const double2* tw = a.ax.tw64;
+  // the axis twiddles, copied to LDS after the exchange plane (read after an
+  // exchange barrier, at LDS rather than L2 latency)
+  double2* twl = reinterpret_cast<double2*>(plane + ((P * pitch + 1) & ~1));
+  const double2* tw = twl;
   const long long ntiles = g.blocks;
   const unsigned tiles_f = (unsigned)g.tiles_f;
   // stage-1 slot i = t R1 + r: pencil p, element j + r nq1
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
     *o = (unsigned)p * in_f + (unsigned)j * in_e;
   };
   T2 raw[16];
-  unsigned char rr[RES == RES_IN ? 16 : 1];
+  unsigned char rr[(RES & RES_IN) != 0 ? 16 : 1];
   auto load = [&](long long tile) __attribute__((always_inline)) {
     const unsigned s = (unsigned)(tile / tiles_f);
     const int f0 = (int)((unsigned)tile - s * tiles_f) * P;
@@ -567,7 +572,7 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
       for (int r = 0; r < R1; ++r) {
         const unsigned oe = o + (unsigned)(r * nq1) * in_e;
         raw[t * R1 + r] = at(src, oe);
-        if constexpr (RES == RES_IN) rr[t * R1 + r] = at(a.res_in + ib, oe);
+        if constexpr ((RES & RES_IN) != 0) rr[t * R1 + r] = at(a.res_in + ib, oe);
       }
     }
   };
@@ -580,6 +585,7 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
   long long tile = (long long)blockIdx.x * a.tpb;
   const long long tend = min(ntiles, tile + a.tpb);
   if (PF) load(tile);
+  for (int i = threadIdx.x; i < n; i += kFftThreads) twl[i] = a.ax.tw64[i];
   for (; tile < tend; ++tile) {
     tid = opaque(tid);
     if (!PF) load(tile);
@@ -592,7 +598,7 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       v[i] = mk((double)raw[i].x, (double)raw[i].y);
-      if constexpr (RES == RES_IN) {
+      if constexpr ((RES & RES_IN) != 0) {
         v[i].x += res_value(rr[i] & 15u, raw[i].x);
         v[i].y += res_value(rr[i] >> 4, raw[i].y);
       }
@@ -607,6 +613,22 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
       }
     }
     if (PF && tile + 1 < tend) load(tile + 1);
+    // the accumulator values this tile adds to, loaded now so that their
+    // latency passes during the transform (pencils past a partial tile read
+    // the last one and are not stored)
+    double accv[EPI == EPI_ACC ? 16 : 1];
+    if constexpr (EPI == EPI_ACC) {
+#pragma unroll
+      for (int t = 0; t < 16 / RL; ++t) {
+        const int b = tid + t * kFftThreads;
+        int p = OUT_E ? b / nsL : b % P;
+        const int k = OUT_E ? b % nsL : b / P;
+        p = p < pc ? p : pc - 1;
+        const unsigned o = (unsigned)p * out_f + (unsigned)k * out_e;
+#pragma unroll
+        for (int r = 0; r < RL; ++r) accv[t * RL + r] = at(a.acc + ob, o + (unsigned)(r * nsL) * out_e);
+      }
+    }
     // stage 1 (span 1): DFT_R1; output position j R1 + r
 #pragma unroll
     for (int t = 0; t < 16 / R1; ++t) dft<R1, INV>(v + t * R1);
@@ -657,14 +679,13 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
         for (int r = 0; r < RL; ++r) {
           const unsigned oe = o + (unsigned)(r * nsL) * out_e;
           if constexpr (EPI == EPI_ACC) {
-            double& d = at(a.acc + ob, oe);
-            d += x[r].x * x[r].x + x[r].y * x[r].y;
+            at(a.acc + ob, oe) = accv[t * RL + r] + (x[r].x * x[r].x + x[r].y * x[r].y);
           } else {
             T2 st;
             st.x = (R)x[r].x;
             st.y = (R)x[r].y;
             at(a.out + ob, oe) = st;
-            if constexpr (RES == RES_OUT)
+            if constexpr ((RES & RES_OUT) != 0)
               at(a.res_out + ob, oe) =
                   (unsigned char)(res_nibble(x[r].x, st.x) | (res_nibble(x[r].y, st.y) << 4));
           }
@@ -728,6 +749,7 @@ struct ZArgs {
   FftAxis ax;                        // length m
   const double2* tw2;                // W_nz^k, k < nz
   int tpb;                           // tiles per block of the register-staged pass
+  unsigned char* res;                // FP32 storage: residual nibbles at the spec index
 };
 
 __device__ __forceinline__ Cx split_bin(Cx z, Cx zc_conj_src, const double2* tw2, int k) {
@@ -741,7 +763,10 @@ __device__ __forceinline__ Cx split_bin(Cx z, Cx zc_conj_src, const double2* tw2
 
 template <typename R>
 __device__ __forceinline__ void z_store(const ZArgs<R>& a, int x, int y, int k, Cx v) {
-  store_cx<R>(a.out, ((long long)x * a.m + k) * a.ny + y, v);
+  const long long o = ((long long)x * a.m + k) * a.ny + y;
+  store_cx<R>(a.out, o, v);
+  if constexpr (sizeof(R) == 4)
+    a.res[o] = (unsigned char)(res_nibble(v.x, (float)v.x) | (res_nibble(v.y, (float)v.y) << 4));
 }
 
 // Register-staged z pass (c2r form below keeps 2048 values per block).
@@ -765,8 +790,11 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z_stream(ZArgs<R> a) {
   constexpr bool PF = sizeof(R) == 4;
   using T2 = typename V2<R>::T;
   __shared__ double plane[P * pitch];
+  // the axis twiddles W_m^k and the split's W_nz^k (k < m) in LDS, read after
+  // an exchange barrier
+  __shared__ double2 twl[m], tw2l[m];
   int tid = threadIdx.x;
-  const double2* tw = a.ax.tw64;
+  const double2* tw = twl;
   const long long ntiles = (long long)a.nx * a.tiles_y;
   T2 raw[16];
   auto load = [&](long long tile) __attribute__((always_inline)) {
@@ -787,6 +815,10 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z_stream(ZArgs<R> a) {
   long long tile = (long long)blockIdx.x * a.tpb;
   const long long tend = min(ntiles, tile + a.tpb);
   if (PF && tile < tend) load(tile);
+  for (int i = threadIdx.x; i < m; i += kFftThreads) {
+    twl[i] = a.ax.tw64[i];
+    tw2l[i] = a.tw2[i];
+  }
   for (; tile < tend; ++tile) {
     tid = opaque(tid);
     if (!PF) load(tile);
@@ -858,8 +890,8 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z_stream(ZArgs<R> a) {
         const int k1 = kA + r * ns, k2 = kB + r * ns;
         const Cx m1 = csel(self, za[r == 0 ? 0 : RL - r], zb[RL - 1 - r]);
         const Cx m2 = csel(self, zb[RL - 1 - r], za[RL - 1 - r]);
-        const Cx x1 = split_bin(za[r], m1, a.tw2, k1);
-        const Cx x2 = split_bin(zb[r], m2, a.tw2, k2);
+        const Cx x1 = split_bin(za[r], m1, tw2l, k1);
+        const Cx x2 = split_bin(zb[r], m2, tw2l, k2);
         T2 s1, s2;
         s1.x = (R)x1.x;
         s1.y = (R)x1.y;
@@ -867,6 +899,13 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z_stream(ZArgs<R> a) {
         s2.y = (R)x2.y;
         at(dst, (unsigned)(k1 * a.ny)) = s1;
         at(dst, (unsigned)(k2 * a.ny)) = s2;
+        if constexpr (sizeof(R) == 4) {
+          unsigned char* rd = a.res + (long long)x * m * a.ny + y0 + p;
+          at(rd, (unsigned)(k1 * a.ny)) =
+              (unsigned char)(res_nibble(x1.x, s1.x) | (res_nibble(x1.y, s1.y) << 4));
+          at(rd, (unsigned)(k2 * a.ny)) =
+              (unsigned char)(res_nibble(x2.x, s2.x) | (res_nibble(x2.y, s2.y) << 4));
+        }
       }
       if (self) a.nyq[(long long)x * a.ny + y0 + p] = make_double2(za[0].x - za[0].y, 0.0);
     }
@@ -1111,7 +1150,9 @@ hipError_t run_pass(PassArgs<R> a, PassKind kind, hipStream_t s, int choice) {
   size_t lds = 0;
   if (reg) {
     a.P = kRegElems / n;
-    lds = (size_t)a.P * (n + n / 16 + 1) * sizeof(double);
+    // exchange plane (rounded to whole double2), then the n axis twiddles
+    lds = (((size_t)a.P * (n + n / 16 + 1) + 1) & ~(size_t)1) * sizeof(double) +
+          (size_t)n * sizeof(double2);
   } else {
     a.P = (choice >= 1 && choice <= 16) ? choice : lds_pencils(n);
     while (a.P > 1 && lds_bytes(n, a.P) > max_lds_per_block()) a.P /= 2;
@@ -1128,10 +1169,10 @@ hipError_t run_pass(PassArgs<R> a, PassKind kind, hipStream_t s, int choice) {
   switch (kind) {
     case PK_Y:
       if (reg) {
-        if (f32) launch_reg_n<R, false, true, false, EPI_STORE, RES_OUT, false>(n, lds, s, a);
+        if (f32) launch_reg_n<R, false, true, false, EPI_STORE, RES_INOUT, false>(n, lds, s, a);
         else launch_reg_n<R, false, true, false, EPI_STORE, RES_NONE, false>(n, lds, s, a);
       } else {
-        if (f32) hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_OUT>), g, dim3(kFftThreads), lds, s, a);
+        if (f32) hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_INOUT>), g, dim3(kFftThreads), lds, s, a);
         else hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_NONE>), g, dim3(kFftThreads), lds, s, a);
       }
       break;
@@ -1181,6 +1222,14 @@ inline int z_reg_rows(int m) {
   }
 }
 
+// FP32 storage: the z pass's residuals follow the y pass's in the plan's
+// residual buffer (nullptr for FP64 storage)
+inline unsigned char* fft_zres(Plan& p) {
+  if (!p.res) return nullptr;
+  return static_cast<unsigned char*>(p.res) +
+         (size_t)p.sys.n[0] * p.sys.n[1] * (p.sys.n[2] / 2);
+}
+
 // Half lengths m the persistent z pass handles (the others take k_r2c_z).
 inline bool z_stream_ok(int m) {
   return m == 16 || m == 32 || m == 64 || m == 128 || m == 256 || m == 512 || m == 1024;
@@ -1207,6 +1256,7 @@ hipError_t pass_z(Plan& p, const void* density, hipStream_t s) {
   a.m = m;
   a.ax = p.ax_zc;
   a.tw2 = p.tw_r2c64;
+  a.res = fft_zres(p);
   if (z_stream_ok(m) && (p.tune_z == 0 || p.tune_z == kTuneReg)) {
     a.P = kRegElems / m;
     a.tiles_y = (ny + a.P - 1) / a.P;
@@ -1246,6 +1296,7 @@ hipError_t pass_y(Plan& p, hipStream_t s) {
   a.in = cx<R>(p.spec);
   a.out = cx<R>(p.rho);
   a.res_out = static_cast<unsigned char*>(p.res);
+  a.res_in = fft_zres(p);
   a.g.in_s = ny;                         // slow = kz
   a.g.in_f = (long long)m * ny;          // fast = x
   a.g.in_e = 1;
@@ -1578,7 +1629,7 @@ size_t fft_spec_bytes(const int n[3], int fft64) {
 }
 
 size_t fft_res_bytes(const int n[3], int fft64) {
-  return fft64 ? 0 : (size_t)n[0] * n[1] * (n[2] / 2);
+  return fft64 ? 0 : 2 * (size_t)n[0] * n[1] * (n[2] / 2);  // y pass, then z pass
 }
 
 // --- plan-free transform for callers that cannot allocate -----------------------
@@ -1589,7 +1640,7 @@ size_t fft_res_bytes(const int n[3], int fft64) {
 // hipGetSymbolAddress on the first call on that device and is kept per device.
 namespace {
 
-constexpr size_t kScratchBytes = (size_t)144 << 20;
+constexpr size_t kScratchBytes = (size_t)148 << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -1632,7 +1683,7 @@ extern "C" int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void*
   const int nn[3] = {n[0], n[1], n[2]};
   const size_t nx = n[0], ny = n[1], nz = n[2], m = nz / 2;
   // layout: the spectrum buffer (the y pass output, transformed in place by
-  // the x pass), the z pass output, the y pass residuals (FP32), the Nyquist
+  // the x pass), the z pass output, the y and z pass residuals (FP32), the Nyquist
   // plane (FP64 complex), then the FP64 twiddles of the x, y, z/2 axes and
   // of the R2C split (length nz)
   const size_t lens[4] = {nx, ny, m, nz};

@@ -7,8 +7,17 @@ Same names, signatures, module globals and ``.npz`` outputs as the reference
 and the output grids run as HIP kernels in ``libmdsf.so``; this module does
 the reference's host-side set-up arithmetic and the ``.npz`` write.
 
-Precision: deposition and FFT in FP32 (twiddles generated in FP64), the
-frame sum of |F|^2 in FP64; home cells, grid sizes and k-grids are exact.
+Precision.  "fp32" plans store the density and the spectrum between FFT
+passes in FP32, evaluate every butterfly, twiddle and the real-to-complex
+split in FP64, keep 4-bit residuals of the z and y passes' FP32 roundings for
+the next pass, and sum |F|^2 over frames in FP64.  "fp64" plans store FP64
+throughout (S(q) within about 1e-13 of FP64 numpy).  The default, "auto",
+runs frame 0 both ways and keeps FP32 when its S(q) agrees with FP64 within
+AUTO_FP32_MAX_ERR (5e-6) on every voxel above 1e-6 of the maximum, which
+leaves the 1e-5 bound a factor of two for the frame sum; FP32 storage of a
+density concentrated in a few hundred voxels (atoms packed at the cell
+corners) is already coarser than that, and such inputs run in FP64.  Home
+cells, grid sizes and k-grids are exact.
 """
 import concurrent.futures
 import math
@@ -25,11 +34,15 @@ theta = np.pi / 3                     # dens.py:16 (set by callers, not read)
 PRECISION = 1.0E-24                   # dens.py:18
 dtyp = np.float64                     # dens.py:20 (FP64 S(q) accumulation)
 
-# FFT precision of the GPU path: "fp32", "fp64" or "auto" (FP64 for grids of
-# at most 128^3 points, where it costs little and FP32 rounding is visible on
-# sparse densities; FP32 above).  S(q) is always summed in FP64.
+# Storage precision of the GPU path: "fp32", "fp64" or "auto" (frame 0 in
+# both; FP32 if its S(q) is within AUTO_FP32_MAX_ERR of FP64 on the voxels
+# above 1e-6 of the maximum, else FP64; RANDOM_NOISE fields run in FP32).
+# S(q) is always summed in FP64.
 FFT_PRECISION = "auto"
-AUTO_FP64_MAX_POINTS = 128 ** 3
+AUTO_FP32_MAX_ERR = 5e-6
+# Outcome of the last compute_sf call's precision choice: {"fp64": bool,
+# "frame0_fp32_err": max relative S(q) error of frame 0 in FP32, or None}.
+LAST_PRECISION = {}
 
 # Threads deflating the .npz members (None: up to 16, one per core); the
 # archive is what np.savez_compressed writes, compressed in parallel.
@@ -39,8 +52,9 @@ NPZ_THREADS = None
 NPZ_LEVEL = 6
 
 # Wall-clock seconds of the phases of the last compute_sf call: "setup",
-# "upload", "frames", "outputs" (S(q) and the output grids on the GPU),
-# "write" (device-to-host copies overlapped with the deflate) (informational).
+# "upload", "plan" (plan creation, with the "auto" precision check on frame
+# 0), "frames", "outputs" (S(q) and the output grids on the GPU), "write"
+# (device-to-host copies overlapped with the deflate) (informational).
 LAST_TIMINGS = {}
 
 
@@ -123,14 +137,45 @@ def get_dplot(dmag):
     return sfplt.cpu().numpy()
 
 
-def _use_fft64(n):
-    if FFT_PRECISION == "fp64":
-        return True
-    if FFT_PRECISION == "fp32":
-        return False
-    if FFT_PRECISION != "auto":
+def _frame0_plans(s, coords, types, scale, box):
+    """FP32 and FP64 plans that have both run frame 0 (on a copy of it), and
+    the max relative error of the FP32 S(q) against the FP64 one on the
+    voxels above 1e-6 of the maximum (0 without such voxels)."""
+    import torch
+    plans, out = [], []
+    try:
+        for fp64 in (False, True):
+            plans.append(engine.Plan.from_setup(s, device=coords.device, fft64=fp64))
+            plans[-1].accumulate_frames(coords[:1].clone(), types, scale[:1], box, s.arith,
+                                        s.wrap)
+            plans[-1].check_errors()
+            out.append(plans[-1].sf())
+    except BaseException:
+        for p in plans:
+            p.close()
+        raise
+    s32, s64 = out
+    mask = s64 > 1e-6 * s64.max()
+    err = float(torch.max(torch.abs(s32[mask] - s64[mask]) / s64[mask])) if bool(mask.any()) else 0.0
+    return plans[0], plans[1], err
+
+
+def _make_plan(s, coords, types, scale, box):
+    """The run's plan in the precision FFT_PRECISION selects (module docstring)."""
+    LAST_PRECISION.clear()
+    if FFT_PRECISION not in ("auto", "fp32", "fp64"):
         raise ValueError("FFT_PRECISION must be 'auto', 'fp32' or 'fp64'")
-    return int(np.prod([int(x) for x in n])) <= AUTO_FP64_MAX_POINTS
+    if FFT_PRECISION == "auto" and RANDOM_NOISE == 0 and s.nframes > 0 and s.natoms > 0:
+        p32, p64, err = _frame0_plans(s, coords, types, scale, box)
+        fp64 = err > AUTO_FP32_MAX_ERR
+        keep, drop = (p64, p32) if fp64 else (p32, p64)
+        drop.close()
+        keep.reset()
+        LAST_PRECISION.update(fp64=fp64, frame0_fp32_err=err)
+        return keep
+    fp64 = FFT_PRECISION == "fp64"
+    LAST_PRECISION.update(fp64=fp64, frame0_fp32_err=None)
+    return engine.Plan.from_setup(s, device=coords.device, fft64=fp64)
 
 
 def _output_plan(n):
@@ -167,21 +212,23 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     s = engine.Setup(r, L, typ, rad, Sres, USE_BETTER_RESOLUTION, ucell)
     if PRINT_DETAILS:
         _print_details(s.L, Sres, s.N)
-    plan = engine.Plan.from_setup(s, fft64=_use_fft64(s.N))
-    dev = plan.device
+    dev = engine._device(None)
     t1 = time.perf_counter()
     coords = torch.from_numpy(np.ascontiguousarray(r)).to(dev)
     scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
+    types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
     box = [float(x) for x in s.L]
     torch.cuda.synchronize(dev)
     t2 = time.perf_counter()
+    plan = _make_plan(s, coords, types, scale, box)
+    torch.cuda.synchronize(dev)
+    t2b = time.perf_counter()
     if RANDOM_NOISE > 0:
         plan.rescale_wrap(coords, scale, box, s.arith, s.wrap)
         for _ in range(s.nframes):
             d1 = np.random.rand(int(s.N[0]), int(s.N[1]), int(s.N[2]))   # dens.py:280
             plan.accumulate_density(torch.from_numpy(d1))   # cast to the plan precision
     else:
-        types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
         plan.accumulate_frames(coords, types, scale, box, s.arith, s.wrap)
     plan.check_errors()
     t3 = time.perf_counter()
@@ -205,5 +252,6 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     del sfplt, kgrid, kgridplt, sf_dev, out
     t5 = time.perf_counter()
     LAST_TIMINGS.clear()
-    LAST_TIMINGS.update(setup=t1 - t0, upload=t2 - t1, frames=t3 - t2, outputs=t4 - t3,
+    LAST_TIMINGS.update(setup=t1 - t0, upload=t2 - t1, plan=t2b - t2, frames=t3 - t2b,
+                        outputs=t4 - t3,
                         write=t5 - t4)
