@@ -207,7 +207,7 @@ int mdsf_rfft3(mdsf_plan* plan, const void* d_density, double* d_spectrum,
  * d_density).  d_spectrum receives complex128 (Nx, Ny, Nz/2+1), bit for bit
  * what mdsf_rfft3 gives on a plan of this grid and precision created without
  * MDSF_AUTOTUNE.  Allocates nothing: its twiddles, pass outputs and Nyquist
- * plane live in one fixed 148 MiB device array inside the library (one copy
+ * plane live in one fixed 144 MiB device array inside the library (one copy
  * per GPU, held until the process ends), which fits every FP32 grid up to
  * 256^3 and every FP64 grid up to 208^3; larger grids are rejected with
  * MDSF_ERR_ARG.  Every call on a GPU uses that GPU's one array, so calls that

@@ -6,16 +6,15 @@
 // Precision.  Storage is the plan precision R (float or double); every
 // butterfly, twiddle product and the real-to-complex split is evaluated in
 // FP64, with FP64 twiddles generated on the device.  An FP32 plan rounds to
-// FP32 only where values are stored between passes, and the z and y pass
-// outputs, whose rounding errors the next passes would spread along whole
-// rows of the spectrum (the error that dominates faint bins next to bright
-// ones), keep a 4-bit residual per value: the rounding error in units of 1/16
-// of the FP32 ulp, so that the next pass reads them 16 times more precisely.
+// FP32 only where values are stored between passes, and the y pass's output,
+// whose rounding error the x pass would spread along whole rows of the
+// spectrum (the error that dominates faint bins next to bright ones), keeps a
+// 4-bit residual per value: the rounding error in units of 1/16 of the FP32
+// ulp, so that the x pass reads it back 16 times more precisely.
 //
 // Layout.  Three passes, each reading whole contiguous rows and writing P
 // pencils side by side (runs of P values):
 //   z  rho[x][y][0..Nz) (density, R)  ->  spec[x][kz][y] (kz < Nz/2), nyq[x][y] (kz = Nz/2)
-//                                         + zres[x][kz][y] (FP32 plans)
 //   y  spec[x][kz][0..Ny)             ->  rho[kz][ky][x] (the density buffer, free by then)
 //                                         + res[kz][ky][x] (FP32 plans)
 //   x  rho[kz][ky][0..Nx)             ->  F in place, or |F|^2 added into sf[kz][ky][kx] (FP64)
@@ -344,7 +343,7 @@ struct Geom {
 
 enum Epilogue { EPI_STORE = 0, EPI_ACC = 1 };
 // residual nibbles read with the input (RES_IN) and / or written with the output (RES_OUT)
-enum Residual { RES_NONE = 0, RES_OUT = 1, RES_IN = 2, RES_INOUT = 3 };
+enum Residual { RES_NONE = 0, RES_OUT = 1, RES_IN = 2 };
 
 template <typename R>
 struct PassArgs {
@@ -543,9 +542,15 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
   const unsigned in_e = IN_E ? 1u : (unsigned)g.in_e, out_e = OUT_E ? 1u : (unsigned)g.out_e;
   const unsigned in_f = (unsigned)g.in_f, out_f = (unsigned)g.out_f;
   // the axis twiddles, copied to LDS after the exchange plane (read after an
-  // exchange barrier, at LDS rather than L2 latency)
+  // exchange barrier, at LDS rather than L2 latency), then the tile's 4096
+  // residual bytes: they move between global memory and LDS 16 contiguous
+  // bytes per lane, and between LDS and the lanes' strided elements a byte
+  // at a time
   double2* twl = reinterpret_cast<double2*>(plane + ((P * pitch + 1) & ~1));
   const double2* tw = twl;
+  unsigned char* resl = reinterpret_cast<unsigned char*>(twl + n);
+  static_assert((RES & RES_IN) == 0 || IN_E, "input residuals: tiles of whole contiguous rows");
+  static_assert((RES & RES_OUT) == 0 || !OUT_E, "output residuals: pencil-fastest stores");
   const long long ntiles = g.blocks;
   const unsigned tiles_f = (unsigned)g.tiles_f;
   // stage-1 slot i = t R1 + r: pencil p, element j + r nq1
@@ -557,7 +562,7 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
     *o = (unsigned)p * in_f + (unsigned)j * in_e;
   };
   T2 raw[16];
-  unsigned char rr[(RES & RES_IN) != 0 ? 16 : 1];
+  uint4 rq;  // RES_IN: bytes 16 tid .. 16 tid + 15 of the tile's residuals (P whole rows)
   auto load = [&](long long tile) __attribute__((always_inline)) {
     const unsigned s = (unsigned)(tile / tiles_f);
     const int f0 = (int)((unsigned)tile - s * tiles_f) * P;
@@ -569,11 +574,12 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
       unsigned o;
       in_pos(t, pc, &o);
 #pragma unroll
-      for (int r = 0; r < R1; ++r) {
-        const unsigned oe = o + (unsigned)(r * nq1) * in_e;
-        raw[t * R1 + r] = at(src, oe);
-        if constexpr ((RES & RES_IN) != 0) rr[t * R1 + r] = at(a.res_in + ib, oe);
-      }
+      for (int r = 0; r < R1; ++r) raw[t * R1 + r] = at(src, o + (unsigned)(r * nq1) * in_e);
+    }
+    if constexpr ((RES & RES_IN) != 0) {
+      // past a partial tile's rows: the tile's first bytes (never used)
+      const unsigned q = 16u * (unsigned)tid < (unsigned)(pc * n) ? (unsigned)tid : 0u;
+      rq = at(reinterpret_cast<const uint4*>(a.res_in + ib), q);
     }
   };
   const long long nblk = (ntiles + a.tpb - 1) / a.tpb;
@@ -596,11 +602,22 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
     const long long ob = (long long)s * g.out_s + (long long)f0 * g.out_f;
     Cx v[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      v[i] = mk((double)raw[i].x, (double)raw[i].y);
-      if constexpr ((RES & RES_IN) != 0) {
-        v[i].x += res_value(rr[i] & 15u, raw[i].x);
-        v[i].y += res_value(rr[i] >> 4, raw[i].y);
+    for (int i = 0; i < 16; ++i) v[i] = mk((double)raw[i].x, (double)raw[i].y);
+    if constexpr ((RES & RES_IN) != 0) {
+      // the previous tile's last reads of resl preceded its exchange barriers
+      reinterpret_cast<uint4*>(resl)[tid] = rq;
+      lds_barrier();
+#pragma unroll
+      for (int t = 0; t < 16 / R1; ++t) {
+        const int b = tid + t * kFftThreads;
+        const int q = (b / nq1) * n + b % nq1;  // row p, element j (in_f = n, in_e = 1)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const unsigned rb = resl[q + r * nq1];
+          const int i = t * R1 + r;
+          v[i].x += res_value(rb & 15u, raw[i].x);
+          v[i].y += res_value(rb >> 4, raw[i].y);
+        }
       }
     }
     if constexpr (WGT) {
@@ -685,10 +702,29 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
             st.x = (R)x[r].x;
             st.y = (R)x[r].y;
             at(a.out + ob, oe) = st;
-            if constexpr ((RES & RES_OUT) != 0)
-              at(a.res_out + ob, oe) =
+            if constexpr ((RES & RES_OUT) != 0)  // element k + r nsL of pencil p
+              resl[(k + r * nsL) * P + p] =
                   (unsigned char)(res_nibble(x[r].x, st.x) | (res_nibble(x[r].y, st.y) << 4));
           }
+        }
+      }
+    }
+    if constexpr ((RES & RES_OUT) != 0) {
+      // the tile's residuals: n rows (elements) of P bytes (pencils), stored
+      // at ob + element out_e + pencil (out_f = 1); the next tile writes resl
+      // only after its exchange barriers
+      lds_barrier();
+      const unsigned q = 16u * (unsigned)tid;
+      const int row = (int)(q / P), col = (int)(q % P);
+      unsigned char* dst = a.res_out + ob;
+      if (P >= 16 && (out_e & 15u) == 0 && col + 16 <= pc) {
+        at(reinterpret_cast<uint4*>(dst + (size_t)row * out_e + col), 0u) =
+            *reinterpret_cast<const uint4*>(resl + q);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int rw = (int)((q + i) / P), cl = (int)((q + i) % P);
+          if (cl < pc) dst[(size_t)rw * out_e + cl] = resl[q + i];
         }
       }
     }
@@ -749,7 +785,6 @@ struct ZArgs {
   FftAxis ax;                        // length m
   const double2* tw2;                // W_nz^k, k < nz
   int tpb;                           // tiles per block of the register-staged pass
-  unsigned char* res;                // FP32 storage: residual nibbles at the spec index
 };
 
 __device__ __forceinline__ Cx split_bin(Cx z, Cx zc_conj_src, const double2* tw2, int k) {
@@ -765,8 +800,6 @@ template <typename R>
 __device__ __forceinline__ void z_store(const ZArgs<R>& a, int x, int y, int k, Cx v) {
   const long long o = ((long long)x * a.m + k) * a.ny + y;
   store_cx<R>(a.out, o, v);
-  if constexpr (sizeof(R) == 4)
-    a.res[o] = (unsigned char)(res_nibble(v.x, (float)v.x) | (res_nibble(v.y, (float)v.y) << 4));
 }
 
 // Register-staged z pass (c2r form below keeps 2048 values per block).
@@ -899,13 +932,6 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_z_stream(ZArgs<R> a) {
         s2.y = (R)x2.y;
         at(dst, (unsigned)(k1 * a.ny)) = s1;
         at(dst, (unsigned)(k2 * a.ny)) = s2;
-        if constexpr (sizeof(R) == 4) {
-          unsigned char* rd = a.res + (long long)x * m * a.ny + y0 + p;
-          at(rd, (unsigned)(k1 * a.ny)) =
-              (unsigned char)(res_nibble(x1.x, s1.x) | (res_nibble(x1.y, s1.y) << 4));
-          at(rd, (unsigned)(k2 * a.ny)) =
-              (unsigned char)(res_nibble(x2.x, s2.x) | (res_nibble(x2.y, s2.y) << 4));
-        }
       }
       if (self) a.nyq[(long long)x * a.ny + y0 + p] = make_double2(za[0].x - za[0].y, 0.0);
     }
@@ -1150,9 +1176,10 @@ hipError_t run_pass(PassArgs<R> a, PassKind kind, hipStream_t s, int choice) {
   size_t lds = 0;
   if (reg) {
     a.P = kRegElems / n;
-    // exchange plane (rounded to whole double2), then the n axis twiddles
+    // exchange plane (rounded to whole double2), the n axis twiddles, the
+    // tile's residual bytes
     lds = (((size_t)a.P * (n + n / 16 + 1) + 1) & ~(size_t)1) * sizeof(double) +
-          (size_t)n * sizeof(double2);
+          (size_t)n * sizeof(double2) + (size_t)kRegElems;
   } else {
     a.P = (choice >= 1 && choice <= 16) ? choice : lds_pencils(n);
     while (a.P > 1 && lds_bytes(n, a.P) > max_lds_per_block()) a.P /= 2;
@@ -1169,10 +1196,10 @@ hipError_t run_pass(PassArgs<R> a, PassKind kind, hipStream_t s, int choice) {
   switch (kind) {
     case PK_Y:
       if (reg) {
-        if (f32) launch_reg_n<R, false, true, false, EPI_STORE, RES_INOUT, false>(n, lds, s, a);
+        if (f32) launch_reg_n<R, false, true, false, EPI_STORE, RES_OUT, false>(n, lds, s, a);
         else launch_reg_n<R, false, true, false, EPI_STORE, RES_NONE, false>(n, lds, s, a);
       } else {
-        if (f32) hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_INOUT>), g, dim3(kFftThreads), lds, s, a);
+        if (f32) hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_OUT>), g, dim3(kFftThreads), lds, s, a);
         else hipLaunchKernelGGL((k_pass_lds<R, false, EPI_STORE, RES_NONE>), g, dim3(kFftThreads), lds, s, a);
       }
       break;
@@ -1222,14 +1249,6 @@ inline int z_reg_rows(int m) {
   }
 }
 
-// FP32 storage: the z pass's residuals follow the y pass's in the plan's
-// residual buffer (nullptr for FP64 storage)
-inline unsigned char* fft_zres(Plan& p) {
-  if (!p.res) return nullptr;
-  return static_cast<unsigned char*>(p.res) +
-         (size_t)p.sys.n[0] * p.sys.n[1] * (p.sys.n[2] / 2);
-}
-
 // Half lengths m the persistent z pass handles (the others take k_r2c_z).
 inline bool z_stream_ok(int m) {
   return m == 16 || m == 32 || m == 64 || m == 128 || m == 256 || m == 512 || m == 1024;
@@ -1256,7 +1275,6 @@ hipError_t pass_z(Plan& p, const void* density, hipStream_t s) {
   a.m = m;
   a.ax = p.ax_zc;
   a.tw2 = p.tw_r2c64;
-  a.res = fft_zres(p);
   if (z_stream_ok(m) && (p.tune_z == 0 || p.tune_z == kTuneReg)) {
     a.P = kRegElems / m;
     a.tiles_y = (ny + a.P - 1) / a.P;
@@ -1296,7 +1314,6 @@ hipError_t pass_y(Plan& p, hipStream_t s) {
   a.in = cx<R>(p.spec);
   a.out = cx<R>(p.rho);
   a.res_out = static_cast<unsigned char*>(p.res);
-  a.res_in = fft_zres(p);
   a.g.in_s = ny;                         // slow = kz
   a.g.in_f = (long long)m * ny;          // fast = x
   a.g.in_e = 1;
@@ -1629,7 +1646,7 @@ size_t fft_spec_bytes(const int n[3], int fft64) {
 }
 
 size_t fft_res_bytes(const int n[3], int fft64) {
-  return fft64 ? 0 : 2 * (size_t)n[0] * n[1] * (n[2] / 2);  // y pass, then z pass
+  return fft64 ? 0 : (size_t)n[0] * n[1] * (n[2] / 2);
 }
 
 // --- plan-free transform for callers that cannot allocate -----------------------
@@ -1640,7 +1657,7 @@ size_t fft_res_bytes(const int n[3], int fft64) {
 // hipGetSymbolAddress on the first call on that device and is kept per device.
 namespace {
 
-constexpr size_t kScratchBytes = (size_t)148 << 20;
+constexpr size_t kScratchBytes = (size_t)144 << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -1683,7 +1700,7 @@ extern "C" int mdsf_rfft3_noalloc(const int32_t n[3], int precision, const void*
   const int nn[3] = {n[0], n[1], n[2]};
   const size_t nx = n[0], ny = n[1], nz = n[2], m = nz / 2;
   // layout: the spectrum buffer (the y pass output, transformed in place by
-  // the x pass), the z pass output, the y and z pass residuals (FP32), the Nyquist
+  // the x pass), the z pass output, the y pass residuals (FP32), the Nyquist
   // plane (FP64 complex), then the FP64 twiddles of the x, y, z/2 axes and
   // of the R2C split (length nz)
   const size_t lens[4] = {nx, ny, m, nz};

@@ -177,8 +177,7 @@ struct Plan {
   // overwrites with the spectrum rho[kz][ky][x] (Nx*Ny*Nz/2 complex)
   void* rho = nullptr;
   void* spec = nullptr;          // z pass output spec[x][kz][y], Nx*Ny*Nz/2 complex
-  void* res = nullptr;           // FP32 plans: rounding residuals, a byte per complex, of the y pass
-                                 // then of the z pass (2 Nx*Ny*Nz/2 bytes)
+  void* res = nullptr;           // y pass rounding residuals, a byte per complex (FP32 plans)
   void* spec_nyq = nullptr;      // Nx*Ny FP64 complex: the kz = Nz/2 plane, nyq[x][y]
   FrameState slot[2];
   // Frame slot b: its atom state and density / spectrum buffer become the
