@@ -44,14 +44,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def fft_bytes(n):
-    """Algorithmic HBM bytes of the three FP32 passes (x pass fused with the FP64 sf RMW)."""
+def fft_bytes(n, fp64=False):
+    """Algorithmic HBM bytes of the three passes: density R (4 / 8 bytes), spectrum
+    complex R between passes, FP64 complex Nyquist plane (kz = Nz/2), a residual
+    byte per y-pass output value (FP32 plans), FP64 S(q) read and written by x."""
     nx, ny, nz = n
-    h = nz // 2 + 1
+    m = nz // 2
+    c = 16 if fp64 else 8        # complex value between passes
+    res = 0 if fp64 else 1       # y pass residual byte
+    plane = nx * ny              # Nyquist plane points
+    vol = nx * ny * m            # half-spectrum points below kz = Nz/2
     return {
-        "z": 4 * nx * ny * nz + 8 * nx * ny * h,   # read FP32 density, write half spectrum
-        "y": 16 * nx * ny * h,                     # read + write complex64
-        "x": 8 * nx * ny * h + 16 * nx * ny * h,   # read complex64, read + write FP64 sf
+        # read the density rows, write spec[x][kz][y] and the Nyquist plane
+        "z": (c // 2) * nx * ny * nz + c * vol + 16 * plane,
+        # read spec, write rho[kz][ky][x] + residuals; the Nyquist plane in place
+        "y": 2 * c * vol + res * vol + 32 * plane,
+        # read rho + residuals, read and write sf; the Nyquist plane into sf_nyq
+        "x": c * vol + res * vol + 16 * vol + 16 * plane + 16 * plane,
     }
 
 
@@ -117,7 +126,10 @@ def copy_calibration(nbytes=256 << 20):
 def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream):
     """Each stage alone: frames run stage by stage on one stream, HIP events
     around every stage (the kernels' own durations, without the overlap of the
-    pipelined loop)."""
+    pipelined loop).  The deposition kernel alone ("deposit_kernel"): one-frame
+    frame-loop calls, whose events on the deposition's stream bracket only the
+    tiled deposition (the parked-atom pass and the FFT follow on another
+    stream, after it)."""
     import torch
     names = ["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"]
     steps = [lambda t: plan.prepare_frame(d_coords[t], d_types, d_scale[t], box, s.arith, s.wrap),
@@ -133,8 +145,19 @@ def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream):
             evs[t][i + 1].record(stream)
     torch.cuda.synchronize()
     plan.check_errors()
-    return {n: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
-            for i, n in enumerate(names)}
+    out = {n: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
+           for i, n in enumerate(names)}
+    plan.set_stage_timing(2)
+    dk = []
+    for t in range(nframes):
+        plan.accumulate_frames(d_coords[t:t + 1], d_types, d_scale[t:t + 1], box, s.arith,
+                               s.wrap)
+        dk.append(plan.stage_times()["deposit"])
+    plan.set_stage_timing(0)
+    torch.cuda.synchronize()
+    plan.check_errors()
+    out["deposit_kernel"] = float(np.mean(dk))
+    return out
 
 
 def adjoint_leg(rad, iters):
@@ -405,10 +428,8 @@ def main():
     box = [float(x) for x in s.L]
     stream = torch.cuda.current_stream(dev)
     # The timed region is the library's own frame loop (mdsf_accumulate_frames):
-    # frame t + 1 prepared on this stream while frame t is deposited and frame
-    # t - 1 transformed on the plan's two other streams.  HIP events bracket
-    # only the deposition (on the stream it runs on), the dominant kernel.
-    plan.set_stage_timing(2)
+    # frame t + 1 prepared on this stream while frame t is deposited and earlier
+    # frames transformed on the plan's two other streams (no stage events).
     log(f"warmup {args.warmup} frames")
     plan.accumulate_frames(d_coords[:args.warmup], d_types, d_scale[:args.warmup], box, s.arith,
                            s.wrap)
@@ -423,24 +444,30 @@ def main():
     torch.cuda.synchronize()
     plan.check_errors()
     total_ms = start.elapsed_time(end)
-    dep_ms = plan.stage_times()["deposit"]
-    log(f"{args.steps / (total_ms * 1e-3):.1f} frames/s; deposition {dep_ms * 1e3:.1f} us "
-        f"per frame in the timed region")
+    log(f"{args.steps / (total_ms * 1e-3):.1f} frames/s")
     iso = None
     if not args.no_isolated:
         iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, min(args.steps, 30), stream)
         log("per-stage avg ms (isolated): " + json.dumps({k: round(v, 4) for k, v in iso.items()}))
 
     n = [int(x) for x in s.N]
-    fb = fft_bytes(n)
+    fb = fft_bytes(n, args.fft == "fp64")
     updates = voxel_updates(s)
-    dep_tflops = 2.0 * updates / (dep_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_deposit_mfma (deposition + fold, FP32 MFMA)",
-                "achieved": round(dep_tflops, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(dep_tflops / FP32_PEAK_TFLOPS, 4),
-                "traffic": DEPOSIT_TRAFFIC,
-                "basis": f"{updates} voxel updates/frame x 2 flop (one FMA each) / avg deposition "
-                         f"{dep_ms * 1e3:.1f} us (HIP events on its stream, timed region)"}
+    # the roofline of the dominant kernel: its average duration alone (HIP
+    # events around it on the stream it runs on, one frame at a time), which
+    # the kernel durations of a rocprofv3 trace of the same run match
+    roofline = None
+    if iso is not None:
+        rdep_ms = iso["deposit_kernel"]
+        rdep_tflops = 2.0 * updates / (rdep_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma",
+                    "kernel": "k_deposit_mfma<0, 16> (deposition + fold, FP32 MFMA)",
+                    "achieved": round(rdep_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(rdep_tflops / FP32_PEAK_TFLOPS, 4),
+                    "traffic": DEPOSIT_TRAFFIC,
+                    "basis": f"{updates} voxel updates/frame x 2 flop (one FMA each) / avg "
+                             f"deposition kernel {rdep_ms * 1e3:.1f} us (alone, one frame at a "
+                             f"time, HIP events on its stream)"}
 
     out = {
         "metric": METRIC,
@@ -459,11 +486,11 @@ def main():
                    "box_A": [float(x) for x in s.L], "fft_precision": args.fft,
                    "sf_accumulation": "f64"},
         "roofline": roofline,
-        "deposit": {"voxel_updates_per_frame": updates,
-                    "updates_per_s": round(updates / (dep_ms * 1e-3), 1),
-                    "tflops_fma": round(dep_tflops, 3), "avg_ms_timed_region": round(dep_ms, 4)},
+        "deposit": {"voxel_updates_per_frame": updates},
     }
     if iso is not None:
+        out["deposit"]["updates_per_s"] = round(updates / (iso["deposit_kernel"] * 1e-3), 1)
+        out["deposit"]["kernel_ms"] = round(iso["deposit_kernel"], 4)
         out["stage_ms_isolated"] = {k: round(v, 4) for k, v in iso.items()}
         out["fft_passes"] = {
             k: {"achieved": round(fb[k] / (iso[st] * 1e-3) / 1e9, 1),
@@ -472,8 +499,6 @@ def main():
             for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"))}
         out["fft_passes_note"] = ("each pass alone on one stream (HIP events around its "
                                   "kernel); algorithmic bytes / time vs 8 TB/s")
-        out["deposit"]["isolated_frac"] = round(
-            2.0 * updates / (iso["deposit"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)
     oracle_sf = None
     if not args.no_cpu:
         log("cpu baseline: frame 0 through the FP64 numpy oracle on one core")

@@ -168,12 +168,12 @@ int mdsf_prepare_frame(mdsf_plan* plan, void* d_coords, int dtype, const int32_t
 int mdsf_deposit(mdsf_plan* plan, void* d_density, mdsf_stream stream);
 int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_stream stream);
 
-/* The plan keeps two frame slots (per-frame atom state and density /
+/* The plan keeps three frame slots (per-frame atom state and density /
  * spectrum buffer) so that mdsf_accumulate_frames can prepare one frame,
- * deposit the previous one and transform the one before on three streams.
- * This selects the slot (0 or 1) that mdsf_prepare_frame, mdsf_deposit,
+ * deposit the previous one and transform the ones before on three streams.
+ * This selects the slot (0, 1 or 2) that mdsf_prepare_frame, mdsf_deposit,
  * mdsf_fft_pass and the other single-frame entry points use from now on; a
- * caller pipelining the stages itself alternates them.  No counterpart in the
+ * caller pipelining the stages itself rotates them.  No counterpart in the
  * reference (its frame loop is sequential, dens.py:277-321). */
 int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which);
 

@@ -71,9 +71,10 @@ hipError_t pipeline_streams(int device, hipStream_t* dep, hipStream_t* fft) {
 
 void free_plan(Plan* p) {
   if (!p) return;
-  for (hipEvent_t e : {p->ev_fork, p->ev_prep[0], p->ev_prep[1], p->ev_dep[0], p->ev_dep[1],
-                       p->ev_fft[0], p->ev_fft[1]})
-    if (e) (void)hipEventDestroy(e);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  for (int b = 0; b < kSlots; ++b)
+    for (hipEvent_t e : {p->ev_prep[b], p->ev_dep[b], p->ev_fft[b]})
+      if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
   for (FrameState& f : p->slot) {
     void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
@@ -82,9 +83,10 @@ void free_plan(Plan* p) {
     for (void* q : fs)
       if (q) (void)hipFree(q);
   }
-  void* ptrs[] = {p->types,     p->err,       p->rho_buf[0], p->nyq_buf[0], p->sf_main,
-                  p->rho_buf[1], p->nyq_buf[1], p->spec_buf[0], p->spec_buf[1],
-                  p->res_buf[0], p->res_buf[1],
+  for (int b = 0; b < kSlots; ++b)
+    for (void* q : {p->rho_buf[b], p->nyq_buf[b], p->spec_buf[b], p->res_buf[b]})
+      if (q) (void)hipFree(q);
+  void* ptrs[] = {p->types,     p->err,       p->sf_main,
                   p->sf_nyq,    p->ax_x.tw,   p->ax_y.tw,    p->ax_zc.tw,   p->ax_x.tw64,
                   p->ax_y.tw64, p->ax_zc.tw64, p->tw_r2c,    p->tw_r2c64,   p->adj_part,
                   p->adj_w,     p->adj_tpart};
@@ -355,7 +357,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   }
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < kSlots; ++b) {
     unsigned char* rho = nullptr;
     unsigned char* spec = nullptr;
     double2* nyq = nullptr;
@@ -395,14 +397,12 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     free_plan(p);
     return rc;
   }
-  if ((e = pipeline_streams(p->device, &p->dep_stream, &p->fft_stream)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_prep[0], hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_prep[1], hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_dep[0], hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_dep[1], hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_fft[0], hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&p->ev_fft[1], hipEventDisableTiming)) != hipSuccess) {
+  e = pipeline_streams(p->device, &p->dep_stream, &p->fft_stream);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+  for (int b = 0; b < kSlots && e == hipSuccess; ++b)
+    for (hipEvent_t* ev : {&p->ev_prep[b], &p->ev_dep[b], &p->ev_fft[b]})
+      if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+  if (e != hipSuccess) {
     rc = hip_fail(e, "frame pipeline stream / events");
     free_plan(p);
     return rc;
@@ -468,13 +468,16 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   if (nframes == 0) return MDSF_OK;
   const size_t esz = dtype == MDSF_F32 ? 4 : 8;
   const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
-  // Three-stage pipeline over the two frame slots (atom state + density
+  // Three-stage pipeline over the kSlots frame slots (atom state + density
   // buffer): frame t is prepared on `s`, deposited on dep_stream and
   // transformed on fft_stream, so that while frame t is deposited frame t + 1
-  // is prepared and frame t - 1 transformed.  A slot is reused two frames
-  // later: preparing frame t waits until frame t - 2's deposition has read
-  // its atom state, depositing it until frame t - 2's FFT has released its
-  // density buffer.  The S(q) sum stays in frame order on one stream.
+  // is prepared and earlier frames transformed.  A slot is reused kSlots
+  // frames later: preparing frame t waits until frame t - kSlots's deposition
+  // has read its atom state, depositing it until frame t - kSlots's FFT has
+  // released its density buffer.  With three slots the deposition does not
+  // wait for the FFT of the frame just before the previous one (the FFT
+  // shares the GPU with the deposition and finishes later than it).  The S(q)
+  // sum stays in frame order on one stream.
   hipStream_t d = p->dep_stream, f = p->fft_stream;
   if (p->timing) {
     while ((int64_t)p->tev.size() < 10 * nframes) {
@@ -493,12 +496,12 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   MDSF_HIP(hipEventRecord(p->ev_fork, s));
   MDSF_HIP(hipStreamWaitEvent(d, p->ev_fork, 0));
   MDSF_HIP(hipStreamWaitEvent(f, p->ev_fork, 0));
-  for (int b = 0; b < 2; ++b) {
+  for (int b = 0; b < kSlots; ++b) {
     MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
     MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }
   for (int64_t t = 0; t < nframes; ++t) {
-    const int b = (int)(t & 1);
+    const int b = (int)(t % kSlots);
     char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
     int32_t* home = (t == nframes - 1) ? d_home_cells : nullptr;
     p->use_slot(b);
@@ -525,7 +528,7 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     }
     MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }
-  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) & 1], 0));
+  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) % kSlots], 0));
   return MDSF_OK;
 }
 
@@ -614,7 +617,8 @@ int mdsf_stage_times(mdsf_plan* plan, double* avg_ms) {
 int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which) {
   Plan* p = reinterpret_cast<Plan*>(plan);
   if (!p) return fail(MDSF_ERR_ARG, "null plan");
-  if (which != 0 && which != 1) return fail(MDSF_ERR_ARG, "frame buffer must be 0 or 1");
+  if (which < 0 || which >= kSlots)
+    return fail(MDSF_ERR_ARG, "frame buffer must be 0, 1 or 2");
   p->use_slot(which);
   return MDSF_OK;
 }
@@ -698,18 +702,18 @@ int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nfra
   if (nframes == 0) return MDSF_OK;
   const size_t esz = dtype == MDSF_F32 ? 4 : 8;
   const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
-  // Same two-slot pipeline as the forward frame loop: frame t is prepared on
+  // Same slot pipeline as the forward frame loop: frame t is prepared on
   // `s`, deposited on dep_stream, and transformed forward, transformed back
   // with the weights and gathered on fft_stream.  Slot b is reused by frame
-  // t + 2 only after frame t's gather (the last reader of its atom state and
-  // density buffer) has finished.
+  // t + kSlots only after frame t's gather (the last reader of its atom state
+  // and density buffer) has finished.
   hipStream_t d = p->dep_stream, f = p->fft_stream;
   MDSF_HIP(hipEventRecord(p->ev_fork, s));
   MDSF_HIP(hipStreamWaitEvent(d, p->ev_fork, 0));
   MDSF_HIP(hipStreamWaitEvent(f, p->ev_fork, 0));
-  for (int b = 0; b < 2; ++b) MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
+  for (int b = 0; b < kSlots; ++b) MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   for (int64_t t = 0; t < nframes; ++t) {
-    const int b = (int)(t & 1);
+    const int b = (int)(t % kSlots);
     char* c = static_cast<char*>(d_coords) + (size_t)t * fstride;
     double* grad_t = d_grad + (size_t)t * p->sys.natoms * 3;
     p->use_slot(b);
@@ -730,7 +734,7 @@ int mdsf_adjoint_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t nfra
                                     /*derivs_ready=*/true, /*grad_zeroed=*/true));
     MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }
-  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) & 1], 0));
+  MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) % kSlots], 0));
   return MDSF_OK;
 }
 

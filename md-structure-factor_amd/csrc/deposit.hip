@@ -1516,7 +1516,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 // ---------------------------------------------------------------------------
 // Corner-quirk spill (dens.py:107): one workgroup per destination tile of the
 // spill grid.  The candidates are the frame's corner atoms in sorted-index
-// order (the list k_gather and k_corner_order build); past kCornerCap of them,
+// order (the list k_rank_gather and k_corner_order build); past kCornerCap of them,
 // per band, the atoms whose windows reach the source tile (the destination
 // tile shifted in z by -Nb for band 0, +Nb for band 1) in the deposition's
 // enumeration order.  A corner atom whose z window does not reach the
@@ -1540,7 +1540,7 @@ __device__ __forceinline__ double spill_exp<double>(double q) { return exp(-q); 
 // stage[block * 16^3 + (ly * 16 + lx) * 16 + lz] (l = cell within the tile)
 // for the tiled deposition to add as it stores; otherwise it adds to rho after
 // the tiled deposition.
-template <typename ACC>
+template <typename ACC, bool GEN>
 __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
                                                             const TypeDev* __restrict__ td,
                                                             const int* __restrict__ boff,
@@ -1614,15 +1614,21 @@ __global__ __launch_bounds__(kThreads) void k_deposit_spill(SysDev sys,
         const ACC exy = (ACC)ty.amp *
                         spill_exp<ACC>((sys.g00 * bx * bx + 2.0 * sys.g01 * bx * by + sys.g11 * by * by) /
                                        ty.two_sig2);
-        // general cells: the z couplings add exp(-2 (g02 bx + g12 by) bz / 2 s^2)
-        const double cz = sys.mode == MODE_GEN ? 2.0 * (sys.g02 * bx + sys.g12 * by) : 0.0;
+        if constexpr (GEN) {
+          // general cells: the z couplings add exp(-2 (g02 bx + g12 by) bz / 2 s^2)
+          // (a kernel of its own: the extra exponential per cell would cost the
+          // other cells' kernel its registers)
+          const double cz = 2.0 * (sys.g02 * bx + sys.g12 * by);
 #pragma unroll
-        for (int k = 0; k < T; ++k) {
-          ACC f = ez ? ez[k] : zfactor(q, ty, r[2], k);
-          if (cz != 0.0)
-            f *= spill_exp<ACC>(cz * (r[2] - (double)(dst.t0[2] + k + shift) * sys.dr[2]) /
-                                ty.two_sig2);
-          acc[k] += exy * f;
+          for (int k = 0; k < T; ++k) {
+            const ACC f = ez ? ez[k] : zfactor(q, ty, r[2], k);
+            acc[k] += exy * f *
+                      spill_exp<ACC>(cz * (r[2] - (double)(dst.t0[2] + k + shift) * sys.dr[2]) /
+                                     ty.two_sig2);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < T; ++k) acc[k] += exy * (ez ? ez[k] : zfactor(q, ty, r[2], k));
         }
       }
     };
@@ -1734,9 +1740,14 @@ template <typename ACC>
 hipError_t launch_spill(Plan& p, ACC* out, ACC* stage, hipStream_t s) {
   const SysDev& sys = p.sys;
   if (p.spill_tiles == 0 || sys.natoms == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_deposit_spill<ACC>, dim3((unsigned)p.spill_tiles), dim3(kThreads), 0, s,
-                     sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.corners, p.spill_flag, out,
-                     stage);
+  if (sys.mode == MODE_GEN)
+    hipLaunchKernelGGL((k_deposit_spill<ACC, true>), dim3((unsigned)p.spill_tiles), dim3(kThreads),
+                       0, s, sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.corners, p.spill_flag,
+                       out, stage);
+  else
+    hipLaunchKernelGGL((k_deposit_spill<ACC, false>), dim3((unsigned)p.spill_tiles),
+                       dim3(kThreads), 0, s, sys, p.types, p.bucket_off, p.s_ir, p.s_r, p.corners,
+                       p.spill_flag, out, stage);
   return hipGetLastError();
 }
 

@@ -138,6 +138,11 @@ struct FrameState {
   int* spill_flag = nullptr;
 };
 
+// Frame slots of a plan (atom state + density / spectrum buffers): frame t
+// uses slot t mod kSlots, so that a frame can be prepared, the previous one
+// deposited and the ones before it transformed without waiting for a buffer.
+constexpr int kSlots = 3;
+
 struct Plan {
   SysDev sys{};
   TypeDev types_host[kMaxTypes]{};
@@ -166,26 +171,27 @@ struct Plan {
   long long spill_tiles = 0;      // sp_cnt[0] sp_cnt[1] sp_cnt[2]
   long long slice_floats = 0;     // regular slices, then the corner components
   int* err = nullptr;
-  // grids: two sets of density / spectrum buffers so that one frame's FFT can
-  // run while the next frame is deposited; rho / spec / res / spec_nyq point at
-  // the current one
-  void* rho_buf[2] = {nullptr, nullptr};
-  void* spec_buf[2] = {nullptr, nullptr};
-  void* res_buf[2] = {nullptr, nullptr};
-  void* nyq_buf[2] = {nullptr, nullptr};
+  // grids: one set of density / spectrum buffers per frame slot, so that a
+  // frame's FFT can run while the next frames are deposited; rho / spec / res /
+  // spec_nyq point at the current one
+  void* rho_buf[kSlots] = {};
+  void* spec_buf[kSlots] = {};
+  void* res_buf[kSlots] = {};
+  void* nyq_buf[kSlots] = {};
   // rho: Nx*Ny*Nz folded density (plan precision), which the y pass then
   // overwrites with the spectrum rho[kz][ky][x] (Nx*Ny*Nz/2 complex)
   void* rho = nullptr;
   void* spec = nullptr;          // z pass output spec[x][kz][y], Nx*Ny*Nz/2 complex
   void* res = nullptr;           // y pass rounding residuals, a byte per complex (FP32 plans)
   void* spec_nyq = nullptr;      // Nx*Ny FP64 complex: the kz = Nz/2 plane, nyq[x][y]
-  FrameState slot[2];
+  FrameState slot[kSlots];
   // Frame slot b: its atom state and density / spectrum buffer become the
   // ones the launches below use (kernel arguments are taken at launch).
   int cur_slot = 0;
   void use_slot(int b) {
-    cur_slot = b & 1;
-    const FrameState& f = slot[b & 1];
+    b %= kSlots;
+    cur_slot = b;
+    const FrameState& f = slot[b];
     ir4 = f.ir4;
     key = f.key;
     rpos = f.rpos;
@@ -203,10 +209,10 @@ struct Plan {
     corners = f.corners;
     spill = f.spill;
     spill_flag = f.spill_flag;
-    rho = rho_buf[b & 1];
-    spec = spec_buf[b & 1];
-    res = res_buf[b & 1];
-    spec_nyq = nyq_buf[b & 1];
+    rho = rho_buf[b];
+    spec = spec_buf[b];
+    res = res_buf[b];
+    spec_nyq = nyq_buf[b];
   }
   double* sf_main = nullptr;     // FP64 accumulator sf[kz][ky][kx], kz < Nz/2
   double* sf_nyq = nullptr;      // FP64 accumulator of the kz = Nz/2 plane, [kx][ky]
@@ -228,8 +234,7 @@ struct Plan {
   std::vector<hipEvent_t> tev;
   int64_t timed_frames = 0;
   hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_prep[2] = {nullptr, nullptr}, ev_dep[2] = {nullptr, nullptr},
-             ev_fft[2] = {nullptr, nullptr};
+  hipEvent_t ev_prep[kSlots] = {}, ev_dep[kSlots] = {}, ev_fft[kSlots] = {};
   // adjoint (allocated on first use)
   long long adj_items = 0;
   double* adj_part = nullptr;    // 3 FP64 partial sums per work item

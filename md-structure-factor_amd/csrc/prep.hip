@@ -113,6 +113,8 @@ __global__ void k_prep(T* __restrict__ coords, SysDev sys, const TypeDev* __rest
 // scans (k_scan_blocks), then block offsets (k_scan_add).  The counts are
 // cleared for the next frame and the fill cursors seeded.
 constexpr int kScanThreads = 1024;
+// bucket counts up to which the frame loop scans in one single-block launch
+constexpr int kScanOneMax = 1 << 16;
 
 __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int* total) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -132,6 +134,42 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int* total) {
   }
   *total = tot;
   return off + x - v;
+}
+
+// One-launch form for the frame loop: a single block walks the counts in
+// rounds of 4 x 1024 consecutive buckets (thread t takes 4 t .. 4 t + 3 of the
+// round, coalesced), one block scan per round plus the running total.  Writes
+// boff[0..nbk] (boff[nbk] = total), seeds the fill cursors, clears the counts
+// and the corner list count.
+__global__ __launch_bounds__(kScanThreads) void k_scan_one(int* __restrict__ bcount,
+                                                           int* __restrict__ boff,
+                                                           int* __restrict__ bcur, int nbk,
+                                                           int* __restrict__ corners) {
+  __shared__ int wsum[kScanThreads / 64];
+  int carry = 0;
+  for (int base = 0; base < nbk; base += 4 * kScanThreads) {
+    const int i0 = base + 4 * (int)threadIdx.x;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < nbk ? bcount[i0 + k] : 0;
+    int total;
+    int run = carry + block_excl_scan(v[0] + v[1] + v[2] + v[3], wsum, &total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < nbk) {
+        boff[i0 + k] = run;
+        bcur[i0 + k] = run;
+        bcount[i0 + k] = 0;
+      }
+      run += v[k];
+    }
+    carry += total;
+    __syncthreads();  // wsum is rewritten by the next round's scan
+  }
+  if (threadIdx.x == 0) {
+    boff[nbk] = carry;
+    corners[0] = 0;  // refilled by k_rank_gather
+  }
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int* __restrict__ bcount,
@@ -154,7 +192,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(int* __restrict__ bof
                                                            const int* __restrict__ aux, int nbk,
                                                            int nblocks, int* __restrict__ corners) {
   __shared__ int base_sh;
-  if (blockIdx.x == 0 && threadIdx.x == 0) corners[0] = 0;  // refilled by k_gather
+  if (blockIdx.x == 0 && threadIdx.x == 0) corners[0] = 0;  // refilled by k_rank_gather
   if (threadIdx.x < 64) {
     int s = 0;
     for (int b = threadIdx.x; b < (int)blockIdx.x; b += 64) s += aux[b];
@@ -186,11 +224,15 @@ __global__ void k_fill(const int* __restrict__ key, int* __restrict__ bcur,
 // Restores atom-index order inside each bucket, so that the gather visits
 // atoms in a canonical order (bitwise-reproducible sums): the entry at
 // position p of bucket b moves to boff[b] + (number of entries of b with a
-// smaller atom index).  One thread per entry; the entries of a bucket are
-// read by the threads of that bucket together (mostly the same wavefront).
-__global__ void k_rank_sort(const int* __restrict__ key, const int* __restrict__ boff,
-                            const int* __restrict__ s_tmp, int* __restrict__ s_idx,
-                            const int* __restrict__ nvalid) {
+// smaller atom index), together with its atom's home cell / type word and
+// position; corner atoms are listed (in arrival order; k_corner_order sorts
+// the list).  One thread per entry; the entries of a bucket are read by the
+// threads of that bucket together (mostly the same wavefront).
+__global__ void k_rank_gather(const int* __restrict__ key, const int* __restrict__ boff,
+                              const int* __restrict__ s_tmp, int* __restrict__ s_idx,
+                              const int4* __restrict__ ir4, const double* __restrict__ rpos,
+                              int4* __restrict__ s_ir, double* __restrict__ s_r,
+                              const int* __restrict__ nvalid, int* __restrict__ corners) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= *nvalid) return;
   const int i = s_tmp[p];
@@ -199,21 +241,11 @@ __global__ void k_rank_sort(const int* __restrict__ key, const int* __restrict__
   int r = 0;
 #pragma unroll 4
   for (int q = lo; q < hi; ++q) r += s_tmp[q] < i ? 1 : 0;
-  s_idx[lo + r] = i;
-}
-
-// Gathers atom data into sorted order (valid sorted positions only).
-__global__ void k_gather(const int* __restrict__ s_idx, const int4* __restrict__ ir4,
-                         const double* __restrict__ rpos, int4* __restrict__ s_ir,
-                         double* __restrict__ s_r, const int* __restrict__ nvalid,
-                         int* __restrict__ corners) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *nvalid) return;
-  const int i = s_idx[s];
+  const int s = lo + r;
+  s_idx[s] = i;
   const int4 q = ir4[i];
   s_ir[s] = q;
   if ((q.w >> 16) & 1) {
-    // corner atom: listed in arrival order; k_corner_order sorts the list
     const int k = atomicAdd(corners, 1);
     if (k < kCornerCap) corners[1 + k] = s;
   }
@@ -224,7 +256,7 @@ __global__ void k_gather(const int* __restrict__ s_idx, const int4* __restrict__
 
 // The corner atoms that spill (spill_bands != 0), in sorted-index order, one
 // block: entry v goes to its rank among them, so the spill pass visits them
-// in a fixed order whatever order k_gather's atomics gave.  Their count goes
+// in a fixed order whatever order k_rank_gather's atomics gave.  Their count goes
 // to corners[1 + 2 kCornerCap].
 constexpr int kOrderThreads = 256;
 __global__ __launch_bounds__(kOrderThreads) void k_corner_order(SysDev sys,
@@ -466,17 +498,21 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
   const int nbk = p.sys.nbuckets;
   const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
   if (nblk == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
-                     p.bucket_off, p.bucket_aux, nbk);
-  hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off, p.bucket_cur,
-                     p.bucket_aux, nbk, nblk, p.corners);
+  if (nbk <= kScanOneMax) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanThreads), 0, s, p.bucket_count,
+                       p.bucket_off, p.bucket_cur, nbk, p.corners);
+  } else {
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
+                       p.bucket_off, p.bucket_aux, nbk);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off,
+                       p.bucket_cur, p.bucket_aux, nbk, nblk, p.corners);
+  }
   if (n > 0) {
     hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
                        p.s_tmp, n);
-    hipLaunchKernelGGL(k_rank_sort, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_off,
-                       p.s_tmp, p.s_idx, p.bucket_off + nbk);
-    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.s_idx, p.ir4, p.rpos,
-                       p.s_ir, p.s_r, p.bucket_off + nbk, p.corners);
+    hipLaunchKernelGGL(k_rank_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.key,
+                       p.bucket_off, p.s_tmp, p.s_idx, p.ir4, p.rpos, p.s_ir, p.s_r,
+                       p.bucket_off + nbk, p.corners);
     hipLaunchKernelGGL(k_corner_order, dim3(1), dim3(kOrderThreads), 0, s, p.sys, p.types,
                        p.s_ir, p.corners, p.spill ? p.spill_flag : nullptr,
                        (int)(p.spill ? p.spill_tiles : 0));

@@ -251,7 +251,7 @@ class Plan:
         return dict(zip(["prepare", "deposit", "fft_z", "fft_y", "fft_x_acc"], list(out)))
 
     def set_frame_buffer(self, which):
-        """Select density / spectrum buffer 0 or 1 for the single-frame entry points."""
+        """Select density / spectrum buffer 0, 1 or 2 for the single-frame entry points."""
         _lib.check(self.lib.mdsf_set_frame_buffer(self._handle, int(which)))
 
     def deposit(self, out=None):
