@@ -514,9 +514,15 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_fft[b], 0));
-    MDSF_HIP(mark(t, 2, d));
-    MDSF_HIP(launch_deposit(*p, p->rho, d));
-    MDSF_HIP(mark(t, 3, d));
+    // the deposition's events are recorded with the tiled kernel's own
+    // dispatch (its begin and end, without the queue's dispatch latency)
+    if (p->timing) {
+      p->dep_ev[0] = p->tev[10 * t + 2];
+      p->dep_ev[1] = p->tev[10 * t + 3];
+    }
+    const hipError_t de = launch_deposit(*p, p->rho, d);
+    p->dep_ev[0] = p->dep_ev[1] = nullptr;
+    MDSF_HIP(de);
     MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
     // parked atoms' clipped windows: on the FFT stream, which has room

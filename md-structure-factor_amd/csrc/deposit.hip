@@ -23,6 +23,8 @@
 // reproducible.  ACC is the arithmetic and output type (float or double).
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "clip.h"
 #include "mdsf_internal.h"
 #include "spill.h"
@@ -1757,32 +1759,42 @@ hipError_t launch_impl(Plan& p, ACC* out, const ACC* stage, hipStream_t s) {
   const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
   if (ntiles == 0) return hipSuccess;
   const dim3 grid((unsigned)ntiles), block(kThreads);
+  // with the plan's deposition events set (stage timing), the events are
+  // recorded with the kernel's own dispatch (the kernel's begin and end)
+#define DEP_LAUNCH(K, ...)                                                              \
+  do {                                                                                  \
+    if (p.dep_ev[0])                                                                    \
+      hipExtLaunchKernelGGL(K, grid, block, 0, s, p.dep_ev[0], p.dep_ev[1], 0, __VA_ARGS__); \
+    else                                                                                \
+      hipLaunchKernelGGL(K, grid, block, 0, s, __VA_ARGS__);                            \
+  } while (0)
   if (sys.mode == MODE_GEN) {
-    hipLaunchKernelGGL((k_deposit_gen<ACC>), grid, block, 0, s, sys, p.types, p.bucket_off,
+    DEP_LAUNCH((k_deposit_gen<ACC>), sys, p.types, p.bucket_off,
                        p.s_ir, p.s_r, stage, p.spill_flag, out);
   } else if constexpr (std::is_same<ACC, float>::value) {
     // blocks size their per-type LDS for 16 types when the system has no more
     const bool few = sys.ntypes <= 16;
     if (sys.mode == MODE_DIAG && few)
-      hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, 16>), grid, block, 0, s, sys, p.types,
+      DEP_LAUNCH((k_deposit_mfma<MODE_DIAG, 16>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
     else if (sys.mode == MODE_DIAG)
-      hipLaunchKernelGGL((k_deposit_mfma<MODE_DIAG, kMaxTypes>), grid, block, 0, s, sys, p.types,
+      DEP_LAUNCH((k_deposit_mfma<MODE_DIAG, kMaxTypes>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
     else if (few)
-      hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, 16>), grid, block, 0, s, sys, p.types,
+      DEP_LAUNCH((k_deposit_mfma<MODE_XY, 16>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
     else
-      hipLaunchKernelGGL((k_deposit_mfma<MODE_XY, kMaxTypes>), grid, block, 0, s, sys, p.types,
+      DEP_LAUNCH((k_deposit_mfma<MODE_XY, kMaxTypes>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
   } else {
     if (sys.mode == MODE_DIAG)
-      hipLaunchKernelGGL((k_deposit<MODE_DIAG, ACC>), grid, block, 0, s, sys, p.types,
+      DEP_LAUNCH((k_deposit<MODE_DIAG, ACC>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.tables, stage, p.spill_flag, out);
     else
-      hipLaunchKernelGGL((k_deposit<MODE_XY, ACC>), grid, block, 0, s, sys, p.types,
+      DEP_LAUNCH((k_deposit<MODE_XY, ACC>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.tables, stage, p.spill_flag, out);
   }
+#undef DEP_LAUNCH
   return hipGetLastError();
 }
 

@@ -231,6 +231,9 @@ struct Plan {
   // optional stage timing of mdsf_accumulate_frames: 10 timing events per
   // frame (prepare, deposit, z, y, x: start / end, each on its stream)
   int timing = 0;  // 0 off, 1 every stage, 2 the deposition only
+  // events the next tiled deposition records with its own dispatch (begin,
+  // end), or null: set by the timed frame loop around launch_deposit
+  hipEvent_t dep_ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> tev;
   int64_t timed_frames = 0;
   hipEvent_t ev_fork = nullptr;

@@ -113,8 +113,6 @@ __global__ void k_prep(T* __restrict__ coords, SysDev sys, const TypeDev* __rest
 // scans (k_scan_blocks), then block offsets (k_scan_add).  The counts are
 // cleared for the next frame and the fill cursors seeded.
 constexpr int kScanThreads = 1024;
-// bucket counts up to which the frame loop scans in one single-block launch
-constexpr int kScanOneMax = 1 << 16;
 
 __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int* total) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -134,42 +132,6 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int* total) {
   }
   *total = tot;
   return off + x - v;
-}
-
-// One-launch form for the frame loop: a single block walks the counts in
-// rounds of 4 x 1024 consecutive buckets (thread t takes 4 t .. 4 t + 3 of the
-// round, coalesced), one block scan per round plus the running total.  Writes
-// boff[0..nbk] (boff[nbk] = total), seeds the fill cursors, clears the counts
-// and the corner list count.
-__global__ __launch_bounds__(kScanThreads) void k_scan_one(int* __restrict__ bcount,
-                                                           int* __restrict__ boff,
-                                                           int* __restrict__ bcur, int nbk,
-                                                           int* __restrict__ corners) {
-  __shared__ int wsum[kScanThreads / 64];
-  int carry = 0;
-  for (int base = 0; base < nbk; base += 4 * kScanThreads) {
-    const int i0 = base + 4 * (int)threadIdx.x;
-    int v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = i0 + k < nbk ? bcount[i0 + k] : 0;
-    int total;
-    int run = carry + block_excl_scan(v[0] + v[1] + v[2] + v[3], wsum, &total);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (i0 + k < nbk) {
-        boff[i0 + k] = run;
-        bcur[i0 + k] = run;
-        bcount[i0 + k] = 0;
-      }
-      run += v[k];
-    }
-    carry += total;
-    __syncthreads();  // wsum is rewritten by the next round's scan
-  }
-  if (threadIdx.x == 0) {
-    boff[nbk] = carry;
-    corners[0] = 0;  // refilled by k_rank_gather
-  }
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int* __restrict__ bcount,
@@ -498,15 +460,10 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
   const int nbk = p.sys.nbuckets;
   const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
   if (nblk == 0) return hipSuccess;
-  if (nbk <= kScanOneMax) {
-    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanThreads), 0, s, p.bucket_count,
-                       p.bucket_off, p.bucket_cur, nbk, p.corners);
-  } else {
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
-                       p.bucket_off, p.bucket_aux, nbk);
-    hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off,
-                       p.bucket_cur, p.bucket_aux, nbk, nblk, p.corners);
-  }
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
+                     p.bucket_off, p.bucket_aux, nbk);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off,
+                     p.bucket_cur, p.bucket_aux, nbk, nblk, p.corners);
   if (n > 0) {
     hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
                        p.s_tmp, n);
