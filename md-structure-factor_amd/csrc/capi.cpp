@@ -473,8 +473,9 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   // transformed on fft_stream, so that while frame t is deposited frame t + 1
   // is prepared and earlier frames transformed.  A slot is reused kSlots
   // frames later: preparing frame t waits until frame t - kSlots's deposition
-  // has read its atom state, depositing it until frame t - kSlots's FFT has
-  // released its density buffer.  With three slots the deposition does not
+  // (tiled and parked passes) has read its atom state, depositing it until
+  // frame t - kSlots's FFT has released its density buffer.  With three
+  // slots the deposition does not
   // wait for the FFT of the frame just before the previous one (the FFT
   // shares the GPU with the deposition and finishes later than it).  The S(q)
   // sum stays in frame order on one stream.
@@ -523,10 +524,12 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     const hipError_t de = launch_deposit(*p, p->rho, d);
     p->dep_ev[0] = p->dep_ev[1] = nullptr;
     MDSF_HIP(de);
+    // parked atoms' clipped windows: on the deposition's stream, before
+    // ev_dep[b], because they read the slot's atom state, which preparing
+    // frame t + kSlots overwrites as soon as ev_dep[b] has passed
+    MDSF_HIP(launch_parked(*p, p->rho, d));
     MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
-    // parked atoms' clipped windows: on the FFT stream, which has room
-    MDSF_HIP(launch_parked(*p, p->rho, f));
     for (int pass = 0; pass < 3; ++pass) {
       MDSF_HIP(mark(t, 4 + 2 * pass, f));
       MDSF_HIP(fft_pass(*p, pass, p->rho, f));

@@ -947,18 +947,30 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
     }
     __syncthreads();
     if constexpr (MODE == MODE_DIAG) {
-      // atoms off the corner regions: copy their three tile slices (float4 per
-      // lane, all waves); the last group's padding is zeroed alongside
-      for (int it = tid; it < nreal * 12; it += kThreads) {
-        const int a = it / 12, r12 = it - 12 * (it / 12), d = r12 >> 2, q4 = r12 & 3;
-        if (((bt.q[a].w >> 16) & 1) != 0) continue;  // corner atom: evaluated below
-        const int off = bt.so[a][d];
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (off >= 0) v = *reinterpret_cast<const float4*>(slices + off + 4 * q4);
-        float* row = d == 0 ? st.vx[bt.vbase[a]] : (d == 1 ? st.vy[bt.vbase[a]] : st.vz[bt.vbase[a]]);
-        *reinterpret_cast<float4*>(row + 4 * q4) = v;
-        if (d == 0)
-          st.vq[bt.vbase[a]][q4] = (v.x != 0.f || v.y != 0.f || v.z != 0.f || v.w != 0.f) ? 1 : 0;
+      // atoms off the corner regions: four lanes per atom (NVK * 4 = the
+      // block), each copying a quarter of the atom's three tile slices; the
+      // three loads are issued before the first store, so the batch waits
+      // for one memory round trip
+      {
+        static_assert(NVK * 4 == kThreads, "one quarter-slice lane per atom and quarter");
+        const int a = tid >> 2, q4 = tid & 3;
+        if (a < nreal && ((bt.q[a].w >> 16) & 1) == 0) {
+          // an axis without a slice (-1) loads the buffer's first quarter
+          // slice and zeroes it: unconditional loads, no branch between them
+          const int o0 = bt.so[a][0], o1 = bt.so[a][1], o2 = bt.so[a][2];
+          float4 v0 = *reinterpret_cast<const float4*>(slices + max(o0, 0) + 4 * q4);
+          float4 v1 = *reinterpret_cast<const float4*>(slices + max(o1, 0) + 4 * q4);
+          float4 v2 = *reinterpret_cast<const float4*>(slices + max(o2, 0) + 4 * q4);
+          const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (o0 < 0) v0 = z4;
+          if (o1 < 0) v1 = z4;
+          if (o2 < 0) v2 = z4;
+          const int vb = bt.vbase[a];
+          *reinterpret_cast<float4*>(&st.vx[vb][4 * q4]) = v0;
+          *reinterpret_cast<float4*>(&st.vy[vb][4 * q4]) = v1;
+          *reinterpret_cast<float4*>(&st.vz[vb][4 * q4]) = v2;
+          st.vq[vb][q4] = (v0.x != 0.f || v0.y != 0.f || v0.z != 0.f || v0.w != 0.f) ? 1 : 0;
+        }
       }
       for (int j = tid; j < (vpad - vtot) * T; j += kThreads) {
         const int v = vtot + j / T, cc = j % T;
@@ -1176,31 +1188,25 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
       kend = (cnt + 3) & ~3;
       if (lane >= cnt && lane < kend) st.wl[wave][lane] = (unsigned char)NVK;
       __builtin_amdgcn_wave_barrier();
-    }
-    for (int k0 = 0; k0 < kend; k0 += 4) {
-      int mu;
-      if constexpr (MODE == MODE_DIAG) {
-        // every x row of the wave's block: testing single rows (LDS read +
-        // ballot per group) costs more than the MFMAs it would skip
-        mu = 0xf;
-      } else {
-        const int m = ((st.vxm[k0] | st.vxm[k0 + 1] | st.vxm[k0 + 2] | st.vxm[k0 + 3]) >> (4 * wave)) & 0xf;
-        mu = __builtin_amdgcn_readfirstlane(m);
-      }
-      if (!mu) continue;
-      int k = k0 + kk;
-      if constexpr (MODE == MODE_DIAG) k = st.wl[wave][k0 + kk];
-      const float zv = st.vz[k][jj];
-      if constexpr (MODE == MODE_DIAG) {
-        const float yv = st.vy[k][jj];
+      // every x row of the wave's block: testing single rows (LDS read +
+      // ballot per group) costs more than the MFMAs it would skip
+      for (int k0 = 0; k0 < kend; k0 += 4) {
+        const int k = st.wl[wave][k0 + kk];
+        const float zv = st.vz[k][jj], yv = st.vy[k][jj];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          if ((mu >> b) & 1) {
-            const float av = st.vx[k][4 * wave + b] * yv;
-            acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, zv, acc[b], 0, 0, 0);
-          }
+          const float av = st.vx[k][4 * wave + b] * yv;
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, zv, acc[b], 0, 0, 0);
         }
-      } else {
+      }
+    }
+    if constexpr (MODE == MODE_XY) for (int k0 = 0; k0 < kend; k0 += 4) {
+      const int m = ((st.vxm[k0] | st.vxm[k0 + 1] | st.vxm[k0 + 2] | st.vxm[k0 + 3]) >> (4 * wave)) & 0xf;
+      const int mu = __builtin_amdgcn_readfirstlane(m);
+      if (!mu) continue;
+      const int k = k0 + kk;
+      const float zv = st.vz[k][jj];
+      {
         const int ra = st.vreal[k], cls = st.vcls[k];
         const float amp = st.amp[ra], k2 = st.k2[ra];
         const int yvb = st.yv[ra][jj];
