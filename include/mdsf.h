@@ -94,7 +94,9 @@ int mdsf_sf_reset(mdsf_plan* plan, mdsf_stream stream);
  * same indices (dens.py:292-308), which numpy clips alike, and so does the
  * deposition (a window across the low edge or past the high edge adds
  * nothing, one cut by the high edge adds its kept part, one wholly at
- * negative indices is counted from the far end of the padded grid). */
+ * negative indices is counted from the far end of the padded grid).
+ * MDSF_ERR_UNSUPPORTED reports an internal size bound the frame exceeded
+ * (the tile-list pool of FP32 orthogonal-cell plans). */
 int mdsf_check_errors(mdsf_plan* plan, mdsf_stream stream);
 
 /* The frame loop of dens.py:277-321 for `nframes` frames.

@@ -41,37 +41,45 @@ hipError_t dalloc(T** p, size_t count) {
   return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
 }
 
-// The frame pipeline's two streams, one pair per device shared by every plan
-// and kept until the process ends.  HIP gives a process few hardware queues
-// (GPU_MAX_HW_QUEUES, 4 by default) and hands them to streams in turn, so
-// private streams per plan would soon share queues, and a deposition could
-// no longer overlap an FFT queued behind it.  Each frame loop starts from and
-// joins back into the caller's stream through its own plan's events, so
-// plans sharing the streams stay correctly ordered.
+// The frame pipeline's three streams (deposition, FFT, and the tile-list
+// branch of the prepare stage), one set per device shared by every plan and
+// kept until the process ends.  HIP gives a process few hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default: these three and the caller's) and hands
+// them to streams in turn, so private streams per plan would soon share
+// queues, and a deposition could no longer overlap an FFT queued behind it.
+// Each frame loop starts from and joins back into the caller's stream through
+// its own plan's events, so plans sharing the streams stay correctly ordered.
+struct PipelineStreams {
+  hipStream_t dep = nullptr, fft = nullptr, aux = nullptr;
+};
 std::mutex g_stream_mu;
-std::map<int, std::pair<hipStream_t, hipStream_t>> g_streams;
+std::map<int, PipelineStreams> g_streams;
 
-hipError_t pipeline_streams(int device, hipStream_t* dep, hipStream_t* fft) {
+hipError_t pipeline_streams(int device, hipStream_t* dep, hipStream_t* fft, hipStream_t* aux) {
   std::lock_guard<std::mutex> lock(g_stream_mu);
   auto it = g_streams.find(device);
   if (it == g_streams.end()) {
-    hipStream_t a = nullptr, b = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b, hipStreamNonBlocking);
+    PipelineStreams ps;
+    hipError_t e = hipStreamCreateWithFlags(&ps.dep, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ps.fft, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ps.aux, hipStreamNonBlocking);
     if (e != hipSuccess) {
-      if (a) (void)hipStreamDestroy(a);
+      for (hipStream_t q : {ps.dep, ps.fft, ps.aux})
+        if (q) (void)hipStreamDestroy(q);
       return e;
     }
-    it = g_streams.emplace(device, std::make_pair(a, b)).first;
+    it = g_streams.emplace(device, ps).first;
   }
-  *dep = it->second.first;
-  *fft = it->second.second;
+  *dep = it->second.dep;
+  *fft = it->second.fft;
+  *aux = it->second.aux;
   return hipSuccess;
 }
 
 void free_plan(Plan* p) {
   if (!p) return;
-  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  for (hipEvent_t e : {p->ev_fork, p->ev_lfork, p->ev_ljoin})
+    if (e) (void)hipEventDestroy(e);
   for (int b = 0; b < kSlots; ++b)
     for (hipEvent_t e : {p->ev_prep[b], p->ev_dep[b], p->ev_fft[b]})
       if (e) (void)hipEventDestroy(e);
@@ -79,7 +87,7 @@ void free_plan(Plan* p) {
   for (FrameState& f : p->slot) {
     void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
                   f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices,
-                  f.dslices, f.corners, f.spill, f.spill_flag};
+                  f.dslices, f.corners, f.spill, f.spill_flag, f.tl_pool, f.tl_head};
     for (void* q : fs)
       if (q) (void)hipFree(q);
   }
@@ -265,6 +273,29 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     free_plan(p);
     return fail(MDSF_ERR_UNSUPPORTED, "too many atoms for the FP32 tile-slice layout (2^31 floats)");
   }
+  // Tile lists: a tile's pool region holds its candidates (atoms of the
+  // bucket columns that can reach it), so the pool is sized by the most
+  // tiles whose candidate range includes one bucket: per axis the tiles whose
+  // cells within A - 1 of the tile (t0 - A + 1 .. t1 + A - 1) meet the B
+  // cells of the bucket, at most (B + 2A + 13) / 16 + 1 of them, plus one
+  // for a partial last tile or bucket at the periodic seam, and at most all
+  // tiles.  Entries carry slice indices and the type in bytes.
+  p->tl_cap = 0;
+  if (p->slice_floats > 0) {
+    long long cap = 0;
+    bool ok = desc->ntypes <= kMaxTypes;
+    for (int t = 0; t < desc->ntypes && ok; ++t) {
+      const TypeDev& ty = p->types_host[t];
+      long long per_atom = 1;
+      for (int d = 0; d < 3; ++d) {
+        if (ty.nsl[d] >= kListNoSlice) ok = false;
+        const long long nt = (desc->n[d] + kTile - 1) / kTile;
+        per_atom *= std::min(nt, (long long)(ty.bsz[d] + 2 * ty.half[d] + 13) / kTile + 2);
+      }
+      cap += (long long)ty.count * per_atom;
+    }
+    if (ok && cap < (1LL << 31)) p->tl_cap = std::max(cap, 1LL);
+  }
   {  // tiled adjoint result slots: per atom one per tile slice triple
     long long slots = 0;
     for (int t = 0; t < desc->ntypes; ++t) {
@@ -354,6 +385,10 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       // never runs the binning)
       TRY(hipMemset(f.spill_flag, 0, sizeof(int) * (size_t)p->spill_tiles));
     }
+    if (p->tl_cap > 0) {
+      TRY(dalloc(&f.tl_pool, (size_t)p->tl_cap));
+      TRY(dalloc(&f.tl_head, (size_t)s.ntiles[0] * s.ntiles[1] * s.ntiles[2]));
+    }
   }
   TRY(dalloc(&p->err, (size_t)1));
   TRY(hipMemset(p->err, 0, sizeof(int)));
@@ -397,8 +432,9 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     free_plan(p);
     return rc;
   }
-  e = pipeline_streams(p->device, &p->dep_stream, &p->fft_stream);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+  e = pipeline_streams(p->device, &p->dep_stream, &p->fft_stream, &p->aux_stream);
+  for (hipEvent_t* ev : {&p->ev_fork, &p->ev_lfork, &p->ev_ljoin})
+    if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   for (int b = 0; b < kSlots && e == hipSuccess; ++b)
     for (hipEvent_t* ev : {&p->ev_prep[b], &p->ev_dep[b], &p->ev_fft[b]})
       if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
@@ -449,6 +485,8 @@ int mdsf_check_errors(mdsf_plan* plan, mdsf_stream stream) {
   MDSF_HIP(hipStreamSynchronize(s));
   if (flag) MDSF_HIP(hipMemsetAsync(p->err, 0, sizeof(int), s));
   if (flag & ERRBIT_TYPE) return fail(MDSF_ERR_TYPE, "atom type index out of range");
+  if (flag & ERRBIT_LIST)
+    return fail(MDSF_ERR_UNSUPPORTED, "tile list pool overflow (its size bound was exceeded)");
   if (flag & ERRBIT_RANGE)
     return fail(MDSF_ERR_RANGE,
                 "an atom's window leaves the padded grid (atom more than the border outside the "
@@ -510,7 +548,7 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(mark(t, 0, s));
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
-    MDSF_HIP(launch_binning(*p, s));
+    MDSF_HIP(launch_binning(*p, s, false, p->aux_stream));
     MDSF_HIP(mark(t, 1, s));
     MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));

@@ -79,11 +79,27 @@ __device__ __forceinline__ bool overlap1(int ir, int A, int t0, int t1, int N) {
 struct TypeRanges {
   int lo[3][2], hi[3][2], rn[3];
   int nx, ny, ncols;
+  int bucket_base, nbk1, nbk2;  // the type's bucket numbering
 };
 
 __device__ __forceinline__ void type_ranges(const TypeDev& ty, const SysDev& sys, const int* t0,
                                             const int* t1, TypeRanges& r) {
-  if (ty.count == 0 || ty.half[0] == 0 || ty.half[1] == 0 || ty.half[2] == 0) {
+  // the type's fields as one batch of loads (the empty asm keeps them ahead
+  // of the branches below, which would otherwise wait for them one by one)
+  const int count = ty.count, bucket_base = ty.bucket_base;
+  int half[3], bsz[3], nbk[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    half[d] = ty.half[d];
+    bsz[d] = ty.bsz[d];
+    nbk[d] = ty.nbk[d];
+  }
+  asm volatile("" ::"v"(count), "v"(half[0]), "v"(half[1]), "v"(half[2]), "v"(bsz[0]),
+               "v"(bsz[1]), "v"(bsz[2]), "v"(nbk[0]), "v"(nbk[1]), "v"(nbk[2]), "v"(bucket_base));
+  r.bucket_base = bucket_base;
+  r.nbk1 = nbk[1];
+  r.nbk2 = nbk[2];
+  if (count == 0 || half[0] == 0 || half[1] == 0 || half[2] == 0) {
     r.ncols = 0;
     r.nx = r.ny = 0;
     r.rn[2] = 0;
@@ -91,8 +107,7 @@ __device__ __forceinline__ void type_ranges(const TypeDev& ty, const SysDev& sys
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d)
-    r.rn[d] = bucket_ranges(t0[d], t1[d], ty.half[d], sys.n[d], ty.bsz[d], ty.nbk[d], r.lo[d],
-                            r.hi[d]);
+    r.rn[d] = bucket_ranges(t0[d], t1[d], half[d], sys.n[d], bsz[d], nbk[d], r.lo[d], r.hi[d]);
   r.nx = (r.hi[0][0] - r.lo[0][0] + 1) + (r.rn[0] == 2 ? r.hi[0][1] - r.lo[0][1] + 1 : 0);
   r.ny = (r.hi[1][0] - r.lo[1][0] + 1) + (r.rn[1] == 2 ? r.hi[1][1] - r.lo[1][1] + 1 : 0);
   r.ncols = r.nx * r.ny * r.rn[2];
@@ -133,6 +148,7 @@ struct StageXY {
 
 struct Tile {
   int t0[3], t1[3];
+  int idx;  // (tx * ntiles_y + ty) * ntiles_z + tz
 };
 
 // value of a 3-array at a runtime axis without dynamic private indexing
@@ -466,13 +482,11 @@ __device__ __forceinline__ int block_scan(SH& sh, int v, int* excl) {
   return tot;
 }
 
-// Streams every atom whose window reaches the tile into the ring sh.list in a
-// fixed order; `process(count, head)` consumes entries head, head+1, ...
-// (mod kListCap) and returns how many it took.
-template <class SH, typename Process>
-__device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td,
-                               const int* __restrict__ boff, const int4* __restrict__ s_ir,
-                               const Tile& tl, SH& sh, int trigger, Process&& process) {
+// The tile's bucket columns: per-type half-widths and bucket ranges, and the
+// columns numbered type by type (col_base).  Returns the column count.
+template <class SH>
+__device__ int enum_setup(const SysDev& sys, const TypeDev* __restrict__ td, const Tile& tl,
+                          SH& sh) {
   const int tid = threadIdx.x;
   for (int i = tid; i < sys.ntypes * 3; i += kThreads) sh.half[i / 3][i % 3] = td[i / 3].half[i % 3];
   for (int t = tid; t < sys.ntypes; t += kThreads) type_ranges(td[t], sys, tl.t0, tl.t1, sh.tr[t]);
@@ -486,29 +500,59 @@ __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td
     sh.col_base[sys.ntypes] = acc;
   }
   __syncthreads();
+  return sh.col_base[sys.ntypes];
+}
+
+// Bucket column cidx of the tile: its first sorted index, its length (the
+// atoms of the column's z bucket range) and its type.
+template <class SH>
+__device__ __forceinline__ void column_at(const int* __restrict__ boff, const SH& sh, int cidx,
+                                          int& s, int& len, int& type) {
+  int t = 0;
+  while (sh.col_base[t + 1] <= cidx) ++t;
+  const TypeRanges& r = sh.tr[t];
+  const int ci = cidx - sh.col_base[t];
+  const int zr = r.rn[2] == 2 ? (ci & 1) : 0;
+  const int cxy = r.rn[2] == 2 ? (ci >> 1) : ci;
+  const int bxi = cxy / r.ny, byi = cxy - bxi * r.ny;
+  const int bx = range_item(r, 0, bxi), by = range_item(r, 1, byi);
+  const int col = r.bucket_base + (bx * r.nbk1 + by) * r.nbk2;
+  const int zlo = zr == 0 ? r.lo[2][0] : r.lo[2][1];
+  const int zhi = zr == 0 ? r.hi[2][0] : r.hi[2][1];
+  s = boff[col + zlo];
+  len = boff[col + zhi + 1] - s;
+  type = t;
+}
+
+// Candidate g of the current column round (col_s / col_pre / col_t): its
+// sorted index and type.
+template <class SH>
+__device__ __forceinline__ void candidate_at(const SH& sh, int g, int& c, int& t) {
+  // last column r with col_pre[r] <= g (col_pre[0] = 0 <= g): a fixed number
+  // of steps, so that several searches of one thread interleave
+  static_assert(kThreads == 256, "8 search steps");
+  int lo = 0;
+#pragma unroll
+  for (int step = kThreads / 2; step > 0; step >>= 1)
+    lo += sh.col_pre[lo + step] <= g ? step : 0;
+  c = sh.col_s[lo] + (g - sh.col_pre[lo]);
+  t = sh.col_t[lo];
+}
+
+// Streams every atom whose window reaches the tile into the ring sh.list in a
+// fixed order; `process(count, head)` consumes entries head, head+1, ...
+// (mod kListCap) and returns how many it took.
+template <class SH, typename Process>
+__device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td,
+                               const int* __restrict__ boff, const int4* __restrict__ s_ir,
+                               const Tile& tl, SH& sh, int trigger, Process&& process) {
+  const int tid = threadIdx.x;
+  const int total_cols = enum_setup(sys, td, tl, sh);
   int count = 0, head = 0;
-  const int total_cols = sh.col_base[sys.ntypes];
   for (int cbase = 0; cbase < total_cols; cbase += kThreads) {
     // phase A: one bucket column per thread -> (start, length, type)
     int my_s = 0, my_len = 0, my_t = 0;
-    const int cidx = cbase + tid;
-    if (cidx < total_cols) {
-      int t = 0;
-      while (sh.col_base[t + 1] <= cidx) ++t;
-      const TypeRanges& r = sh.tr[t];
-      const int ci = cidx - sh.col_base[t];
-      const int zr = r.rn[2] == 2 ? (ci & 1) : 0;
-      const int cxy = r.rn[2] == 2 ? (ci >> 1) : ci;
-      const int bxi = cxy / r.ny, byi = cxy - bxi * r.ny;
-      const TypeDev& ty = td[t];
-      const int bx = range_item(r, 0, bxi), by = range_item(r, 1, byi);
-      const int col = ty.bucket_base + (bx * ty.nbk[1] + by) * ty.nbk[2];
-      const int zlo = zr == 0 ? r.lo[2][0] : r.lo[2][1];
-      const int zhi = zr == 0 ? r.hi[2][0] : r.hi[2][1];
-      my_s = boff[col + zlo];
-      my_len = boff[col + zhi + 1] - my_s;
-      my_t = t;
-    }
+    if (cbase + tid < total_cols) column_at(boff, sh, cbase + tid, my_s, my_len, my_t);
     int excl;
     const int ncand = block_scan(sh, my_len, &excl);
     sh.col_s[tid] = my_s;
@@ -522,14 +566,8 @@ __device__ void enumerate_tile(const SysDev& sys, const TypeDev* __restrict__ td
       bool hit = false;
       int entry = 0;
       if (g < ncand) {
-        int lo = 0, hi = kThreads;  // last column r with col_pre[r] <= g
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (sh.col_pre[mid] <= g) lo = mid;
-          else hi = mid;
-        }
-        const int c = sh.col_s[lo] + (g - sh.col_pre[lo]);
-        const int t = sh.col_t[lo];
+        int c, t;
+        candidate_at(sh, g, c, t);
         const int4 q = s_ir[c];
         hit = overlap1(q.x, sh.half[t][0], tl.t0[0], tl.t1[0], sys.n[0]) &&
               overlap1(q.y, sh.half[t][1], tl.t0[1], tl.t1[1], sys.n[1]) &&
@@ -563,6 +601,7 @@ __device__ __forceinline__ void tile_coords(const SysDev& sys, Tile& tl) {
   const int tile = bid < 8 * q8 ? (bid % 8) * q8 + bid / 8 : bid;
   const int ntz = sys.ntiles[2], nty = sys.ntiles[1];
   const int tz = tile % ntz, ty_ = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  tl.idx = tile;
   tl.t0[0] = tx * T;
   tl.t0[1] = ty_ * T;
   tl.t0[2] = tz * T;
@@ -1278,6 +1317,371 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
 }
 
 // ---------------------------------------------------------------------------
+// Tile lists (FP32 orthogonal cells).  The candidate enumeration of the tiled
+// deposition runs in the prepare stage as kernels of its own.  Each tile
+// gets a region of the slot's pool with one entry per candidate (the atoms of
+// the bucket columns that can reach it): k_tile_count counts them,
+// k_tile_scan places the regions in tile order (no atomics, which one
+// counter shared by every tile would serialise).  k_tile_lists then streams
+// the tile's candidates in the enumeration's fixed order, tests them exactly
+// and writes the hits into the region, atoms off the corner regions upward
+// from its start and corner atoms downward from its end.  The tile's head is
+// {region start, regular atoms, corner atoms, region size}.  An entry holds
+// what the deposition needs to find the atom's tile slices:
+//   x  regular atom: float offset of its slices; corner atom: sorted index
+//   y  slice index j_d of the tile on axis d in byte d (kListNoSlice: none),
+//      type in bits 24..29
+template <int MT>
+__global__ __launch_bounds__(kThreads) void k_tile_count(SysDev sys, const TypeDev* __restrict__ td,
+                                                         const int* __restrict__ boff,
+                                                         int4* __restrict__ heads) {
+  __shared__ EnumShared<MT> sh;
+  const int tid = threadIdx.x;
+  Tile tl;
+  tile_coords(sys, tl);
+  const int total_cols = enum_setup(sys, td, tl, sh);
+  int nc = 0;
+  for (int cbase = 0; cbase < total_cols; cbase += kThreads) {
+    int cs = 0, len = 0, ct = 0;
+    if (cbase + tid < total_cols) column_at(boff, sh, cbase + tid, cs, len, ct);
+    int excl;
+    nc += block_scan(sh, len, &excl);
+  }
+  if (tid == 0) heads[tl.idx] = make_int4(0, 0, 0, nc);
+}
+
+// Region starts: exclusive scan of the region sizes in tile order (one
+// block, 4 tiles per thread and round).
+constexpr int kTileScanThreads = 1024;
+__global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(int4* __restrict__ heads,
+                                                                int ntiles) {
+  __shared__ int wsum[kTileScanThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int carry = 0;
+  for (int r0 = 0; r0 < ntiles; r0 += 4 * kTileScanThreads) {
+    const int i0 = r0 + 4 * tid;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < ntiles ? heads[i0 + k].w : 0;
+    const int sum = v[0] + v[1] + v[2] + v[3];
+    int x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kTileScanThreads / 64; ++w) {
+      const int s = wsum[w];
+      off += w < wave ? s : 0;
+      tot += s;
+    }
+    int run = carry + off + x - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < ntiles) heads[i0 + k].x = run;
+      run += v[k];
+    }
+    carry += tot;
+    __syncthreads();  // wsum is rewritten by the next round
+  }
+}
+
+// Candidates per thread and phase-B round of k_tile_lists (consecutive ones,
+// their searches and atom loads in flight together)
+constexpr int kListUnroll = 4;
+
+template <int MT>
+__global__ __launch_bounds__(kThreads) void k_tile_lists(SysDev sys, const TypeDev* __restrict__ td,
+                                                         const int* __restrict__ boff,
+                                                         const int4* __restrict__ s_ir,
+                                                         int2* __restrict__ pool, int cap,
+                                                         int4* __restrict__ heads,
+                                                         int* __restrict__ err) {
+  __shared__ EnumShared<MT> sh;
+  __shared__ struct {
+    int slice_base[MT], slice_stride[MT], type_start[MT], count[MT], nsl[MT][3];
+  } tyl;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < sys.ntypes; i += kThreads) {
+    tyl.slice_base[i] = (int)td[i].slice_base;
+    tyl.slice_stride[i] = td[i].slice_stride;
+    tyl.type_start[i] = td[i].type_start;
+    tyl.count[i] = td[i].count;
+    tyl.nsl[i][0] = td[i].nsl[0];
+    tyl.nsl[i][1] = td[i].nsl[1];
+    tyl.nsl[i][2] = td[i].nsl[2];
+  }
+  Tile tl;
+  tile_coords(sys, tl);
+  const int4 hd = heads[tl.idx];
+  const int base = hd.x, nc = hd.w;
+  const int total_cols = enum_setup(sys, td, tl, sh);  // (its barriers order tyl too)
+  if (base > cap - nc) {  // the plan's size bound did not hold (uniform over the block)
+    if (tid == 0) {
+      atomicOr(err, ERRBIT_LIST);
+      heads[tl.idx] = make_int4(0, 0, 0, 0);
+    }
+    return;
+  }
+  int nreg = 0, ncor = 0;
+  for (int cbase = 0; cbase < total_cols; cbase += kThreads) {
+    int my_s = 0, my_len = 0, my_t = 0;
+    if (cbase + tid < total_cols) column_at(boff, sh, cbase + tid, my_s, my_len, my_t);
+    int excl;
+    const int ncand = block_scan(sh, my_len, &excl);
+    sh.col_s[tid] = my_s;
+    sh.col_t[tid] = my_t;
+    sh.col_pre[tid] = excl;
+    if (tid == 0) sh.col_pre[kThreads] = ncand;
+    __syncthreads();
+    for (int g0 = 0; g0 < ncand; g0 += kListUnroll * kThreads) {
+      // candidates gb .. gb + kListUnroll - 1 of this thread; those past the
+      // end read the last one, so that the loads go out unconditionally
+      const int gb = g0 + kListUnroll * tid;
+      int cc[kListUnroll], ct[kListUnroll];
+      int4 q[kListUnroll];
+#pragma unroll
+      for (int k = 0; k < kListUnroll; ++k) {
+        candidate_at(sh, min(gb + k, ncand - 1), cc[k], ct[k]);
+        q[k] = s_ir[cc[k]];
+      }
+      int2 e[kListUnroll];
+      int fl[kListUnroll];  // 1: regular hit, 1 << 16: corner hit
+      int h = 0;
+#pragma unroll
+      for (int k = 0; k < kListUnroll; ++k) {
+        fl[k] = 0;
+        e[k] = make_int2(0, 0);
+        const int t = ct[k];
+        if (gb + k < ncand &&
+            overlap1(q[k].x, sh.half[t][0], tl.t0[0], tl.t1[0], sys.n[0]) &&
+            overlap1(q[k].y, sh.half[t][1], tl.t0[1], tl.t1[1], sys.n[1]) &&
+            overlap1(q[k].z, sh.half[t][2], tl.t0[2], tl.t1[2], sys.n[2])) {
+          const bool corner = (q[k].w & kAtomCorner) != 0;
+          const int slot = cc[k] - tyl.type_start[t];
+          const bool ok = slot >= 0 && slot < tyl.count[t];  // false only after a reported error
+          // this tile's slice per axis: j = (tile - T0) mod ntiles, T0 the
+          // tile of the window start (k_slices' numbering); the start lies
+          // in [-Nb, N + Nb), Nb <= N, so one wrap suffices
+          int packed = t << 24;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const int ir = d == 0 ? q[k].x : (d == 1 ? q[k].y : q[k].z);
+            const int N = sys.n[d];
+            int lo = ir - sh.half[t][d];
+            lo = lo < 0 ? lo + N : (lo >= N ? lo - N : lo);
+            int j = (tl.t0[d] >> 4) - (lo >> 4);
+            if (j < 0) j += sys.ntiles[d];
+            packed |= ((ok && j < tyl.nsl[t][d]) ? j : kListNoSlice) << (8 * d);
+          }
+          e[k] = make_int2(corner ? cc[k] : tyl.slice_base[t] + slot * tyl.slice_stride[t],
+                           packed);
+          fl[k] = corner ? (1 << 16) : 1;
+        }
+        h += fl[k];
+      }
+      int pos;
+      const int nhit = block_scan(sh, h, &pos);
+      int pr = nreg + (pos & 0xffff), pc = ncor + (pos >> 16);
+#pragma unroll
+      for (int k = 0; k < kListUnroll; ++k) {
+        if (fl[k] == 1) pool[base + pr++] = e[k];
+        else if (fl[k]) pool[base + nc - 1 - pc++] = e[k];
+      }
+      nreg += nhit & 0xffff;
+      ncor += nhit >> 16;
+    }
+    __syncthreads();  // the column arrays are rewritten by the next round
+  }
+  if (tid == 0) heads[tl.idx] = make_int4(base, nreg, ncor, nc);
+}
+
+// Tiled deposition from the tile lists (FP32 orthogonal cells): the rank-1
+// MFMA sums of k_deposit_mfma without the enumeration.  Regular atoms come in
+// batches of 64, four lanes per atom, each loading a quarter of the atom's
+// three tile slices; corner atoms in batches of 16, four lanes per virtual
+// atom.  The next batch's slices and the entries of the batch after it are
+// loaded while the current batch's MFMAs run.  Sum order: the regular atoms
+// in list order, then the corner atoms in list order (fixed, so results are
+// bitwise reproducible).
+template <int MT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6)))
+void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __restrict__ pool,
+                     const int4* __restrict__ heads, const float* __restrict__ slices,
+                     const float* __restrict__ spill, const int* __restrict__ spill_flag,
+                     float* __restrict__ rho) {
+  __shared__ StageM st;
+  // per type: slice axis offsets (in slices) of the regular and the corner
+  // layout, corner component stride per axis (floats), layout bases
+  __shared__ struct {
+    int pre[MT][3], cpre[MT][3], cs[MT][3];
+    int slice_base[MT], slice_stride[MT], corner_base[MT], corner_stride[MT], type_start[MT];
+  } ty;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < sys.ntypes; i += kThreads) {
+    const TypeDev& d = td[i];
+    ty.pre[i][0] = 0;
+    ty.pre[i][1] = d.nsl[0];
+    ty.pre[i][2] = d.nsl[0] + d.nsl[1];
+    ty.cpre[i][0] = 0;
+    ty.cpre[i][1] = 2 * d.nsl[0];
+    ty.cpre[i][2] = 2 * d.nsl[0] + 3 * d.nsl[1];
+    ty.cs[i][0] = T * d.nsl[0];
+    ty.cs[i][1] = T * d.nsl[1];
+    ty.cs[i][2] = T * d.nsl[2];
+    ty.slice_base[i] = (int)d.slice_base;
+    ty.slice_stride[i] = d.slice_stride;
+    ty.corner_base[i] = (int)d.corner_base;
+    ty.corner_stride[i] = d.corner_stride;
+    ty.type_start[i] = d.type_start;
+  }
+  if (tid < T) st.vx[NV][tid] = st.vy[NV][tid] = st.vz[NV][tid] = 0.f;
+  Tile tl;
+  tile_coords(sys, tl);
+  const int4 hd = heads[tl.idx];
+  const int base = hd.x, nreg = hd.y, ncor = hd.z, nc = hd.w;
+  __syncthreads();
+
+  // acc: this batch's MFMA sums; tot: the tile's running sum, added batch by
+  // batch with round-to-nearest adds (the MFMA accumulation rounds toward
+  // zero on its long sums)
+  v4f acc[4], tot[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) acc[b] = tot[b] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int q4 = tid & 3;
+  // three quarter slices at float offsets o0..o2 (-1: none, zero); a missing
+  // one loads the buffer's first quarter slice and zeroes it, so the three
+  // loads go out unconditionally
+  auto load3 = [&](int o0, int o1, int o2, float4& v0, float4& v1, float4& v2) {
+    v0 = *reinterpret_cast<const float4*>(slices + max(o0, 0) + 4 * q4);
+    v1 = *reinterpret_cast<const float4*>(slices + max(o1, 0) + 4 * q4);
+    v2 = *reinterpret_cast<const float4*>(slices + max(o2, 0) + 4 * q4);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (o0 < 0) v0 = z4;
+    if (o1 < 0) v1 = z4;
+    if (o2 < 0) v2 = z4;
+  };
+  auto stage = [&](int row, const float4& v0, const float4& v1, const float4& v2) {
+    *reinterpret_cast<float4*>(&st.vx[row][4 * q4]) = v0;
+    *reinterpret_cast<float4*>(&st.vy[row][4 * q4]) = v1;
+    *reinterpret_cast<float4*>(&st.vz[row][4 * q4]) = v2;
+    st.vq[row][q4] = (v0.x != 0.f || v0.y != 0.f || v0.z != 0.f || v0.w != 0.f) ? 1 : 0;
+  };
+  // MFMA accumulation of staged rows [0, vtot): the wave's list holds the
+  // rows with nonzero X on its 4 x rows, padded with the zero row NV
+  auto accumulate = [&](int vtot) {
+    const int kk = lane >> 4, jj = lane & 15;
+    const bool on = lane < vtot && st.vq[lane][wave] != 0;
+    const unsigned long long bm = __ballot(on);
+    const int pos = __popcll(bm & ((1ull << lane) - 1));
+    if (on) st.wl[wave][pos] = (unsigned char)lane;
+    const int cnt = __popcll(bm);
+    const int kend = (cnt + 3) & ~3;
+    if (lane >= cnt && lane < kend) st.wl[wave][lane] = (unsigned char)NV;
+    __builtin_amdgcn_wave_barrier();
+    for (int k0 = 0; k0 < kend; k0 += 4) {
+      const int k = st.wl[wave][k0 + kk];
+      const float zv = st.vz[k][jj], yv = st.vy[k][jj];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const float av = st.vx[k][4 * wave + b] * yv;
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, zv, acc[b], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      tot[b] += acc[b];
+      acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+
+  // regular atoms: lanes 4a .. 4a + 3 take atom a of the batch
+  {
+    const int a = tid >> 2;
+    auto offsets = [&](const int2& e, bool valid, int& o0, int& o1, int& o2) {
+      const int t = (int)((unsigned)e.y >> 24);
+      const int j0 = e.y & 255, j1 = (e.y >> 8) & 255, j2 = (e.y >> 16) & 255;
+      o0 = valid && j0 != kListNoSlice ? e.x + T * (ty.pre[t][0] + j0) : -1;
+      o1 = valid && j1 != kListNoSlice ? e.x + T * (ty.pre[t][1] + j1) : -1;
+      o2 = valid && j2 != kListNoSlice ? e.x + T * (ty.pre[t][2] + j2) : -1;
+    };
+    float4 v0, v1, v2;
+    int2 en = make_int2(0, 0);
+    {
+      int o0, o1, o2;
+      const int2 e = a < nreg ? pool[base + a] : make_int2(0, 0);
+      offsets(e, a < nreg, o0, o1, o2);
+      load3(o0, o1, o2, v0, v1, v2);
+      if (NV + a < nreg) en = pool[base + NV + a];
+    }
+    for (int b0 = 0; b0 < nreg; b0 += NV) {
+      __syncthreads();  // the previous batch's MFMAs have read the staging rows
+      stage(a, v0, v1, v2);  // rows past the batch's atoms get zeros
+      __syncthreads();
+      if (b0 + NV < nreg) {
+        int o0, o1, o2;
+        offsets(en, b0 + NV + a < nreg, o0, o1, o2);
+        load3(o0, o1, o2, v0, v1, v2);
+        if (b0 + 2 * NV + a < nreg) en = pool[base + b0 + 2 * NV + a];
+      }
+      accumulate(min(NV, nreg - b0));
+    }
+  }
+  // corner atoms: 4 virtual atoms each, (Xc, Yall, Zs), (Xn, Yc, Zs),
+  // (Xn, Yl, ZL), (Xn, Yh, ZH), from the regular slices and the corner
+  // components (x {C, L+H}, y {C, L, H}, z {ZL, ZH}); lanes 16 ca .. 16 ca +
+  // 15 take corner atom ca, four lanes per virtual atom v
+  for (int c0 = 0; c0 < ncor; c0 += NV / 4) {
+    const int nca = min(NV / 4, ncor - c0);
+    const int ca = tid >> 4, v = (tid >> 2) & 3;
+    int o0 = -1, o1 = -1, o2 = -1;
+    if (ca < nca) {
+      const int2 e = pool[base + nc - 1 - (c0 + ca)];
+      const int t = (int)((unsigned)e.y >> 24), slot = e.x - ty.type_start[t];
+      const int ab = ty.slice_base[t] + slot * ty.slice_stride[t];
+      const int cb = ty.corner_base[t] + slot * ty.corner_stride[t];
+      const int j0 = e.y & 255, j1 = (e.y >> 8) & 255, j2 = (e.y >> 16) & 255;
+      const int so1 = j1 != kListNoSlice ? ab + T * (ty.pre[t][1] + j1) : -1;
+      const int so2 = j2 != kListNoSlice ? ab + T * (ty.pre[t][2] + j2) : -1;
+      const int cb0 = j0 != kListNoSlice ? cb + T * (ty.cpre[t][0] + j0) : -1;
+      const int cb1 = j1 != kListNoSlice ? cb + T * (ty.cpre[t][1] + j1) : -1;
+      const int cb2 = j2 != kListNoSlice ? cb + T * (ty.cpre[t][2] + j2) : -1;
+      o0 = cb0 >= 0 ? cb0 + (v == 0 ? 0 : ty.cs[t][0]) : -1;
+      o1 = v == 0 ? so1 : (cb1 >= 0 ? cb1 + (v - 1) * ty.cs[t][1] : -1);
+      o2 = v < 2 ? so2 : (cb2 >= 0 ? cb2 + (v - 2) * ty.cs[t][2] : -1);
+    }
+    float4 w0, w1, w2;
+    load3(o0, o1, o2, w0, w1, w2);
+    __syncthreads();
+    stage(tid >> 2, w0, w1, w2);  // row 4 ca + v
+    __syncthreads();
+    accumulate(4 * nca);
+  }
+
+  // D[y][z] of block b: lane l holds y = 4 (l / 16) + r, z = l % 16
+  const int gz = tl.t0[2] + (lane & 15);
+  const float* sp = spill_stage_tile(sys, spill, spill_flag, tl);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int gx = tl.t0[0] + 4 * wave + b;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gy = tl.t0[1] + 4 * (lane >> 4) + r;
+      if (gx < tl.t1[0] && gy < tl.t1[1] && gz < tl.t1[2]) {
+        float v = tot[b][r];
+        if (sp) v += sp[((gy - tl.t0[1]) * T + (gx - tl.t0[0])) * T + (gz - tl.t0[2])];
+        rho[((size_t)gx * sys.n[1] + gy) * sys.n[2] + gz] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint of the MFMA deposition (FP32 orthogonal cells).  With G = dL/drho
 // on the folded grid and a virtual atom's tile slices X, Y, Z and their
 // derivatives in the atom position dX, dY, dZ, the atom's gradient from one
@@ -1780,7 +2184,13 @@ hipError_t launch_impl(Plan& p, ACC* out, const ACC* stage, hipStream_t s) {
   } else if constexpr (std::is_same<ACC, float>::value) {
     // blocks size their per-type LDS for 16 types when the system has no more
     const bool few = sys.ntypes <= 16;
-    if (sys.mode == MODE_DIAG && few)
+    if (sys.mode == MODE_DIAG && p.tl_cap > 0 && few)
+      DEP_LAUNCH((k_deposit_lists<16>), sys, p.types, p.tl_pool, p.tl_head, p.slices, stage,
+                 p.spill_flag, out);
+    else if (sys.mode == MODE_DIAG && p.tl_cap > 0)
+      DEP_LAUNCH((k_deposit_lists<kMaxTypes>), sys, p.types, p.tl_pool, p.tl_head, p.slices,
+                 stage, p.spill_flag, out);
+    else if (sys.mode == MODE_DIAG && few)
       DEP_LAUNCH((k_deposit_mfma<MODE_DIAG, 16>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);
     else if (sys.mode == MODE_DIAG)
@@ -1839,6 +2249,26 @@ hipError_t launch_spill_stage(Plan& p, hipStream_t s) {
   if (!p.spill) return hipSuccess;
   if (p.fft64) return launch_spill<double>(p, nullptr, static_cast<double*>(p.spill), s);
   return launch_spill<float>(p, nullptr, static_cast<float*>(p.spill), s);
+}
+
+hipError_t launch_tile_lists(Plan& p, hipStream_t s) {
+  const SysDev& sys = p.sys;
+  const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
+  if (p.tl_cap == 0 || ntiles == 0 || sys.natoms == 0) return hipSuccess;
+  const dim3 grid((unsigned)ntiles), block(kThreads);
+  if (sys.ntypes <= 16)
+    hipLaunchKernelGGL(k_tile_count<16>, grid, block, 0, s, sys, p.types, p.bucket_off, p.tl_head);
+  else
+    hipLaunchKernelGGL(k_tile_count<kMaxTypes>, grid, block, 0, s, sys, p.types, p.bucket_off,
+                       p.tl_head);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s, p.tl_head, (int)ntiles);
+  if (sys.ntypes <= 16)
+    hipLaunchKernelGGL(k_tile_lists<16>, grid, block, 0, s, sys, p.types, p.bucket_off, p.s_ir,
+                       p.tl_pool, (int)p.tl_cap, p.tl_head, p.err);
+  else
+    hipLaunchKernelGGL(k_tile_lists<kMaxTypes>, grid, block, 0, s, sys, p.types, p.bucket_off,
+                       p.s_ir, p.tl_pool, (int)p.tl_cap, p.tl_head, p.err);
+  return hipGetLastError();
 }
 
 hipError_t launch_adjoint_tiles(Plan& p, const float* grho, hipStream_t s) {

@@ -18,6 +18,7 @@ constexpr int kMaxStages = 16;  // FFT stages per axis
 constexpr int kTuneReg = 100;   // FFT pencil choice: register-staged kernel
 constexpr int kMaxSpillBufTiles = 4096;  // spill tiles staged per frame slot (else added in place)
 constexpr int kCornerCap = 512;     // corner atoms listed per frame for the spill pass
+constexpr int kListNoSlice = 255;  // tile-list entry: no slice on that axis (a byte per axis)
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
 // Cell metric G = U U^T of the plan: diagonal, xy-coupled (g01 != 0), or
@@ -45,7 +46,7 @@ __host__ __device__ inline void numpy_slice(long long i0, long long i1, long lon
 }
 
 // Error bits raised by kernels into the plan's device flag word.
-enum ErrBit { ERRBIT_RANGE = 1, ERRBIT_TYPE = 2 };
+enum ErrBit { ERRBIT_RANGE = 1, ERRBIT_TYPE = 2, ERRBIT_LIST = 4 };
 
 // Per-type constants on the device.
 struct TypeDev {
@@ -136,6 +137,10 @@ struct FrameState {
   // and per spill tile whether any atom spills into it
   void* spill = nullptr;
   int* spill_flag = nullptr;
+  // tile lists (FP32 orthogonal-cell plans): entry pool and per-tile heads
+  // {region start, regular atoms, corner atoms, region size}
+  int2* tl_pool = nullptr;
+  int4* tl_head = nullptr;
 };
 
 // Frame slots of a plan (atom state + density / spectrum buffers): frame t
@@ -170,6 +175,10 @@ struct Plan {
   int* spill_flag = nullptr;      // their occupancy flags
   long long spill_tiles = 0;      // sp_cnt[0] sp_cnt[1] sp_cnt[2]
   long long slice_floats = 0;     // regular slices, then the corner components
+  // tile lists of the current slot (null when the plan does not use them)
+  int2* tl_pool = nullptr;
+  int4* tl_head = nullptr;
+  long long tl_cap = 0;           // pool entries per slot (0: no tile lists)
   int* err = nullptr;
   // grids: one set of density / spectrum buffers per frame slot, so that a
   // frame's FFT can run while the next frames are deposited; rho / spec / res /
@@ -209,6 +218,8 @@ struct Plan {
     corners = f.corners;
     spill = f.spill;
     spill_flag = f.spill_flag;
+    tl_pool = f.tl_pool;
+    tl_head = f.tl_head;
     rho = rho_buf[b];
     spec = spec_buf[b];
     res = res_buf[b];
@@ -228,6 +239,9 @@ struct Plan {
   // t is deposited on dep_stream and frame t - 1 transformed on fft_stream
   // (both shared by all plans on the device, never destroyed)
   hipStream_t dep_stream = nullptr, fft_stream = nullptr;
+  // the prepare stage's tile-list branch (frame loop; shared like the two above)
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_lfork = nullptr, ev_ljoin = nullptr;
   // optional stage timing of mdsf_accumulate_frames: 10 timing events per
   // frame (prepare, deposit, z, y, x: start / end, each on its stream)
   int timing = 0;  // 0 off, 1 every stage, 2 the deposition only
@@ -263,8 +277,10 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
                        int64_t wrap_hi, uint32_t* occ, int32_t* home, hipStream_t s,
                        bool coords_only = false, bool write_back = true);
 // sorts the prepared atoms and builds their tile slices; with `derivs` (and
-// dslices allocated) also the slices' derivatives for the tiled adjoint
-hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false);
+// dslices allocated) also the slices' derivatives for the tiled adjoint.
+// With `aux`, the tile lists are built on that stream, in parallel with the
+// rest, and joined back into `s`.
+hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false, hipStream_t aux = nullptr);
 // only the slice derivatives of the last sorted frame
 hipError_t launch_slice_derivs(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);
@@ -274,6 +290,8 @@ hipError_t launch_parked(Plan& p, void* out, hipStream_t s);
 // corner-fold spill of the sorted frame into the slot's staging tiles (when the
 // plan stages them; launch_deposit adds them as it stores the tiles)
 hipError_t launch_spill_stage(Plan& p, hipStream_t s);
+// per-tile atom lists of the sorted frame (plans with tile lists)
+hipError_t launch_tile_lists(Plan& p, hipStream_t s);
 // tiled adjoint: adj_tpart slots <- per (atom, tile) gradient sums from
 // dL/drho (FP32 DIAG plans; slots of tiles an atom does not reach are left
 // untouched)

@@ -455,7 +455,7 @@ hipError_t launch_slice_derivs(Plan& p, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
+hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux) {
   const int n = p.sys.natoms;
   const int nbk = p.sys.nbuckets;
   const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
@@ -470,12 +470,22 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
     hipLaunchKernelGGL(k_rank_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.key,
                        p.bucket_off, p.s_tmp, p.s_idx, p.ir4, p.rpos, p.s_ir, p.s_r,
                        p.bucket_off + nbk, p.corners);
+    // the tile lists: on `aux` (when given) while the corner list, the spill
+    // staging and the slices go on here
+    const bool split = aux && p.tl_cap > 0;
+    hipError_t e = hipSuccess;
+    if (split) {
+      if ((e = hipEventRecord(p.ev_lfork, s)) != hipSuccess ||
+          (e = hipStreamWaitEvent(aux, p.ev_lfork, 0)) != hipSuccess)
+        return e;
+    }
+    if ((e = launch_tile_lists(p, split ? aux : s)) != hipSuccess) return e;
+    if (split && (e = hipEventRecord(p.ev_ljoin, aux)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_corner_order, dim3(1), dim3(kOrderThreads), 0, s, p.sys, p.types,
                        p.s_ir, p.corners, p.spill ? p.spill_flag : nullptr,
                        (int)(p.spill ? p.spill_tiles : 0));
     // the corner-fold spill, staged for the deposition of this frame
-    const hipError_t e = launch_spill_stage(p, s);
-    if (e != hipSuccess) return e;
+    if ((e = launch_spill_stage(p, s)) != hipSuccess) return e;
     if (p.slice_floats > 0)
       hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0,
                          s, p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices,
@@ -491,6 +501,7 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs) {
         hipLaunchKernelGGL(k_tables<float>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
                            static_cast<float*>(p.tables), total);
     }
+    if (split && (e = hipStreamWaitEvent(s, p.ev_ljoin, 0)) != hipSuccess) return e;
   }
   return hipGetLastError();
 }
