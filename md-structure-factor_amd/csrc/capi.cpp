@@ -569,9 +569,15 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
     for (int pass = 0; pass < 3; ++pass) {
+      // with stage timing, the pass's stream kernel records the pass events
+      // with its own dispatch (the marks stand when another kernel runs it)
       MDSF_HIP(mark(t, 4 + 2 * pass, f));
-      MDSF_HIP(fft_pass(*p, pass, p->rho, f));
-      MDSF_HIP(mark(t, 5 + 2 * pass, f));
+      if (p->timing == 1) fft_pass_events(p->tev[10 * t + 4 + 2 * pass], p->tev[10 * t + 5 + 2 * pass]);
+      const hipError_t fe = fft_pass(*p, pass, p->rho, f);
+      const bool took = fft_pass_events_used();
+      fft_pass_events(nullptr, nullptr);
+      MDSF_HIP(fe);
+      if (!took) MDSF_HIP(mark(t, 5 + 2 * pass, f));
     }
     MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
   }

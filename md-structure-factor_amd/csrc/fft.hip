@@ -34,6 +34,8 @@
 #include <map>
 #include <mutex>
 
+#include <hip/hip_ext.h>
+
 #include "mdsf_internal.h"
 
 namespace mdsf {
@@ -1117,6 +1119,22 @@ inline int lds_pencils(int n) {
 }
 inline size_t lds_bytes(int n, int P) { return (size_t)P * lds_pitch(n) * 4 * sizeof(double); }
 
+// Events the next stream-kernel launch of a pass records with its own
+// dispatch (its begin and end: the frame loop's stage timing), or null.
+thread_local hipEvent_t t_pass_ev[2] = {nullptr, nullptr};
+thread_local bool t_pass_ev_used = false;
+
+template <typename K, typename A>
+void pass_launch(K kernel, dim3 grid, size_t lds, hipStream_t s, const A& a) {
+  if (t_pass_ev[0]) {
+    hipExtLaunchKernelGGL(kernel, grid, dim3(kFftThreads), lds, s, t_pass_ev[0], t_pass_ev[1], 0,
+                          a);
+    t_pass_ev_used = true;
+  } else {
+    hipLaunchKernelGGL(kernel, grid, dim3(kFftThreads), lds, s, a);
+  }
+}
+
 // Radix split of the register-staged passes for n, or 0.
 inline int reg_split(int n, int* r1, int* r2, int* r3) {
   switch (n) {
@@ -1139,8 +1157,8 @@ void launch_stream(size_t lds, hipStream_t s, PassArgs<R> a) {
   a.tpb = stream_tpb(k, lds, a.g.blocks);
   const long long grid = (a.g.blocks + a.tpb - 1) / a.tpb + (a.nyq ? a.nq_pencils : 0);
   if (grid == 0) return;
-  hipLaunchKernelGGL((k_pass_stream<R, R1, R2, R3, INV, IN_E, OUT_E, EPI, RES, WGT>),
-                     dim3((unsigned)grid), dim3(kFftThreads), lds, s, a);
+  pass_launch(k_pass_stream<R, R1, R2, R3, INV, IN_E, OUT_E, EPI, RES, WGT>, dim3((unsigned)grid),
+              lds, s, a);
 }
 
 template <typename R, bool INV, bool IN_E, bool OUT_E, int EPI, int RES, bool WGT>
@@ -1259,8 +1277,7 @@ void launch_z_stream(long long ntiles, hipStream_t s, ZArgs<R> a) {
   a.tpb = stream_tpb(reinterpret_cast<const void*>(&k_r2c_z_stream<R, R1, R2, R3>), 0, ntiles);
   const long long grid = (ntiles + a.tpb - 1) / a.tpb;
   if (grid == 0) return;
-  hipLaunchKernelGGL((k_r2c_z_stream<R, R1, R2, R3>), dim3((unsigned)grid), dim3(kFftThreads), 0,
-                     s, a);
+  pass_launch(k_r2c_z_stream<R, R1, R2, R3>, dim3((unsigned)grid), 0, s, a);
 }
 
 template <typename R>
@@ -1616,6 +1633,14 @@ hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, void*
   return p.fft64 ? inverse_impl<double>(p, w_main, w_nyq, out, s)
                  : inverse_impl<float>(p, w_main, w_nyq, out, s);
 }
+
+void fft_pass_events(hipEvent_t begin, hipEvent_t end) {
+  t_pass_ev[0] = begin;
+  t_pass_ev[1] = end;
+  t_pass_ev_used = false;
+}
+
+bool fft_pass_events_used() { return t_pass_ev_used; }
 
 hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s) {
   switch (pass) {

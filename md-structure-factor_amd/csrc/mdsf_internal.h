@@ -307,6 +307,10 @@ hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, void*
                        hipStream_t s);
 // one forward pass: 0 = z (R2C from `density`), 1 = y, 2 = x with |F|^2 accumulation
 hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s);
+// events (or null) that the next pass's stream kernel records with its own
+// dispatch on this thread, and whether a launch took them
+void fft_pass_events(hipEvent_t begin, hipEvent_t end);
+bool fft_pass_events_used();
 // stored spectrum -> complex128 (Nx, Ny, Nz/2+1)
 hipError_t fft_spectrum_read(Plan& p, double* out, hipStream_t s);
 bool fft_factor(int n, int* radix, int* nstages);
