@@ -9,8 +9,8 @@ the reference's host-side set-up arithmetic and the ``.npz`` write.
 
 Precision.  "fp32" plans store the density and the spectrum between FFT
 passes in FP32, evaluate every butterfly, twiddle and the real-to-complex
-split in FP64, keep 4-bit residuals of the z and y passes' FP32 roundings for
-the next pass, and sum |F|^2 over frames in FP64.  "fp64" plans store FP64
+split in FP64, keep 4-bit residuals of the y pass's FP32 roundings for the x
+pass, and sum |F|^2 over frames in FP64.  "fp64" plans store FP64
 throughout (S(q) within about 1e-13 of FP64 numpy).  The default, "auto",
 runs frame 0 both ways and keeps FP32 when its S(q) agrees with FP64 within
 AUTO_FP32_MAX_ERR (5e-6) on every voxel above 1e-6 of the maximum, which
