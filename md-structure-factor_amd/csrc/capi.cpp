@@ -78,7 +78,7 @@ hipError_t pipeline_streams(int device, hipStream_t* dep, hipStream_t* fft, hipS
 
 void free_plan(Plan* p) {
   if (!p) return;
-  for (hipEvent_t e : {p->ev_fork, p->ev_lfork, p->ev_ljoin})
+  for (hipEvent_t e : {p->ev_fork, p->ev_lfork, p->ev_lgath, p->ev_ljoin})
     if (e) (void)hipEventDestroy(e);
   for (int b = 0; b < kSlots; ++b)
     for (hipEvent_t e : {p->ev_prep[b], p->ev_dep[b], p->ev_fft[b]})
@@ -433,7 +433,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
     return rc;
   }
   e = pipeline_streams(p->device, &p->dep_stream, &p->fft_stream, &p->aux_stream);
-  for (hipEvent_t* ev : {&p->ev_fork, &p->ev_lfork, &p->ev_ljoin})
+  for (hipEvent_t* ev : {&p->ev_fork, &p->ev_lfork, &p->ev_lgath, &p->ev_ljoin})
     if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
   for (int b = 0; b < kSlots && e == hipSuccess; ++b)
     for (hipEvent_t* ev : {&p->ev_prep[b], &p->ev_dep[b], &p->ev_fft[b]})
@@ -507,9 +507,11 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   const size_t esz = dtype == MDSF_F32 ? 4 : 8;
   const size_t fstride = (size_t)p->sys.natoms * 3 * esz;
   // Three-stage pipeline over the kSlots frame slots (atom state + density
-  // buffer): frame t is prepared on `s`, deposited on dep_stream and
-  // transformed on fft_stream, so that while frame t is deposited frame t + 1
-  // is prepared and earlier frames transformed.  A slot is reused kSlots
+  // buffer): frame t is prepared on `s` (its tile lists on aux_stream, its
+  // corner list and spill staging on dep_stream ahead of its deposition),
+  // deposited on dep_stream and transformed on fft_stream, so that while
+  // frame t is deposited frame t + 1 is prepared and earlier frames
+  // transformed.  A slot is reused kSlots
   // frames later: preparing frame t waits until frame t - kSlots's deposition
   // (tiled and parked passes) has read its atom state, depositing it until
   // frame t - kSlots's FFT has released its density buffer.  With three
@@ -518,6 +520,10 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   // shares the GPU with the deposition and finishes later than it).  The S(q)
   // sum stays in frame order on one stream.
   hipStream_t d = p->dep_stream, f = p->fft_stream;
+  // the prepare stage's branches go on their own streams on large grids; on
+  // small ones the cross-stream waits cost more than the branches overlap
+  const long long ntiles = (long long)p->sys.ntiles[0] * p->sys.ntiles[1] * p->sys.ntiles[2];
+  const bool split = ntiles >= kSplitPrepTiles;
   if (p->timing) {
     while ((int64_t)p->tev.size() < 10 * nframes) {
       hipEvent_t e;
@@ -548,7 +554,7 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(mark(t, 0, s));
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
-    MDSF_HIP(launch_binning(*p, s, false, p->aux_stream));
+    MDSF_HIP(launch_binning(*p, s, false, split ? p->aux_stream : nullptr, split ? d : nullptr));
     MDSF_HIP(mark(t, 1, s));
     MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));

@@ -2251,7 +2251,7 @@ hipError_t launch_spill_stage(Plan& p, hipStream_t s) {
   return launch_spill<float>(p, nullptr, static_cast<float*>(p.spill), s);
 }
 
-hipError_t launch_tile_lists(Plan& p, hipStream_t s) {
+hipError_t launch_tile_regions(Plan& p, hipStream_t s) {
   const SysDev& sys = p.sys;
   const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
   if (p.tl_cap == 0 || ntiles == 0 || sys.natoms == 0) return hipSuccess;
@@ -2262,6 +2262,14 @@ hipError_t launch_tile_lists(Plan& p, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_count<kMaxTypes>, grid, block, 0, s, sys, p.types, p.bucket_off,
                        p.tl_head);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s, p.tl_head, (int)ntiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_lists(Plan& p, hipStream_t s) {
+  const SysDev& sys = p.sys;
+  const long long ntiles = (long long)sys.ntiles[0] * sys.ntiles[1] * sys.ntiles[2];
+  if (p.tl_cap == 0 || ntiles == 0 || sys.natoms == 0) return hipSuccess;
+  const dim3 grid((unsigned)ntiles), block(kThreads);
   if (sys.ntypes <= 16)
     hipLaunchKernelGGL(k_tile_lists<16>, grid, block, 0, s, sys, p.types, p.bucket_off, p.s_ir,
                        p.tl_pool, (int)p.tl_cap, p.tl_head, p.err);

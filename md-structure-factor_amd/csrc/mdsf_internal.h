@@ -19,6 +19,7 @@ constexpr int kTuneReg = 100;   // FFT pencil choice: register-staged kernel
 constexpr int kMaxSpillBufTiles = 4096;  // spill tiles staged per frame slot (else added in place)
 constexpr int kCornerCap = 512;     // corner atoms listed per frame for the spill pass
 constexpr int kListNoSlice = 255;  // tile-list entry: no slice on that axis (a byte per axis)
+constexpr long long kSplitPrepTiles = 2048;  // tiles from which the frame loop splits its prepare stage
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
 // Cell metric G = U U^T of the plan: diagonal, xy-coupled (g01 != 0), or
@@ -241,7 +242,7 @@ struct Plan {
   hipStream_t dep_stream = nullptr, fft_stream = nullptr;
   // the prepare stage's tile-list branch (frame loop; shared like the two above)
   hipStream_t aux_stream = nullptr;
-  hipEvent_t ev_lfork = nullptr, ev_ljoin = nullptr;
+  hipEvent_t ev_lfork = nullptr, ev_lgath = nullptr, ev_ljoin = nullptr;
   // optional stage timing of mdsf_accumulate_frames: 10 timing events per
   // frame (prepare, deposit, z, y, x: start / end, each on its stream)
   int timing = 0;  // 0 off, 1 every stage, 2 the deposition only
@@ -279,8 +280,11 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
 // sorts the prepared atoms and builds their tile slices; with `derivs` (and
 // dslices allocated) also the slices' derivatives for the tiled adjoint.
 // With `aux`, the tile lists are built on that stream, in parallel with the
-// rest, and joined back into `s`.
-hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false, hipStream_t aux = nullptr);
+// rest, and joined back into `s`; with `side`, the corner list and the spill
+// staging go on that stream (the one the frame's deposition then runs on),
+// after the sorted atoms are ready.
+hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false, hipStream_t aux = nullptr,
+                          hipStream_t side = nullptr);
 // only the slice derivatives of the last sorted frame
 hipError_t launch_slice_derivs(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);
@@ -290,7 +294,9 @@ hipError_t launch_parked(Plan& p, void* out, hipStream_t s);
 // corner-fold spill of the sorted frame into the slot's staging tiles (when the
 // plan stages them; launch_deposit adds them as it stores the tiles)
 hipError_t launch_spill_stage(Plan& p, hipStream_t s);
-// per-tile atom lists of the sorted frame (plans with tile lists)
+// per-tile atom lists (plans with tile lists): the tiles' pool regions (after
+// the bucket scan), then their entries (after the atoms are sorted)
+hipError_t launch_tile_regions(Plan& p, hipStream_t s);
 hipError_t launch_tile_lists(Plan& p, hipStream_t s);
 // tiled adjoint: adj_tpart slots <- per (atom, tile) gradient sums from
 // dL/drho (FP32 DIAG plans; slots of tiles an atom does not reach are left

@@ -455,7 +455,8 @@ hipError_t launch_slice_derivs(Plan& p, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux) {
+hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux,
+                          hipStream_t side) {
   const int n = p.sys.natoms;
   const int nbk = p.sys.nbuckets;
   const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
@@ -464,45 +465,59 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux) 
                      p.bucket_off, p.bucket_aux, nbk);
   hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off,
                      p.bucket_cur, p.bucket_aux, nbk, nblk, p.corners);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
-                       p.s_tmp, n);
-    hipLaunchKernelGGL(k_rank_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.key,
-                       p.bucket_off, p.s_tmp, p.s_idx, p.ir4, p.rpos, p.s_ir, p.s_r,
-                       p.bucket_off + nbk, p.corners);
-    // the tile lists: on `aux` (when given) while the corner list, the spill
-    // staging and the slices go on here
-    const bool split = aux && p.tl_cap > 0;
-    hipError_t e = hipSuccess;
-    if (split) {
-      if ((e = hipEventRecord(p.ev_lfork, s)) != hipSuccess ||
-          (e = hipStreamWaitEvent(aux, p.ev_lfork, 0)) != hipSuccess)
-        return e;
-    }
-    if ((e = launch_tile_lists(p, split ? aux : s)) != hipSuccess) return e;
-    if (split && (e = hipEventRecord(p.ev_ljoin, aux)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_corner_order, dim3(1), dim3(kOrderThreads), 0, s, p.sys, p.types,
-                       p.s_ir, p.corners, p.spill ? p.spill_flag : nullptr,
-                       (int)(p.spill ? p.spill_tiles : 0));
-    // the corner-fold spill, staged for the deposition of this frame
-    if ((e = launch_spill_stage(p, s)) != hipSuccess) return e;
-    if (p.slice_floats > 0)
-      hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0,
-                         s, p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices,
-                         p.dslices, (derivs && p.dslices) ? 1 : 0);
-    // factor tables feed the FP64 gather
-    const long long total = p.fft64 ? p.table_floats : 0;
-    if (total > 0) {
-      const dim3 grid((unsigned)((total + 255) / 256)), block(256);
-      if (p.fft64)
-        hipLaunchKernelGGL(k_tables<double>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
-                           static_cast<double*>(p.tables), total);
-      else
-        hipLaunchKernelGGL(k_tables<float>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
-                           static_cast<float*>(p.tables), total);
-    }
-    if (split && (e = hipStreamWaitEvent(s, p.ev_ljoin, 0)) != hipSuccess) return e;
+  if (n == 0) return hipGetLastError();
+  // Branches after the bucket scan (when the streams are given): the tile
+  // lists on `aux` (their regions need only the bucket offsets, their
+  // entries the sorted atoms), the corner list and the spill staging on
+  // `side` (the stream whose deposition reads them), the slices here.
+  const bool split = aux && p.tl_cap > 0;
+  hipError_t e = hipSuccess;
+  if (split) {
+    if ((e = hipEventRecord(p.ev_lfork, s)) != hipSuccess ||
+        (e = hipStreamWaitEvent(aux, p.ev_lfork, 0)) != hipSuccess ||
+        (e = launch_tile_regions(p, aux)) != hipSuccess)
+      return e;
   }
+  hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
+                     p.s_tmp, n);
+  hipLaunchKernelGGL(k_rank_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.key,
+                     p.bucket_off, p.s_tmp, p.s_idx, p.ir4, p.rpos, p.s_ir, p.s_r,
+                     p.bucket_off + nbk, p.corners);
+  if (split || side) {
+    if ((e = hipEventRecord(p.ev_lgath, s)) != hipSuccess) return e;
+  }
+  if (split) {
+    if ((e = hipStreamWaitEvent(aux, p.ev_lgath, 0)) != hipSuccess ||
+        (e = launch_tile_lists(p, aux)) != hipSuccess ||
+        (e = hipEventRecord(p.ev_ljoin, aux)) != hipSuccess)
+      return e;
+  } else {
+    if ((e = launch_tile_regions(p, s)) != hipSuccess || (e = launch_tile_lists(p, s)) != hipSuccess)
+      return e;
+  }
+  const hipStream_t cs = side ? side : s;
+  if (side && (e = hipStreamWaitEvent(side, p.ev_lgath, 0)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_corner_order, dim3(1), dim3(kOrderThreads), 0, cs, p.sys, p.types, p.s_ir,
+                     p.corners, p.spill ? p.spill_flag : nullptr,
+                     (int)(p.spill ? p.spill_tiles : 0));
+  // the corner-fold spill, staged for the deposition of this frame
+  if ((e = launch_spill_stage(p, cs)) != hipSuccess) return e;
+  if (p.slice_floats > 0)
+    hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0, s,
+                       p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices, p.dslices,
+                       (derivs && p.dslices) ? 1 : 0);
+  // factor tables feed the FP64 gather
+  const long long total = p.fft64 ? p.table_floats : 0;
+  if (total > 0) {
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    if (p.fft64)
+      hipLaunchKernelGGL(k_tables<double>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
+                         static_cast<double*>(p.tables), total);
+    else
+      hipLaunchKernelGGL(k_tables<float>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
+                         static_cast<float*>(p.tables), total);
+  }
+  if (split && (e = hipStreamWaitEvent(s, p.ev_ljoin, 0)) != hipSuccess) return e;
   return hipGetLastError();
 }
 
