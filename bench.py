@@ -127,28 +127,25 @@ def copy_calibration(nbytes=256 << 20):
 
 def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream):
     """Each stage alone: one-frame calls of the library's frame loop with stage
-    timing on, so that every stage of the frame runs by itself (the prepare
-    stage, then the deposition, then the FFT passes, each waiting for the one
-    before).  Prepare: events on its stream around the whole stage.  The
-    deposition's tiled kernel and each FFT pass's kernel record their events
-    with their own dispatch (begin and end of the kernel: the durations a
-    rocprofv3 kernel trace reports for them)."""
+    timing on, queued back to back (each call joins back into the stream the
+    next one starts from, so every stage of a frame runs by itself: the
+    prepare stage, then the deposition, then the FFT passes, and the next
+    frame's prepare after them).  Prepare: events on its stream around the
+    whole stage.  The deposition's tiled kernel and each FFT pass's kernel
+    record their events with their own dispatch (begin and end of the kernel:
+    the durations a rocprofv3 kernel trace reports for them)."""
     import torch
-    names = ["prepare", "deposit_kernel", "fft_z", "fft_y", "fft_x_acc"]
-    acc = {k: [] for k in names}
     plan.set_stage_timing(1)
+    torch.cuda.synchronize()
     for t in range(nframes):
         plan.accumulate_frames(d_coords[t:t + 1], d_types, d_scale[t:t + 1], box, s.arith,
                                s.wrap)
-        st = plan.stage_times()
-        acc["prepare"].append(st["prepare"])
-        acc["deposit_kernel"].append(st["deposit"])
-        for k in ("fft_z", "fft_y", "fft_x_acc"):
-            acc[k].append(st[k])
+    st = plan.stage_times()
     plan.set_stage_timing(0)
     torch.cuda.synchronize()
     plan.check_errors()
-    return {k: float(np.mean(v)) for k, v in acc.items()}
+    return {"prepare": st["prepare"], "deposit_kernel": st["deposit"], "fft_z": st["fft_z"],
+            "fft_y": st["fft_y"], "fft_x_acc": st["fft_x_acc"]}
 
 
 def adjoint_leg(rad, iters):

@@ -180,13 +180,14 @@ int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_str
 int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which);
 
 /* Stage timing of mdsf_accumulate_frames (off by default): with enable = 1
- * the frame loop records HIP events around each frame's preparation,
- * deposition and three FFT passes on the stream each runs on; with enable = 2
- * only around the deposition (the least perturbation of the overlapped loop).
- * mdsf_stage_times then waits for the last call's events and returns the
- * average milliseconds per frame of {prepare, deposit, z pass, y pass, x pass
- * + |F|^2 sum} in avg_ms[5] (0 for the stages not timed).  No counterpart in
- * the reference. */
+ * the frame loop records HIP events for each frame's preparation (around the
+ * stage, on its stream), deposition and three FFT passes (with the tiled
+ * deposition kernel's and each pass kernel's own dispatch); with enable = 2
+ * only for the deposition (the least perturbation of the overlapped loop).
+ * Timed frames accumulate over calls; mdsf_stage_times waits for them and
+ * returns the average milliseconds per frame of {prepare, deposit, z pass,
+ * y pass, x pass + |F|^2 sum} in avg_ms[5] (0 for the stages not timed),
+ * then starts the count over.  No counterpart in the reference. */
 int mdsf_set_stage_timing(mdsf_plan* plan, int32_t enable);
 int mdsf_stage_times(mdsf_plan* plan, double* avg_ms);
 

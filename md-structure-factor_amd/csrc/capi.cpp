@@ -524,19 +524,24 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   // small ones the cross-stream waits cost more than the branches overlap
   const long long ntiles = (long long)p->sys.ntiles[0] * p->sys.ntiles[1] * p->sys.ntiles[2];
   const bool split = ntiles >= kSplitPrepTiles;
+  // timed frames accumulate over calls until mdsf_stage_times reads them
+  // (frame t of this call is timed frame tb + t), so that one-frame calls
+  // can be queued back to back
+  int64_t tb = 0;
   if (p->timing) {
-    while ((int64_t)p->tev.size() < 10 * nframes) {
+    tb = p->timed_frames < kMaxTimedFrames ? p->timed_frames : 0;
+    while ((int64_t)p->tev.size() < 10 * (tb + nframes)) {
       hipEvent_t e;
       MDSF_HIP(hipEventCreate(&e));
       p->tev.push_back(e);
     }
-    p->timed_frames = nframes;
+    p->timed_frames = tb + nframes;
   }
   // timing mode 1: every stage; mode 2: only the deposition (two events per
   // frame on its stream, the least perturbation of the overlapped loop)
   auto mark = [&](int64_t t, int k, hipStream_t st) -> hipError_t {
     if (!p->timing || (p->timing == 2 && k != 2 && k != 3)) return hipSuccess;
-    return hipEventRecord(p->tev[10 * t + k], st);
+    return hipEventRecord(p->tev[10 * (tb + t) + k], st);
   };
   MDSF_HIP(hipEventRecord(p->ev_fork, s));
   MDSF_HIP(hipStreamWaitEvent(d, p->ev_fork, 0));
@@ -562,8 +567,8 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     // the deposition's events are recorded with the tiled kernel's own
     // dispatch (its begin and end, without the queue's dispatch latency)
     if (p->timing) {
-      p->dep_ev[0] = p->tev[10 * t + 2];
-      p->dep_ev[1] = p->tev[10 * t + 3];
+      p->dep_ev[0] = p->tev[10 * (tb + t) + 2];
+      p->dep_ev[1] = p->tev[10 * (tb + t) + 3];
     }
     const hipError_t de = launch_deposit(*p, p->rho, d);
     p->dep_ev[0] = p->dep_ev[1] = nullptr;
@@ -578,7 +583,8 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
       // with stage timing, the pass's stream kernel records the pass events
       // with its own dispatch (the marks stand when another kernel runs it)
       MDSF_HIP(mark(t, 4 + 2 * pass, f));
-      if (p->timing == 1) fft_pass_events(p->tev[10 * t + 4 + 2 * pass], p->tev[10 * t + 5 + 2 * pass]);
+      if (p->timing == 1)
+        fft_pass_events(p->tev[10 * (tb + t) + 4 + 2 * pass], p->tev[10 * (tb + t) + 5 + 2 * pass]);
       const hipError_t fe = fft_pass(*p, pass, p->rho, f);
       const bool took = fft_pass_events_used();
       fft_pass_events(nullptr, nullptr);
@@ -670,6 +676,7 @@ int mdsf_stage_times(mdsf_plan* plan, double* avg_ms) {
     }
   }
   for (int k = 0; k < 5; ++k) avg_ms[k] /= (double)n;
+  p->timed_frames = 0;
   return MDSF_OK;
 }
 

@@ -20,6 +20,7 @@ constexpr int kMaxSpillBufTiles = 4096;  // spill tiles staged per frame slot (e
 constexpr int kCornerCap = 512;     // corner atoms listed per frame for the spill pass
 constexpr int kListNoSlice = 255;  // tile-list entry: no slice on that axis (a byte per axis)
 constexpr long long kSplitPrepTiles = 2048;  // tiles from which the frame loop splits its prepare stage
+constexpr int64_t kMaxTimedFrames = 1 << 16;  // stage-timing events kept before they start over
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
 // Cell metric G = U U^T of the plan: diagonal, xy-coupled (g01 != 0), or
@@ -250,7 +251,7 @@ struct Plan {
   // end), or null: set by the timed frame loop around launch_deposit
   hipEvent_t dep_ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> tev;
-  int64_t timed_frames = 0;
+  int64_t timed_frames = 0;  // frames timed since the last mdsf_stage_times
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_prep[kSlots] = {}, ev_dep[kSlots] = {}, ev_fft[kSlots] = {};
   // adjoint (allocated on first use)

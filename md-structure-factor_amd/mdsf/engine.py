@@ -244,7 +244,8 @@ class Plan:
         _lib.check(self.lib.mdsf_set_stage_timing(self._handle, mode))
 
     def stage_times(self):
-        """Average ms per frame of the last timed accumulate_frames call:
+        """Average ms per frame of the frames accumulate_frames timed since the
+        last call (timed frames accumulate over calls):
         {prepare, deposit, fft_z, fft_y, fft_x_acc}."""
         out = (_lib.c_double * 5)()
         _lib.check(self.lib.mdsf_stage_times(self._handle, out))

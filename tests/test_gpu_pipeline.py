@@ -80,3 +80,28 @@ def test_pipelined_adjoint_matches_single_frame_entry_points(fp64, rad, gpu):
     pb.check_errors()
     assert np.array_equal(ca.cpu().numpy(), cb.cpu().numpy())
     assert np.array_equal(got, want), float(np.max(np.abs(got - want)))
+
+
+def test_stage_timing_accumulates_over_calls(rad, gpu):
+    """Stage timing of one-frame calls queued back to back: the timed frames
+    add up until stage_times reads them, every stage has a positive time, and
+    a read with no frame timed since the last one is an error."""
+    from mdsf import engine, synth
+    coords, dims, typ = synth.uniform(11, 3000, 30.0, ["H", "C", "O"], 4)
+    s = engine.Setup(coords, dims, typ, rad, 30.0 / 64.5, False, np.eye(3))
+    dev = torch.device("cuda", 0)
+    types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
+    scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
+    box = [float(x) for x in s.L]
+    c = torch.from_numpy(coords.copy()).to(dev)
+    p = engine.Plan.from_setup(s)
+    p.set_stage_timing(1)
+    for t in range(4):
+        p.accumulate_frames(c[t:t + 1], types, scale[t:t + 1], box, s.arith, s.wrap)
+    times = p.stage_times()
+    assert all(v > 0 for v in times.values()), times
+    with pytest.raises(ValueError):  # MDSF_ERR_ARG: nothing timed since the last read
+        p.stage_times()
+    p.set_stage_timing(0)
+    p.check_errors()
+    p.close()
