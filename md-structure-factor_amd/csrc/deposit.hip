@@ -1504,17 +1504,28 @@ __global__ __launch_bounds__(kThreads) void k_tile_lists(SysDev sys, const TypeD
 // MFMA sums of k_deposit_mfma without the enumeration.  Regular atoms come in
 // batches of 64, four lanes per atom, each loading a quarter of the atom's
 // three tile slices; corner atoms in batches of 16, four lanes per virtual
-// atom.  The next batch's slices and the entries of the batch after it are
-// loaded while the current batch's MFMAs run.  Sum order: the regular atoms
-// in list order, then the corner atoms in list order (fixed, so results are
-// bitwise reproducible).
+// atom.  The staging rows are double-buffered: batch i + 1 is written to the
+// other buffer right after batch i's MFMAs are issued (one barrier per
+// batch), and its slices and the entries of batch i + 2 were loaded during
+// the batch before.  Sum order: the regular atoms in list order, then the
+// corner atoms in list order (fixed, so results are bitwise reproducible).
+struct StageRows {
+  __align__(16) float vx[NV + 1][T];  // row NV stays zero: the lists' padding row
+  __align__(16) float vy[NV + 1][T];
+  __align__(16) float vz[NV + 1][T];
+  // per row and wave w: nonzero X on x rows 4w..4w+3
+  __align__(4) unsigned char vq[NV][4];
+};
+
 template <int MT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6)))
 void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __restrict__ pool,
                      const int4* __restrict__ heads, const float* __restrict__ slices,
                      const float* __restrict__ spill, const int* __restrict__ spill_flag,
                      float* __restrict__ rho) {
-  __shared__ StageM st;
+  __shared__ StageRows st[2];
+  // per wave: the rows it accumulates, padded to a multiple of 4 with NV
+  __shared__ unsigned char wl[kThreads / 64][NV + 4];
   // per type: slice axis offsets (in slices) of the regular and the corner
   // layout, corner component stride per axis (floats), layout bases
   __shared__ struct {
@@ -1540,19 +1551,24 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
     ty.corner_stride[i] = d.corner_stride;
     ty.type_start[i] = d.type_start;
   }
-  if (tid < T) st.vx[NV][tid] = st.vy[NV][tid] = st.vz[NV][tid] = 0.f;
+  if (tid < T) {
+    st[0].vx[NV][tid] = st[0].vy[NV][tid] = st[0].vz[NV][tid] = 0.f;
+    st[1].vx[NV][tid] = st[1].vy[NV][tid] = st[1].vz[NV][tid] = 0.f;
+  }
   Tile tl;
   tile_coords(sys, tl);
   const int4 hd = heads[tl.idx];
   const int base = hd.x, nreg = hd.y, ncor = hd.z, nc = hd.w;
   __syncthreads();
 
-  // acc: this batch's MFMA sums; tot: the tile's running sum, added batch by
-  // batch with round-to-nearest adds (the MFMA accumulation rounds toward
-  // zero on its long sums)
-  v4f acc[4], tot[4];
+  // tot: the tile's running sum.  Each batch's MFMA chain starts from zero
+  // and is added into tot with round-to-nearest adds: the MFMA accumulation
+  // rounds its partial sums toward zero (measured: 99.5% of a dense tile's
+  // voxels came out low, by 9e-7 relative after a few hundred rank-1 terms),
+  // so the biased roundings act on batch-sized partial sums only.
+  v4f tot[4];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) acc[b] = tot[b] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int b = 0; b < 4; ++b) tot[b] = v4f{0.f, 0.f, 0.f, 0.f};
   const int q4 = tid & 3;
   // three quarter slices at float offsets o0..o2 (-1: none, zero); a missing
   // one loads the buffer's first quarter slice and zeroes it, so the three
@@ -1566,38 +1582,45 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
     if (o1 < 0) v1 = z4;
     if (o2 < 0) v2 = z4;
   };
-  auto stage = [&](int row, const float4& v0, const float4& v1, const float4& v2) {
-    *reinterpret_cast<float4*>(&st.vx[row][4 * q4]) = v0;
-    *reinterpret_cast<float4*>(&st.vy[row][4 * q4]) = v1;
-    *reinterpret_cast<float4*>(&st.vz[row][4 * q4]) = v2;
-    st.vq[row][q4] = (v0.x != 0.f || v0.y != 0.f || v0.z != 0.f || v0.w != 0.f) ? 1 : 0;
+  auto stage = [&](StageRows& sb, int row, const float4& v0, const float4& v1, const float4& v2) {
+    *reinterpret_cast<float4*>(&sb.vx[row][4 * q4]) = v0;
+    *reinterpret_cast<float4*>(&sb.vy[row][4 * q4]) = v1;
+    *reinterpret_cast<float4*>(&sb.vz[row][4 * q4]) = v2;
+    sb.vq[row][q4] = (v0.x != 0.f || v0.y != 0.f || v0.z != 0.f || v0.w != 0.f) ? 1 : 0;
   };
-  // MFMA accumulation of staged rows [0, vtot): the wave's list holds the
-  // rows with nonzero X on its 4 x rows, padded with the zero row NV
-  auto accumulate = [&](int vtot) {
+  // MFMA accumulation of staged rows [0, vtot) of sb: the wave's list holds
+  // the rows with nonzero X on its 4 x rows, padded with the zero row NV;
+  // the first group starts from a zero accumulator
+  auto accumulate = [&](const StageRows& sb, int vtot) {
     const int kk = lane >> 4, jj = lane & 15;
-    const bool on = lane < vtot && st.vq[lane][wave] != 0;
+    const bool on = lane < vtot && sb.vq[lane][wave] != 0;
     const unsigned long long bm = __ballot(on);
     const int pos = __popcll(bm & ((1ull << lane) - 1));
-    if (on) st.wl[wave][pos] = (unsigned char)lane;
+    if (on) wl[wave][pos] = (unsigned char)lane;
     const int cnt = __popcll(bm);
     const int kend = (cnt + 3) & ~3;
-    if (lane >= cnt && lane < kend) st.wl[wave][lane] = (unsigned char)NV;
+    if (lane >= cnt && lane < kend) wl[wave][lane] = (unsigned char)NV;
     __builtin_amdgcn_wave_barrier();
-    for (int k0 = 0; k0 < kend; k0 += 4) {
-      const int k = st.wl[wave][k0 + kk];
-      const float zv = st.vz[k][jj], yv = st.vy[k][jj];
+    if (kend == 0) return;
+    v4f acc[4];
+    {
+      const int k = wl[wave][kk];
+      const float zv = sb.vz[k][jj], yv = sb.vy[k][jj];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const float av = st.vx[k][4 * wave + b] * yv;
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, zv, acc[b], 0, 0, 0);
-      }
+      for (int b = 0; b < 4; ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(sb.vx[k][4 * wave + b] * yv, zv,
+                                                      v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    }
+    for (int k0 = 4; k0 < kend; k0 += 4) {
+      const int k = wl[wave][k0 + kk];
+      const float zv = sb.vz[k][jj], yv = sb.vy[k][jj];
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(sb.vx[k][4 * wave + b] * yv, zv, acc[b], 0,
+                                                      0, 0);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      tot[b] += acc[b];
-      acc[b] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
+    for (int b = 0; b < 4; ++b) tot[b] += acc[b];
   };
 
   // regular atoms: lanes 4a .. 4a + 3 take atom a of the batch
@@ -1610,32 +1633,47 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
       o1 = valid && j1 != kListNoSlice ? e.x + T * (ty.pre[t][1] + j1) : -1;
       o2 = valid && j2 != kListNoSlice ? e.x + T * (ty.pre[t][2] + j2) : -1;
     };
-    float4 v0, v1, v2;
-    int2 en = make_int2(0, 0);
-    {
-      int o0, o1, o2;
-      const int2 e = a < nreg ? pool[base + a] : make_int2(0, 0);
-      offsets(e, a < nreg, o0, o1, o2);
-      load3(o0, o1, o2, v0, v1, v2);
-      if (NV + a < nreg) en = pool[base + NV + a];
-    }
-    for (int b0 = 0; b0 < nreg; b0 += NV) {
-      __syncthreads();  // the previous batch's MFMAs have read the staging rows
-      stage(a, v0, v1, v2);  // rows past the batch's atoms get zeros
-      __syncthreads();
-      if (b0 + NV < nreg) {
+    if (nreg > 0) {
+      float4 v0, v1, v2;
+      int2 en = make_int2(0, 0);
+      {
         int o0, o1, o2;
-        offsets(en, b0 + NV + a < nreg, o0, o1, o2);
+        const int2 e = a < nreg ? pool[base + a] : make_int2(0, 0);
+        offsets(e, a < nreg, o0, o1, o2);
         load3(o0, o1, o2, v0, v1, v2);
-        if (b0 + 2 * NV + a < nreg) en = pool[base + b0 + 2 * NV + a];
+        if (NV + a < nreg) en = pool[base + NV + a];
       }
-      accumulate(min(NV, nreg - b0));
+      stage(st[0], a, v0, v1, v2);  // rows past the batch's atoms get zeros
+      if (NV < nreg) {  // batch 1 into the registers, the entries of batch 2
+        int o0, o1, o2;
+        offsets(en, NV + a < nreg, o0, o1, o2);
+        load3(o0, o1, o2, v0, v1, v2);
+        if (2 * NV + a < nreg) en = pool[base + 2 * NV + a];
+      }
+      __syncthreads();
+      int buf = 0;
+      for (int b0 = 0; b0 < nreg; b0 += NV, buf ^= 1) {
+        accumulate(st[buf], min(NV, nreg - b0));
+        if (b0 + NV < nreg) {
+          // batch i + 1 into the other buffer (last read by batch i - 1's
+          // MFMAs, which every wave finished before the barrier below it)
+          stage(st[buf ^ 1], a, v0, v1, v2);
+          if (b0 + 2 * NV < nreg) {
+            int o0, o1, o2;
+            offsets(en, b0 + 2 * NV + a < nreg, o0, o1, o2);
+            load3(o0, o1, o2, v0, v1, v2);
+            if (b0 + 3 * NV + a < nreg) en = pool[base + b0 + 3 * NV + a];
+          }
+        }
+        __syncthreads();
+      }
     }
   }
   // corner atoms: 4 virtual atoms each, (Xc, Yall, Zs), (Xn, Yc, Zs),
   // (Xn, Yl, ZL), (Xn, Yh, ZH), from the regular slices and the corner
   // components (x {C, L+H}, y {C, L, H}, z {ZL, ZH}); lanes 16 ca .. 16 ca +
-  // 15 take corner atom ca, four lanes per virtual atom v
+  // 15 take corner atom ca, four lanes per virtual atom v (buffer 0, whose
+  // last MFMA reads ended before the regular loop's last barrier)
   for (int c0 = 0; c0 < ncor; c0 += NV / 4) {
     const int nca = min(NV / 4, ncor - c0);
     const int ca = tid >> 4, v = (tid >> 2) & 3;
@@ -1658,9 +1696,9 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
     float4 w0, w1, w2;
     load3(o0, o1, o2, w0, w1, w2);
     __syncthreads();
-    stage(tid >> 2, w0, w1, w2);  // row 4 ca + v
+    stage(st[0], tid >> 2, w0, w1, w2);  // row 4 ca + v
     __syncthreads();
-    accumulate(4 * nca);
+    accumulate(st[0], 4 * nca);
   }
 
   // D[y][z] of block b: lane l holds y = 4 (l / 16) + r, z = l % 16
