@@ -87,7 +87,8 @@ void free_plan(Plan* p) {
   for (FrameState& f : p->slot) {
     void* fs[] = {f.ir4,   f.key,   f.rpos, f.bucket_count, f.bucket_off, f.bucket_cur,
                   f.bucket_aux, f.s_tmp, f.s_idx, f.s_ir, f.s_r, f.tables, f.slices,
-                  f.dslices, f.corners, f.spill, f.spill_flag, f.tl_pool, f.tl_head};
+                  f.dslices, f.bslices, f.corners, f.spill, f.spill_flag, f.tl_pool,
+                  f.tl_head};
     for (void* q : fs)
       if (q) (void)hipFree(q);
   }
@@ -263,12 +264,29 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
   }
   s.nbuckets = bucket_base;
   p->table_floats = table_floats;
+  const long long regular_floats = slice_floats;
   for (int t = 0; t < desc->ntypes; ++t) {  // corner components follow the regular slices
     TypeDev& ty = p->types_host[t];
     ty.corner_base = slice_floats;
     slice_floats += (long long)ty.count * ty.corner_stride;
   }
-  p->slice_floats = (!p->fft64 && s.mode == MODE_DIAG) ? slice_floats : 0;
+  // FP32 xy-coupled cells deposit from the tile slices and lists too when
+  // every type's window meets each x and each y tile through one periodic
+  // image (2A <= N - 16: two images in one tile need 2A >= N - 14), so that
+  // an atom's factor over a tile is X(x) Y(y) 2^(c b_x b_y), and when the
+  // cross exponent |c b_x b_y| <= c (A_x + 1) dr_x (A_y + 1) dr_y stays
+  // within 100 (2^(...) finite in FP32); otherwise they keep the enumerating
+  // kernel, which sums the images of each cell pair.
+  bool xy_lists = !p->fft64 && s.mode == MODE_XY && slice_floats < (1LL << 31);
+  for (int t = 0; t < desc->ntypes && xy_lists; ++t) {
+    const TypeDev& ty = p->types_host[t];
+    for (int d = 0; d < 2; ++d)
+      if (2 * ty.half[d] > desc->n[d] - kTile) xy_lists = false;
+    const double c = 2.0 * std::fabs(s.g01) * 1.4426950408889634 / ty.two_sig2;
+    if (c * (ty.half[0] + 1) * s.dr[0] * (ty.half[1] + 1) * s.dr[1] > 100.0) xy_lists = false;
+  }
+  p->slice_floats = (!p->fft64 && (s.mode == MODE_DIAG || xy_lists)) ? slice_floats : 0;
+  p->bslice_floats = xy_lists ? regular_floats : 0;
   if (p->slice_floats >= (1LL << 31)) {  // the deposition addresses slices with 32-bit offsets
     free_plan(p);
     return fail(MDSF_ERR_UNSUPPORTED, "too many atoms for the FP32 tile-slice layout (2^31 floats)");
@@ -303,7 +321,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       ty.adjt_base = slots;
       slots += (long long)ty.count * ty.nsl[0] * ty.nsl[1] * ty.nsl[2];
     }
-    p->adj_tslots = p->slice_floats > 0 ? slots : 0;
+    p->adj_tslots = (p->slice_floats > 0 && s.mode == MODE_DIAG) ? slots : 0;
   }
   // adjoint work items (atom, x-chunk of <= 64 planes, chunk of (y, z)
   // pairs sized for ~16k window voxels), heaviest windows numbered first so
@@ -374,6 +392,7 @@ int mdsf_plan_create(const mdsf_grid_desc* desc, mdsf_plan** out) {
       f.tables = tb;
     }
     TRY(dalloc(&f.slices, (size_t)p->slice_floats));
+    if (p->bslice_floats > 0) TRY(dalloc(&f.bslices, (size_t)p->bslice_floats));
     TRY(dalloc(&f.corners, (size_t)2 + 2 * kCornerCap));
     TRY(hipMemset(f.corners, 0, sizeof(int) * ((size_t)2 + 2 * kCornerCap)));
     if (p->spill_tiles > 0 && p->spill_tiles <= kMaxSpillBufTiles) {

@@ -1509,21 +1509,33 @@ __global__ __launch_bounds__(kThreads) void k_tile_lists(SysDev sys, const TypeD
 // batch), and its slices and the entries of batch i + 2 were loaded during
 // the batch before.  Sum order: the regular atoms in list order, then the
 // corner atoms in list order (fixed, so results are bitwise reproducible).
+//
+// XY: xy-coupled cells whose windows meet each x and y tile through one
+// image.  The y rows hold LY = log2 Y and two more rows per virtual atom hold
+// CX (per x cell) and BY (per y cell) from bslices, so that the A operand is
+// X[x] 2^(LY[y] + CX[x] BY[y]): the cross factor exp(-2 g01 b_x b_y / 2 s^2)
+// costs one fma and one v_exp_f32 per operand element.
+template <bool XY>
 struct StageRows {
   __align__(16) float vx[NV + 1][T];  // row NV stays zero: the lists' padding row
   __align__(16) float vy[NV + 1][T];
   __align__(16) float vz[NV + 1][T];
+  __align__(16) float cx[XY ? NV + 1 : 1][T];
+  __align__(16) float by[XY ? NV + 1 : 1][T];
   // per row and wave w: nonzero X on x rows 4w..4w+3
   __align__(4) unsigned char vq[NV][4];
 };
 
-template <int MT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6)))
-void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __restrict__ pool,
-                     const int4* __restrict__ heads, const float* __restrict__ slices,
-                     const float* __restrict__ spill, const int* __restrict__ spill_flag,
-                     float* __restrict__ rho) {
-  __shared__ StageRows st[2];
+template <int MT, bool XY>
+__device__ __forceinline__ void deposit_lists(SysDev sys, const TypeDev* __restrict__ td,
+                                              const int2* __restrict__ pool,
+                                              const int4* __restrict__ heads,
+                                              const float* __restrict__ slices,
+                                              const float* __restrict__ bslices,
+                                              const float* __restrict__ spill,
+                                              const int* __restrict__ spill_flag,
+                                              float* __restrict__ rho) {
+  __shared__ StageRows<XY> st[2];
   // per wave: the rows it accumulates, padded to a multiple of 4 with NV
   __shared__ unsigned char wl[kThreads / 64][NV + 4];
   // per type: slice axis offsets (in slices) of the regular and the corner
@@ -1552,8 +1564,11 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
     ty.type_start[i] = d.type_start;
   }
   if (tid < T) {
-    st[0].vx[NV][tid] = st[0].vy[NV][tid] = st[0].vz[NV][tid] = 0.f;
-    st[1].vx[NV][tid] = st[1].vy[NV][tid] = st[1].vz[NV][tid] = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      st[b].vx[NV][tid] = st[b].vy[NV][tid] = st[b].vz[NV][tid] = 0.f;
+      if constexpr (XY) st[b].cx[NV][tid] = st[b].by[NV][tid] = 0.f;
+    }
   }
   Tile tl;
   tile_coords(sys, tl);
@@ -1570,28 +1585,51 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
 #pragma unroll
   for (int b = 0; b < 4; ++b) tot[b] = v4f{0.f, 0.f, 0.f, 0.f};
   const int q4 = tid & 3;
-  // three quarter slices at float offsets o0..o2 (-1: none, zero); a missing
-  // one loads the buffer's first quarter slice and zeroes it, so the three
-  // loads go out unconditionally
-  auto load3 = [&](int o0, int o1, int o2, float4& v0, float4& v1, float4& v2) {
-    v0 = *reinterpret_cast<const float4*>(slices + max(o0, 0) + 4 * q4);
-    v1 = *reinterpret_cast<const float4*>(slices + max(o1, 0) + 4 * q4);
-    v2 = *reinterpret_cast<const float4*>(slices + max(o2, 0) + 4 * q4);
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (o0 < 0) v0 = z4;
-    if (o1 < 0) v1 = z4;
-    if (o2 < 0) v2 = z4;
+  // a virtual atom's quarter rows: X, Y (XY: LY), Z and (XY) CX, BY
+  struct Rows {
+    float4 x, y, z, cx, by;
   };
-  auto stage = [&](StageRows& sb, int row, const float4& v0, const float4& v1, const float4& v2) {
-    *reinterpret_cast<float4*>(&sb.vx[row][4 * q4]) = v0;
-    *reinterpret_cast<float4*>(&sb.vy[row][4 * q4]) = v1;
-    *reinterpret_cast<float4*>(&sb.vz[row][4 * q4]) = v2;
-    sb.vq[row][q4] = (v0.x != 0.f || v0.y != 0.f || v0.z != 0.f || v0.w != 0.f) ? 1 : 0;
+  // quarter slices at float offsets o0..o2 (-1: none, zero; XY: a missing LY
+  // is -inf) and (XY) the cross rows at ox, oy (-1: none, zero); a missing one
+  // loads the buffer's first quarter slice and replaces it, so the loads go
+  // out unconditionally
+  auto load = [&](int o0, int o1, int o2, int ox, int oy, Rows& v) {
+    v.x = *reinterpret_cast<const float4*>(slices + max(o0, 0) + 4 * q4);
+    v.y = *reinterpret_cast<const float4*>(slices + max(o1, 0) + 4 * q4);
+    v.z = *reinterpret_cast<const float4*>(slices + max(o2, 0) + 4 * q4);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (XY) {
+      v.cx = *reinterpret_cast<const float4*>(bslices + max(ox, 0) + 4 * q4);
+      v.by = *reinterpret_cast<const float4*>(bslices + max(oy, 0) + 4 * q4);
+      if (ox < 0) v.cx = z4;
+      if (oy < 0) v.by = z4;
+    }
+    const float ly0 = XY ? -__builtin_inff() : 0.f;
+    if (o0 < 0) v.x = z4;
+    if (o1 < 0) v.y = make_float4(ly0, ly0, ly0, ly0);
+    if (o2 < 0) v.z = z4;
+  };
+  auto stage = [&](StageRows<XY>& sb, int row, const Rows& v) {
+    *reinterpret_cast<float4*>(&sb.vx[row][4 * q4]) = v.x;
+    *reinterpret_cast<float4*>(&sb.vy[row][4 * q4]) = v.y;
+    *reinterpret_cast<float4*>(&sb.vz[row][4 * q4]) = v.z;
+    if constexpr (XY) {
+      *reinterpret_cast<float4*>(&sb.cx[row][4 * q4]) = v.cx;
+      *reinterpret_cast<float4*>(&sb.by[row][4 * q4]) = v.by;
+    }
+    sb.vq[row][q4] = (v.x.x != 0.f || v.x.y != 0.f || v.x.z != 0.f || v.x.w != 0.f) ? 1 : 0;
+  };
+  // A operand of virtual atom k at x row xr, y = jj
+  auto aop = [&](const StageRows<XY>& sb, int k, int xr, float yv, float bv) -> float {
+    if constexpr (XY)
+      return sb.vx[k][xr] * __builtin_amdgcn_exp2f(fmaf(sb.cx[k][xr], bv, yv));
+    else
+      return sb.vx[k][xr] * yv;
   };
   // MFMA accumulation of staged rows [0, vtot) of sb: the wave's list holds
   // the rows with nonzero X on its 4 x rows, padded with the zero row NV;
   // the first group starts from a zero accumulator
-  auto accumulate = [&](const StageRows& sb, int vtot) {
+  auto accumulate = [&](const StageRows<XY>& sb, int vtot) {
     const int kk = lane >> 4, jj = lane & 15;
     const bool on = lane < vtot && sb.vq[lane][wave] != 0;
     const unsigned long long bm = __ballot(on);
@@ -1605,19 +1643,19 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
     v4f acc[4];
     {
       const int k = wl[wave][kk];
-      const float zv = sb.vz[k][jj], yv = sb.vy[k][jj];
+      const float zv = sb.vz[k][jj], yv = sb.vy[k][jj], bv = XY ? sb.by[k][jj] : 0.f;
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(sb.vx[k][4 * wave + b] * yv, zv,
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop(sb, k, 4 * wave + b, yv, bv), zv,
                                                       v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
     for (int k0 = 4; k0 < kend; k0 += 4) {
       const int k = wl[wave][k0 + kk];
-      const float zv = sb.vz[k][jj], yv = sb.vy[k][jj];
+      const float zv = sb.vz[k][jj], yv = sb.vy[k][jj], bv = XY ? sb.by[k][jj] : 0.f;
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(sb.vx[k][4 * wave + b] * yv, zv, acc[b], 0,
-                                                      0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop(sb, k, 4 * wave + b, yv, bv), zv, acc[b],
+                                                      0, 0, 0);
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) tot[b] += acc[b];
@@ -1626,28 +1664,27 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
   // regular atoms: lanes 4a .. 4a + 3 take atom a of the batch
   {
     const int a = tid >> 2;
-    auto offsets = [&](const int2& e, bool valid, int& o0, int& o1, int& o2) {
+    // the atom's regular slice offsets (XY: the cross rows at the x and y
+    // slice offsets)
+    auto fetch = [&](const int2& e, bool valid, Rows& v) {
       const int t = (int)((unsigned)e.y >> 24);
       const int j0 = e.y & 255, j1 = (e.y >> 8) & 255, j2 = (e.y >> 16) & 255;
-      o0 = valid && j0 != kListNoSlice ? e.x + T * (ty.pre[t][0] + j0) : -1;
-      o1 = valid && j1 != kListNoSlice ? e.x + T * (ty.pre[t][1] + j1) : -1;
-      o2 = valid && j2 != kListNoSlice ? e.x + T * (ty.pre[t][2] + j2) : -1;
+      const int o0 = valid && j0 != kListNoSlice ? e.x + T * (ty.pre[t][0] + j0) : -1;
+      const int o1 = valid && j1 != kListNoSlice ? e.x + T * (ty.pre[t][1] + j1) : -1;
+      const int o2 = valid && j2 != kListNoSlice ? e.x + T * (ty.pre[t][2] + j2) : -1;
+      load(o0, o1, o2, o0, o1, v);
     };
     if (nreg > 0) {
-      float4 v0, v1, v2;
+      Rows v;
       int2 en = make_int2(0, 0);
       {
-        int o0, o1, o2;
         const int2 e = a < nreg ? pool[base + a] : make_int2(0, 0);
-        offsets(e, a < nreg, o0, o1, o2);
-        load3(o0, o1, o2, v0, v1, v2);
+        fetch(e, a < nreg, v);
         if (NV + a < nreg) en = pool[base + NV + a];
       }
-      stage(st[0], a, v0, v1, v2);  // rows past the batch's atoms get zeros
+      stage(st[0], a, v);  // rows past the batch's atoms get zeros
       if (NV < nreg) {  // batch 1 into the registers, the entries of batch 2
-        int o0, o1, o2;
-        offsets(en, NV + a < nreg, o0, o1, o2);
-        load3(o0, o1, o2, v0, v1, v2);
+        fetch(en, NV + a < nreg, v);
         if (2 * NV + a < nreg) en = pool[base + 2 * NV + a];
       }
       __syncthreads();
@@ -1657,11 +1694,9 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
         if (b0 + NV < nreg) {
           // batch i + 1 into the other buffer (last read by batch i - 1's
           // MFMAs, which every wave finished before the barrier below it)
-          stage(st[buf ^ 1], a, v0, v1, v2);
+          stage(st[buf ^ 1], a, v);
           if (b0 + 2 * NV < nreg) {
-            int o0, o1, o2;
-            offsets(en, b0 + 2 * NV + a < nreg, o0, o1, o2);
-            load3(o0, o1, o2, v0, v1, v2);
+            fetch(en, b0 + 2 * NV + a < nreg, v);
             if (b0 + 3 * NV + a < nreg) en = pool[base + b0 + 3 * NV + a];
           }
         }
@@ -1677,13 +1712,14 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
   for (int c0 = 0; c0 < ncor; c0 += NV / 4) {
     const int nca = min(NV / 4, ncor - c0);
     const int ca = tid >> 4, v = (tid >> 2) & 3;
-    int o0 = -1, o1 = -1, o2 = -1;
+    int o0 = -1, o1 = -1, o2 = -1, ox = -1, oy = -1;
     if (ca < nca) {
       const int2 e = pool[base + nc - 1 - (c0 + ca)];
       const int t = (int)((unsigned)e.y >> 24), slot = e.x - ty.type_start[t];
       const int ab = ty.slice_base[t] + slot * ty.slice_stride[t];
       const int cb = ty.corner_base[t] + slot * ty.corner_stride[t];
       const int j0 = e.y & 255, j1 = (e.y >> 8) & 255, j2 = (e.y >> 16) & 255;
+      const int so0 = j0 != kListNoSlice ? ab + T * (ty.pre[t][0] + j0) : -1;
       const int so1 = j1 != kListNoSlice ? ab + T * (ty.pre[t][1] + j1) : -1;
       const int so2 = j2 != kListNoSlice ? ab + T * (ty.pre[t][2] + j2) : -1;
       const int cb0 = j0 != kListNoSlice ? cb + T * (ty.cpre[t][0] + j0) : -1;
@@ -1692,11 +1728,15 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
       o0 = cb0 >= 0 ? cb0 + (v == 0 ? 0 : ty.cs[t][0]) : -1;
       o1 = v == 0 ? so1 : (cb1 >= 0 ? cb1 + (v - 1) * ty.cs[t][1] : -1);
       o2 = v < 2 ? so2 : (cb2 >= 0 ? cb2 + (v - 2) * ty.cs[t][2] : -1);
+      // XY: the cross rows at the regular x and y slices (one image per
+      // tile, so the 4 virtual atoms share them)
+      ox = so0;
+      oy = so1;
     }
-    float4 w0, w1, w2;
-    load3(o0, o1, o2, w0, w1, w2);
+    Rows w;
+    load(o0, o1, o2, ox, oy, w);
     __syncthreads();
-    stage(st[0], tid >> 2, w0, w1, w2);  // row 4 ca + v
+    stage(st[0], tid >> 2, w);  // row 4 ca + v
     __syncthreads();
     accumulate(st[0], 4 * nca);
   }
@@ -1717,6 +1757,26 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
       }
     }
   }
+}
+
+// Orthogonal cells: 26.6 KB LDS, 6 waves per SIMD.
+template <int MT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6)))
+void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __restrict__ pool,
+                     const int4* __restrict__ heads, const float* __restrict__ slices,
+                     const float* __restrict__ bslices, const float* __restrict__ spill,
+                     const int* __restrict__ spill_flag, float* __restrict__ rho) {
+  deposit_lists<MT, false>(sys, td, pool, heads, slices, bslices, spill, spill_flag, rho);
+}
+
+// xy-coupled cells: five rows per virtual atom, 42 KB LDS (3 blocks per CU).
+template <int MT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8)))
+void k_deposit_lists_xy(SysDev sys, const TypeDev* __restrict__ td, const int2* __restrict__ pool,
+                        const int4* __restrict__ heads, const float* __restrict__ slices,
+                        const float* __restrict__ bslices, const float* __restrict__ spill,
+                        const int* __restrict__ spill_flag, float* __restrict__ rho) {
+  deposit_lists<MT, true>(sys, td, pool, heads, slices, bslices, spill, spill_flag, rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -2223,11 +2283,17 @@ hipError_t launch_impl(Plan& p, ACC* out, const ACC* stage, hipStream_t s) {
     // blocks size their per-type LDS for 16 types when the system has no more
     const bool few = sys.ntypes <= 16;
     if (sys.mode == MODE_DIAG && p.tl_cap > 0 && few)
-      DEP_LAUNCH((k_deposit_lists<16>), sys, p.types, p.tl_pool, p.tl_head, p.slices, stage,
-                 p.spill_flag, out);
+      DEP_LAUNCH((k_deposit_lists<16>), sys, p.types, p.tl_pool, p.tl_head, p.slices,
+                 p.bslices, stage, p.spill_flag, out);
     else if (sys.mode == MODE_DIAG && p.tl_cap > 0)
-      DEP_LAUNCH((k_deposit_lists<kMaxTypes>), sys, p.types, p.tl_pool, p.tl_head, p.slices,
-                 stage, p.spill_flag, out);
+      DEP_LAUNCH((k_deposit_lists<kMaxTypes>), sys, p.types, p.tl_pool, p.tl_head,
+                 p.slices, p.bslices, stage, p.spill_flag, out);
+    else if (sys.mode == MODE_XY && p.tl_cap > 0 && few)
+      DEP_LAUNCH((k_deposit_lists_xy<16>), sys, p.types, p.tl_pool, p.tl_head, p.slices,
+                 p.bslices, stage, p.spill_flag, out);
+    else if (sys.mode == MODE_XY && p.tl_cap > 0)
+      DEP_LAUNCH((k_deposit_lists_xy<kMaxTypes>), sys, p.types, p.tl_pool, p.tl_head,
+                 p.slices, p.bslices, stage, p.spill_flag, out);
     else if (sys.mode == MODE_DIAG && few)
       DEP_LAUNCH((k_deposit_mfma<MODE_DIAG, 16>), sys, p.types,
                          p.bucket_off, p.s_ir, p.s_r, p.slices, stage, p.spill_flag, out);

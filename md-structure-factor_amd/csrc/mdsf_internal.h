@@ -131,6 +131,9 @@ struct FrameState {
   void* tables = nullptr;
   float* slices = nullptr;
   float* dslices = nullptr;  // derivatives of the slices in the atom position (adjoint)
+  // xy-coupled cells on tile lists: per regular x slice cell c_xy b_x, per
+  // regular y slice cell b_y (same offsets as the slices), for the cross factor
+  float* bslices = nullptr;
   // corner atoms (window off the core on all three axes): [0] count, then
   // kCornerCap sorted indices in arrival order, then those that spill in index
   // order, then their count
@@ -172,11 +175,13 @@ struct Plan {
   void* tables = nullptr;         // per-atom factor tables (FP64 plans)
   float* slices = nullptr;        // per-atom tile slices (FP32 plans, orthogonal-axis cells)
   float* dslices = nullptr;       // their derivatives (adjoint only)
+  float* bslices = nullptr;       // cross-term rows (xy-coupled cells on tile lists)
   int* corners = nullptr;         // corner-atom list of the current slot
   void* spill = nullptr;          // staged spill tiles of the current slot (or null)
   int* spill_flag = nullptr;      // their occupancy flags
   long long spill_tiles = 0;      // sp_cnt[0] sp_cnt[1] sp_cnt[2]
   long long slice_floats = 0;     // regular slices, then the corner components
+  long long bslice_floats = 0;    // the regular part, when bslices are used (else 0)
   // tile lists of the current slot (null when the plan does not use them)
   int2* tl_pool = nullptr;
   int4* tl_head = nullptr;
@@ -217,6 +222,7 @@ struct Plan {
     tables = f.tables;
     slices = f.slices;
     dslices = f.dslices;
+    bslices = f.bslices;
     corners = f.corners;
     spill = f.spill;
     spill_flag = f.spill_flag;

@@ -316,9 +316,19 @@ __global__ void k_tables(SysDev sys, const TypeDev* __restrict__ td, const int4*
 // window [lo, lo + 2A) of amp^[d==0] exp(-g_dd (r - u dr)^2 / 2 s^2): the
 // exponent in FP64, exp in FP32 (dens.py:252-256, 299-308).  Cells past N
 // are zero.
+//
+// xy-coupled cells (FP32 plans whose windows meet each x and y tile through
+// one image, so at most one of the three terms is nonzero): the factor
+// amp exp(-(g00 b_x^2 + 2 g01 b_x b_y + g11 b_y^2) / 2 s^2) of a cell pair is
+// X(x) 2^(LY(y) + CX(x) BY(y)), with X as above, and in the y slices (regular
+// and corner components) LY = log2 of the y factor (-inf outside the window)
+// instead of the factor; bslices holds CX = -2 g01 log2(e) / 2 s^2 b_x at the
+// regular x slice cells and BY = b_y at the regular y slice cells (0 outside
+// the window).
 __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4* __restrict__ s_ir,
                          const double* __restrict__ s_r, const int* __restrict__ nvalid,
-                         float* __restrict__ slices, float* __restrict__ dslices, int mode) {
+                         float* __restrict__ slices, float* __restrict__ dslices,
+                         float* __restrict__ bslices, int mode) {
 #pragma clang fp contract(off)
   const long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const int pair = (int)(g / kTile), i = (int)(g - (long long)pair * kTile);
@@ -364,6 +374,46 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
     if (wd) dslices[o] = dv;
   };
   int T = pos_mod(lo, N) / kTile;
+  if (sys.mode == MODE_XY && d < 2) {
+    constexpr double kLog2e = 1.4426950408889634;
+    const double cxy = -2.0 * sys.g01 * kLog2e / ty.two_sig2;
+    auto inw = [&](int u) { return u >= lo && u < hi; };
+    // log2 of the y factor of the image at u, -inf outside the window
+    auto lg = [&](int u) -> float {
+      if (!inw(u)) return -__builtin_inff();
+      const double b = r - (double)u * dr;
+      return (float)(-(gk * (b * b)) * kLog2e);
+    };
+    for (int j = 0; j < nsl; ++j) {
+      const int c = T * kTile + i;
+      const bool in = c < N;
+      const int us = inw(c) ? c : (inw(c - N) ? c - N : c + N);
+      const bool any = in && inw(us);
+      const double bs = r - (double)us * dr;
+      const long long o = aoff + j * kTile;
+      if (d == 0) {
+        float dd;
+        const float vL = in ? f(c - N, dd) : 0.f, vC = in ? f(c, dd) : 0.f;
+        const float vH = in ? f(c + N, dd) : 0.f;
+        slices[o] = vC + vL + vH;
+        bslices[o] = any ? (float)(cxy * bs) : 0.f;
+        if (corner) {
+          slices[acoff + j * kTile] = vC;
+          slices[acoff + (nsl + j) * kTile] = vL + vH;
+        }
+      } else {
+        slices[o] = any ? lg(us) : -__builtin_inff();
+        bslices[o] = any ? (float)bs : 0.f;
+        if (corner) {
+          slices[acoff + j * kTile] = in ? lg(c) : -__builtin_inff();
+          slices[acoff + (nsl + j) * kTile] = in ? lg(c - N) : -__builtin_inff();
+          slices[acoff + (2 * nsl + j) * kTile] = in ? lg(c + N) : -__builtin_inff();
+        }
+      }
+      T = T + 1 == nt ? 0 : T + 1;
+    }
+    return;
+  }
   for (int j = 0; j < nsl; ++j) {
     const int c = T * kTile + i;
     const bool in = c < N;
@@ -451,7 +501,7 @@ hipError_t launch_slice_derivs(Plan& p, hipStream_t s) {
   if (n == 0 || p.slice_floats == 0 || !p.dslices) return hipSuccess;
   hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0, s,
                      p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + p.sys.nbuckets, p.slices,
-                     p.dslices, 2);
+                     p.dslices, p.bslices, 2);
   return hipGetLastError();
 }
 
@@ -505,7 +555,7 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux,
   if (p.slice_floats > 0)
     hipLaunchKernelGGL(k_slices, dim3((unsigned)((3LL * kTile * n + 255) / 256)), dim3(256), 0, s,
                        p.sys, p.types, p.s_ir, p.s_r, p.bucket_off + nbk, p.slices, p.dslices,
-                       (derivs && p.dslices) ? 1 : 0);
+                       p.bslices, (derivs && p.dslices) ? 1 : 0);
   // factor tables feed the FP64 gather
   const long long total = p.fft64 ? p.table_floats : 0;
   if (total > 0) {
