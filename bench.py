@@ -354,10 +354,14 @@ def mono_leg(coords, dims, typ, rad, sres, steps, warmup):
     torch.cuda.synchronize()
     plan.check_errors()
     ms = e0.elapsed_time(e1)
+    iso = isolated_stages(plan, d_c, d_t, d_s, box, s, min(steps, 30), stream)
     plan.close()
     return {"workload": "hii_phase_256^3_50k_atoms in the 120-degree cell (main_gromacs.py "
                         "-ct 120 transform)", "frames": steps, "frames_per_s": steps / (ms * 1e-3),
-            "ms_per_frame": ms / steps, "deposition": "xy-coupled (MODE_XY)"}
+            "ms_per_frame": ms / steps,
+            "deposition": "xy-coupled (MODE_XY): k_deposit_lists_xy, the tile lists with the "
+                          "cross factor 2^(c b_x b_y) per MFMA operand element",
+            "stage_ms_isolated": {k: round(v, 4) for k, v in iso.items()}}
 
 
 def main():

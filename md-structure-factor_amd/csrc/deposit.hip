@@ -1515,6 +1515,11 @@ __global__ __launch_bounds__(kThreads) void k_tile_lists(SysDev sys, const TypeD
 // CX (per x cell) and BY (per y cell) from bslices, so that the A operand is
 // X[x] 2^(LY[y] + CX[x] BY[y]): the cross factor exp(-2 g01 b_x b_y / 2 s^2)
 // costs one fma and one v_exp_f32 per operand element.
+// xy-coupled cells stage into one buffer (two barriers per batch): with two
+// their 43 KB of LDS held the kernel to 3 blocks per CU, and one buffer at 5
+// waves per SIMD ran 13% faster (126 -> 109 us at 256^3, 50 k atoms)
+constexpr int kXYStageBuffers = 1;
+
 template <bool XY>
 struct StageRows {
   __align__(16) float vx[NV + 1][T];  // row NV stays zero: the lists' padding row
@@ -1535,7 +1540,10 @@ __device__ __forceinline__ void deposit_lists(SysDev sys, const TypeDev* __restr
                                               const float* __restrict__ spill,
                                               const int* __restrict__ spill_flag,
                                               float* __restrict__ rho) {
-  __shared__ StageRows<XY> st[2];
+  // staging buffers: two (the next batch is staged while this one's MFMAs
+  // run) or, for xy-coupled cells, kXYStageBuffers
+  constexpr int NB = XY ? kXYStageBuffers : 2;
+  __shared__ StageRows<XY> st[NB];
   // per wave: the rows it accumulates, padded to a multiple of 4 with NV
   __shared__ unsigned char wl[kThreads / 64][NV + 4];
   // per type: slice axis offsets (in slices) of the regular and the corner
@@ -1565,7 +1573,7 @@ __device__ __forceinline__ void deposit_lists(SysDev sys, const TypeDev* __restr
   }
   if (tid < T) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NB; ++b) {
       st[b].vx[NV][tid] = st[b].vy[NV][tid] = st[b].vz[NV][tid] = 0.f;
       if constexpr (XY) st[b].cx[NV][tid] = st[b].by[NV][tid] = 0.f;
     }
@@ -1689,12 +1697,14 @@ __device__ __forceinline__ void deposit_lists(SysDev sys, const TypeDev* __restr
       }
       __syncthreads();
       int buf = 0;
-      for (int b0 = 0; b0 < nreg; b0 += NV, buf ^= 1) {
+      for (int b0 = 0; b0 < nreg; b0 += NV, buf ^= NB - 1) {
         accumulate(st[buf], min(NV, nreg - b0));
         if (b0 + NV < nreg) {
           // batch i + 1 into the other buffer (last read by batch i - 1's
-          // MFMAs, which every wave finished before the barrier below it)
-          stage(st[buf ^ 1], a, v);
+          // MFMAs, which every wave finished before the barrier below it);
+          // with one buffer, once every wave's MFMAs of batch i have read it
+          if constexpr (NB == 1) __syncthreads();
+          stage(st[buf ^ (NB - 1)], a, v);
           if (b0 + 2 * NV < nreg) {
             fetch(en, b0 + 2 * NV + a < nreg, v);
             if (b0 + 3 * NV + a < nreg) en = pool[base + b0 + 3 * NV + a];
@@ -1769,9 +1779,10 @@ void k_deposit_lists(SysDev sys, const TypeDev* __restrict__ td, const int2* __r
   deposit_lists<MT, false>(sys, td, pool, heads, slices, bslices, spill, spill_flag, rho);
 }
 
-// xy-coupled cells: five rows per virtual atom, 42 KB LDS (3 blocks per CU).
+// xy-coupled cells: five rows per virtual atom in one staging buffer, 22 KB
+// LDS, 5 waves per SIMD (92 VGPRs).
 template <int MT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8)))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void k_deposit_lists_xy(SysDev sys, const TypeDev* __restrict__ td, const int2* __restrict__ pool,
                         const int4* __restrict__ heads, const float* __restrict__ slices,
                         const float* __restrict__ bslices, const float* __restrict__ spill,

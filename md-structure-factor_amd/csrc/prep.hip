@@ -373,53 +373,40 @@ __global__ void k_slices(SysDev sys, const TypeDev* __restrict__ td, const int4*
     if (wv) slices[o] = v;
     if (wd) dslices[o] = dv;
   };
+  // xy-coupled cells (the branch is uniform across the grid; within it only
+  // the bslices store depends on the axis)
+  const bool xy = sys.mode == MODE_XY;
+  const bool ylog = xy && d == 1;
+  constexpr double kLog2e = 1.4426950408889634;
+  const double cxy = -2.0 * sys.g01 * kLog2e / ty.two_sig2;
+  auto inw = [&](int u) { return u >= lo && u < hi; };
+  // log2 of the y factor of the image at u, -inf outside the window
+  auto lg = [&](int u) -> float {
+    if (!inw(u)) return -__builtin_inff();
+    const double b = r - (double)u * dr;
+    return (float)(-(gk * (b * b)) * kLog2e);
+  };
   int T = pos_mod(lo, N) / kTile;
-  if (sys.mode == MODE_XY && d < 2) {
-    constexpr double kLog2e = 1.4426950408889634;
-    const double cxy = -2.0 * sys.g01 * kLog2e / ty.two_sig2;
-    auto inw = [&](int u) { return u >= lo && u < hi; };
-    // log2 of the y factor of the image at u, -inf outside the window
-    auto lg = [&](int u) -> float {
-      if (!inw(u)) return -__builtin_inff();
-      const double b = r - (double)u * dr;
-      return (float)(-(gk * (b * b)) * kLog2e);
-    };
-    for (int j = 0; j < nsl; ++j) {
-      const int c = T * kTile + i;
-      const bool in = c < N;
-      const int us = inw(c) ? c : (inw(c - N) ? c - N : c + N);
-      const bool any = in && inw(us);
-      const double bs = r - (double)us * dr;
-      const long long o = aoff + j * kTile;
-      if (d == 0) {
-        float dd;
-        const float vL = in ? f(c - N, dd) : 0.f, vC = in ? f(c, dd) : 0.f;
-        const float vH = in ? f(c + N, dd) : 0.f;
-        slices[o] = vC + vL + vH;
-        bslices[o] = any ? (float)(cxy * bs) : 0.f;
-        if (corner) {
-          slices[acoff + j * kTile] = vC;
-          slices[acoff + (nsl + j) * kTile] = vL + vH;
-        }
-      } else {
-        slices[o] = any ? lg(us) : -__builtin_inff();
-        bslices[o] = any ? (float)bs : 0.f;
-        if (corner) {
-          slices[acoff + j * kTile] = in ? lg(c) : -__builtin_inff();
-          slices[acoff + (nsl + j) * kTile] = in ? lg(c - N) : -__builtin_inff();
-          slices[acoff + (2 * nsl + j) * kTile] = in ? lg(c + N) : -__builtin_inff();
-        }
-      }
-      T = T + 1 == nt ? 0 : T + 1;
-    }
-    return;
-  }
   for (int j = 0; j < nsl; ++j) {
     const int c = T * kTile + i;
     const bool in = c < N;
     float dL = 0.f, dC = 0.f, dH = 0.f;
-    const float vL = in ? f(c - N, dL) : 0.f, vC = in ? f(c, dC) : 0.f, vH = in ? f(c + N, dH) : 0.f;
-    put(aoff + j * kTile, vC + vL + vH, dC + dL + dH);
+    float vL = in ? f(c - N, dL) : 0.f, vC = in ? f(c, dC) : 0.f, vH = in ? f(c + N, dH) : 0.f;
+    if (xy) {
+      // at most one image in the window (the plan's condition): its b gives
+      // CX (x) or BY (y); y values become log2 (then the regular value, the
+      // one image's, is the largest of the three)
+      const int us = inw(c) ? c : (inw(c - N) ? c - N : c + N);
+      const bool any = in && inw(us);
+      const double bs = r - (double)us * dr;
+      if (d < 2) bslices[aoff + j * kTile] = any ? (float)(d == 0 ? cxy * bs : bs) : 0.f;
+      if (ylog) {
+        vL = in ? lg(c - N) : -__builtin_inff();
+        vC = in ? lg(c) : -__builtin_inff();
+        vH = in ? lg(c + N) : -__builtin_inff();
+      }
+    }
+    put(aoff + j * kTile, ylog ? fmaxf(fmaxf(vL, vC), vH) : vC + vL + vH, dC + dL + dH);
     if (corner) {
       // the components the corner virtual atoms take (dens.py:101-107): on z
       // the slots where the reference's corner copy lands for y on L / on H
