@@ -63,6 +63,9 @@ def fft_bytes(n, fp64=False):
         "y": 2 * c * vol + res * vol + 32 * plane,
         # read rho + residuals, read and write sf; the Nyquist plane into sf_nyq
         "x": c * vol + res * vol + 16 * vol + 16 * plane + 16 * plane,
+        # x with frames in pairs (the frame loop's form), per frame: the frame's
+        # rho + residuals + Nyquist plane, half of one read and write of sf / sf_nyq
+        "x_pair": c * vol + res * vol + 8 * vol + 16 * plane + 8 * plane,
     }
 
 
@@ -146,6 +149,25 @@ def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream):
     plan.check_errors()
     return {"prepare": st["prepare"], "deposit_kernel": st["deposit"], "fft_z": st["fft_z"],
             "fft_y": st["fft_y"], "fft_x_acc": st["fft_x_acc"]}
+
+
+def paired_x(plan, d_coords, d_types, d_scale, box, s, npairs):
+    """The x pass as the frame loop runs it, two frames per launch: two-frame
+    calls queued back to back with stage timing.  The pair's launch is timed as
+    the second frame's x pass (with the kernel's own dispatch) and the first
+    frame's as zero, so the average is the per-frame cost.  The launch runs
+    alone: the next call starts after the previous one has joined its stream."""
+    import torch
+    plan.set_stage_timing(1)
+    torch.cuda.synchronize()
+    for k in range(npairs):
+        plan.accumulate_frames(d_coords[2 * k:2 * k + 2], d_types, d_scale[2 * k:2 * k + 2], box,
+                               s.arith, s.wrap)
+    st = plan.stage_times()
+    plan.set_stage_timing(0)
+    torch.cuda.synchronize()
+    plan.check_errors()
+    return st["fft_x_acc"]
 
 
 def adjoint_leg(rad, iters):
@@ -441,6 +463,10 @@ def main():
     if not args.no_isolated:
         iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, min(args.steps, 30), stream)
         log("per-stage avg ms (isolated): " + json.dumps({k: round(v, 4) for k, v in iso.items()}))
+        if args.steps >= 2:
+            iso["fft_x_pair"] = paired_x(plan, d_coords, d_types, d_scale, box, s,
+                                         min(args.steps // 2, 15))
+            log(f"x pass per frame, frames in pairs: {iso['fft_x_pair']:.4f} ms")
 
     n = [int(x) for x in s.N]
     fb = fft_bytes(n, args.fft == "fp64")
@@ -489,10 +515,16 @@ def main():
             k: {"achieved": round(fb[k] / (iso[st] * 1e-3) / 1e9, 1),
                 "frac": round(fb[k] / (iso[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "bytes": fb[k], "avg_ms": round(iso[st], 4)}
-            for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"))}
+            for k, st in (("z", "fft_z"), ("y", "fft_y"), ("x", "fft_x_acc"), ("x_pair", "fft_x_pair"))
+            if st in iso}
         out["fft_passes_note"] = ("each pass alone (one frame at a time; HIP events recorded "
-                                  "with the pass kernel's own dispatch); algorithmic bytes / "
-                                  "time vs 8 TB/s")
+                                  "with the pass kernel's own dispatch); x_pair: the x pass as "
+                                  "the frame loop runs it, two frames per launch, per frame; "
+                                  "algorithmic bytes / time vs 8 TB/s")
+        if "fft_x_pair" in iso:
+            out["fft_ms_per_frame"] = {
+                "one_frame": round(iso["fft_z"] + iso["fft_y"] + iso["fft_x_acc"], 4),
+                "frame_loop": round(iso["fft_z"] + iso["fft_y"] + iso["fft_x_pair"], 4)}
     oracle_sf = None
     if not args.no_cpu:
         log("cpu baseline: frame 0 through the FP64 numpy oracle on one core")

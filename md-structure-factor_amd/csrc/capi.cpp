@@ -543,6 +543,7 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   // small ones the cross-stream waits cost more than the branches overlap
   const long long ntiles = (long long)p->sys.ntiles[0] * p->sys.ntiles[1] * p->sys.ntiles[2];
   const bool split = ntiles >= kSplitPrepTiles;
+  const bool pair = nframes >= 2 && fft_x_pairs(*p);
   // timed frames accumulate over calls until mdsf_stage_times reads them
   // (frame t of this call is timed frame tb + t), so that one-frame calls
   // can be queued back to back
@@ -598,19 +599,35 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(launch_parked(*p, p->rho, d));
     MDSF_HIP(hipEventRecord(p->ev_dep[b], d));
     MDSF_HIP(hipStreamWaitEvent(f, p->ev_dep[b], 0));
+    // x passes in pairs: frame t (even) waits for frame t + 1, and one launch
+    // adds both frames' |F|^2, reading and writing the FP64 sums once; frame
+    // t's slot is released with frame t + 1's (preparing and depositing frame
+    // t + kSlots wait for it)
+    const bool defer = pair && (t % 2) == 0 && t + 1 < nframes;
+    const bool paired = pair && (t % 2) == 1;
     for (int pass = 0; pass < 3; ++pass) {
       // with stage timing, the pass's stream kernel records the pass events
-      // with its own dispatch (the marks stand when another kernel runs it)
+      // with its own dispatch (the marks stand when another kernel runs it);
+      // a deferred x pass is timed as zero, its pair's launch as frame t + 1's
       MDSF_HIP(mark(t, 4 + 2 * pass, f));
+      if (pass == 2 && defer) {
+        MDSF_HIP(mark(t, 5 + 2 * pass, f));
+        continue;
+      }
       if (p->timing == 1)
         fft_pass_events(p->tev[10 * (tb + t) + 4 + 2 * pass], p->tev[10 * (tb + t) + 5 + 2 * pass]);
-      const hipError_t fe = fft_pass(*p, pass, p->rho, f);
+      const hipError_t fe = (pass == 2 && paired)
+                                ? fft_x_pair(*p, (int)((t - 1) % kSlots), b, f)
+                                : fft_pass(*p, pass, p->rho, f);
       const bool took = fft_pass_events_used();
       fft_pass_events(nullptr, nullptr);
       MDSF_HIP(fe);
       if (!took) MDSF_HIP(mark(t, 5 + 2 * pass, f));
     }
-    MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
+    if (!defer) {
+      if (paired) MDSF_HIP(hipEventRecord(p->ev_fft[(t - 1) % kSlots], f));
+      MDSF_HIP(hipEventRecord(p->ev_fft[b], f));
+    }
   }
   MDSF_HIP(hipStreamWaitEvent(s, p->ev_fft[(nframes - 1) % kSlots], 0));
   return MDSF_OK;

@@ -367,6 +367,13 @@ struct PassArgs {
   long long nq_p, nq_e;
   int nq_pencils;
   int tpb;                      // tiles per block of the register-staged pass
+  // EPI_ACC, register-staged pass: a second spectrum (the next frame's, with
+  // its residuals and Nyquist plane) transformed by the same blocks, its
+  // |X|^2 added after the first one's, so that the FP64 sums are read and
+  // written once for two frames (null: one spectrum)
+  const typename V2<R>::T* in2;
+  const unsigned char* res_in2;
+  double2* nyq2;
 };
 
 // One Nyquist-plane pencil per block, every stage in LDS (FP64).
@@ -565,12 +572,18 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
   };
   T2 raw[16];
   uint4 rq;  // RES_IN: bytes 16 tid .. 16 tid + 15 of the tile's residuals (P whole rows)
-  auto load = [&](long long tile) __attribute__((always_inline)) {
+  // EPI_ACC with a second spectrum: each tile is transformed twice (sub-tile
+  // h = 0: a.in, h = 1: a.in2), into the same accumulator values
+  const int nsub = (EPI == EPI_ACC && a.in2) ? 2 : 1;
+  // sub-tile vt = nsub tile + h
+  auto load = [&](long long vt) __attribute__((always_inline)) {
+    const long long tile = nsub == 2 ? vt >> 1 : vt;
+    const bool second = nsub == 2 && (vt & 1);
     const unsigned s = (unsigned)(tile / tiles_f);
     const int f0 = (int)((unsigned)tile - s * tiles_f) * P;
     const int pc = min(P, g.nf - f0);
     const long long ib = (long long)s * g.in_s + (long long)f0 * g.in_f;
-    const T2* src = a.in + ib;
+    const T2* src = (second ? a.in2 : a.in) + ib;
 #pragma unroll
     for (int t = 0; t < 16 / R1; ++t) {
       unsigned o;
@@ -581,22 +594,33 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
     if constexpr ((RES & RES_IN) != 0) {
       // past a partial tile's rows: the tile's first bytes (never used)
       const unsigned q = 16u * (unsigned)tid < (unsigned)(pc * n) ? (unsigned)tid : 0u;
-      rq = at(reinterpret_cast<const uint4*>(a.res_in + ib), q);
+      rq = at(reinterpret_cast<const uint4*>((second ? a.res_in2 : a.res_in) + ib), q);
     }
   };
   const long long nblk = (ntiles + a.tpb - 1) / a.tpb;
   if ((long long)blockIdx.x >= nblk) {
-    // the Nyquist-plane pencils: one per block after the tile blocks
+    // the Nyquist-plane pencils: one per block after the tile blocks (the
+    // second spectrum's after the first's, into the same sums)
     if (a.nyq) nyq_pencil<INV, EPI>(a, (int)((long long)blockIdx.x - nblk), plane);
+    if constexpr (EPI == EPI_ACC) {
+      if (a.nyq2) {
+        PassArgs<R> b = a;
+        b.nyq = a.nyq2;
+        nyq_pencil<INV, EPI>(b, (int)((long long)blockIdx.x - nblk), plane);
+      }
+    }
     return;
   }
-  long long tile = (long long)blockIdx.x * a.tpb;
-  const long long tend = min(ntiles, tile + a.tpb);
-  if (PF) load(tile);
+  const long long vend = min(ntiles, (long long)(blockIdx.x + 1) * a.tpb) * nsub;
+  long long vt = (long long)blockIdx.x * a.tpb * nsub;
+  if (PF) load(vt);
   for (int i = threadIdx.x; i < n; i += kFftThreads) twl[i] = a.ax.tw64[i];
-  for (; tile < tend; ++tile) {
+  double accv[EPI == EPI_ACC ? 16 : 1];  // this tile's accumulator values (EPI_ACC)
+  for (; vt < vend; ++vt) {
     tid = opaque(tid);
-    if (!PF) load(tile);
+    if (!PF) load(vt);
+    const long long tile = nsub == 2 ? vt >> 1 : vt;
+    const bool first = nsub == 2 && !(vt & 1);  // a first spectrum's sub-tile (accumulator kept)
     const unsigned s = (unsigned)(tile / tiles_f);
     const int f0 = (int)((unsigned)tile - s * tiles_f) * P;
     const int pc = min(P, g.nf - f0);
@@ -631,21 +655,23 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
         for (int r = 0; r < R1; ++r) v[t * R1 + r] = cscale(at(a.w + ib, o + (unsigned)(r * nq1) * in_e), v[t * R1 + r]);
       }
     }
-    if (PF && tile + 1 < tend) load(tile + 1);
+    if (PF && vt + 1 < vend) load(vt + 1);
     // the accumulator values this tile adds to, loaded now so that their
     // latency passes during the transform (pencils past a partial tile read
-    // the last one and are not stored)
-    double accv[EPI == EPI_ACC ? 16 : 1];
+    // the last one and are not stored); with two spectra, loaded with the
+    // first and kept in registers through the second
     if constexpr (EPI == EPI_ACC) {
+      if (!(nsub == 2 && (vt & 1))) {
 #pragma unroll
-      for (int t = 0; t < 16 / RL; ++t) {
-        const int b = tid + t * kFftThreads;
-        int p = OUT_E ? b / nsL : b % P;
-        const int k = OUT_E ? b % nsL : b / P;
-        p = p < pc ? p : pc - 1;
-        const unsigned o = (unsigned)p * out_f + (unsigned)k * out_e;
+        for (int t = 0; t < 16 / RL; ++t) {
+          const int b = tid + t * kFftThreads;
+          int p = OUT_E ? b / nsL : b % P;
+          const int k = OUT_E ? b % nsL : b / P;
+          p = p < pc ? p : pc - 1;
+          const unsigned o = (unsigned)p * out_f + (unsigned)k * out_e;
 #pragma unroll
-        for (int r = 0; r < RL; ++r) accv[t * RL + r] = at(a.acc + ob, o + (unsigned)(r * nsL) * out_e);
+          for (int r = 0; r < RL; ++r) accv[t * RL + r] = at(a.acc + ob, o + (unsigned)(r * nsL) * out_e);
+        }
       }
     }
     // stage 1 (span 1): DFT_R1; output position j R1 + r
@@ -698,7 +724,9 @@ __global__ __launch_bounds__(kFftThreads) void k_pass_stream(PassArgs<R> a) {
         for (int r = 0; r < RL; ++r) {
           const unsigned oe = o + (unsigned)(r * nsL) * out_e;
           if constexpr (EPI == EPI_ACC) {
-            at(a.acc + ob, oe) = accv[t * RL + r] + (x[r].x * x[r].x + x[r].y * x[r].y);
+            const double sum = accv[t * RL + r] + (x[r].x * x[r].x + x[r].y * x[r].y);
+            if (first) accv[t * RL + r] = sum;
+            else at(a.acc + ob, oe) = sum;
           } else {
             T2 st;
             st.x = (R)x[r].x;
@@ -1349,13 +1377,13 @@ hipError_t pass_y(Plan& p, hipStream_t s) {
 // x pass: rho[kz][ky][0..Nx) in place (F) or |F|^2 into sf_main; the Nyquist
 // plane nyq[:][y] in place or into sf_nyq
 template <typename R>
-hipError_t pass_x(Plan& p, bool accumulate, hipStream_t s) {
+PassArgs<R> x_args(Plan& p, void* rho, void* res, void* nyq) {
   const int nx = p.sys.n[0], ny = p.sys.n[1], m = p.sys.n[2] / 2;
   PassArgs<R> a = base_args<R>(p, nx, p.ax_x);
-  a.in = cx<R>(p.rho);
-  a.out = cx<R>(p.rho);
+  a.in = cx<R>(rho);
+  a.out = cx<R>(rho);
   a.acc = p.sf_main;
-  a.res_in = static_cast<const unsigned char*>(p.res);
+  a.res_in = static_cast<const unsigned char*>(res);
   a.g.in_s = 0;
   a.g.in_f = nx;                         // fast = row kz * Ny + ky
   a.g.in_e = 1;
@@ -1364,12 +1392,30 @@ hipError_t pass_x(Plan& p, bool accumulate, hipStream_t s) {
   a.g.out_e = 1;
   a.g.nf = m * ny;
   a.g.blocks = 1;
-  a.nyq = static_cast<double2*>(p.spec_nyq);
+  a.nyq = static_cast<double2*>(nyq);
   a.nyq_acc = p.sf_nyq;
   a.nq_p = 1;
   a.nq_e = ny;
   a.nq_pencils = ny;
-  return run_pass<R>(a, accumulate ? PK_X_ACC : PK_X_STORE, s, p.tune_x);
+  return a;
+}
+
+template <typename R>
+hipError_t pass_x(Plan& p, bool accumulate, hipStream_t s) {
+  return run_pass<R>(x_args<R>(p, p.rho, p.res, p.spec_nyq), accumulate ? PK_X_ACC : PK_X_STORE, s,
+                     p.tune_x);
+}
+
+// Both frames' x passes with |F|^2 into sf_main / sf_nyq, the frame in slot
+// sa first: one read and one write of the FP64 sums for the two frames, the
+// same additions in the same order as two single passes.
+template <typename R>
+hipError_t pass_x_pair(Plan& p, int sa, int sb, hipStream_t s) {
+  PassArgs<R> a = x_args<R>(p, p.rho_buf[sa], p.res_buf[sa], p.nyq_buf[sa]);
+  a.in2 = cx<R>(p.rho_buf[sb]);
+  a.res_in2 = static_cast<const unsigned char*>(p.res_buf[sb]);
+  a.nyq2 = static_cast<double2*>(p.nyq_buf[sb]);
+  return run_pass<R>(a, PK_X_ACC, s, p.tune_x);
 }
 
 template <typename R>
@@ -1641,6 +1687,16 @@ void fft_pass_events(hipEvent_t begin, hipEvent_t end) {
 }
 
 bool fft_pass_events_used() { return t_pass_ev_used; }
+
+bool fft_x_pairs(const Plan& p) {
+  int r1, r2, r3;
+  return (p.tune_x == 0 || p.tune_x == kTuneReg) && reg_split(p.sys.n[0], &r1, &r2, &r3);
+}
+
+hipError_t fft_x_pair(Plan& p, int slot_a, int slot_b, hipStream_t s) {
+  return p.fft64 ? pass_x_pair<double>(p, slot_a, slot_b, s)
+                 : pass_x_pair<float>(p, slot_a, slot_b, s);
+}
 
 hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s) {
   switch (pass) {

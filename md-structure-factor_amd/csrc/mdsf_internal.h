@@ -320,6 +320,11 @@ hipError_t fft_inverse(Plan& p, const double* w_main, const double* w_nyq, void*
                        hipStream_t s);
 // one forward pass: 0 = z (R2C from `density`), 1 = y, 2 = x with |F|^2 accumulation
 hipError_t fft_pass(Plan& p, int pass, const void* density, hipStream_t s);
+// Whether the plan's x pass can take two frames in one launch, and that
+// launch: |F|^2 of the frame in slot_a, then of the one in slot_b, into the
+// S(q) sums (the same FP64 additions in the same order as two x passes).
+bool fft_x_pairs(const Plan& p);
+hipError_t fft_x_pair(Plan& p, int slot_a, int slot_b, hipStream_t s);
 // events (or null) that the next pass's stream kernel records with its own
 // dispatch on this thread, and whether a launch took them
 void fft_pass_events(hipEvent_t begin, hipEvent_t end);

@@ -1,8 +1,10 @@
-"""The pipelined frame loop (three streams, two frame slots) against the same
-frames run stage by stage on one stream: S(q) must be bit-identical (every
-kernel is deterministic and the S(q) sum keeps frame order), for both plan
-precisions and for an odd frame count; the stage timing API reports positive
-per-stage averages."""
+"""The pipelined frame loop (three streams, three frame slots) against the
+same frames run stage by stage on one stream: S(q) must be bit-identical
+(every kernel is deterministic and the S(q) sum keeps frame order), for both
+plan precisions and for an odd frame count, on a grid whose x pass takes two
+frames per launch (64: the FP64 sums read and written once per pair, with the
+additions in frame order) and one whose x pass does not (80); the stage timing
+API reports positive per-stage averages."""
 import numpy as np
 import pytest
 import torch
@@ -10,13 +12,14 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("grid", [80, 64])
 @pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("nframes", [1, 2, 5])
-def test_pipelined_loop_matches_sequential(nframes, fp64, rad, gpu):
+def test_pipelined_loop_matches_sequential(nframes, fp64, grid, rad, gpu):
     from mdsf import engine, synth
     coords, dims, typ = synth.uniform(7, 6000, 36.0, ["H", "C", "O", "Na"], nframes)
-    s = engine.Setup(coords, dims, typ, rad, 36.0 / 80.5, False, np.eye(3))
-    assert list(s.N) == [80, 80, 80]
+    s = engine.Setup(coords, dims, typ, rad, 36.0 / (grid + 0.5), False, np.eye(3))
+    assert list(s.N) == [grid, grid, grid]
     dev = torch.device("cuda", 0)
     types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
     scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
