@@ -579,10 +579,17 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(mark(t, 0, s));
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
-    MDSF_HIP(launch_binning(*p, s, false, split ? p->aux_stream : nullptr, split ? d : nullptr));
+    // the deposition waits for the tile-list branch itself (one cross-stream
+    // wait instead of two in a row through `s`), except under stage timing,
+    // whose prepare interval ends when every branch has
+    const bool join = p->timing == 1;
+    MDSF_HIP(launch_binning(*p, s, false, split ? p->aux_stream : nullptr, split ? d : nullptr,
+                            join));
     MDSF_HIP(mark(t, 1, s));
     MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));
+    if (split && !join && p->tl_cap > 0 && p->sys.natoms > 0)
+      MDSF_HIP(hipStreamWaitEvent(d, p->ev_ljoin, 0));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_fft[b], 0));
     // the deposition's events are recorded with the tiled kernel's own
     // dispatch (its begin and end, without the queue's dispatch latency)

@@ -524,6 +524,27 @@ __device__ __forceinline__ void column_at(const int* __restrict__ boff, const SH
   type = t;
 }
 
+// Bucket column cidx's length from the bucket counts themselves (before the
+// bucket scan has turned them into offsets): the sum over its z buckets.
+template <class SH>
+__device__ __forceinline__ int column_count(const int* __restrict__ bcount, const SH& sh,
+                                            int cidx) {
+  int t = 0;
+  while (sh.col_base[t + 1] <= cidx) ++t;
+  const TypeRanges& r = sh.tr[t];
+  const int ci = cidx - sh.col_base[t];
+  const int zr = r.rn[2] == 2 ? (ci & 1) : 0;
+  const int cxy = r.rn[2] == 2 ? (ci >> 1) : ci;
+  const int bxi = cxy / r.ny, byi = cxy - bxi * r.ny;
+  const int bx = range_item(r, 0, bxi), by = range_item(r, 1, byi);
+  const int col = r.bucket_base + (bx * r.nbk1 + by) * r.nbk2;
+  const int zlo = zr == 0 ? r.lo[2][0] : r.lo[2][1];
+  const int zhi = zr == 0 ? r.hi[2][0] : r.hi[2][1];
+  int len = 0;
+  for (int z = zlo; z <= zhi; ++z) len += bcount[col + z];
+  return len;
+}
+
 // Candidate g of the current column round (col_s / col_pre / col_t): its
 // sorted index and type.
 template <class SH>
@@ -1331,9 +1352,11 @@ void k_deposit_mfma(SysDev sys, const TypeDev* __restrict__ td,
 //   x  regular atom: float offset of its slices; corner atom: sorted index
 //   y  slice index j_d of the tile on axis d in byte d (kListNoSlice: none),
 //      type in bits 24..29
+// The counts come from the bucket counts of k_prep, so that the regions can
+// be placed while the bucket scan runs.
 template <int MT>
 __global__ __launch_bounds__(kThreads) void k_tile_count(SysDev sys, const TypeDev* __restrict__ td,
-                                                         const int* __restrict__ boff,
+                                                         const int* __restrict__ bcount,
                                                          int4* __restrict__ heads) {
   __shared__ EnumShared<MT> sh;
   const int tid = threadIdx.x;
@@ -1342,8 +1365,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_count(SysDev sys, const TypeD
   const int total_cols = enum_setup(sys, td, tl, sh);
   int nc = 0;
   for (int cbase = 0; cbase < total_cols; cbase += kThreads) {
-    int cs = 0, len = 0, ct = 0;
-    if (cbase + tid < total_cols) column_at(boff, sh, cbase + tid, cs, len, ct);
+    const int len = cbase + tid < total_cols ? column_count(bcount, sh, cbase + tid) : 0;
     int excl;
     nc += block_scan(sh, len, &excl);
   }
@@ -1400,8 +1422,13 @@ __global__ __launch_bounds__(kThreads) void k_tile_lists(SysDev sys, const TypeD
                                                          const int4* __restrict__ s_ir,
                                                          int2* __restrict__ pool, int cap,
                                                          int4* __restrict__ heads,
-                                                         int* __restrict__ err) {
+                                                         int* __restrict__ err,
+                                                         int* __restrict__ bcount, int nbk) {
   __shared__ EnumShared<MT> sh;
+  // the bucket counts, read by k_tile_count and the bucket scan before this
+  // kernel, start the slot's next frame at zero
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < nbk; i += gridDim.x * kThreads)
+    bcount[i] = 0;
   __shared__ struct {
     int slice_base[MT], slice_stride[MT], type_start[MT], count[MT], nsl[MT][3];
   } tyl;
@@ -2372,9 +2399,10 @@ hipError_t launch_tile_regions(Plan& p, hipStream_t s) {
   if (p.tl_cap == 0 || ntiles == 0 || sys.natoms == 0) return hipSuccess;
   const dim3 grid((unsigned)ntiles), block(kThreads);
   if (sys.ntypes <= 16)
-    hipLaunchKernelGGL(k_tile_count<16>, grid, block, 0, s, sys, p.types, p.bucket_off, p.tl_head);
+    hipLaunchKernelGGL(k_tile_count<16>, grid, block, 0, s, sys, p.types, p.bucket_count,
+                       p.tl_head);
   else
-    hipLaunchKernelGGL(k_tile_count<kMaxTypes>, grid, block, 0, s, sys, p.types, p.bucket_off,
+    hipLaunchKernelGGL(k_tile_count<kMaxTypes>, grid, block, 0, s, sys, p.types, p.bucket_count,
                        p.tl_head);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s, p.tl_head, (int)ntiles);
   return hipGetLastError();
@@ -2387,10 +2415,12 @@ hipError_t launch_tile_lists(Plan& p, hipStream_t s) {
   const dim3 grid((unsigned)ntiles), block(kThreads);
   if (sys.ntypes <= 16)
     hipLaunchKernelGGL(k_tile_lists<16>, grid, block, 0, s, sys, p.types, p.bucket_off, p.s_ir,
-                       p.tl_pool, (int)p.tl_cap, p.tl_head, p.err);
+                       p.tl_pool, (int)p.tl_cap, p.tl_head, p.err, p.bucket_count,
+                       sys.nbuckets);
   else
     hipLaunchKernelGGL(k_tile_lists<kMaxTypes>, grid, block, 0, s, sys, p.types, p.bucket_off,
-                       p.s_ir, p.tl_pool, (int)p.tl_cap, p.tl_head, p.err);
+                       p.s_ir, p.tl_pool, (int)p.tl_cap, p.tl_head, p.err, p.bucket_count,
+                       sys.nbuckets);
   return hipGetLastError();
 }
 

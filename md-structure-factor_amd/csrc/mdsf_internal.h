@@ -287,11 +287,12 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
 // sorts the prepared atoms and builds their tile slices; with `derivs` (and
 // dslices allocated) also the slices' derivatives for the tiled adjoint.
 // With `aux`, the tile lists are built on that stream, in parallel with the
-// rest, and joined back into `s`; with `side`, the corner list and the spill
-// staging go on that stream (the one the frame's deposition then runs on),
-// after the sorted atoms are ready.
+// rest, and joined back into `s` (without `join`, the caller waits for
+// p.ev_ljoin on the stream that reads them); with `side`, the corner list and
+// the spill staging go on that stream (the one the frame's deposition then
+// runs on), after the sorted atoms are ready.
 hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false, hipStream_t aux = nullptr,
-                          hipStream_t side = nullptr);
+                          hipStream_t side = nullptr, bool join = true);
 // only the slice derivatives of the last sorted frame
 hipError_t launch_slice_derivs(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);

@@ -134,9 +134,12 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int* total) {
   return off + x - v;
 }
 
+// clear: zero the counts for the slot's next frame (plans with tile lists
+// leave them to k_tile_lists, after k_tile_count has read them)
 __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int* __restrict__ bcount,
                                                               int* __restrict__ boff,
-                                                              int* __restrict__ aux, int nbk) {
+                                                              int* __restrict__ aux, int nbk,
+                                                              int clear) {
   __shared__ int wsum[kScanThreads / 64];
   const int i = blockIdx.x * kScanThreads + threadIdx.x;
   const int v = i < nbk ? bcount[i] : 0;
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(int* __restrict__ 
   const int ex = block_excl_scan(v, wsum, &total);
   if (i < nbk) {
     boff[i] = ex;
-    bcount[i] = 0;
+    if (clear) bcount[i] = 0;
   }
   if (threadIdx.x == 0) aux[blockIdx.x] = total;
 }
@@ -493,21 +496,18 @@ hipError_t launch_slice_derivs(Plan& p, hipStream_t s) {
 }
 
 hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux,
-                          hipStream_t side) {
+                          hipStream_t side, bool join) {
   const int n = p.sys.natoms;
   const int nbk = p.sys.nbuckets;
   const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
   if (nblk == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
-                     p.bucket_off, p.bucket_aux, nbk);
-  hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off,
-                     p.bucket_cur, p.bucket_aux, nbk, nblk, p.corners);
-  if (n == 0) return hipGetLastError();
-  // Branches after the bucket scan (when the streams are given): the tile
-  // lists on `aux` (their regions need only the bucket offsets, their
-  // entries the sorted atoms), the corner list and the spill staging on
-  // `side` (the stream whose deposition reads them), the slices here.
-  const bool split = aux && p.tl_cap > 0;
+  // Branches (when the streams are given): the tile lists on `aux` (their
+  // regions need only the bucket counts, from here on; their entries the
+  // sorted atoms), the corner list and the spill staging on `side` (the
+  // stream whose deposition reads them), the bucket scan, the sort and the
+  // slices here.
+  const bool lists = p.tl_cap > 0 && n > 0;
+  const bool split = aux && lists;
   hipError_t e = hipSuccess;
   if (split) {
     if ((e = hipEventRecord(p.ev_lfork, s)) != hipSuccess ||
@@ -515,6 +515,11 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux,
         (e = launch_tile_regions(p, aux)) != hipSuccess)
       return e;
   }
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_count,
+                     p.bucket_off, p.bucket_aux, nbk, lists ? 0 : 1);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanThreads), 0, s, p.bucket_off,
+                     p.bucket_cur, p.bucket_aux, nbk, nblk, p.corners);
+  if (n == 0) return hipGetLastError();
   hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, p.key, p.bucket_cur,
                      p.s_tmp, n);
   hipLaunchKernelGGL(k_rank_gather, dim3((n + 255) / 256), dim3(256), 0, s, p.key,
@@ -554,7 +559,7 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux,
       hipLaunchKernelGGL(k_tables<float>, grid, block, 0, s, p.sys, p.types, p.s_ir, p.s_r,
                          static_cast<float*>(p.tables), total);
   }
-  if (split && (e = hipStreamWaitEvent(s, p.ev_ljoin, 0)) != hipSuccess) return e;
+  if (split && join && (e = hipStreamWaitEvent(s, p.ev_ljoin, 0)) != hipSuccess) return e;
   return hipGetLastError();
 }
 
