@@ -39,7 +39,7 @@ METRIC = "frames/s for S(q) at 256^3, 50k atoms"
 # HBM bytes per deposition launch on this workload, from two rocprofv3 --pmc
 # runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by the same
 # counters on a 256 MiB device copy (FETCH x 2.000, WRITE x 1.000)
-DEPOSIT_TRAFFIC = 144391168
+DEPOSIT_TRAFFIC = 143718400
 
 
 def log(msg):
