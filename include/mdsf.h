@@ -113,7 +113,10 @@ int mdsf_check_errors(mdsf_plan* plan, mdsf_stream stream);
  *   d_occupancy optional uint32[Nx*Ny*Nz]: += 1 at (ir mod N) per atom and frame.
  *   d_home_cells optional int32[natoms*3]: ir of the LAST frame (dens.py:285);
  *              INT32_MIN for a non-finite coordinate or |r / dr| >= 1e9.
- * Adds sum_frames |rfftn(folded density)|^2 into the plan accumulator. */
+ * Adds sum_frames |rfftn(folded density)|^2 into the plan accumulator, in
+ * frame order (the x passes of frames 2k and 2k + 1 may share one launch,
+ * which adds the two frames' terms one after the other: the same result as
+ * frame by frame). */
 int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype,
                            int64_t nframes, const int32_t* d_types,
                            const double* d_scale, const double box[3], int arith,
@@ -187,7 +190,9 @@ int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which);
  * Timed frames accumulate over calls; mdsf_stage_times waits for them and
  * returns the average milliseconds per frame of {prepare, deposit, z pass,
  * y pass, x pass + |F|^2 sum} in avg_ms[5] (0 for the stages not timed),
- * then starts the count over.  No counterpart in the reference. */
+ * then starts the count over.  An x pass shared by two frames is timed as the
+ * second frame's and the first frame's as zero, so the average is the cost
+ * per frame.  No counterpart in the reference. */
 int mdsf_set_stage_timing(mdsf_plan* plan, int32_t enable);
 int mdsf_stage_times(mdsf_plan* plan, double* avg_ms);
 
