@@ -609,9 +609,14 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     // x passes in pairs: frame t (even) waits for frame t + 1, and one launch
     // adds both frames' |F|^2, reading and writing the FP64 sums once; frame
     // t's slot is released with frame t + 1's (preparing and depositing frame
-    // t + kSlots wait for it)
-    const bool defer = pair && (t % 2) == 0 && t + 1 < nframes;
-    const bool paired = pair && (t % 2) == 1;
+    // t + kSlots wait for it).  The last two frames of a longer call are not
+    // paired: both x passes would wait for the last y pass and lengthen the
+    // call's tail by about one x pass.
+    auto defers = [&](int64_t u) {
+      return pair && (u % 2) == 0 && (u + 2 < nframes || nframes == 2);
+    };
+    const bool defer = defers(t);
+    const bool paired = t > 0 && defers(t - 1);
     for (int pass = 0; pass < 3; ++pass) {
       // with stage timing, the pass's stream kernel records the pass events
       // with its own dispatch (the marks stand when another kernel runs it);

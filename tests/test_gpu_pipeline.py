@@ -3,8 +3,9 @@ same frames run stage by stage on one stream: S(q) must be bit-identical
 (every kernel is deterministic and the S(q) sum keeps frame order), for both
 plan precisions and for an odd frame count, on a grid whose x pass takes two
 frames per launch (64: the FP64 sums read and written once per pair, with the
-additions in frame order) and one whose x pass does not (80); the stage timing
-API reports positive per-stage averages."""
+additions in frame order; 4 frames: one pair, then the last two unpaired) and
+one whose x pass does not (80); the stage timing API reports positive
+per-stage averages."""
 import numpy as np
 import pytest
 import torch
@@ -14,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("grid", [80, 64])
 @pytest.mark.parametrize("fp64", [False, True])
-@pytest.mark.parametrize("nframes", [1, 2, 5])
+@pytest.mark.parametrize("nframes", [1, 2, 4, 5])
 def test_pipelined_loop_matches_sequential(nframes, fp64, grid, rad, gpu):
     from mdsf import engine, synth
     coords, dims, typ = synth.uniform(7, 6000, 36.0, ["H", "C", "O", "Na"], nframes)
