@@ -22,7 +22,7 @@ import torch
 
 from conftest import load_golden
 from oracle import dens_oracle as O
-from test_gpu_coverage import _density_check, _monoclinic, _sq_dropin
+from test_gpu_coverage import _density_check, _many_types_system, _monoclinic, _sq_dropin
 from test_gpu_parity import _corner_case, _gpu, assert_sq_close
 
 pytestmark = pytest.mark.gpu
@@ -81,6 +81,51 @@ def test_xy_lists_density_and_sq(theta, rad, gpu, tmp_path):
     print(f"  S(q) max rel {err.max():.2e} median {np.median(err):.2e}", flush=True)
 
 
+def test_xy_lists_many_parameter_sets(rad, gpu, tmp_path):
+    """40 parameter sets (the list kernel's per-type layout sized for 64
+    types) in a 60 A cell at Sres 0.6 (N = 100, windows up to 56 cells, corner
+    atoms): density of two frames and S(q) against the oracle."""
+    from mdsf import engine
+    coords, dims, typ = _many_types_system(33, frames=2, box=60.0)
+    ucell = _monoclinic(120.0)
+    s = engine.Setup(coords, dims, typ, rad, 0.6, False, ucell)
+    assert len(s.amp) == 40 and xy_lists_taken(s)
+    _density_check(coords, dims, typ, rad, 0.6, ucell, False, frames=(0, 1))
+    want = O.compute_sf(np.array(coords), np.array(dims), typ, None, rad, ucell, 0.6,
+                        better=False)["sf"]
+    _, got = _sq_dropin(coords, dims, typ, rad, 0.6, ucell, "fp32", tmp_path)
+    assert_sq_close(got["sf"], want)
+
+
+def test_xy_lists_adjoint_frames(rad, gpu):
+    """Forward (the xy list deposition) and backward (the gather adjoint of
+    xy-coupled cells) over two frames of a 120-degree system on the lists,
+    against autograd of the torch reference (tolerances of
+    test_gpu_adjoint.py: S(q) 1e-4 relative, gradient 2e-4 of its maximum)."""
+    import sf_torch_ref as T
+    from mdsf.autograd import SFModel
+    coords, dims, typ = _xy_system(63, frames=2, n=200)
+    coords = coords.astype(np.float64)
+    ucell = _monoclinic(120.0)
+    model = SFModel(coords, dims, typ, rad, SRES, ucell=ucell, fp64=False)
+    from mdsf import engine
+    assert xy_lists_taken(engine.Setup(np.array(coords), dims, typ, rad, SRES, False, ucell))
+    rng = np.random.default_rng(4)
+    x = torch.tensor(coords, device=model.plan.device, requires_grad=True)
+    S = model(x)
+    w = torch.tensor(rng.normal(size=tuple(S.shape)), device=S.device) / S.detach().max()
+    torch.sum(w * S).backward()
+    got = x.grad.cpu().numpy()
+    xr = torch.tensor(coords, requires_grad=True)
+    Sr = T.structure_factor(xr, dims, typ, rad, SRES, ucell)
+    np.testing.assert_allclose(S.detach().cpu().numpy(), Sr.detach().numpy(), rtol=1e-4,
+                               atol=1e-6 * float(Sr.max()))
+    torch.sum(w.cpu() * Sr).backward()
+    want = xr.grad.numpy()
+    err = float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    assert err <= 2e-4, err
+
+
 def test_xy_lists_reproducible(rad, gpu):
     coords, dims, typ = _xy_system(62, frames=1)
     args = (coords, dims, typ, rad, SRES, False, _monoclinic(120.0))
@@ -96,7 +141,8 @@ def test_xy_lists_reproducible(rad, gpu):
 
 
 def test_xy_kernel_choice(rad):
-    """Both xy kernels are reached: the small-grid parity cases keep the
+    """Both xy kernels are reached, with 16 and with 64 types: the small-grid
+    parity cases and the 40-type system of test_gpu_coverage.py keep the
     enumerating kernel (a window wider than N - 16 cells), FP64 plans never
     take lists, and the 120-degree HII workload of the bench takes them."""
     from mdsf import engine, synth
@@ -108,6 +154,8 @@ def test_xy_kernel_choice(rad):
     r, dims, typ, ucell = _corner_case(rng, np.array([7.0, 8.0, 9.0]), 2, ["C", "O", "H", "N"],
                                        120.0)
     assert not xy_lists_taken(engine.Setup(r, dims, typ, rad, 0.3, True, ucell))
+    c40, d40, t40 = _many_types_system(31)
+    assert not xy_lists_taken(engine.Setup(c40, d40, t40, rad, 0.6, False, _monoclinic(120.0)))
     coords, dims, typ = _xy_system(61, frames=1)
     s = engine.Setup(np.array(coords), dims, typ, rad, SRES, False, _monoclinic(120.0))
     assert xy_lists_taken(s) and not xy_lists_taken(s, fp64=True)
