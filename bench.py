@@ -8,10 +8,11 @@ axis), FP32 deposition and FFT, FP64 frame sum of |F|^2.
 One step = one frame through the whole path (rescale/wrap, binning, factor
 tables, deposition + fold, z/y/x FFT passes, |F|^2 accumulation), with the
 trajectory already resident in HBM.  The timed region is the library's own
-pipelined frame loop without stage events; a second pass runs frames one at
-a time with the library's stage timing (the deposition kernel and each FFT
-pass kernel timed by events recorded with their own dispatch) for the
-roofline and the per-pass FFT figures.  The CPU
+pipelined frame loop without stage events; a second pass runs 30 frames one
+at a time (after 100 untimed ones) with the library's stage timing (the
+deposition kernel and each FFT pass kernel timed by events recorded with
+their own dispatch) for the roofline and the per-pass FFT figures, whatever
+the step count.  The CPU
 baseline runs one whole frame through the FP64 numpy oracle (the reference's
 per-atom op sequence) on one host core; the precision leg compares the FP32
 and FP64 plans with that oracle frame.
@@ -128,7 +129,7 @@ def copy_calibration(nbytes=256 << 20):
     print(json.dumps({"copy_bytes": nbytes, "reps": 3}), flush=True)
 
 
-def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream):
+def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream, warm=100):
     """Each stage alone: one-frame calls of the library's frame loop with stage
     timing on, queued back to back (each call joins back into the stream the
     next one starts from, so every stage of a frame runs by itself: the
@@ -136,12 +137,21 @@ def isolated_stages(plan, d_coords, d_types, d_scale, box, s, nframes, stream):
     frame's prepare after them).  Prepare: events on its stream around the
     whole stage.  The deposition's tiled kernel and each FFT pass's kernel
     record their events with their own dispatch (begin and end of the kernel:
-    the durations a rocprofv3 kernel trace reports for them)."""
+    the durations a rocprofv3 kernel trace reports for them).  `warm`
+    untimed one-frame calls first, so that a short run is timed at the same
+    steady clocks as a long one; the frames are taken in turn from the ones
+    on the device."""
     import torch
+    nf = int(d_coords.shape[0])
+    for t in range(warm):
+        i = t % nf
+        plan.accumulate_frames(d_coords[i:i + 1], d_types, d_scale[i:i + 1], box, s.arith, s.wrap)
+    torch.cuda.synchronize()
     plan.set_stage_timing(1)
     torch.cuda.synchronize()
     for t in range(nframes):
-        plan.accumulate_frames(d_coords[t:t + 1], d_types, d_scale[t:t + 1], box, s.arith,
+        i = t % nf
+        plan.accumulate_frames(d_coords[i:i + 1], d_types, d_scale[i:i + 1], box, s.arith,
                                s.wrap)
     st = plan.stage_times()
     plan.set_stage_timing(0)
@@ -158,11 +168,12 @@ def paired_x(plan, d_coords, d_types, d_scale, box, s, npairs):
     frame's as zero, so the average is the per-frame cost.  The launch runs
     alone: the next call starts after the previous one has joined its stream."""
     import torch
+    nf = int(d_coords.shape[0])
     plan.set_stage_timing(1)
     torch.cuda.synchronize()
     for k in range(npairs):
-        plan.accumulate_frames(d_coords[2 * k:2 * k + 2], d_types, d_scale[2 * k:2 * k + 2], box,
-                               s.arith, s.wrap)
+        i = (2 * k) % (nf - 1)
+        plan.accumulate_frames(d_coords[i:i + 2], d_types, d_scale[i:i + 2], box, s.arith, s.wrap)
     st = plan.stage_times()
     plan.set_stage_timing(0)
     torch.cuda.synchronize()
@@ -376,7 +387,7 @@ def mono_leg(coords, dims, typ, rad, sres, steps, warmup):
     torch.cuda.synchronize()
     plan.check_errors()
     ms = e0.elapsed_time(e1)
-    iso = isolated_stages(plan, d_c, d_t, d_s, box, s, min(steps, 30), stream)
+    iso = isolated_stages(plan, d_c, d_t, d_s, box, s, 30, stream)
     plan.close()
     return {"workload": "hii_phase_256^3_50k_atoms in the 120-degree cell (main_gromacs.py "
                         "-ct 120 transform)", "frames": steps, "frames_per_s": steps / (ms * 1e-3),
@@ -461,11 +472,10 @@ def main():
     log(f"{args.steps / (total_ms * 1e-3):.1f} frames/s")
     iso = None
     if not args.no_isolated:
-        iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, min(args.steps, 30), stream)
+        iso = isolated_stages(plan, d_coords, d_types, d_scale, box, s, 30, stream)
         log("per-stage avg ms (isolated): " + json.dumps({k: round(v, 4) for k, v in iso.items()}))
-        if args.steps >= 2:
-            iso["fft_x_pair"] = paired_x(plan, d_coords, d_types, d_scale, box, s,
-                                         min(args.steps // 2, 15))
+        if d_coords.shape[0] >= 2:
+            iso["fft_x_pair"] = paired_x(plan, d_coords, d_types, d_scale, box, s, 15)
             log(f"x pass per frame, frames in pairs: {iso['fft_x_pair']:.4f} ms")
 
     n = [int(x) for x in s.N]
