@@ -183,8 +183,9 @@ int mdsf_fft_pass(mdsf_plan* plan, int32_t pass, const void* d_density, mdsf_str
 int mdsf_set_frame_buffer(mdsf_plan* plan, int32_t which);
 
 /* Stage timing of mdsf_accumulate_frames (off by default): with enable = 1
- * the frame loop records HIP events for each frame's preparation (around the
- * stage, on its stream), deposition and three FFT passes (with the tiled
+ * the frame loop records HIP events for each frame's preparation (from its
+ * start on the caller's stream to the end of the later of its caller-stream
+ * part and its tile-list branch), deposition and three FFT passes (with the tiled
  * deposition kernel's and each pass kernel's own dispatch); with enable = 2
  * only for the deposition (the least perturbation of the overlapped loop).
  * Timed frames accumulate over calls; mdsf_stage_times waits for them and

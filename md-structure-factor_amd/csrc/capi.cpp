@@ -550,7 +550,7 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   int64_t tb = 0;
   if (p->timing) {
     tb = p->timed_frames < kMaxTimedFrames ? p->timed_frames : 0;
-    while ((int64_t)p->tev.size() < 10 * (tb + nframes)) {
+    while ((int64_t)p->tev.size() < kTimedEvents * (tb + nframes)) {
       hipEvent_t e;
       MDSF_HIP(hipEventCreate(&e));
       p->tev.push_back(e);
@@ -561,7 +561,7 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   // frame on its stream, the least perturbation of the overlapped loop)
   auto mark = [&](int64_t t, int k, hipStream_t st) -> hipError_t {
     if (!p->timing || (p->timing == 2 && k != 2 && k != 3)) return hipSuccess;
-    return hipEventRecord(p->tev[10 * (tb + t) + k], st);
+    return hipEventRecord(p->tev[kTimedEvents * (tb + t) + k], st);
   };
   MDSF_HIP(hipEventRecord(p->ev_fork, s));
   MDSF_HIP(hipStreamWaitEvent(d, p->ev_fork, 0));
@@ -580,22 +580,25 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
     // the deposition waits for the tile-list branch itself (one cross-stream
-    // wait instead of two in a row through `s`), except under stage timing,
-    // whose prepare interval ends when every branch has
-    const bool join = p->timing == 1;
+    // wait instead of two in a row through `s`); under stage timing the
+    // branch records its own end (event 10), and the prepare interval ends
+    // with the later of the two
+    const bool join = false;
+    const bool lsplit = split && p->tl_cap > 0 && p->sys.natoms > 0;  // lists on aux_stream
+    hipEvent_t aux_end = (p->timing == 1 && lsplit) ? p->tev[kTimedEvents * (tb + t) + 10] : nullptr;
     MDSF_HIP(launch_binning(*p, s, false, split ? p->aux_stream : nullptr, split ? d : nullptr,
-                            join));
+                            join, aux_end));
     MDSF_HIP(mark(t, 1, s));
+    if (p->timing == 1 && !aux_end) MDSF_HIP(mark(t, 10, s));
     MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));
-    if (split && !join && p->tl_cap > 0 && p->sys.natoms > 0)
-      MDSF_HIP(hipStreamWaitEvent(d, p->ev_ljoin, 0));
+    if (lsplit && !join) MDSF_HIP(hipStreamWaitEvent(d, p->ev_ljoin, 0));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_fft[b], 0));
     // the deposition's events are recorded with the tiled kernel's own
     // dispatch (its begin and end, without the queue's dispatch latency)
     if (p->timing) {
-      p->dep_ev[0] = p->tev[10 * (tb + t) + 2];
-      p->dep_ev[1] = p->tev[10 * (tb + t) + 3];
+      p->dep_ev[0] = p->tev[kTimedEvents * (tb + t) + 2];
+      p->dep_ev[1] = p->tev[kTimedEvents * (tb + t) + 3];
     }
     const hipError_t de = launch_deposit(*p, p->rho, d);
     p->dep_ev[0] = p->dep_ev[1] = nullptr;
@@ -627,7 +630,8 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
         continue;
       }
       if (p->timing == 1)
-        fft_pass_events(p->tev[10 * (tb + t) + 4 + 2 * pass], p->tev[10 * (tb + t) + 5 + 2 * pass]);
+        fft_pass_events(p->tev[kTimedEvents * (tb + t) + 4 + 2 * pass],
+                        p->tev[kTimedEvents * (tb + t) + 5 + 2 * pass]);
       const hipError_t fe = (pass == 2 && paired)
                                 ? fft_x_pair(*p, (int)((t - 1) % kSlots), b, f)
                                 : fft_pass(*p, pass, p->rho, f);
@@ -713,13 +717,22 @@ int mdsf_stage_times(mdsf_plan* plan, double* avg_ms) {
   for (int k = 0; k < 5; ++k) avg_ms[k] = 0.0;
   if (n == 0) return fail(MDSF_ERR_ARG, "no timed mdsf_accumulate_frames call");
   const bool all = p->timing != 2;
-  if (all) MDSF_HIP(hipEventSynchronize(p->tev[10 * (n - 1) + 9]));
-  MDSF_HIP(hipEventSynchronize(p->tev[10 * (n - 1) + 3]));
+  if (all) MDSF_HIP(hipEventSynchronize(p->tev[kTimedEvents * (n - 1) + 9]));
+  MDSF_HIP(hipEventSynchronize(p->tev[kTimedEvents * (n - 1) + 3]));
   for (int64_t t = 0; t < n; ++t) {
     for (int k = 0; k < 5; ++k) {
       if (!all && k != 1) continue;
       float ms = 0.f;
-      MDSF_HIP(hipEventElapsedTime(&ms, p->tev[10 * t + 2 * k], p->tev[10 * t + 2 * k + 1]));
+      MDSF_HIP(hipEventElapsedTime(&ms, p->tev[kTimedEvents * t + 2 * k],
+                                   p->tev[kTimedEvents * t + 2 * k + 1]));
+      if (k == 0) {
+        // the prepare stage ends with the later of its caller-stream part and
+        // its tile-list branch
+        float ms_aux = 0.f;
+        MDSF_HIP(hipEventSynchronize(p->tev[kTimedEvents * t + 10]));
+        MDSF_HIP(hipEventElapsedTime(&ms_aux, p->tev[kTimedEvents * t], p->tev[kTimedEvents * t + 10]));
+        ms = ms_aux > ms ? ms_aux : ms;
+      }
       avg_ms[k] += ms;
     }
   }

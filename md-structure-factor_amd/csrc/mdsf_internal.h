@@ -20,6 +20,10 @@ constexpr int kMaxSpillBufTiles = 4096;  // spill tiles staged per frame slot (e
 constexpr int kCornerCap = 512;     // corner atoms listed per frame for the spill pass
 constexpr int kListNoSlice = 255;  // tile-list entry: no slice on that axis (a byte per axis)
 constexpr long long kSplitPrepTiles = 2048;  // tiles from which the frame loop splits its prepare stage
+// stage-timing events per frame: prepare start / end (caller's stream), the
+// deposition kernel's begin / end, the three FFT passes' begin / end, and the
+// end of the prepare stage's tile-list branch
+constexpr int kTimedEvents = 11;
 constexpr int64_t kMaxTimedFrames = 1 << 16;  // stage-timing events kept before they start over
 
 enum Side { SIDE_L = 0, SIDE_C = 1, SIDE_H = 2 };
@@ -288,11 +292,13 @@ hipError_t launch_prep(Plan& p, void* coords, int dtype, int arith, const int32_
 // dslices allocated) also the slices' derivatives for the tiled adjoint.
 // With `aux`, the tile lists are built on that stream, in parallel with the
 // rest, and joined back into `s` (without `join`, the caller waits for
-// p.ev_ljoin on the stream that reads them); with `side`, the corner list and
-// the spill staging go on that stream (the one the frame's deposition then
-// runs on), after the sorted atoms are ready.
+// p.ev_ljoin on the stream that reads them; `aux_end`, if given, is recorded
+// on `aux` after the lists); with `side`, the corner list and the spill
+// staging go on that stream (the one the frame's deposition then runs on),
+// after the sorted atoms are ready.
 hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs = false, hipStream_t aux = nullptr,
-                          hipStream_t side = nullptr, bool join = true);
+                          hipStream_t side = nullptr, bool join = true,
+                          hipEvent_t aux_end = nullptr);
 // only the slice derivatives of the last sorted frame
 hipError_t launch_slice_derivs(Plan& p, hipStream_t s);
 hipError_t launch_deposit(Plan& p, void* out, hipStream_t s);

@@ -496,7 +496,7 @@ hipError_t launch_slice_derivs(Plan& p, hipStream_t s) {
 }
 
 hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux,
-                          hipStream_t side, bool join) {
+                          hipStream_t side, bool join, hipEvent_t aux_end) {
   const int n = p.sys.natoms;
   const int nbk = p.sys.nbuckets;
   const int nblk = (nbk + kScanThreads - 1) / kScanThreads;
@@ -533,6 +533,7 @@ hipError_t launch_binning(Plan& p, hipStream_t s, bool derivs, hipStream_t aux,
         (e = launch_tile_lists(p, aux)) != hipSuccess ||
         (e = hipEventRecord(p.ev_ljoin, aux)) != hipSuccess)
       return e;
+    if (aux_end && (e = hipEventRecord(aux_end, aux)) != hipSuccess) return e;
   } else {
     if ((e = launch_tile_regions(p, s)) != hipSuccess || (e = launch_tile_lists(p, s)) != hipSuccess)
       return e;
