@@ -109,3 +109,46 @@ def test_stage_timing_accumulates_over_calls(rad, gpu):
     p.set_stage_timing(0)
     p.check_errors()
     p.close()
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_split_prepare_stage(fp64, rad, gpu):
+    """A grid of 2,048 tiles (256 x 256 x 128), where the frame loop runs the
+    prepare stage in branches (tile lists on a third stream, the deposition
+    waiting for them directly; FP64 plans have no lists): S(q) of 4 frames
+    bitwise equal with and without stage timing (which records the list
+    branch's end) and equal to the stage-by-stage loop; every stage time is
+    positive."""
+    from mdsf import engine, synth
+    coords, dims, typ = synth.uniform(21, 3000, 40.0, ["H", "C", "O"], 4)
+    s = engine.Setup(coords, dims, typ, rad, np.array([40 / 256.5, 40 / 256.5, 40 / 128.5]),
+                     False, np.eye(3))
+    assert list(s.N) == [256, 256, 128]
+    dev = torch.device("cuda", 0)
+    types = torch.from_numpy(s.type_index.astype(np.int32).reshape(-1)).to(dev)
+    scale = torch.from_numpy(np.ascontiguousarray(s.scale, dtype=np.float64)).to(dev)
+    box = [float(x) for x in s.L]
+    out = []
+    for timing in (0, 1):
+        p = engine.Plan.from_setup(s, fft64=fp64)
+        p.set_stage_timing(timing)
+        c = torch.from_numpy(coords.copy()).to(dev)
+        p.accumulate_frames(c, types, scale, box, s.arith, s.wrap)
+        p.check_errors()
+        out.append(p.sf().cpu().numpy())
+        if timing:
+            times = p.stage_times()
+            assert all(v > 0 for v in times.values()), times
+        p.close()
+    assert np.array_equal(out[0], out[1])
+    pb = engine.Plan.from_setup(s, fft64=fp64)
+    cb = torch.from_numpy(coords.copy()).to(dev)
+    for t in range(4):
+        pb.set_frame_buffer(t % 3)
+        pb.prepare_frame(cb[t], types, scale[t], box, s.arith, s.wrap)
+        pb.deposit()
+        for i in range(3):
+            pb.fft_pass(i)
+    pb.check_errors()
+    assert np.array_equal(out[0], pb.sf().cpu().numpy())
+    pb.close()
