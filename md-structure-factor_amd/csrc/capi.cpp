@@ -530,14 +530,13 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
   // corner list and spill staging on dep_stream ahead of its deposition),
   // deposited on dep_stream and transformed on fft_stream, so that while
   // frame t is deposited frame t + 1 is prepared and earlier frames
-  // transformed.  A slot is reused kSlots
-  // frames later: preparing frame t waits until frame t - kSlots's deposition
-  // (tiled and parked passes) has read its atom state, depositing it until
-  // frame t - kSlots's FFT has released its density buffer.  With three
-  // slots the deposition does not
+  // transformed.  A slot is reused kSlots frames later: preparing frame t
+  // waits until frame t - kSlots's deposition (tiled and parked passes) has
+  // read its atom state, depositing it until frame t - kSlots's FFT has
+  // released its density buffer.  With three slots the deposition does not
   // wait for the FFT of the frame just before the previous one (the FFT
-  // shares the GPU with the deposition and finishes later than it).  The S(q)
-  // sum stays in frame order on one stream.
+  // shares the GPU with the deposition and finishes later than it).  The
+  // S(q) sum stays in frame order on one stream.
   hipStream_t d = p->dep_stream, f = p->fft_stream;
   // the prepare stage's branches go on their own streams on large grids; on
   // small ones the cross-stream waits cost more than the branches overlap
@@ -580,19 +579,19 @@ int mdsf_accumulate_frames(mdsf_plan* plan, void* d_coords, int dtype, int64_t n
     MDSF_HIP(launch_prep(*p, c, dtype, arith, d_types, d_scale + 3 * t, box, wrap_lo, wrap_hi,
                          d_occupancy, home, s));
     // the deposition waits for the tile-list branch itself (one cross-stream
-    // wait instead of two in a row through `s`); under stage timing the
-    // branch records its own end (event 10), and the prepare interval ends
-    // with the later of the two
-    const bool join = false;
+    // wait instead of two in a row through `s`, so the binning does not join
+    // the branch back into `s`); under stage timing the branch records its
+    // own end (event 10), and the prepare interval ends with the later of the
+    // two
     const bool lsplit = split && p->tl_cap > 0 && p->sys.natoms > 0;  // lists on aux_stream
     hipEvent_t aux_end = (p->timing == 1 && lsplit) ? p->tev[kTimedEvents * (tb + t) + 10] : nullptr;
     MDSF_HIP(launch_binning(*p, s, false, split ? p->aux_stream : nullptr, split ? d : nullptr,
-                            join, aux_end));
+                            /*join=*/false, aux_end));
     MDSF_HIP(mark(t, 1, s));
     if (p->timing == 1 && !aux_end) MDSF_HIP(mark(t, 10, s));
     MDSF_HIP(hipEventRecord(p->ev_prep[b], s));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_prep[b], 0));
-    if (lsplit && !join) MDSF_HIP(hipStreamWaitEvent(d, p->ev_ljoin, 0));
+    if (lsplit) MDSF_HIP(hipStreamWaitEvent(d, p->ev_ljoin, 0));
     MDSF_HIP(hipStreamWaitEvent(d, p->ev_fft[b], 0));
     // the deposition's events are recorded with the tiled kernel's own
     // dispatch (its begin and end, without the queue's dispatch latency)
