@@ -1413,8 +1413,10 @@ __global__ __launch_bounds__(kTileScanThreads) void k_tile_scan(int4* __restrict
 }
 
 // Candidates per thread and phase-B round of k_tile_lists (consecutive ones,
-// their searches and atom loads in flight together)
-constexpr int kListUnroll = 4;
+// their searches and atom loads in flight together).  Measured at 256^3,
+// 50 k atoms, pipelined: 1, 2, 3, 4, 8 gave 4,202, 4,216, 4,173, 4,140,
+// 3,922 frames/s (fewer registers, more blocks per CU beside the deposition)
+constexpr int kListUnroll = 2;
 
 template <int MT>
 __global__ __launch_bounds__(kThreads) void k_tile_lists(SysDev sys, const TypeDev* __restrict__ td,
