@@ -14,8 +14,9 @@ deposition kernel and each FFT pass kernel timed by events recorded with
 their own dispatch) for the roofline and the per-pass FFT figures, whatever
 the step count.  The CPU
 baseline runs one whole frame through the FP64 numpy oracle (the reference's
-per-atom op sequence) on one host core; the precision leg compares the FP32
-and FP64 plans with that oracle frame.
+per-atom op sequence) on one host core, and, as a stronger baseline, one frame
+per host core at once in separate processes (`cpu_baseline_all_cores`, up to
+16); the precision leg compares the FP32 and FP64 plans with that oracle frame.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu]
 """
@@ -117,6 +118,55 @@ def cpu_baseline(coords0, dims, typ, rad, sres):
         "host": f"{_cpu_name()}; nproc={os.cpu_count()}; {platform.machine()}",
     }
     return rec, sf
+
+
+def _oracle_frame_seconds(job):
+    """Worker of the all-cores baseline: one frame through the FP64 numpy
+    oracle (deposition, fold, rfftn + |F|^2) in this process, on one thread;
+    returns its wall time."""
+    r, dims, typ, rad, sres = job
+    from threadpoolctl import threadpool_limits
+    from oracle import dens_oracle as O
+    with threadpool_limits(limits=1):
+        o = O.setup(r, dims, typ, rad, sres, better=False)
+        n, nb = o["N"], o["Nb"]
+        padded = tuple(int(n[i]) + 2 * nb for i in range(3))
+        t0 = time.perf_counter()
+        d0, _ = O.deposit_frame(r[0], list(typ), rad, o["bdict"], o["dr"], nb, np.eye(3), padded)
+        d1 = O.fold(d0, *O.fold_tables(n, nb))
+        del d0
+        O.frame_power(d1)
+        return time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(coords, dims, typ, rad, sres, max_procs=16):
+    """The stronger CPU baseline: P frames of the workload through the FP64
+    numpy oracle at once, one process and one thread per frame, P = the
+    host cores this process may use (at most max_procs, at most the frames);
+    frames/s = P / the wall time of the P frames (worker start-up excluded)."""
+    import multiprocessing as mp
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    nproc = max(1, min(max_procs, ncpu, coords.shape[0]))
+    jobs = [(np.array(coords[i][None], copy=True), np.array(dims[i:i + 1]), typ, rad, sres)
+            for i in range(nproc)]
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        # start the workers (imports) before the clock; both waits are bounded
+        pool.map_async(abs, range(nproc)).get(timeout=120)
+        t0 = time.perf_counter()
+        per = pool.map_async(_oracle_frame_seconds, jobs, chunksize=1).get(timeout=600)
+        wall = time.perf_counter() - t0
+    return {
+        "value": nproc / wall,
+        "unit": "frames/s",
+        "cores": nproc,
+        "kind": "port",
+        "sample": (f"frames 0..{nproc - 1} of the workload, one per process, through the FP64 "
+                   f"numpy oracle: {wall:.1f} s wall, {min(per):.1f}-{max(per):.1f} s per frame"),
+        "host": f"{_cpu_name()}; usable cores={ncpu}; {platform.machine()}",
+    }
 
 
 def copy_calibration(nbytes=256 << 20):
@@ -404,6 +454,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline leg (and the precision leg's oracle comparison)")
+    ap.add_argument("--no-cpu-all", action="store_true",
+                    help="skip the all-cores CPU baseline (one oracle frame per host core)")
     ap.add_argument("--fft", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+bwd leg")
     ap.add_argument("--no-precision", action="store_true",
@@ -438,12 +490,18 @@ def main():
     from mdsf import engine, synth
 
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     rad = dens.load_radii(os.path.join(PKG, "radii.txt"))
     nframes = max(args.steps, args.warmup, 1)
     log(f"generating workload: {nframes} frames, 50,000 atoms")
     coords, dims, typ = synth.hii(seed=2002, nframes=nframes)
     sres = synth.hii_sres()
+    cpu_all = None
+    if not args.no_cpu and not args.no_cpu_all:
+        # before this process touches the GPU: the worker processes start
+        # from a process without a GPU context
+        log("cpu baseline, all cores: one frame per process through the FP64 numpy oracle")
+        cpu_all = cpu_baseline_all_cores(coords, dims, typ, rad, sres)
+    torch.cuda.set_device(dev)
     s = engine.Setup(coords, dims, typ, rad, sres, False, np.eye(3))
     assert list(s.N) == [256, 256, 256], s.N
     plan = engine.Plan.from_setup(s, fft64=(args.fft == "fp64"), autotune=not args.no_autotune)
@@ -539,6 +597,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline: frame 0 through the FP64 numpy oracle on one core")
         out["cpu_baseline"], oracle_sf = cpu_baseline(coords[0], dims, typ, rad, sres)
+        if cpu_all is not None:
+            out["cpu_baseline_all_cores"] = cpu_all
     if not args.no_precision:
         log("precision: fp32 / fp64 plans vs the oracle (frame 0), fp32 vs fp64 (2 frames)")
         out["precision"] = precision_leg(coords, s, oracle_sf)
