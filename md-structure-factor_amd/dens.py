@@ -51,6 +51,17 @@ NPZ_THREADS = None
 # several times faster and somewhat larger files that np.load reads the same)
 NPZ_LEVEL = 6
 
+# Frames per checkpoint block (0: off).  With n > 0, compute_sf runs the
+# frames in blocks of n, adds each block's FP64 sum to a running total in
+# frame order and, after each block, saves (frames done, the FP64 total) to
+# out_filename + ".ckpt.npz"; a later call on the same inputs (coordinates,
+# boxes, types, Sres, cell, grid) resumes after the last block saved, and
+# the result is bitwise that of an uninterrupted call with the same n.  It
+# can differ in the last bits from a call without checkpoints (blocks summed
+# apart, then added).  The file is removed once the output is written.  The
+# reference saves only at the end (dens.py:346); RANDOM_NOISE runs ignore it.
+CHECKPOINT_FRAMES = 0
+
 # Wall-clock seconds of the phases of the last compute_sf call: "setup",
 # "upload", "plan" (plan creation, with the "auto" precision check on frame
 # 0), "frames", "outputs" (S(q) and the output grids on the GPU), "write"
@@ -194,6 +205,45 @@ def _print_details(L, Sres, N):
     print("=" * 50)
 
 
+def _checkpoint_key(r, L, typ, Sres, ucell, N):
+    """Fingerprint of a compute_sf call's inputs (before the in-place rescale)."""
+    import hashlib
+    h = hashlib.blake2b(digest_size=16)
+    for a in (np.ascontiguousarray(r), np.asarray(L, dtype=np.float64),
+              np.asarray(ucell, dtype=np.float64), np.asarray(N, dtype=np.int64)):
+        h.update(str(a.dtype).encode() + str(a.shape).encode())
+        h.update(a.tobytes())
+    h.update(("\0".join(str(t) for t in typ) + "|" + repr(float(Sres))).encode())
+    return h.hexdigest()
+
+
+def _checkpoint_path(out_filename):
+    return str(out_filename) + ".ckpt.npz"
+
+
+def _checkpoint_save(out_filename, key, frames_done, total):
+    """Writes the checkpoint through a temporary file and an atomic rename."""
+    import os
+    path = _checkpoint_path(out_filename)
+    tmp = path + ".tmp.npz"
+    np.savez(tmp, key=np.frombuffer(key.encode(), dtype=np.uint8),
+             frames_done=np.int64(frames_done), sf=total)
+    os.replace(tmp, path)
+
+
+def _checkpoint_load(out_filename, key, shape):
+    """(frames done, FP64 partial S(q)) of a checkpoint of these inputs, or
+    (0, None) when there is none or it belongs to other inputs."""
+    import os
+    path = _checkpoint_path(out_filename)
+    if not os.path.exists(path):
+        return 0, None
+    with np.load(path, allow_pickle=False) as z:
+        if bytes(z["key"]).decode() != key or tuple(z["sf"].shape) != tuple(shape):
+            return 0, None
+        return int(z["frames_done"]), np.array(z["sf"], dtype=np.float64)
+
+
 def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     """Compute the 3-D structure factor and save it to ``out_filename + '.npz'``.
 
@@ -210,6 +260,8 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     if ucell is None:
         ucell = np.eye(3)
     s = engine.Setup(r, L, typ, rad, Sres, USE_BETTER_RESOLUTION, ucell)
+    ckpt = CHECKPOINT_FRAMES > 0 and RANDOM_NOISE <= 0 and s.nframes > 0
+    key = _checkpoint_key(r, L, typ, Sres, ucell, s.N) if ckpt else None
     if PRINT_DETAILS:
         _print_details(s.L, Sres, s.N)
     dev = engine._device(None)
@@ -228,12 +280,27 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
         for _ in range(s.nframes):
             d1 = np.random.rand(int(s.N[0]), int(s.N[1]), int(s.N[2]))   # dens.py:280
             plan.accumulate_density(torch.from_numpy(d1))   # cast to the plan precision
+    elif ckpt:
+        done, part = _checkpoint_load(out_filename, key, plan.half_shape)
+        total = torch.from_numpy(part).to(dev) if part is not None else None
+        if done > 0:   # frames already summed: only their in-place rescale
+            plan.rescale_wrap(coords[:done], scale[:done], box, s.arith, s.wrap)
+        k = done
+        while k < s.nframes:
+            c = min(int(CHECKPOINT_FRAMES), s.nframes - k)
+            plan.reset()
+            plan.accumulate_frames(coords[k:k + c], types, scale[k:k + c], box, s.arith, s.wrap)
+            plan.check_errors()
+            block = plan.sf()
+            total = block if total is None else total + block
+            k += c
+            _checkpoint_save(out_filename, key, k, total.cpu().numpy())
     else:
         plan.accumulate_frames(coords, types, scale, box, s.arith, s.wrap)
     plan.check_errors()
     t3 = time.perf_counter()
     r[...] = coords.cpu().numpy()
-    sf_dev = plan.sf()
+    sf_dev = total if ckpt else plan.sf()
     sfplt, kgrid, kgridplt = plan.outputs(sf_dev, box, s.box_f32)
     stream = torch.cuda.current_stream(dev)
 
@@ -248,6 +315,9 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
                    kgrid=pool.submit(fetch, kgrid), kgridplt=pool.submit(fetch, kgridplt))
         t4 = time.perf_counter()
         npzio.savez_compressed(out_filename, threads=NPZ_THREADS, level=NPZ_LEVEL, **out)
+    if ckpt:
+        import os
+        os.remove(_checkpoint_path(out_filename))
     plan.close()
     del sfplt, kgrid, kgridplt, sf_dev, out
     t5 = time.perf_counter()
@@ -255,3 +325,5 @@ def compute_sf(r, L, typ, out_filename, rad, ucell=None, Sres=1.0):
     LAST_TIMINGS.update(setup=t1 - t0, upload=t2 - t1, plan=t2b - t2, frames=t3 - t2b,
                         outputs=t4 - t3,
                         write=t5 - t4)
+    if ckpt:
+        LAST_TIMINGS["resumed_from"] = done
